@@ -1,0 +1,143 @@
+"""ctypes binding to libfmm2d_amd.so (the C-ABI declared in include/fmm2d_amd.h).
+
+The library must have been built (``python -c "import __graft_entry__ as g;
+g.build()"`` or ``make -C fmm2d_amd/csrc``).  There is no CPU fallback: every
+entry point raises if the library or a ROCm device is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libfmm2d_amd.so")
+
+_lib = None
+
+# (name, restype, argtypes) for every exported symbol of include/fmm2d_amd.h
+_vp, _i, _i64, _d = C.c_void_p, C.c_int, C.c_int64, C.c_double
+SIGNATURES = {
+    "fmm2d_abi_version": (_i, []),
+    "fmm2d_c2d_direct": (_i, [_i, _vp, _i64, _i, _vp, _i, _vp, _vp, _i64, _i, _vp, _vp, _vp,
+                              _d, _vp]),
+    "fmm2d_l2d_direct": (_i, [_i, _vp, _i64, _i, _vp, _i, _vp, _vp, _vp, _i64, _i, _vp, _vp,
+                              _vp, _d, _vp]),
+    "fmm2d_h2d_direct": (_i, [_i, _d, _d, _vp, _i64, _i, _vp, _i, _vp, _vp, _vp, _i64, _i, _vp,
+                              _vp, _vp, _d, _vp]),
+    "fmm2d_hank103": (_i, [_vp, _i64, _vp, _vp, _vp]),
+}
+# FMM drivers: (nd, eps, [zk_re, zk_im,] ns, sources, ifcharge, charge, ifdipole,
+# dipstr, [dipvec,] iper, ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg,
+# gradtarg, hesstarg, opts, stream)
+_FMM_TAIL = [_i64, _vp, _i, _vp, _i, _vp]
+_FMM_OUT = [_i, _i, _vp, _vp, _vp, _i64, _vp, _i, _vp, _vp, _vp, _vp, _vp]
+SIGNATURES["fmm2d_cfmm2d"] = (_i, [_i, _d] + _FMM_TAIL + _FMM_OUT)
+SIGNATURES["fmm2d_lfmm2d"] = (_i, [_i, _d] + _FMM_TAIL + [_vp] + _FMM_OUT)
+SIGNATURES["fmm2d_hfmm2d"] = (_i, [_i, _d, _d, _d] + _FMM_TAIL + [_vp] + _FMM_OUT)
+
+
+class FmmOpts(C.Structure):
+    """struct fmm2d_opts of include/fmm2d_amd.h."""
+    _fields_ = [("ndiv", C.c_int32), ("idivflag", C.c_int32), ("ifnear", C.c_int32),
+                ("precision", C.c_int32), ("n_nterms", C.c_int32), ("pad0", C.c_int32),
+                ("nterms", C.POINTER(C.c_int32)), ("timeinfo", C.c_double * 16),
+                ("nlevels", C.c_int32), ("nboxes", C.c_int32), ("pad1", C.c_int32),
+                ("pad2", C.c_int32)]
+
+    def timeinfo_array(self):
+        return np.array(self.timeinfo[:], dtype=np.float64)
+
+
+def opts_struct(opts=None):
+    """Build an FmmOpts from a dict: ndiv, idivflag, ifnear, precision ("fp64" or
+    "mixed"), nterms (per-level list).  Missing keys take the reference defaults."""
+    o = FmmOpts()
+    opts = opts or {}
+    o.ndiv = int(opts.get("ndiv", 0))
+    o.idivflag = int(opts.get("idivflag", 0))
+    o.ifnear = int(opts.get("ifnear", 1))
+    prec = opts.get("precision", "fp64")
+    o.precision = {"fp64": 0, "mixed": 1}[prec] if isinstance(prec, str) else int(prec)
+    nt = opts.get("nterms")
+    if nt is not None:
+        arr = (C.c_int32 * len(nt))(*[int(v) for v in nt])
+        o._nterms_keep = arr
+        o.nterms = C.cast(arr, C.POINTER(C.c_int32))
+        o.n_nterms = len(nt)
+    return o
+
+
+def byref_opts(o):
+    return C.byref(o)
+
+
+class NativeLibraryError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libfmm2d_amd.so once; raise NativeLibraryError if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise NativeLibraryError(
+                f"{LIBPATH} is missing: build it with `make -C fmm2d_amd/csrc` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue  # reported by the symbol-export test; calling it raises
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def torch_device():
+    """The ROCm device used for all work; raises when no GPU is present."""
+    import torch
+    if not torch.cuda.is_available():
+        raise NativeLibraryError("fmm2d_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def check(ier, what):
+    if ier < 0:
+        raise RuntimeError(f"fmm2d_amd: {what} failed with code {ier}")
+    return ier
+
+
+# --------------------------------------------------------------- layouts
+def to_dev_fortran(a, dtype):
+    """Device tensor holding ``a`` in Fortran (column-major) memory order.
+
+    A Fortran array of shape (n1,...,nk) has the memory of a C array of
+    shape (nk,...,n1); the returned tensor has that reversed shape.
+    """
+    import torch
+    arr = np.ascontiguousarray(np.transpose(np.asarray(a, dtype=dtype)))
+    return torch.from_numpy(arr).to(torch_device())
+
+
+def dev_zeros_fortran(shape, dtype):
+    import torch
+    return torch.zeros(tuple(reversed(shape)), dtype=dtype, device=torch_device())
+
+
+def fortran_to_numpy(t):
+    """Inverse of dev_zeros_fortran: a numpy array with the Fortran shape
+    (Fortran-ordered strides, as f2py returns)."""
+    return np.transpose(t.cpu().numpy())

@@ -1,0 +1,85 @@
+// Shared types and helpers for the fmm2d_amd HIP library (gfx950).
+//
+// Complex numbers are a plain {re, im} pair of doubles: layout-compatible with
+// C99 `double complex`, numpy complex128 and Fortran complex*16, which is what
+// the C-ABI hands us.  All arithmetic is written out so the compiler sees
+// ordinary FP64 operations (no library complex type on the device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cmath>
+
+#define FMM2D_HD __host__ __device__ __forceinline__
+
+struct cplx {
+  double re, im;
+};
+
+FMM2D_HD cplx mkc(double r, double i = 0.0) { return cplx{r, i}; }
+FMM2D_HD cplx operator+(cplx a, cplx b) { return cplx{a.re + b.re, a.im + b.im}; }
+FMM2D_HD cplx operator-(cplx a, cplx b) { return cplx{a.re - b.re, a.im - b.im}; }
+FMM2D_HD cplx operator-(cplx a) { return cplx{-a.re, -a.im}; }
+FMM2D_HD cplx operator*(cplx a, cplx b) {
+  return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+FMM2D_HD cplx operator*(double s, cplx a) { return cplx{s * a.re, s * a.im}; }
+FMM2D_HD cplx operator*(cplx a, double s) { return cplx{s * a.re, s * a.im}; }
+FMM2D_HD cplx& operator+=(cplx& a, cplx b) { a.re += b.re; a.im += b.im; return a; }
+FMM2D_HD cplx& operator-=(cplx& a, cplx b) { a.re -= b.re; a.im -= b.im; return a; }
+FMM2D_HD cplx conjc(cplx a) { return cplx{a.re, -a.im}; }
+FMM2D_HD double abs2(cplx a) { return a.re * a.re + a.im * a.im; }
+FMM2D_HD double absc(cplx a) { return hypot(a.re, a.im); }
+// a / b, scaled to avoid overflow for large |b| (Smith's algorithm).
+FMM2D_HD cplx divc(cplx a, cplx b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    double r = b.im / b.re, d = b.re + b.im * r;
+    return cplx{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  } else {
+    double r = b.re / b.im, d = b.re * r + b.im;
+    return cplx{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+  }
+}
+FMM2D_HD cplx operator/(cplx a, cplx b) { return divc(a, b); }
+FMM2D_HD cplx operator/(cplx a, double s) { return cplx{a.re / s, a.im / s}; }
+FMM2D_HD cplx invc(cplx b) { return divc(cplx{1.0, 0.0}, b); }
+FMM2D_HD cplx expc(cplx a) {
+  double e = exp(a.re), s, c;
+  sincos(a.im, &s, &c);
+  return cplx{e * c, e * s};
+}
+// principal log
+FMM2D_HD cplx logc(cplx a) { return cplx{log(absc(a)), atan2(a.im, a.re)}; }
+// principal square root (branch cut on the negative real axis)
+FMM2D_HD cplx sqrtc(cplx a) {
+  double m = absc(a);
+  if (m == 0.0) return cplx{0.0, 0.0};
+  double t = sqrt(0.5 * (m + fabs(a.re)));
+  if (a.re >= 0.0) return cplx{t, 0.5 * a.im / t};
+  return cplx{0.5 * fabs(a.im) / t, copysign(t, a.im)};
+}
+
+// ---- error codes shared by the C-ABI -------------------------------------
+// 0 = ok, 4 = workspace allocation failure (the reference's ier=4),
+// 13 = expansion-order search exhausted (the reference's ier=13),
+// negative = errors the reference cannot produce (bad argument, HIP failure).
+enum {
+  FMM2D_OK = 0,
+  FMM2D_ERR_ALLOC = 4,
+  FMM2D_ERR_NTERMS = 13,
+  FMM2D_ERR_ARG = -1,
+  FMM2D_ERR_HIP = -2,
+  FMM2D_ERR_UNSUPPORTED = -3,
+};
+
+#define FMM2D_HIP_TRY(expr)                                                   \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      fprintf(stderr, "fmm2d_amd: HIP error %s at %s:%d: %s\n",               \
+              hipGetErrorString(_e), __FILE__, __LINE__, #expr);              \
+      return _e == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;     \
+    }                                                                         \
+  } while (0)
+
+static inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }

@@ -1,0 +1,26 @@
+// Direct (all-pairs) evaluation entry used by the C-ABI.
+#pragma once
+#include "common.h"
+
+namespace fmm2d {
+
+struct DirectArgs {
+  int family;        // 0 Cauchy (c2d), 1 Laplace (l2d), 2 Helmholtz (h2d)
+  int nd;
+  cplx zk;           // Helmholtz only
+  const double* src; // (2,ns) column-major, device
+  int64_t ns;
+  int ifcharge, ifdipole;
+  const cplx* charge;   // (nd,ns)
+  const cplx* dipstr;   // (nd,ns)
+  const double* dipvec; // (nd,2,ns), Laplace/Helmholtz only
+  const double* targ;   // (2,nt)
+  int64_t nt;
+  int pg;               // 1 pot, 2 +grad, 3 +hess
+  double thresh;
+  cplx *pot, *grad, *hess;  // incremented
+};
+
+int direct_eval(const DirectArgs& a, hipStream_t st);
+
+}  // namespace fmm2d

@@ -1,0 +1,196 @@
+// All-pairs direct evaluation (the *2d_direct* evaluators and the l2ddir /
+// c2ddir / h2ddir Python entry points).
+//
+// Reference: laplace/cauchykernels2d.f, laplace/lapkernels2d.f,
+// helmholtz/helmkernels2d.f -- O(ns*nt) double loops that INCREMENT pot/grad/
+// hess.  Here: one thread per target, sources streamed through LDS in tiles of
+// 256, per-thread register accumulation over NDT densities.  When there are
+// too few target blocks to fill the 256 CUs the source range is split across
+// blocks; each split writes a private copy of the outputs and a second kernel
+// adds the copies into the caller's buffers in split order (deterministic).
+#include "p2p.h"
+#include "p2p_math.h"
+
+namespace fmm2d {
+
+constexpr int kTile = 256;
+
+template <int FAM, int IFC, int IFD, int NDT> struct TileSmem {
+  double x[kTile], y[kTile];
+  cplx ch[IFC ? kTile * NDT : 1];
+  cplx ds[IFD ? kTile * NDT : 1];
+  double dv[(IFD && FAM != CAUCHY) ? kTile * NDT * 2 : 1];
+};
+
+// Output layout (reference, column-major): pot(nd,n); Cauchy grad(nd,n),
+// hess(nd,n); Laplace/Helmholtz grad(nd,2,n), hess(nd,3,n).
+template <int FAM, int IFC, int IFD, int PG, int NDT>
+__global__ __launch_bounds__(kTile) void direct_kernel(
+    int nd, cplx zk, const double* __restrict__ src, int64_t ns, int64_t split_len,
+    const cplx* __restrict__ charge, const cplx* __restrict__ dipstr,
+    const double* __restrict__ dipvec, const double* __restrict__ targ, int64_t nt,
+    double thresh, cplx* __restrict__ pot, cplx* __restrict__ grad, cplx* __restrict__ hess,
+    int64_t split_stride_pot, int64_t split_stride_grad, int64_t split_stride_hess) {
+  constexpr int NG = FamDims<FAM>::NG, NH = FamDims<FAM>::NH;
+  __shared__ TileSmem<FAM, IFC, IFD, NDT> sm;
+  const int64_t it = (int64_t)blockIdx.x * kTile + threadIdx.x;
+  const int d0 = blockIdx.z * NDT;
+  const int64_t s_begin = (int64_t)blockIdx.y * split_len;
+  const int64_t s_end = min(ns, s_begin + split_len);
+  const bool active = it < nt;
+  double tx = 0.0, ty = 0.0;
+  if (active) { tx = targ[2 * it]; ty = targ[2 * it + 1]; }
+  const double thresh2 = thresh * thresh;
+  Acc<FAM, PG, NDT> acc;
+  acc.zero();
+  for (int64_t t0 = s_begin; t0 < s_end; t0 += kTile) {
+    const int nload = (int)min((int64_t)kTile, s_end - t0);
+    __syncthreads();
+    if ((int)threadIdx.x < nload) {
+      const int64_t j = t0 + threadIdx.x;
+      sm.x[threadIdx.x] = src[2 * j];
+      sm.y[threadIdx.x] = src[2 * j + 1];
+#pragma unroll
+      for (int d = 0; d < NDT; ++d) {
+        const bool ok = d0 + d < nd;
+        if (IFC) sm.ch[threadIdx.x * NDT + d] = ok ? charge[(int64_t)nd * j + d0 + d] : mkc(0.0);
+        if (IFD) sm.ds[threadIdx.x * NDT + d] = ok ? dipstr[(int64_t)nd * j + d0 + d] : mkc(0.0);
+        if (IFD && FAM != CAUCHY) {
+          sm.dv[(threadIdx.x * NDT + d) * 2 + 0] = ok ? dipvec[(int64_t)nd * (2 * j) + d0 + d] : 0.0;
+          sm.dv[(threadIdx.x * NDT + d) * 2 + 1] = ok ? dipvec[(int64_t)nd * (2 * j + 1) + d0 + d] : 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll 2
+      for (int k = 0; k < nload; ++k) {
+        SrcStr<NDT> s;
+#pragma unroll
+        for (int d = 0; d < NDT; ++d) {
+          if (IFC) s.charge[d] = sm.ch[k * NDT + d];
+          if (IFD) s.dipstr[d] = sm.ds[k * NDT + d];
+          if (IFD && FAM != CAUCHY) {
+            s.dv[d][0] = sm.dv[(k * NDT + d) * 2];
+            s.dv[d][1] = sm.dv[(k * NDT + d) * 2 + 1];
+          }
+        }
+        const double dx = tx - sm.x[k], dy = ty - sm.y[k];
+        if constexpr (FAM == CAUCHY)
+          pair_cauchy<IFC, IFD, PG, NDT>(dx, dy, s, thresh, thresh2, acc);
+        else if constexpr (FAM == LAPLACE)
+          pair_laplace<IFC, IFD, PG, NDT>(dx, dy, s, thresh, thresh2, acc);
+        else
+          pair_helm<IFC, IFD, PG, NDT>(dx, dy, zk, s, thresh, acc);
+      }
+    }
+  }
+  if (!active) return;
+  cplx* P = pot + blockIdx.y * split_stride_pot;
+  cplx* G = grad + blockIdx.y * split_stride_grad;
+  cplx* H = hess + blockIdx.y * split_stride_hess;
+#pragma unroll
+  for (int d = 0; d < NDT; ++d) {
+    if (d0 + d >= nd) break;
+    const int dd = d0 + d;
+    P[(int64_t)nd * it + dd] += acc.pot[d];
+    if (PG >= 2) {
+#pragma unroll
+      for (int c = 0; c < NG; ++c) G[(int64_t)nd * (NG * it + c) + dd] += acc.grad[d][c];
+    }
+    if (PG >= 3) {
+#pragma unroll
+      for (int c = 0; c < NH; ++c) H[(int64_t)nd * (NH * it + c) + dd] += acc.hess[d][c];
+    }
+  }
+}
+
+// out[i] += sum_s part[s*n + i]
+__global__ void add_splits_kernel(cplx* __restrict__ out, const cplx* __restrict__ part,
+                                  int64_t n, int nsplit) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cplx s = out[i];
+  for (int k = 0; k < nsplit; ++k) s += part[(int64_t)k * n + i];
+  out[i] = s;
+}
+
+template <int FAM, int IFC, int IFD, int PG, int NDT>
+static void launch_direct(dim3 grid, hipStream_t st, const DirectArgs& a, int64_t split_len,
+                          cplx* pot, cplx* grad, cplx* hess, int64_t sp, int64_t sg, int64_t sh) {
+  hipLaunchKernelGGL((direct_kernel<FAM, IFC, IFD, PG, NDT>), grid, dim3(kTile), 0, st, a.nd,
+                     a.zk, a.src, a.ns, split_len, a.charge, a.dipstr, a.dipvec, a.targ, a.nt,
+                     a.thresh, pot, grad, hess, sp, sg, sh);
+}
+
+template <int FAM, int IFC, int IFD, int NDT>
+static void dispatch_pg(int pg, dim3 grid, hipStream_t st, const DirectArgs& a, int64_t sl,
+                        cplx* p, cplx* g, cplx* h, int64_t sp, int64_t sg, int64_t sh) {
+  if (pg == 1) launch_direct<FAM, IFC, IFD, 1, NDT>(grid, st, a, sl, p, g, h, sp, sg, sh);
+  else if (pg == 2) launch_direct<FAM, IFC, IFD, 2, NDT>(grid, st, a, sl, p, g, h, sp, sg, sh);
+  else launch_direct<FAM, IFC, IFD, 3, NDT>(grid, st, a, sl, p, g, h, sp, sg, sh);
+}
+
+template <int FAM, int NDT>
+static void dispatch_cd(const DirectArgs& a, dim3 grid, hipStream_t st, int64_t sl, cplx* p,
+                        cplx* g, cplx* h, int64_t sp, int64_t sg, int64_t sh) {
+  if (a.ifcharge && !a.ifdipole) dispatch_pg<FAM, 1, 0, NDT>(a.pg, grid, st, a, sl, p, g, h, sp, sg, sh);
+  else if (!a.ifcharge && a.ifdipole) dispatch_pg<FAM, 0, 1, NDT>(a.pg, grid, st, a, sl, p, g, h, sp, sg, sh);
+  else dispatch_pg<FAM, 1, 1, NDT>(a.pg, grid, st, a, sl, p, g, h, sp, sg, sh);
+}
+
+int direct_eval(const DirectArgs& a, hipStream_t st) {
+  if (a.nd < 1 || a.ns < 0 || a.nt < 0 || a.pg < 1 || a.pg > 3) return FMM2D_ERR_ARG;
+  if (!a.ifcharge && !a.ifdipole) return FMM2D_ERR_ARG;
+  if (a.family < CAUCHY || a.family > HELM) return FMM2D_ERR_ARG;
+  if (a.nt == 0 || a.ns == 0) return FMM2D_OK;
+  const int NDT = (a.nd == 1) ? 1 : 2;
+  const int nchunk = (a.nd + NDT - 1) / NDT;
+  const int64_t tblocks = ceil_div(a.nt, kTile);
+  // aim for >= 2048 blocks (8 per CU) before splitting the source range
+  int64_t nsplit = 1;
+  const int64_t want = 2048;
+  if (tblocks * nchunk < want) nsplit = (want + tblocks * nchunk - 1) / (tblocks * nchunk);
+  const int64_t max_split = (a.ns + kTile - 1) / kTile;
+  if (nsplit > max_split) nsplit = max_split;
+  if (nsplit > 65535) nsplit = 65535;
+  const int64_t split_len = ((a.ns + nsplit - 1) / nsplit + kTile - 1) / kTile * kTile;
+  nsplit = (a.ns + split_len - 1) / split_len;
+  const int NG = (a.family == CAUCHY) ? 1 : 2, NH = (a.family == CAUCHY) ? 1 : 3;
+  const int64_t np = (int64_t)a.nd * a.nt;
+  const int64_t ng = (a.pg >= 2) ? np * NG : 0;
+  const int64_t nh = (a.pg >= 3) ? np * NH : 0;
+  cplx *p = a.pot, *g = a.grad, *h = a.hess;
+  cplx* work = nullptr;
+  if (nsplit > 1) {
+    const int64_t per = np + ng + nh;
+    FMM2D_HIP_TRY(hipMallocAsync((void**)&work, sizeof(cplx) * per * nsplit, st));
+    FMM2D_HIP_TRY(hipMemsetAsync(work, 0, sizeof(cplx) * per * nsplit, st));
+    p = work;
+    g = work + np * nsplit;
+    h = work + (np + ng) * nsplit;
+  }
+  const int64_t sp = (nsplit > 1) ? np : 0, sg = (nsplit > 1) ? ng : 0, sh = (nsplit > 1) ? nh : 0;
+  dim3 grid((unsigned)tblocks, (unsigned)nsplit, (unsigned)nchunk);
+  if (NDT == 1) {
+    if (a.family == CAUCHY) dispatch_cd<CAUCHY, 1>(a, grid, st, split_len, p, g, h, sp, sg, sh);
+    else if (a.family == LAPLACE) dispatch_cd<LAPLACE, 1>(a, grid, st, split_len, p, g, h, sp, sg, sh);
+    else dispatch_cd<HELM, 1>(a, grid, st, split_len, p, g, h, sp, sg, sh);
+  } else {
+    if (a.family == CAUCHY) dispatch_cd<CAUCHY, 2>(a, grid, st, split_len, p, g, h, sp, sg, sh);
+    else if (a.family == LAPLACE) dispatch_cd<LAPLACE, 2>(a, grid, st, split_len, p, g, h, sp, sg, sh);
+    else dispatch_cd<HELM, 2>(a, grid, st, split_len, p, g, h, sp, sg, sh);
+  }
+  FMM2D_HIP_TRY(hipGetLastError());
+  if (nsplit > 1) {
+    hipLaunchKernelGGL(add_splits_kernel, dim3(ceil_div(np, 256)), dim3(256), 0, st, a.pot, p, np,
+                       (int)nsplit);
+    if (ng) hipLaunchKernelGGL(add_splits_kernel, dim3(ceil_div(ng, 256)), dim3(256), 0, st, a.grad, g, ng, (int)nsplit);
+    if (nh) hipLaunchKernelGGL(add_splits_kernel, dim3(ceil_div(nh, 256)), dim3(256), 0, st, a.hess, h, nh, (int)nsplit);
+    FMM2D_HIP_TRY(hipGetLastError());
+    FMM2D_HIP_TRY(hipFreeAsync(work, st));
+  }
+  return FMM2D_OK;
+}
+
+}  // namespace fmm2d

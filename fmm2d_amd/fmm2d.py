@@ -1,0 +1,427 @@
+"""Host-side mirror of the reference Python API (python/fmm2dpy/fmm2d.py).
+
+Same public names, keyword-only signatures, assertion messages, dispatch
+rules and output shapes as the reference; underneath, every computation is a
+call through the C-ABI (include/fmm2d_amd.h) into hand-written gfx950 HIP
+kernels.  Inputs may be numpy arrays (copied to the GPU) and outputs are
+numpy arrays laid out as f2py returns them (Fortran-ordered).
+
+Reference quirks that are reproduced on purpose (python/fmm2dpy/fmm2d.py):
+  * the source-evaluation gate ``iftarg == 0 or pgt != 1 or pgt != 2`` is
+    always true (lines 98, 294, 550, 802), so with targets and pg in {1,2,3}
+    the sources+targets call decides the outputs and ``pg == pgt`` is asserted;
+  * ``hfmm2d`` and ``h2ddir`` have no Hessian (pg/pgt = 3) path;
+  * ``l2ddir``/``c2ddir`` with nd == 1 and pgt == 3 reshape ``hesstarg`` but
+    leave ``gradtarg`` with its leading density axis (lines 1604-1611, 1717-1725);
+  * "Nothing to compute" is printed and an empty Output returned when no
+    output is requested.
+"""
+import numpy as np
+import numpy.linalg as la
+
+from . import _native as nat
+
+
+class Output():
+    """Result container (python/fmm2dpy/fmm2d.py:9-18)."""
+    pot = None
+    grad = None
+    hess = None
+    pottarg = None
+    gradtarg = None
+    hesstarg = None
+    pre = None
+    pretarg = None
+    ier = 0
+
+
+# --------------------------------------------------------------------------
+# shared validation (same messages as the reference)
+def _nsrc(sources):
+    assert sources.shape[0] == 2, "The first dimension of sources must be 2"
+    if np.size(np.shape(sources)) == 2:
+        return sources.shape[1]
+    return 1
+
+
+def _check_charges(charges, nd, ns):
+    if charges is None:
+        return 0
+    if nd == 1:
+        assert charges.shape[0] == ns, "Charges must be same length as second dimension of sources"
+    if nd > 1:
+        assert charges.shape[0] == nd and charges.shape[1] == ns, \
+            "Charges must be of shape [nd,ns] where nd is number of densities, and ns is number of sources"
+    return 1
+
+
+def _check_dipoles(dipstr, dipvec, nd, ns, need_vec=True):
+    if need_vec:
+        if dipvec is None and dipstr is None:
+            return 0
+    elif dipstr is None:
+        return 0
+    if nd == 1:
+        assert dipstr.shape[0] == ns, "Dipole strengths must be same length as second dimension of sources"
+        if need_vec:
+            if ns > 1:
+                assert dipvec.shape[0] == 2 and dipvec.shape[1] == ns, \
+                    "dipole vectors must be of shape [2,number of sources]"
+            else:
+                assert dipvec.shape[0] == 2, "dipole vectors must be of shape [2,number of sources]"
+    if nd > 1:
+        if need_vec:
+            assert dipvec.shape[0] == nd and dipvec.shape[1] == 2 and dipvec.shape[2] == ns, \
+                "Dipole vectors must be of shape [nd,2,ns] where nd is number of densities, and ns is number of sources"
+        assert dipstr.shape[0] == nd and dipstr.shape[1] == ns, \
+            "Dipole strengths must be of shape [nd,ns] where nd is number of densities, and ns is number of sources"
+    return 1
+
+
+class _DevInputs:
+    """Device copies of the inputs in the C-ABI's (Fortran) layout."""
+
+    def __init__(self, sources, ns, nd, charges, dipstr, dipvec, targets, need_vec):
+        import torch
+        self.ns = ns
+        self.nd = nd
+        src = np.asarray(sources, dtype=np.float64).reshape(2, ns)
+        self.src = nat.to_dev_fortran(src, np.float64)
+        self.charge = None
+        self.dipstr = None
+        self.dipvec = None
+        if charges is not None:
+            self.charge = nat.to_dev_fortran(np.reshape(charges, (nd, ns)), np.complex128)
+        if dipstr is not None:
+            self.dipstr = nat.to_dev_fortran(np.reshape(dipstr, (nd, ns)), np.complex128)
+            if need_vec:
+                self.dipvec = nat.to_dev_fortran(np.reshape(dipvec, (nd, 2, ns)), np.float64)
+        self.targ = None
+        self.nt = 0
+        if targets is not None:
+            t = np.asarray(targets, dtype=np.float64)
+            self.nt = t.shape[1] if t.ndim == 2 else 1
+            self.targ = nat.to_dev_fortran(t.reshape(2, self.nt), np.float64)
+        self.torch = torch
+
+
+# --------------------------------------------------------------------------
+# FMM drivers
+def _fmm(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
+         opts=None):
+    """One call of the native driver; returns (dict of f2py-shaped outputs, ier).
+
+    kind: "c" (cfmm2d), "l" (lfmm2d) or "h" (hfmm2d).  pg/pgt are the
+    evaluation flags at sources/targets (0 = none); targets may be None.
+    """
+    import torch
+    ns = _nsrc(sources)
+    need_vec = kind != "c"
+    dev = _DevInputs(sources, ns, nd, charges, dipstr, dipvec,
+                     targets if pgt > 0 else None, need_vec)
+    nt = dev.nt
+    ng = 1 if kind == "c" else 2
+    nh = 1 if kind == "c" else 3
+
+    def shapes(n):
+        return ((nd, n), (nd, n) if ng == 1 else (nd, ng, n), (nd, n) if nh == 1 else (nd, nh, n))
+
+    bufs = {}
+    for flag, n, names in ((pg, ns, ("pot", "grad", "hess")),
+                           (pgt, nt, ("pottarg", "gradtarg", "hesstarg"))):
+        for lvl, (name, shp) in enumerate(zip(names, shapes(n))):
+            bufs[name] = nat.dev_zeros_fortran(shp, torch.complex128) if flag > lvl else None
+    L = nat.lib()
+    ifc = 1 if dev.charge is not None else 0
+    ifd = 1 if dev.dipstr is not None else 0
+    o = nat.opts_struct(opts)
+    head = [nd, float(eps)]
+    if kind == "h":
+        zk = complex(zk)
+        head += [zk.real, zk.imag]
+    args = head + [ns, nat.ptr(dev.src), ifc, nat.ptr(dev.charge), ifd, nat.ptr(dev.dipstr)]
+    if kind != "c":
+        args.append(nat.ptr(dev.dipvec))
+    args += [0, pg, nat.ptr(bufs["pot"]), nat.ptr(bufs["grad"]), nat.ptr(bufs["hess"]),
+             nt, nat.ptr(dev.targ), pgt, nat.ptr(bufs["pottarg"]), nat.ptr(bufs["gradtarg"]),
+             nat.ptr(bufs["hesstarg"]), nat.byref_opts(o), nat.stream_ptr()]
+    fn = {"c": L.fmm2d_cfmm2d, "l": L.fmm2d_lfmm2d, "h": L.fmm2d_hfmm2d}[kind]
+    ier = fn(*args)
+    nat.check(ier, kind + "fmm2d")
+    res = {}
+    for name, t in bufs.items():
+        if t is None:
+            continue
+        a = nat.fortran_to_numpy(t)
+        res[name] = a[0] if nd == 1 else a
+    if opts is not None and "timeinfo" in opts:
+        opts["timeinfo"][:] = o.timeinfo_array()
+    return res, ier
+
+
+def _fmm_dispatch(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
+                  maxpg):
+    """The reference's branch structure (python/fmm2dpy/fmm2d.py:98-196, 294-454,
+    550-710, 802-962): evaluation at sources, at targets, or at both."""
+    out = Output()
+    ns = _nsrc(sources)
+    if pg == 0 and pgt == 0:
+        print("Nothing to compute, set either pg or pgt to non-zero")
+        return out
+    ifcharge = _check_charges(charges, nd, ns)
+    ifdipole = _check_dipoles(dipstr, dipvec, nd, ns, need_vec=(kind != "c"))
+    if targets is not None:
+        assert targets.shape[0] == 2, "The first dimension of targets must be 2"
+    if not (ifcharge or ifdipole):
+        return out
+    ch = charges if ifcharge else None
+    ds = dipstr if ifdipole else None
+    src_ok = 1 <= pg <= maxpg
+    if src_ok and targets is not None:
+        msg = ("if both potential or potential at gradient are requested at sources and "
+               "targets, then the same pg must be equal to pgt") if kind == "h" else \
+              ("if output is requested at both sources and targets, then the same pg must "
+               "be equal to pgt")
+        if not (1 <= pgt <= maxpg):
+            # the reference evaluates at the sources first, then asserts
+            res, ier = _fmm(kind, eps, zk, sources, ch, ds, dipvec, None, pg, 0, nd)
+            _fill(out, res, ier)
+        assert pg == pgt, msg
+        res, ier = _fmm(kind, eps, zk, sources, ch, ds, dipvec, targets, pg, pgt, nd)
+        _fill(out, res, ier)
+    elif src_ok:
+        res, ier = _fmm(kind, eps, zk, sources, ch, ds, dipvec, None, pg, 0, nd)
+        _fill(out, res, ier)
+    elif targets is not None and 1 <= pgt <= maxpg:
+        res, ier = _fmm(kind, eps, zk, sources, ch, ds, dipvec, targets, 0, pgt, nd)
+        _fill(out, res, ier)
+    return out
+
+
+def _fill(out, res, ier):
+    for k, v in res.items():
+        setattr(out, k, v)
+    out.ier = ier
+
+
+def hfmm2d(*, eps, zk, sources, charges=None, dipstr=None, dipvec=None,
+           targets=None, pg=0, pgt=0, nd=1):
+    """Helmholtz FMM (python/fmm2dpy/fmm2d.py:20-198; helmholtz/hfmm2d.f:19).
+
+    u(x) = (i/4) sum_j [c_j H0(k|x-x_j|) + k d_j (v_j.(x-x_j))/|x-x_j| H1(k|x-x_j|)];
+    pot and gradient (pg, pgt in {1, 2}) at sources and/or targets.
+    """
+    return _fmm_dispatch("h", eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt,
+                         nd, maxpg=2)
+
+
+def lfmm2d(*, eps, sources, charges=None, dipstr=None, dipvec=None,
+           targets=None, pg=0, pgt=0, nd=1):
+    """Laplace FMM with complex densities (python/fmm2dpy/fmm2d.py:459-712;
+    laplace/lfmm2d.f:29).  pg, pgt in {1, 2, 3}."""
+    return _fmm_dispatch("l", eps, None, sources, charges, dipstr, dipvec, targets, pg, pgt,
+                         nd, maxpg=3)
+
+
+def cfmm2d(*, eps, sources, charges=None, dipstr=None,
+           targets=None, pg=0, pgt=0, nd=1):
+    """Cauchy-kernel FMM (python/fmm2dpy/fmm2d.py:715-964; laplace/cfmm2d.f:23).
+
+    pot = sum c_j log|z-xi_j| + d_j/(z-xi_j) evaluated with analytic (complex
+    log) expansions in the far field, exactly as the reference does, so for
+    complex charges only Re(pot) of real-charge parts is a log-potential;
+    grad = d/dz and hess = d^2/dz^2 are single-valued.
+    """
+    return _fmm_dispatch("c", eps, None, sources, charges, dipstr, None, targets, pg, pgt,
+                         nd, maxpg=3)
+
+
+# --------------------------------------------------------------------------
+# direct evaluators
+def _direct(family, zk, sources, targets, charges, dipstr, dipvec, pgt, nd, thresh):
+    """Run one direct evaluation; returns (pot, grad, hess) with f2py shapes
+    (nd,nt), (nd,nt)|(nd,2,nt), (nd,nt)|(nd,3,nt)."""
+    import torch
+    ns = _nsrc(sources)
+    need_vec = family != "c"
+    dev = _DevInputs(sources, ns, nd, charges, dipstr, dipvec, targets, need_vec)
+    nt = dev.nt
+    ng = 1 if family == "c" else 2
+    nh = 1 if family == "c" else 3
+    pot = nat.dev_zeros_fortran((nd, nt), torch.complex128)
+    gshape = (nd, nt) if ng == 1 else (nd, ng, nt)
+    hshape = (nd, nt) if nh == 1 else (nd, nh, nt)
+    grad = nat.dev_zeros_fortran(gshape, torch.complex128) if pgt >= 2 else None
+    hess = nat.dev_zeros_fortran(hshape, torch.complex128) if pgt >= 3 else None
+    L = nat.lib()
+    ifc = 1 if dev.charge is not None else 0
+    ifd = 1 if dev.dipstr is not None else 0
+    common = (nat.ptr(dev.src), ns, ifc, nat.ptr(dev.charge), ifd, nat.ptr(dev.dipstr))
+    outs = (nat.ptr(pot), nat.ptr(grad), nat.ptr(hess), float(thresh), nat.stream_ptr())
+    if family == "c":
+        ier = L.fmm2d_c2d_direct(nd, *common, nat.ptr(dev.targ), nt, pgt, *outs)
+    elif family == "l":
+        ier = L.fmm2d_l2d_direct(nd, *common, nat.ptr(dev.dipvec), nat.ptr(dev.targ), nt, pgt,
+                                 *outs)
+    else:
+        zk = complex(zk)
+        ier = L.fmm2d_h2d_direct(nd, zk.real, zk.imag, *common, nat.ptr(dev.dipvec),
+                                 nat.ptr(dev.targ), nt, pgt, *outs)
+    nat.check(ier, family + "2d_direct")
+    res = [nat.fortran_to_numpy(pot),
+           nat.fortran_to_numpy(grad) if grad is not None else None,
+           nat.fortran_to_numpy(hess) if hess is not None else None]
+    return res, nt
+
+
+def h2ddir(*, zk, sources, targets, charges=None, dipstr=None, dipvec=None,
+           pgt=0, nd=1, thresh=1e-16):
+    """Helmholtz direct evaluation (python/fmm2dpy/fmm2d.py:1250-1367).
+
+    u(x) = sum_j c_j (i/4) H0(k|x-x_j|) + dipole terms; pairs with
+    |k|*|x-x_j| below thresh are omitted.  pgt in {1, 2}.
+    """
+    out = Output()
+    ns = _nsrc(sources)
+    if pgt == 0:
+        print("Nothing to compute, set either pg or pgt to non-zero")
+        return out
+    ifcharge = _check_charges(charges, nd, ns)
+    ifdipole = _check_dipoles(dipstr, dipvec, nd, ns)
+    assert targets.shape[0] == 2, "The first dimension of targets must be 2"
+    if pgt not in (1, 2) or not (ifcharge or ifdipole):
+        return _reshape_ddir(out, nd, 0, targets, ndim_g=2)
+    (p, g, _), nt = _direct("h", zk, sources, targets, charges if ifcharge else None,
+                            dipstr if ifdipole else None, dipvec, pgt, nd, thresh)
+    out.pottarg = p
+    if pgt == 2:
+        out.gradtarg = g
+    return _reshape_ddir(out, nd, pgt, targets, ndim_g=2, grad_only_pgt2=True)
+
+
+def l2ddir(*, sources, targets, charges=None, dipstr=None, dipvec=None,
+           pgt=0, nd=1, thresh=1e-16):
+    """Laplace direct evaluation (python/fmm2dpy/fmm2d.py:1493-1613).
+
+    u(x) = sum_j c_j log|x-x_j| - d_j v_j.(x-x_j)/|x-x_j|^2 (the reference
+    code's normalisation; no 1/(2 pi)).  pgt in {1, 2, 3}.
+    """
+    out = Output()
+    ns = _nsrc(sources)
+    if pgt == 0:
+        print("Nothing to compute, set either pg or pgt to non-zero")
+        return out
+    ifcharge = _check_charges(charges, nd, ns)
+    ifdipole = _check_dipoles(dipstr, dipvec, nd, ns)
+    assert targets.shape[0] == 2, "The first dimension of targets must be 2"
+    if pgt not in (1, 2, 3) or not (ifcharge or ifdipole):
+        return _reshape_ddir(out, nd, 0, targets, ndim_g=2)
+    (p, g, h), nt = _direct("l", None, sources, targets, charges if ifcharge else None,
+                            dipstr if ifdipole else None, dipvec, pgt, nd, thresh)
+    out.pottarg = p
+    if pgt >= 2:
+        out.gradtarg = g
+    if pgt >= 3:
+        out.hesstarg = h
+    return _reshape_ddir(out, nd, pgt, targets, ndim_g=2)
+
+
+def c2ddir(*, sources, targets, charges=None, dipstr=None,
+           pgt=0, nd=1, thresh=1e-16):
+    """Cauchy direct evaluation (python/fmm2dpy/fmm2d.py:1616-1727).
+
+    u(z) = sum_j c_j log|z-xi_j| + d_j/(z-xi_j); grad = d/dz, hess = d^2/dz^2.
+    """
+    out = Output()
+    ns = _nsrc(sources)
+    if pgt == 0:
+        print("Nothing to compute, set either pg or pgt to non-zero")
+        return out
+    ifcharge = _check_charges(charges, nd, ns)
+    ifdipole = _check_dipoles(dipstr, None, nd, ns, need_vec=False)
+    assert targets.shape[0] == 2, "The first dimension of targets must be 2"
+    if pgt not in (1, 2, 3) or not (ifcharge or ifdipole):
+        return _reshape_ddir(out, nd, 0, targets, ndim_g=1)
+    (p, g, h), nt = _direct("c", None, sources, targets, charges if ifcharge else None,
+                            dipstr if ifdipole else None, None, pgt, nd, thresh)
+    out.pottarg = p
+    if pgt >= 2:
+        out.gradtarg = g
+    if pgt >= 3:
+        out.hesstarg = h
+    return _reshape_ddir(out, nd, pgt, targets, ndim_g=1)
+
+
+def _reshape_ddir(out, nd, pgt, targets, ndim_g, grad_only_pgt2=False):
+    """nd == 1 reshapes of the reference (gradtarg only when pgt == 2)."""
+    if nd != 1 or pgt == 0:
+        return out
+    nt = targets.shape[1]
+    out.pottarg = out.pottarg.reshape(nt,)
+    if pgt == 2:
+        out.gradtarg = out.gradtarg.reshape((nt,) if ndim_g == 1 else (2, nt))
+    if pgt == 3 and not grad_only_pgt2:
+        out.hesstarg = out.hesstarg.reshape((nt,) if ndim_g == 1 else (3, nt))
+    return out
+
+
+# --------------------------------------------------------------------------
+def comperr(*, ntest, out, outex, pg=0, pgt=0, nd=1, cauchy=0, bh=0):
+    """Relative L2 error over the first ntest points (python/fmm2dpy/fmm2d.py:2017-2139).
+
+    With cauchy=1 the potentials are compared through their real parts only,
+    as the reference does.
+    """
+    r = 0.0
+    err = 0.0
+
+    def acc(a, b):
+        nonlocal r, err
+        r += la.norm(b) ** 2
+        err += la.norm(b - a) ** 2
+
+    def sl(x):
+        return x[..., 0:ntest]
+
+    comps = 3 if bh == 1 else 2
+    for flag, pname, gname, hname in ((pg, "pot", "grad", "hess"),
+                                      (pgt, "pottarg", "gradtarg", "hesstarg")):
+        if flag <= 0:
+            continue
+        p, pex = getattr(out, pname), getattr(outex, pname)
+        if pname == "pot":
+            real = cauchy != 0
+        else:
+            real = not (cauchy == 0 and bh == 0)
+        if nd > 1:
+            real = cauchy != 0
+        if real:
+            acc(sl(p.real).ravel(), sl(pex.real).ravel())
+        else:
+            acc(sl(p).ravel(), sl(pex).ravel())
+        if flag >= 2:
+            g, gex = getattr(out, gname), getattr(outex, gname)
+            acc(sl(g).ravel(), sl(gex).ravel())
+        if flag >= 3:
+            h, hex_ = getattr(out, hname), getattr(outex, hname)
+            acc(sl(h).ravel(), sl(hex_).ravel())
+    return np.sqrt(err / r)
+
+
+# --------------------------------------------------------------------------
+# families of the reference that are outside this library's scope
+def _out_of_scope(name):
+    def f(*args, **kwargs):
+        raise NotImplementedError(
+            f"{name} is not provided by fmm2d_amd (only the Laplace, Cauchy and "
+            "Helmholtz FMMs and direct evaluators are)")
+    f.__name__ = name
+    return f
+
+
+rfmm2d = _out_of_scope("rfmm2d")
+bhfmm2d = _out_of_scope("bhfmm2d")
+stfmm2d = _out_of_scope("stfmm2d")
+r2ddir = _out_of_scope("r2ddir")
+bh2ddir = _out_of_scope("bh2ddir")
+st2ddir = _out_of_scope("st2ddir")
