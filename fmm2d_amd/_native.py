@@ -26,6 +26,13 @@ SIGNATURES = {
                               _vp, _vp, _d, _vp]),
     "fmm2d_hank103": (_i, [_vp, _i64, _vp, _vp, _vp]),
 }
+SIGNATURES["fmm2d_tree_build"] = (_i, [_vp, _i64, _vp, _i64, _i, _i, _i, C.POINTER(_i),
+                                         C.POINTER(_i), C.POINTER(_vp), _vp])
+SIGNATURES["fmm2d_tree_get"] = (_i, [_vp] * 22)
+SIGNATURES["fmm2d_tree_free"] = (None, [_vp])
+SIGNATURES["fmm2d_l2dterms"] = (_i, [_d, C.POINTER(_i)])
+SIGNATURES["fmm2d_h2dterms"] = (_i, [_d, _d, _d, _d, C.POINTER(_i)])
+SIGNATURES["fmm2d_ndiv"] = (_i, [_d, _i64, _i64, _i, C.POINTER(_i), C.POINTER(_i)])
 # FMM drivers: (nd, eps, [zk_re, zk_im,] ns, sources, ifcharge, charge, ifdipole,
 # dipstr, [dipvec,] iper, ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg,
 # gradtarg, hesstarg, opts, stream)

@@ -1,0 +1,233 @@
+// C-ABI entry points of the FMM drivers, the tree export and the parameter
+// rules.  Declarations and reference citations: include/fmm2d_amd.h.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/fmm2d_amd.h"
+#include "helm_ops.h"
+#include "lap_ops.h"
+#include "tree.h"
+
+using namespace fmm2d;
+
+namespace {
+
+CauchyArgs cauchy_args(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
+                       const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dipstr,
+                       const double* dipvec, int ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad,
+                       fmm2d_cplx* hess, int64_t nt, const double* targ, int ifpghtarg,
+                       fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg,
+                       const fmm2d_opts* opts) {
+  CauchyArgs a;
+  a.nd = nd;
+  a.eps = eps;
+  a.ns = ns;
+  a.nt = nt;
+  a.src = sources;
+  a.targ = targ;
+  a.ifcharge = ifcharge;
+  a.ifdipole = ifdipole;
+  a.charge = reinterpret_cast<const cplx*>(charge);
+  a.dipstr = reinterpret_cast<const cplx*>(dipstr);
+  a.dipvec = dipvec;
+  a.ifpgh = ifpgh;
+  a.ifpghtarg = ifpghtarg;
+  a.pot = reinterpret_cast<cplx*>(pot);
+  a.grad = reinterpret_cast<cplx*>(grad);
+  a.hess = reinterpret_cast<cplx*>(hess);
+  a.pottarg = reinterpret_cast<cplx*>(pottarg);
+  a.gradtarg = reinterpret_cast<cplx*>(gradtarg);
+  a.hesstarg = reinterpret_cast<cplx*>(hesstarg);
+  a.opts = opts;
+  return a;
+}
+
+struct TreeHandle {
+  Tree t;
+  Lists l;
+  bool have_lists = false;
+  hipStream_t st = nullptr;
+};
+
+}  // namespace
+
+extern "C" {
+
+int fmm2d_cfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
+                 const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dipstr, int iper,
+                 int ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, int64_t nt,
+                 const double* targ, int ifpghtarg, fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg,
+                 fmm2d_cplx* hesstarg, fmm2d_opts* opts, void* stream) {
+  if (iper != 0) return FMM2D_ERR_UNSUPPORTED;
+  CauchyArgs a = cauchy_args(nd, eps, ns, sources, ifcharge, charge, ifdipole, dipstr, nullptr,
+                             ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg,
+                             hesstarg, opts);
+  return laplace_fmm(0, a, reinterpret_cast<hipStream_t>(stream));
+}
+
+int fmm2d_lfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
+                 const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dipstr,
+                 const double* dipvec, int iper, int ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad,
+                 fmm2d_cplx* hess, int64_t nt, const double* targ, int ifpghtarg,
+                 fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg,
+                 fmm2d_opts* opts, void* stream) {
+  if (iper != 0) return FMM2D_ERR_UNSUPPORTED;
+  CauchyArgs a = cauchy_args(nd, eps, ns, sources, ifcharge, charge, ifdipole, dipstr, dipvec,
+                             ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg,
+                             hesstarg, opts);
+  return laplace_fmm(1, a, reinterpret_cast<hipStream_t>(stream));
+}
+
+int fmm2d_hfmm2d(int nd, double eps, double zk_re, double zk_im, int64_t ns,
+                 const double* sources, int ifcharge, const fmm2d_cplx* charge, int ifdipole,
+                 const fmm2d_cplx* dipstr, const double* dipvec, int iper, int ifpgh,
+                 fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, int64_t nt,
+                 const double* targ, int ifpghtarg, fmm2d_cplx* pottarg,
+                 fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg, fmm2d_opts* opts, void* stream) {
+  if (iper != 0) return FMM2D_ERR_UNSUPPORTED;
+  HelmArgs a;
+  a.nd = nd;
+  a.eps = eps;
+  a.zk = cplx{zk_re, zk_im};
+  a.ns = ns;
+  a.nt = nt;
+  a.src = sources;
+  a.targ = targ;
+  a.ifcharge = ifcharge;
+  a.ifdipole = ifdipole;
+  a.charge = reinterpret_cast<const cplx*>(charge);
+  a.dipstr = reinterpret_cast<const cplx*>(dipstr);
+  a.dipvec = dipvec;
+  a.ifpgh = ifpgh;
+  a.ifpghtarg = ifpghtarg;
+  a.pot = reinterpret_cast<cplx*>(pot);
+  a.grad = reinterpret_cast<cplx*>(grad);
+  a.hess = reinterpret_cast<cplx*>(hess);
+  a.pottarg = reinterpret_cast<cplx*>(pottarg);
+  a.gradtarg = reinterpret_cast<cplx*>(gradtarg);
+  a.hesstarg = reinterpret_cast<cplx*>(hesstarg);
+  a.opts = opts;
+  return helmholtz_fmm(a, reinterpret_cast<hipStream_t>(stream));
+}
+
+int fmm2d_tree_build(const double* sources, int64_t ns, const double* targ, int64_t nt,
+                     int idivflag, int ndiv, int nlmax, int* nlevels, int* nboxes,
+                     void** handle, void* stream) {
+  if (!handle || ns < 1 || nt < 0 || (nt > 0 && !targ) || !sources || ndiv < 1 ||
+      idivflag < 0 || idivflag > 2 || nlmax < 1)
+    return FMM2D_ERR_ARG;
+  TreeHandle* h = new TreeHandle;
+  h->st = reinterpret_cast<hipStream_t>(stream);
+  try {
+    build_tree(sources, ns, targ, nt, ndiv, idivflag, nlmax, h->t, h->st);
+    hip_throw(hipStreamSynchronize(h->st), "sync");
+  } catch (const HipError& e) {
+    delete h;
+    return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  }
+  if (nlevels) *nlevels = h->t.nlevels;
+  if (nboxes) *nboxes = h->t.nboxes;
+  *handle = h;
+  return FMM2D_OK;
+}
+
+int fmm2d_tree_get(void* handle, int* laddr, int* ilevel, int* iparent, int* nchild,
+                   int* ichild, int* ncoll, int* coll, double* centers, double* boxsize,
+                   int* isrc, int* isrcse, int* itarg, int* itargse, int* nlist1, int* list1,
+                   int* nlist2, int* list2, int* nlist3, int* list3, int* nlist4, int* list4) {
+  if (!handle) return FMM2D_ERR_ARG;
+  TreeHandle* h = static_cast<TreeHandle*>(handle);
+  Tree& t = h->t;
+  const int nb = t.nboxes, nl = t.nlevels;
+  auto get = [&](const DBuf<int>& d, size_t n, int* out, bool plus1) {
+    if (!out || n == 0) return;
+    hip_throw(hipMemcpyAsync(out, d.p, sizeof(int) * n, hipMemcpyDeviceToHost, h->st), "copy");
+    hip_throw(hipStreamSynchronize(h->st), "sync");
+    if (plus1)
+      for (size_t i = 0; i < n; ++i) out[i] = out[i] < 0 ? -1 : out[i] + 1;
+  };
+  try {
+    if (laddr)
+      for (int l = 0; l <= nl; ++l) {
+        laddr[2 * l] = t.lstart[l] + 1;
+        laddr[2 * l + 1] = t.lend[l];
+      }
+    if (boxsize)
+      for (int l = 0; l <= nl; ++l) boxsize[l] = t.boxsize[l];
+    get(t.ilevel, nb, ilevel, false);
+    get(t.iparent, nb, iparent, true);
+    get(t.nchild, nb, nchild, false);
+    get(t.ichild, 4 * (size_t)nb, ichild, true);
+    get(t.ncoll, nb, ncoll, false);
+    get(t.coll, 9 * (size_t)nb, coll, true);
+    if (centers) {
+      hip_throw(hipMemcpyAsync(centers, t.centers.p, sizeof(double) * 2 * nb,
+                               hipMemcpyDeviceToHost, h->st), "copy");
+      hip_throw(hipStreamSynchronize(h->st), "sync");
+    }
+    get(t.isrc, t.ns, isrc, true);
+    get(t.itarg, t.nt, itarg, true);
+    // [start, end) 0-based -> (first, last) 1-based
+    for (int k = 0; k < 2; ++k) {
+      int* out = k ? itargse : isrcse;
+      if (!out) continue;
+      get(k ? t.itargse : t.isrcse, 2 * (size_t)nb, out, false);
+      for (int b = 0; b < nb; ++b) out[2 * b] += 1;
+    }
+    if (nlist1) {
+      if (!h->have_lists) {
+        build_lists(t, h->l, h->st);
+        h->have_lists = true;
+      }
+      const int caps[4] = {kMaxList1, kMaxList2, kMaxList3, kMaxList4};
+      int* ns[4] = {nlist1, nlist2, nlist3, nlist4};
+      int* ls[4] = {list1, list2, list3, list4};
+      const DBuf<int>* dn[4] = {&h->l.n1, &h->l.n2, &h->l.n3, &h->l.n4};
+      const DBuf<int>* dl[4] = {&h->l.l1, &h->l.l2, &h->l.l3, &h->l.l4};
+      for (int k = 0; k < 4; ++k) {
+        if (!ns[k] || !ls[k]) continue;
+        get(*dn[k], nb, ns[k], false);
+        get(*dl[k], (size_t)caps[k] * nb, ls[k], true);
+        for (int b = 0; b < nb; ++b) {
+          int* row = ls[k] + (size_t)caps[k] * b;
+          std::sort(row, row + ns[k][b]);
+          for (int j = ns[k][b]; j < caps[k]; ++j) row[j] = 0;
+        }
+      }
+    }
+  } catch (const HipError& e) {
+    return FMM2D_ERR_HIP;
+  }
+  return FMM2D_OK;
+}
+
+void fmm2d_tree_free(void* handle) {
+  TreeHandle* h = static_cast<TreeHandle*>(handle);
+  if (!h) return;
+  (void)hipStreamSynchronize(h->st);
+  delete h;
+}
+
+int fmm2d_l2dterms(double eps, int* nterms) {
+  if (!nterms) return FMM2D_ERR_ARG;
+  *nterms = l2dterms(eps);
+  return FMM2D_OK;
+}
+
+int fmm2d_h2dterms(double bsize, double zk_re, double zk_im, double eps, int* nterms) {
+  if (!nterms) return FMM2D_ERR_ARG;
+  int ier = 0;
+  *nterms = h2dterms(bsize, cplx{zk_re, zk_im}, eps, &ier);
+  return ier;
+}
+
+int fmm2d_ndiv(double eps, int64_t ns, int64_t nt, int helmholtz, int* ndiv, int* idivflag) {
+  (void)helmholtz;  // lndiv2d and hndiv2d are the same table
+  if (!ndiv || !idivflag) return FMM2D_ERR_ARG;
+  *ndiv = lndiv2d(eps, ns, nt);
+  *idivflag = 0;
+  return FMM2D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,31 @@
+// Helmholtz FMM (hfmm2d) on the device.
+#pragma once
+#include "../../include/fmm2d_amd.h"
+#include "common.h"
+
+namespace fmm2d {
+
+struct HelmArgs {
+  int nd;
+  double eps;
+  cplx zk;
+  int64_t ns, nt;
+  const double* src;
+  const double* targ;
+  int ifcharge, ifdipole;
+  const cplx* charge;
+  const cplx* dipstr;
+  const double* dipvec;
+  int ifpgh, ifpghtarg;
+  cplx *pot, *grad, *hess, *pottarg, *gradtarg, *hesstarg;
+  const fmm2d_opts* opts;
+};
+
+// helmholtz/hfmm2d.f:19
+int helmholtz_fmm(const HelmArgs& a, hipStream_t st);
+
+// h2dterms (helmholtz/h2dterms.f:35-96); *ier = 13 when no order up to 50000
+// meets the criterion.
+int h2dterms(double bsize, cplx zk, double eps, int* ier);
+
+}  // namespace fmm2d

@@ -1,0 +1,737 @@
+// Laplace / Cauchy-kernel FMM on the device (FP64).
+//
+// Reference: laplace/cfmm2d.f (driver 23-436, cfmm2dmain 442-1341),
+// laplace/lfmm2d.f (complex-density split 103-223), laplace/laprouts2d.f
+// (expansions and translations).  Expansions are the reference's: multipole
+// M_0 log(z) + sum M_n (r/z)^n and local sum L_n (z/r)^n about each box
+// centre, scaled by the box size r (rscale = boxsize of the level).
+//
+// Work decomposition (one launch per step, or per level where the reference's
+// loops carry a level dependency):
+//   P2M        one 64-lane wavefront per leaf, lane = coefficient index
+//   M2M / L2L  one wavefront per box, fixed 4 child-offset matrices
+//   M2L        one wavefront per target box, lane = output coefficient; the
+//              translation depends only on the offset (dx,dy) in [-3,3]^2 of
+//              the source box (in box sizes) and is precomputed once
+//              (49 (p+1)x(p+1) matrices; the level enters only through
+//              log(boxsize) M_0 added to L_0)
+//   list 4     one wavefront per box (sources of coarser leaves -> local)
+//   list 3+L2P one thread per evaluation point
+//   near field one thread per evaluation point over its leaf's list 1
+// All kernels read and write the box expansions in place; there are no
+// atomics in the FMM itself (pull form, as the reference's loops).
+#include <algorithm>
+#include <complex>
+#include <vector>
+
+#include "devbuf.h"
+#include "lap_ops.h"
+#include "p2p_math.h"
+#include "tree.h"
+
+namespace fmm2d {
+
+// --------------------------------------------------------------- rules
+int lndiv2d(double eps, int64_t ns, int64_t nt) {
+  if (eps >= 0.5e-0) return 3;
+  if (eps >= 0.5e-1) return 5;
+  if (eps >= 0.5e-2) return 8;
+  if (eps >= 0.5e-3) return 10;
+  if (eps >= 0.5e-6) return 15;
+  if (eps >= 0.5e-9) return 20;
+  if (eps >= 0.5e-12) return 25;
+  if (eps >= 0.5e-15) return 45;
+  return (int)(ns + nt);
+}
+
+int l2dterms(double eps) {
+  // first j >= 2 with (sqrt(2)/2)^j / 1.5^(j+1) < eps (j <= 100), else 1
+  const double a = std::sqrt(2.0) / 2.0;
+  for (int j = 2; j <= 100; ++j)
+    if (std::pow(a, j) / std::pow(1.5, j + 1) < eps) return j;
+  return 1;
+}
+
+namespace {
+
+constexpr int kW = 64;  // wavefront size
+constexpr int kOff = 7;  // offsets -3..3 per axis
+
+FMM2D_HD cplx cpowi(cplx w, int k) {
+  cplx r = mkc(1.0), b = w;
+  while (k) {
+    if (k & 1) r = r * b;
+    b = b * b;
+    k >>= 1;
+  }
+  return r;
+}
+
+__device__ __forceinline__ int npts_eval(int b, const int* sse, const int* tse, int ifpgh,
+                                         int ifpght) {
+  int n = 0;
+  if (ifpgh > 0) n += sse[2 * b + 1] - sse[2 * b];
+  if (ifpght > 0) n += tse[2 * b + 1] - tse[2 * b];
+  return n;
+}
+
+// ---------------------------------------------------------- gather / scatter
+__global__ void k_gather_xy(int64_t n, const int* perm, const double* xy, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = perm[i];
+  out[2 * i] = xy[2 * (int64_t)j];
+  out[2 * i + 1] = xy[2 * (int64_t)j + 1];
+}
+
+// cfmm2d: densities copied as they are.
+__global__ void k_gather_dens(int64_t n, int nd, const int* perm, const cplx* in, cplx* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = perm[i];
+  for (int d = 0; d < nd; ++d) out[i * nd + d] = in[j * nd + d];
+}
+
+// lfmm2d split (laplace/lfmm2d.f:139-156): density d of the Laplace problem
+// becomes Cauchy densities 2d (real part) and 2d+1 (imaginary part); a dipole
+// (d, v) becomes -(v1 + i v2) times each part.
+__global__ void k_gather_split(int64_t n, int nd, const int* perm, const cplx* ch,
+                               const cplx* ds, const double* dv, cplx* och, cplx* ods) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = perm[i];
+  for (int d = 0; d < nd; ++d) {
+    if (ch) {
+      const cplx c = ch[j * nd + d];
+      och[i * 2 * nd + 2 * d] = mkc(c.re);
+      och[i * 2 * nd + 2 * d + 1] = mkc(c.im);
+    }
+    if (ds) {
+      const cplx s = ds[j * nd + d];
+      const cplx v = cplx{-dv[nd * (2 * j) + d], -dv[nd * (2 * j + 1) + d]};
+      ods[i * 2 * nd + 2 * d] = s.re * v;
+      ods[i * 2 * nd + 2 * d + 1] = s.im * v;
+    }
+  }
+}
+
+// out[perm[i]] = in[i] for m values per point
+__global__ void k_scatter(int64_t n, int m, const int* perm, const cplx* in, cplx* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = perm[i];
+  for (int k = 0; k < m; ++k) out[j * m + k] = in[i * m + k];
+}
+
+// lfmm2d unpack (laplace/lfmm2d.f:169-223).  in: Cauchy outputs of 2nd
+// densities; pot(nd,n), grad(nd,2,n), hess(nd,3,n).
+__global__ void k_scatter_unpack(int64_t n, int nd, int pg, const int* perm, const cplx* p1,
+                                 const cplx* g1, const cplx* h1, cplx* pot, cplx* grad,
+                                 cplx* hess) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = perm[i];
+  for (int d = 0; d < nd; ++d) {
+    const int64_t a = i * 2 * nd + 2 * d;
+    pot[j * nd + d] = cplx{p1[a].re, p1[a + 1].re};
+    if (pg >= 2) {
+      grad[nd * (2 * j) + d] = cplx{g1[a].re, g1[a + 1].re};
+      grad[nd * (2 * j + 1) + d] = cplx{-g1[a].im, -g1[a + 1].im};
+    }
+    if (pg >= 3) {
+      const cplx hxx = cplx{h1[a].re, h1[a + 1].re};
+      hess[nd * (3 * j) + d] = hxx;
+      hess[nd * (3 * j + 1) + d] = cplx{-h1[a].im, -h1[a + 1].im};
+      hess[nd * (3 * j + 2) + d] = -hxx;
+    }
+  }
+}
+
+__global__ void k_leaf_of(int64_t n, const int* nchild, const int* se, int* leaf) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n || nchild[b] != 0) return;
+  for (int i = se[2 * b]; i < se[2 * b + 1]; ++i) leaf[i] = (int)b;
+}
+
+// ---------------------------------------------------------- P2M
+// l2dformmp{c,d,cd} (laprouts2d.f:84-282) on leaves at levels >= 2.
+template <int IFC, int IFD>
+__global__ __launch_bounds__(kW) void k_p2m(int nd, int P, const int* ilevel, const int* nchild,
+                                            const int* nterms, const double* rsc,
+                                            const double* centers, const int* sse,
+                                            const double* xs, const cplx* chs, const cplx* dps,
+                                            cplx* mp) {
+  const int b = blockIdx.x;
+  const int lev = ilevel[b];
+  if (lev < 2 || nchild[b] != 0) return;
+  const int s0 = sse[2 * b], s1 = sse[2 * b + 1];
+  if (s1 <= s0) return;
+  const int p = nterms[lev];
+  const double r = rsc[lev], rinv = 1.0 / r;
+  const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  for (int nn = threadIdx.x; nn <= p; nn += kW) {
+    for (int d = 0; d < nd; ++d) {
+      cplx acc = mkc(0.0);
+      for (int j = s0; j < s1; ++j) {
+        const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
+        if (nn == 0) {
+          if (IFC) acc += chs[(int64_t)j * nd + d];
+        } else {
+          const cplx pw = cpowi(w, nn - 1);  // (z/r)^(n-1)
+          if (IFC) acc -= (pw * w) * chs[(int64_t)j * nd + d] * (1.0 / nn);
+          if (IFD) acc += pw * dps[(int64_t)j * nd + d] * rinv;
+        }
+      }
+      mp[((int64_t)b * nd + d) * P + nn] += acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------- M2M
+// l2dmpmp (laprouts2d.f:905-1015): parents at one level from their children.
+// A[q][j][n]: child quadrant q, child coefficient j, parent coefficient n.
+__global__ __launch_bounds__(kW) void k_m2m(int lo, int nd, int P, int pc, int pp,
+                                            const int* nchild, const int* ichild, const int* sse,
+                                            const cplx* A, cplx* mp) {
+  const int b = lo + blockIdx.x;
+  if (nchild[b] == 0) return;
+  for (int nn = threadIdx.x; nn <= pp; nn += kW) {
+    for (int d = 0; d < nd; ++d) {
+      cplx acc = mkc(0.0);
+      for (int q = 0; q < 4; ++q) {
+        const int c = ichild[4 * b + q];
+        if (sse[2 * c + 1] <= sse[2 * c]) continue;
+        const cplx* m = mp + ((int64_t)c * nd + d) * P;
+        const int jm = min(nn, pc);
+        for (int j = 0; j <= jm; ++j) acc += A[((int64_t)q * P + j) * P + nn] * m[j];
+      }
+      mp[((int64_t)b * nd + d) * P + nn] += acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------- M2L
+// l2dmploc (laprouts2d.f:1130-1244) over list 2, all levels at once.
+// T[o][j][n] for offset o = (dx+3)*7 + (dy+3) of the source box.
+__global__ __launch_bounds__(kW) void k_m2l(int nd, int P, const int* ilevel, const int* nterms,
+                                            const double* rsc, const double* logr,
+                                            const double* centers, const int* n2, const int* l2,
+                                            const int* sse, const int* tse, int ifpgh,
+                                            int ifpght, const cplx* T, const cplx* mp,
+                                            cplx* loc) {
+  const int b = blockIdx.x;
+  const int lev = ilevel[b];
+  const int cnt = n2[b];
+  if (lev < 2 || cnt == 0 || npts_eval(b, sse, tse, ifpgh, ifpght) == 0) return;
+  const int p = nterms[lev];
+  const double rinv = 1.0 / rsc[lev], lr = logr[lev];
+  const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  __shared__ int so[kMaxList2], sc[kMaxList2];
+  if ((int)threadIdx.x < cnt) {
+    const int c = l2[(int64_t)kMaxList2 * b + threadIdx.x];
+    const int ox = (int)rint((centers[2 * c] - cx) * rinv);
+    const int oy = (int)rint((centers[2 * c + 1] - cy) * rinv);
+    so[threadIdx.x] = (ox + 3) * kOff + (oy + 3);
+    sc[threadIdx.x] = c;
+  }
+  __syncthreads();
+  for (int nn = threadIdx.x; nn <= p; nn += kW) {
+    for (int d = 0; d < nd; ++d) {
+      cplx acc = mkc(0.0);
+      for (int k = 0; k < cnt; ++k) {
+        const cplx* m = mp + ((int64_t)sc[k] * nd + d) * P;
+        const cplx* t = T + (int64_t)so[k] * P * P + nn;
+        for (int j = 0; j <= p; ++j) acc += t[(int64_t)j * P] * m[j];
+        if (nn == 0) acc += lr * m[0];
+      }
+      loc[((int64_t)b * nd + d) * P + nn] += acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------- list 4 (P2L)
+// l2dformta{c,d,cd} (laprouts2d.f:501-690): sources of coarser leaves in list 4.
+template <int IFC, int IFD>
+__global__ __launch_bounds__(kW) void k_p2l(int nd, int P, const int* ilevel, const int* nterms,
+                                            const double* rsc, const double* centers,
+                                            const int* n4, const int* l4, const int* sse,
+                                            const int* tse, int ifpgh, int ifpght,
+                                            const double* xs, const cplx* chs, const cplx* dps,
+                                            cplx* loc) {
+  const int b = blockIdx.x;
+  const int lev = ilevel[b];
+  const int cnt = n4[b];
+  if (lev < 2 || cnt == 0 || npts_eval(b, sse, tse, ifpgh, ifpght) == 0) return;
+  const int p = nterms[lev];
+  const double r = rsc[lev];
+  const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  for (int nn = threadIdx.x; nn <= p; nn += kW) {
+    for (int d = 0; d < nd; ++d) {
+      cplx acc = mkc(0.0);
+      for (int k = 0; k < cnt; ++k) {
+        const int c = l4[(int64_t)kMaxList4 * b + k];
+        for (int j = sse[2 * c]; j < sse[2 * c + 1]; ++j) {
+          const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
+          const cplx zi = invc(z0);
+          const cplx wn = cpowi(r * zi, nn);  // (r/z0)^n
+          if (IFC) {
+            const cplx c0 = chs[(int64_t)j * nd + d];
+            if (nn == 0) acc += log(absc(z0)) * c0;
+            else acc -= (wn * c0) * (1.0 / nn);
+          }
+          if (IFD) acc -= (zi * wn) * dps[(int64_t)j * nd + d];
+        }
+      }
+      loc[((int64_t)b * nd + d) * P + nn] += acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------- L2L
+// l2dlocloc (laprouts2d.f:1020-1123): children at one level from their parent.
+// B[q][j][n]: child quadrant q, parent coefficient j, child coefficient n.
+__global__ __launch_bounds__(kW) void k_l2l(int lo, int nd, int P, int pp, int pc,
+                                            const int* iparent, const int* ichild,
+                                            const int* sse, const int* tse, int ifpgh,
+                                            int ifpght, const cplx* B, cplx* loc) {
+  const int b = lo + blockIdx.x;
+  const int dad = iparent[b];
+  if (npts_eval(dad, sse, tse, ifpgh, ifpght) == 0) return;
+  int q = 0;
+  while (q < 3 && ichild[4 * dad + q] != b) ++q;
+  for (int nn = threadIdx.x; nn <= pc; nn += kW) {
+    for (int d = 0; d < nd; ++d) {
+      const cplx* L = loc + ((int64_t)dad * nd + d) * P;
+      cplx acc = mkc(0.0);
+      for (int j = nn; j <= pp; ++j) acc += B[((int64_t)q * P + j) * P + nn] * L[j];
+      loc[((int64_t)b * nd + d) * P + nn] += acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------- list 3 + L2P
+// l2dtaeval{p,g,h} (laprouts2d.f:696-897) from the point's leaf and
+// l2dmpeval{p,g,h} (laprouts2d.f:289-495) of the leaf's list-3 boxes.
+template <int PG>
+__global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* leaf,
+                       const int* ilevel, const int* nterms, const double* rsc,
+                       const double* centers, const int* n3, const int* l3, const cplx* mp,
+                       const cplx* loc, cplx* pot, cplx* grad, cplx* hess) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = leaf[i];
+  const double x = xy[2 * i], y = xy[2 * i + 1];
+  const int lev = ilevel[b];
+  const int p = nterms[lev];
+  const double r = rsc[lev], rinv = 1.0 / r;
+  const cplx w = cplx{(x - centers[2 * b]) * rinv, (y - centers[2 * b + 1]) * rinv};
+  const int c3 = n3[b];
+  for (int d = 0; d < nd; ++d) {
+    cplx ap = mkc(0.0), ag = mkc(0.0), ah = mkc(0.0);
+    // local expansion: sum L_n w^n, d/dz = sum n L_n w^(n-1) / r, ...
+    {
+      const cplx* L = loc + ((int64_t)b * nd + d) * P;
+      cplx wm2 = mkc(0.0), wm1 = mkc(0.0), wn = mkc(1.0);
+      for (int k = 0; k <= p; ++k) {
+        const cplx Lk = L[k];
+        ap += Lk * wn;
+        if (PG >= 2 && k >= 1) ag += (double)k * (Lk * wm1);
+        if (PG >= 3 && k >= 2) ah += (double)(k * (k - 1)) * (Lk * wm2);
+        wm2 = wm1;
+        wm1 = wn;
+        wn = wn * w;
+      }
+      if (PG >= 2) ag = ag * rinv;
+      if (PG >= 3) ah = ah * (rinv * rinv);
+    }
+    // multipoles of the list-3 boxes
+    for (int k3 = 0; k3 < c3; ++k3) {
+      const int c = l3[(int64_t)kMaxList3 * b + k3];
+      const int lc = ilevel[c];
+      const int pc = nterms[lc];
+      const double rc = rsc[lc], rci = 1.0 / rc;
+      const cplx z = cplx{x - centers[2 * c], y - centers[2 * c + 1]};
+      const cplx u = rc * invc(z);  // r/z
+      const cplx* M = mp + ((int64_t)c * nd + d) * P;
+      ap += log(absc(z)) * M[0];
+      if (PG >= 2) ag += (u * rci) * M[0];
+      if (PG >= 3) ah -= (u * u) * (rci * rci) * M[0];
+      cplx un = u;  // u^n
+      for (int k = 1; k <= pc; ++k) {
+        const cplx Mk = M[k];
+        ap += Mk * un;
+        const cplx un1 = un * u;
+        if (PG >= 2) ag -= ((double)k * rci) * (Mk * un1);
+        if (PG >= 3) ah += ((double)(k * (k + 1)) * rci * rci) * (Mk * (un1 * u));
+        un = un1;
+      }
+    }
+    pot[i * nd + d] += ap;
+    if (PG >= 2) grad[i * nd + d] += ag;
+    if (PG >= 3) hess[i * nd + d] += ah;
+  }
+}
+
+// ---------------------------------------------------------- near field
+// cfmm2dpart_direct (laprouts2d.f -> cfmm2d.f:1450-1616): list-1 sources of
+// the point's leaf through the c2d_direct* variant matching the flags.
+template <int IFC, int IFD, int PG, int NDT>
+__global__ void k_near(int64_t n, int nd, int d0, const double* xy, const int* leaf,
+                       const int* n1, const int* l1, const int* sse, const double* xs,
+                       const cplx* chs, const cplx* dps, double thresh, cplx* pot, cplx* grad,
+                       cplx* hess) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = leaf[i];
+  const double x = xy[2 * i], y = xy[2 * i + 1];
+  const double thresh2 = thresh * thresh;
+  Acc<CAUCHY, PG, NDT> a;
+  a.zero();
+  const int c1 = n1[b];
+  for (int k = 0; k < c1; ++k) {
+    const int c = l1[(int64_t)kMaxList1 * b + k];
+    for (int j = sse[2 * c]; j < sse[2 * c + 1]; ++j) {
+      SrcStr<NDT> s;
+#pragma unroll
+      for (int dd = 0; dd < NDT; ++dd) {
+        const bool ok = d0 + dd < nd;
+        if (IFC) s.charge[dd] = ok ? chs[(int64_t)j * nd + d0 + dd] : mkc(0.0);
+        if (IFD) s.dipstr[dd] = ok ? dps[(int64_t)j * nd + d0 + dd] : mkc(0.0);
+      }
+      pair_cauchy<IFC, IFD, PG, NDT>(x - xs[2 * j], y - xs[2 * j + 1], s, thresh, thresh2, a);
+    }
+  }
+#pragma unroll
+  for (int dd = 0; dd < NDT; ++dd) {
+    if (d0 + dd >= nd) break;
+    pot[i * nd + d0 + dd] += a.pot[dd];
+    if (PG >= 2) grad[i * nd + d0 + dd] += a.grad[dd][0];
+    if (PG >= 3) hess[i * nd + d0 + dd] += a.hess[dd][0];
+  }
+}
+
+// ---------------------------------------------------------- host side
+using lcplx = std::complex<long double>;
+
+long double binom(int n, int k) {
+  if (k < 0 || k > n) return 0.0L;
+  long double r = 1.0L;
+  for (int i = 1; i <= k; ++i) r = r * (long double)(n - k + i) / (long double)i;
+  return r;
+}
+
+cplx tocplx(lcplx z) { return cplx{(double)z.real(), (double)z.imag()}; }
+
+// The fixed translation matrices (level independent for equal orders).
+void make_matrices(int P, std::vector<cplx>& A, std::vector<cplx>& B, std::vector<cplx>& T) {
+  A.assign(4 * (size_t)P * P, cplx{0, 0});
+  B.assign(4 * (size_t)P * P, cplx{0, 0});
+  T.assign(kOff * kOff * (size_t)P * P, cplx{0, 0});
+  for (int q = 0; q < 4; ++q) {
+    const long double sx = (q & 1) ? 1.0L : -1.0L, sy = (q & 2) ? 1.0L : -1.0L;
+    // M2M: child - parent = (sx, sy) * r_child/2; r_parent = 2 r_child
+    const lcplx w = lcplx(sx, sy) / 2.0L, wh = w / 2.0L, wi = 1.0L / w;
+    for (int n = 0; n < P; ++n)
+      for (int j = 0; j <= n; ++j) {
+        lcplx v;
+        if (n == 0) v = 1.0L;
+        else if (j == 0) v = -std::pow(wh, n) / (long double)n;
+        else v = binom(n - 1, j - 1) * std::pow(wi, j) * std::pow(wh, n);
+        A[((size_t)q * P + j) * P + n] = tocplx(v);
+      }
+    // L2L: child - parent = (sx, sy) * r_parent/4; r_child = r_parent/2
+    const lcplx v4 = lcplx(sx, sy) / 4.0L;
+    for (int n = 0; n < P; ++n)
+      for (int j = n; j < P; ++j) {
+        const lcplx v = binom(j, n) * std::pow(v4, j - n) * std::pow(0.5L, n);
+        B[((size_t)q * P + j) * P + n] = tocplx(v);
+      }
+  }
+  // M2L: source - target = u * r with u = dx + i dy, max(|dx|,|dy|) >= 2
+  for (int dx = -3; dx <= 3; ++dx)
+    for (int dy = -3; dy <= 3; ++dy) {
+      if (std::abs(dx) <= 1 && std::abs(dy) <= 1) continue;
+      const int o = (dx + 3) * kOff + (dy + 3);
+      const lcplx u((long double)dx, (long double)dy), ui = 1.0L / u, mui = -ui;
+      for (int n = 0; n < P; ++n)
+        for (int j = 0; j < P; ++j) {
+          lcplx v;
+          if (n == 0 && j == 0) v = std::log(std::abs(u));
+          else if (n == 0) v = std::pow(mui, j);
+          else if (j == 0) v = -std::pow(ui, n) / (long double)n;
+          else v = binom(n + j - 1, j - 1) * std::pow(mui, j) * std::pow(ui, n);
+          T[((size_t)o * P + j) * P + n] = tocplx(v);
+        }
+    }
+}
+
+template <typename T> DBuf<T> upload(const std::vector<T>& v, hipStream_t st) {
+  DBuf<T> d(v.size(), st);
+  if (!v.empty())
+    hip_throw(hipMemcpyAsync(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st),
+              "upload");
+  hip_throw(hipStreamSynchronize(st), "sync");  // v may die after return
+  return d;
+}
+
+template <int PG, int NDT>
+void launch_near_pg(int ifc, int ifd, int64_t n, hipStream_t st, int nd, int d0, const double* xy,
+                    const int* leaf, const Lists& L, const int* sse, const double* xs,
+                    const cplx* chs, const cplx* dps, double thresh, cplx* p, cplx* g, cplx* h) {
+  if (ifc && !ifd)
+    launch1d(k_near<1, 0, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
+             thresh, p, g, h);
+  else if (!ifc && ifd)
+    launch1d(k_near<0, 1, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
+             thresh, p, g, h);
+  else
+    launch1d(k_near<1, 1, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
+             thresh, p, g, h);
+}
+
+void launch_near(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd, const double* xy,
+                 const int* leaf, const Lists& L, const int* sse, const double* xs,
+                 const cplx* chs, const cplx* dps, double thresh, cplx* p, cplx* g, cplx* h) {
+  const int NDT = (nd == 1) ? 1 : 2;
+  for (int d0 = 0; d0 < nd; d0 += NDT) {
+    if (NDT == 1) {
+      if (pg == 1) launch_near_pg<1, 1>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+      else if (pg == 2) launch_near_pg<2, 1>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+      else launch_near_pg<3, 1>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+    } else {
+      if (pg == 1) launch_near_pg<1, 2>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+      else if (pg == 2) launch_near_pg<2, 2>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+      else launch_near_pg<3, 2>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+    }
+  }
+}
+
+void launch_eval(int pg, int64_t n, hipStream_t st, int nd, int P, const double* xy,
+                 const int* leaf, const Tree& t, const int* nterms, const double* rsc,
+                 const Lists& L, const cplx* mp, const cplx* loc, cplx* p, cplx* g, cplx* h) {
+  if (pg == 1)
+    launch1d(k_eval<1>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
+             L.l3.p, mp, loc, p, g, h);
+  else if (pg == 2)
+    launch1d(k_eval<2>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
+             L.l3.p, mp, loc, p, g, h);
+  else
+    launch1d(k_eval<3>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
+             L.l3.p, mp, loc, p, g, h);
+}
+
+struct Timer {
+  bool on;
+  hipStream_t st;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> slot;
+  Timer(bool enable, hipStream_t s) : on(enable), st(s) { mark(-1); }
+  void mark(int s) {
+    if (!on) return;
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, st);
+    ev.push_back(e);
+    slot.push_back(s);
+  }
+  // after a stream sync: accumulate interval k (ending at event k) into slot
+  void collect(double* out) {
+    if (!on) return;
+    for (size_t k = 1; k < ev.size(); ++k) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[k - 1], ev[k]);
+      if (slot[k] >= 0) out[slot[k]] += 1e-3 * ms;
+    }
+    float tot = 0.f;
+    (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
+    out[11] = 1e-3 * tot;
+  }
+  ~Timer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
+  if (a.nd < 1 || a.ns < 1 || a.nt < 0 || a.ifpgh < 0 || a.ifpgh > 3 || a.ifpghtarg < 0 ||
+      a.ifpghtarg > 3)
+    return FMM2D_ERR_ARG;
+  if (!a.src || (a.nt > 0 && !a.targ) || (a.ifcharge && !a.charge) ||
+      (a.ifdipole && !a.dipstr) || (kind == 1 && a.ifdipole && !a.dipvec))
+    return FMM2D_ERR_ARG;
+  if ((a.ifpgh >= 1 && !a.pot) || (a.ifpgh >= 2 && !a.grad) || (a.ifpgh >= 3 && !a.hess) ||
+      (a.ifpghtarg >= 1 && !a.pottarg) || (a.ifpghtarg >= 2 && !a.gradtarg) ||
+      (a.ifpghtarg >= 3 && !a.hesstarg))
+    return FMM2D_ERR_ARG;
+  if (a.ns > INT32_MAX || a.nt > INT32_MAX) return FMM2D_ERR_ARG;
+  if (!a.ifcharge && !a.ifdipole) return FMM2D_OK;
+  const fmm2d_opts* o = a.opts;
+  if (o && o->precision != 0) return FMM2D_ERR_UNSUPPORTED;
+  try {
+    double tinfo[16] = {0};
+    Timer tm(o != nullptr, st);
+    const int64_t ns = a.ns;
+    const int64_t nt = (a.ifpghtarg > 0) ? a.nt : 0;
+    const int nd = (kind == 1) ? 2 * a.nd : a.nd;  // Cauchy densities
+    int ndiv = lndiv2d(a.eps, ns, nt);
+    int idivflag = 0;
+    if (o && o->ndiv > 0) {
+      ndiv = o->ndiv;
+      idivflag = o->idivflag;
+    }
+    // ---- tree and lists
+    Tree t;
+    build_tree(a.src, ns, a.targ, nt, ndiv, idivflag, 51, t, st);
+    Lists L;
+    build_lists(t, L, st);
+    tm.mark(8);
+    const int nl = t.nlevels, nb = t.nboxes;
+    std::vector<int> nterms(nl + 1, l2dterms(a.eps));
+    if (o && o->n_nterms > 0 && o->nterms)
+      for (int l = 0; l <= nl; ++l) nterms[l] = o->nterms[std::min(l, o->n_nterms - 1)];
+    int pmax = 0;
+    for (int v : nterms) pmax = std::max(pmax, v);
+    const int P = pmax + 1;
+    std::vector<double> rsc(t.boxsize), logr(nl + 1);
+    for (int l = 0; l <= nl; ++l) logr[l] = std::log(rsc[l]);
+    std::vector<cplx> A, B, T;
+    make_matrices(P, A, B, T);
+    DBuf<int> d_nterms = upload(nterms, st);
+    DBuf<double> d_rsc = upload(rsc, st), d_logr = upload(logr, st);
+    DBuf<cplx> d_A = upload(A, st), d_B = upload(B, st), d_T = upload(T, st);
+
+    // ---- sort and gather
+    DBuf<double> xs(2 * ns, st), xt(2 * std::max<int64_t>(nt, 1), st);
+    DBuf<cplx> chs(a.ifcharge ? ns * nd : 1, st), dps(a.ifdipole ? ns * nd : 1, st);
+    launch1d(k_gather_xy, ns, st, t.isrc.p, a.src, xs.p);
+    if (nt > 0) launch1d(k_gather_xy, nt, st, t.itarg.p, a.targ, xt.p);
+    if (kind == 0) {
+      if (a.ifcharge) launch1d(k_gather_dens, ns, st, nd, t.isrc.p, a.charge, chs.p);
+      if (a.ifdipole) launch1d(k_gather_dens, ns, st, nd, t.isrc.p, a.dipstr, dps.p);
+    } else {
+      launch1d(k_gather_split, ns, st, a.nd, t.isrc.p, a.ifcharge ? a.charge : nullptr,
+               a.ifdipole ? a.dipstr : nullptr, a.dipvec, chs.p, dps.p);
+    }
+    DBuf<int> leaf_s(ns, st), leaf_t(std::max<int64_t>(nt, 1), st);
+    launch1d(k_leaf_of, nb, st, t.nchild.p, t.isrcse.p, leaf_s.p);
+    if (nt > 0) launch1d(k_leaf_of, nb, st, t.nchild.p, t.itargse.p, leaf_t.p);
+    const int64_t nexp = (int64_t)nb * nd * P;
+    DBuf<cplx> mp(nexp, st), loc(nexp, st);
+    mp.zero();
+    loc.zero();
+    const int pg = a.ifpgh, pgt = a.ifpghtarg;
+    DBuf<cplx> ps(pg >= 1 ? ns * nd : 1, st), gs(pg >= 2 ? ns * nd : 1, st),
+        hs(pg >= 3 ? ns * nd : 1, st);
+    DBuf<cplx> pt(pgt >= 1 ? nt * nd : 1, st), gt(pgt >= 2 ? nt * nd : 1, st),
+        ht(pgt >= 3 ? nt * nd : 1, st);
+    ps.zero(); gs.zero(); hs.zero(); pt.zero(); gt.zero(); ht.zero();
+    tm.mark(9);
+
+    const int* sse = t.isrcse.p;
+    const int* tse = t.itargse.p;
+    // ---- step 1: P2M
+    if (a.ifcharge && !a.ifdipole)
+      hipLaunchKernelGGL((k_p2m<1, 0>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, t.nchild.p,
+                         d_nterms.p, d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p);
+    else if (!a.ifcharge && a.ifdipole)
+      hipLaunchKernelGGL((k_p2m<0, 1>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, t.nchild.p,
+                         d_nterms.p, d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p);
+    else
+      hipLaunchKernelGGL((k_p2m<1, 1>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, t.nchild.p,
+                         d_nterms.p, d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p);
+    hip_throw(hipGetLastError(), "p2m");
+    tm.mark(0);
+    // ---- step 2: list 4
+    if (a.ifcharge && !a.ifdipole)
+      hipLaunchKernelGGL((k_p2l<1, 0>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p,
+                         d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p,
+                         dps.p, loc.p);
+    else if (!a.ifcharge && a.ifdipole)
+      hipLaunchKernelGGL((k_p2l<0, 1>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p,
+                         d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p,
+                         dps.p, loc.p);
+    else
+      hipLaunchKernelGGL((k_p2l<1, 1>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p,
+                         d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p,
+                         dps.p, loc.p);
+    hip_throw(hipGetLastError(), "p2l");
+    tm.mark(1);
+    // ---- step 3: M2M, finest parents first
+    for (int l = nl - 1; l >= 1; --l) {
+      const int nlev = t.lend[l] - t.lstart[l];
+      if (nlev > 0)
+        hipLaunchKernelGGL(k_m2m, dim3(nlev), dim3(kW), 0, st, t.lstart[l], nd, P, nterms[l + 1],
+                           nterms[l], t.nchild.p, t.ichild.p, sse, d_A.p, mp.p);
+    }
+    hip_throw(hipGetLastError(), "m2m");
+    tm.mark(2);
+    // ---- step 4: M2L
+    hipLaunchKernelGGL(k_m2l, dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p, d_rsc.p,
+                       d_logr.p, t.centers.p, L.n2.p, L.l2.p, sse, tse, pg, pgt, d_T.p, mp.p,
+                       loc.p);
+    hip_throw(hipGetLastError(), "m2l");
+    tm.mark(3);
+    // ---- step 5: L2L, coarsest first
+    for (int l = 1; l <= nl - 1; ++l) {
+      const int nlev = t.lend[l + 1] - t.lstart[l + 1];
+      if (nlev > 0)
+        hipLaunchKernelGGL(k_l2l, dim3(nlev), dim3(kW), 0, st, t.lstart[l + 1], nd, P,
+                           nterms[l], nterms[l + 1], t.iparent.p, t.ichild.p, sse, tse, pg, pgt,
+                           d_B.p, loc.p);
+    }
+    hip_throw(hipGetLastError(), "l2l");
+    tm.mark(4);
+    // ---- steps 6+7: list 3 and L2P
+    if (pg > 0)
+      launch_eval(pg, ns, st, nd, P, xs.p, leaf_s.p, t, d_nterms.p, d_rsc.p, L, mp.p, loc.p,
+                  ps.p, gs.p, hs.p);
+    if (pgt > 0)
+      launch_eval(pgt, nt, st, nd, P, xt.p, leaf_t.p, t, d_nterms.p, d_rsc.p, L, mp.p, loc.p,
+                  pt.p, gt.p, ht.p);
+    tm.mark(6);
+    // ---- step 8: near field, thresh = boxsize(0) 2^-51 (cfmm2d.f:1276)
+    const int ifnear = o ? o->ifnear : 1;
+    if (ifnear) {
+      const double thresh = t.boxsize[0] * std::ldexp(1.0, -51);
+      if (pg > 0)
+        launch_near(pg, a.ifcharge, a.ifdipole, ns, st, nd, xs.p, leaf_s.p, L, sse, xs.p, chs.p,
+                    dps.p, thresh, ps.p, gs.p, hs.p);
+      if (pgt > 0)
+        launch_near(pgt, a.ifcharge, a.ifdipole, nt, st, nd, xt.p, leaf_t.p, L, sse, xs.p,
+                    chs.p, dps.p, thresh, pt.p, gt.p, ht.p);
+    }
+    tm.mark(7);
+    // ---- unsort
+    if (kind == 0) {
+      if (pg >= 1) launch1d(k_scatter, ns, st, nd, t.isrc.p, ps.p, a.pot);
+      if (pg >= 2) launch1d(k_scatter, ns, st, nd, t.isrc.p, gs.p, a.grad);
+      if (pg >= 3) launch1d(k_scatter, ns, st, nd, t.isrc.p, hs.p, a.hess);
+      if (pgt >= 1) launch1d(k_scatter, nt, st, nd, t.itarg.p, pt.p, a.pottarg);
+      if (pgt >= 2) launch1d(k_scatter, nt, st, nd, t.itarg.p, gt.p, a.gradtarg);
+      if (pgt >= 3) launch1d(k_scatter, nt, st, nd, t.itarg.p, ht.p, a.hesstarg);
+    } else {
+      if (pg >= 1)
+        launch1d(k_scatter_unpack, ns, st, a.nd, pg, t.isrc.p, ps.p, gs.p, hs.p, a.pot, a.grad,
+                 a.hess);
+      if (pgt >= 1)
+        launch1d(k_scatter_unpack, nt, st, a.nd, pgt, t.itarg.p, pt.p, gt.p, ht.p, a.pottarg,
+                 a.gradtarg, a.hesstarg);
+    }
+    tm.mark(10);
+    hip_throw(hipStreamSynchronize(st), "sync");
+    if (o) {
+      fmm2d_opts* ow = const_cast<fmm2d_opts*>(o);
+      tm.collect(tinfo);
+      for (int k = 0; k < 16; ++k) ow->timeinfo[k] = tinfo[k];
+      ow->nlevels = nl;
+      ow->nboxes = nb;
+    }
+    return FMM2D_OK;
+  } catch (const HipError& e) {
+    fprintf(stderr, "fmm2d_amd: %s failed: %s\n", e.what, hipGetErrorString(e.err));
+    return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  }
+}
+
+}  // namespace fmm2d

@@ -1,0 +1,42 @@
+// Adaptive quadtree on the device, numbered exactly as the reference numbers
+// it, plus the interaction lists.
+#pragma once
+#include <vector>
+
+#include "devbuf.h"
+
+namespace fmm2d {
+
+constexpr int kMaxColl = 9;    // colleagues incl. self
+constexpr int kMaxList1 = 13;  // computemnlists, common/tree_routs2d.f:459-479
+constexpr int kMaxList2 = 27;
+constexpr int kMaxList3 = 20;
+constexpr int kMaxList4 = 5;
+
+struct Tree {
+  int nlevels = 0;
+  int nboxes = 0;
+  int64_t ns = 0, nt = 0;
+  std::vector<int> lstart, lend;  // level l boxes are [lstart[l], lend[l]) (0-based)
+  std::vector<double> boxsize;    // boxsize[l], l = 0..nlevels
+  // device, indexed by 0-based box id; "none" is -1
+  DBuf<double> centers;  // 2 per box
+  DBuf<int> ilevel, iparent, nchild, ichild /*4*/, ncoll, coll /*9*/;
+  DBuf<int> isrcse, itargse;  // [start, end) per box, 2 per box
+  DBuf<int> isrc, itarg;      // sorted position -> original point index
+};
+
+struct Lists {
+  DBuf<int> n1, l1, n2, l2, n3, l3, n4, l4;  // counts and [box][cap] tables
+};
+
+// Build the tree of the reference's pts_tree_build + pts_tree_sort
+// (common/pts_tree2d.f:406-655, 1222-1246) over the given points.
+// src/trg: (2,n) interleaved device arrays.  Throws HipError on HIP failure.
+void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, int ndiv,
+                int idivflag, int nlmax, Tree& t, hipStream_t st);
+
+// Lists 1-4 of computelists (common/tree_routs2d.f:486-710), as sets.
+void build_lists(const Tree& t, Lists& l, hipStream_t st);
+
+}  // namespace fmm2d

@@ -37,6 +37,29 @@ typedef struct {
   double re, im;
 } fmm2d_cplx;
 
+/*
+ * Optional controls and per-call diagnostics of the FMM drivers (NULL = the
+ * reference's behaviour).  They generalise the reference's only tuning hook,
+ * the *_ndiv entry points (laplace/lfmm2d_ndiv.f:29-32, laplace/cfmm2d_ndiv.f,
+ * helmholtz/hfmm2d_ndiv.f:19-22: ndiv, idivflag, ifnear, timeinfo(8)).
+ */
+typedef struct {
+  int32_t ndiv;        /* max points per leaf; <= 0: reference rule (lndiv2d / hndiv2d) */
+  int32_t idivflag;    /* 0: refine on sources, 1: on targets, 2: on the max (pts_tree_build) */
+  int32_t ifnear;      /* 1: add the list-1 near field (reference default), 0: omit it */
+  int32_t precision;   /* 0: FP64 throughout; 1: mixed FP32/FP64 */
+  int32_t n_nterms;    /* entries in nterms; 0: reference rule (l2dterms / h2dterms) */
+  int32_t pad0;
+  const int32_t* nterms; /* HOST array: expansion order for levels 0..n_nterms-1 */
+  /* out: seconds per stage.  [0..7] are the reference's timeinfo(1..8): P2M,
+   * list-4 P2L, M2M, M2L, L2L, list-3 M2P, L2P, near field (list-3 and L2P
+   * run fused and are reported together in [6]); [8] tree + lists, [9] sort
+   * and gather, [10] scatter, [11] whole call. */
+  double timeinfo[16];
+  int32_t nlevels, nboxes; /* out: tree depth and box count */
+  int32_t pad1, pad2;
+} fmm2d_opts;
+
 /* Library ABI version (FMM2D_ABI_VERSION).  No reference counterpart. */
 int fmm2d_abi_version(void);
 
@@ -74,6 +97,80 @@ int fmm2d_h2d_direct(int nd, double zk_re, double zk_im, const double* sources, 
  *   replaces common/hank103.f:116 hank103(z,h0,h1,ifexpon) with ifexpon=1. */
 int fmm2d_hank103(const fmm2d_cplx* z, int64_t n, fmm2d_cplx* h0, fmm2d_cplx* h1,
                   void* stream);
+
+/*
+ * Cauchy-kernel FMM; OVERWRITES the requested outputs.
+ *   replaces laplace/cfmm2d.f:23 cfmm2d(nd,eps,ns,sources,ifcharge,charge,
+ *   ifdipole,dipstr,iper,ifpgh,pot,grad,hess,nt,targ,ifpghtarg,pottarg,
+ *   gradtarg,hesstarg,ier) -- ier is the return value; iper must be 0.
+ * pot = sum c log|z-xi| + d/(z-xi) with the reference's analytic far-field
+ * expansions; grad(nd,n) = d/dz, hess(nd,n) = d^2/dz^2.  Returns after the
+ * outputs are complete on the device.
+ */
+int fmm2d_cfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
+                 const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dipstr, int iper,
+                 int ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, int64_t nt,
+                 const double* targ, int ifpghtarg, fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg,
+                 fmm2d_cplx* hesstarg, fmm2d_opts* opts, void* stream);
+
+/*
+ * Laplace FMM with complex densities and dipole vectors; OVERWRITES outputs.
+ *   replaces laplace/lfmm2d.f:29 lfmm2d(nd,eps,ns,sources,ifcharge,charge,
+ *   ifdipole,dipstr,dipvec,iper,ifpgh,pot,grad,hess,nt,targ,ifpghtarg,
+ *   pottarg,gradtarg,hesstarg,ier); grad(nd,2,n), hess(nd,3,n).
+ */
+int fmm2d_lfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
+                 const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dipstr,
+                 const double* dipvec, int iper, int ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad,
+                 fmm2d_cplx* hess, int64_t nt, const double* targ, int ifpghtarg,
+                 fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg,
+                 fmm2d_opts* opts, void* stream);
+
+/*
+ * Helmholtz FMM; OVERWRITES outputs (ifpgh, ifpghtarg <= 2).
+ *   replaces helmholtz/hfmm2d.f:19 hfmm2d(nd,eps,zk,ns,sources,ifcharge,
+ *   charge,ifdipole,dipstr,dipvec,iper,ifpgh,pot,grad,hess,nt,targ,
+ *   ifpghtarg,pottarg,gradtarg,hesstarg,ier).
+ */
+int fmm2d_hfmm2d(int nd, double eps, double zk_re, double zk_im, int64_t ns,
+                 const double* sources, int ifcharge, const fmm2d_cplx* charge, int ifdipole,
+                 const fmm2d_cplx* dipstr, const double* dipvec, int iper, int ifpgh,
+                 fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, int64_t nt,
+                 const double* targ, int ifpghtarg, fmm2d_cplx* pottarg,
+                 fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg, fmm2d_opts* opts, void* stream);
+
+/*
+ * Adaptive quadtree (pts_tree_mem + pts_tree_build + pts_tree_sort,
+ * common/pts_tree2d.f:34, 406, 1222) over device-resident points, numbered
+ * box-for-box as the reference numbers it.  *handle receives an opaque tree;
+ * copy it out with fmm2d_tree_get and release it with fmm2d_tree_free.
+ */
+int fmm2d_tree_build(const double* sources, int64_t ns, const double* targ, int64_t nt,
+                     int idivflag, int ndiv, int nlmax, int* nlevels, int* nboxes,
+                     void** handle, void* stream);
+
+/*
+ * Copy a tree to HOST arrays in the reference's conventions (1-based box and
+ * point indices, -1 = none): laddr(2,0:nlevels), ilevel(nb), iparent(nb),
+ * nchild(nb), ichild(4,nb), ncoll(nb), coll(9,nb), centers(2,nb),
+ * boxsize(0:nlevels), isrc(ns), isrcse(2,nb), itarg(nt), itargse(2,nb).
+ * When nlist1 is non-NULL the lists of computelists (common/tree_routs2d.f:486)
+ * are also returned: nlistK(nb) and listK(mK,nb) with mK = 13, 27, 20, 5,
+ * each row in ascending box order.  Any output pointer may be NULL.
+ */
+int fmm2d_tree_get(void* handle, int* laddr, int* ilevel, int* iparent, int* nchild,
+                   int* ichild, int* ncoll, int* coll, double* centers, double* boxsize,
+                   int* isrc, int* isrcse, int* itarg, int* itargse, int* nlist1, int* list1,
+                   int* nlist2, int* list2, int* nlist3, int* list3, int* nlist4, int* list4);
+
+void fmm2d_tree_free(void* handle);
+
+/* Expansion-order and leaf-size rules on the host:
+ *   l2dterms  laplace/l2dterms.f:38     h2dterms helmholtz/h2dterms.f:35
+ *   lndiv2d   laplace/lndiv2d.f         hndiv2d  helmholtz/hndiv2d.f */
+int fmm2d_l2dterms(double eps, int* nterms);
+int fmm2d_h2dterms(double bsize, double zk_re, double zk_im, double eps, int* nterms);
+int fmm2d_ndiv(double eps, int64_t ns, int64_t nt, int helmholtz, int* ndiv, int* idivflag);
 
 #ifdef __cplusplus
 }

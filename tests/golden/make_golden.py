@@ -76,8 +76,93 @@ def hankel_fixture():
                         h0_mp=h0m, h1_mp=h1m)
 
 
+def point_sets():
+    """Named point sets (sources, targets) for the tree and FMM fixtures."""
+    rng = np.random.default_rng(5)
+    sets = {}
+    sets["uniform"] = (rng.random((2, 3000)), rng.random((2, 1500)))
+    # clusters deep enough to exercise the level-restriction passes
+    c = []
+    for cx, cy, s, n in ((0.3, 0.3, 2e-3, 1500), (0.71, 0.64, 1e-2, 1200), (0.5, 0.9, 3e-4, 600)):
+        c.append(np.array([[cx], [cy]]) + s * rng.standard_normal((2, n)))
+    c.append(rng.random((2, 300)))
+    sets["cluster"] = (np.concatenate(c, axis=1), rng.random((2, 700)))
+    t = rng.uniform(0, 2 * np.pi, 2500)
+    r = 1 + 0.3 * np.cos(5 * t)
+    curve = np.stack([r * np.cos(t), r * np.sin(t)]) + 1e-4 * rng.standard_normal((2, 2500))
+    sets["curve"] = (curve, None)
+    return sets
+
+
+def tree_fixtures():
+    out = {}
+    for name, (src, trg) in point_sets().items():
+        for ndiv in (5, 15):
+            tr = R.tree(src, trg, ndiv)
+            li = R.lists(tr)
+            pre = f"{name}_{ndiv}_"
+            for k in ("nlevels", "nboxes", "laddr", "ilevel", "iparent", "nchild", "ichild",
+                      "ncoll", "coll", "centers", "boxsize", "isrc", "isrcse", "itarg",
+                      "itargse"):
+                out[pre + k] = np.asarray(tr[k])
+            for k in range(1, 5):
+                n = li[f"nlist{k}"]
+                L = li[f"list{k}"].copy()
+                for b in range(tr["nboxes"]):
+                    row = np.sort(L[: n[b], b])
+                    L[:, b] = 0
+                    L[: n[b], b] = row
+                out[pre + f"nlist{k}"] = n
+                out[pre + f"list{k}"] = L
+        out[name + "_src"] = src
+        if trg is not None:
+            out[name + "_trg"] = trg
+    np.savez_compressed(os.path.join(HERE, "tree_ref.npz"), **out)
+
+
+def fmm_fixtures():
+    """cfmm2d / lfmm2d / hfmm2d outputs of the reference on small problems."""
+    rng = np.random.default_rng(7)
+    out = {}
+    ns, nt = 1200, 700
+    src = rng.random((2, ns))
+    src[:, :300] = 0.2 + 3e-3 * rng.standard_normal((2, 300))
+    trg = rng.random((2, nt))
+    out["src"], out["trg"] = src, trg
+    for nd in (1, 2):
+        ch = rng.standard_normal((nd, ns)) + 1j * rng.standard_normal((nd, ns))
+        ds = rng.standard_normal((nd, ns)) + 1j * rng.standard_normal((nd, ns))
+        dv = rng.standard_normal((nd, 2, ns))
+        out[f"nd{nd}_charge"], out[f"nd{nd}_dipstr"], out[f"nd{nd}_dipvec"] = ch, ds, dv
+        cases = [(1, 0, 1, 0), (1, 0, 2, 2), (0, 1, 0, 2), (1, 1, 3, 3)] if nd == 1 else \
+                [(1, 1, 2, 2)]
+        for kind in ("c", "l", "h"):
+            for eps in ((1e-6, 1e-12) if kind != "h" else (1e-6, 1e-10)):
+                for ifc, ifd, pg, pgt in cases:
+                    if kind == "h" and (pg > 2 or pgt > 2):
+                        continue
+                    kw = dict(nd=nd, eps=eps, sources=src, charges=ch if ifc else None,
+                              dipstr=ds if ifd else None, dipvec=dv if ifd else None,
+                              targets=trg if pgt else None, ifpgh=pg, ifpghtarg=pgt)
+                    if kind == "c":
+                        kw.pop("dipvec")
+                        r = R.cfmm2d(**kw)
+                    elif kind == "l":
+                        r = R.lfmm2d(**kw)
+                    else:
+                        r = R.hfmm2d(zk=12.0 + 0.0j, **kw)
+                    tag = f"nd{nd}_{kind}_{eps:.0e}_{ifc}{ifd}{pg}{pgt}_"
+                    for k, v in r.items():
+                        if k in ("pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"):
+                            out[tag + k] = v
+                    out[tag + "ier"] = np.array(r["ier"])
+    np.savez_compressed(os.path.join(HERE, "fmm_ref.npz"), **out)
+
+
 if __name__ == "__main__":
     direct_fixtures()
     hankel_fixture()
+    tree_fixtures()
+    fmm_fixtures()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))
