@@ -1,0 +1,198 @@
+"""Benchmark: FMM points/sec at N = 1e7, eps = 1e-6 (lfmm2d, clustered points).
+
+Workload (one "step" = one full lfmm2d call, tree build and unsort included):
+  N = 1e7 sources, 90% drawn from 64 isotropic Gaussians (centres U[0.05,0.95]^2,
+  sigma log-uniform in [1e-3, 1e-2]) and 10% uniform in [0,1]^2, seed 3; real
+  charges N(0,1) passed as complex128 (lfmm2d semantics); eps = 1e-6; potential
+  and gradient at the sources (pg = 2).  Inputs and outputs stay in HBM; the
+  timed region is K back-to-back calls bracketed by device synchronisation.
+
+Reported: points/sec (value), per-stage times from HIP events recorded by the
+library on the call's stream, the dominant stage's roofline fraction against
+MI355X's published peak, relative L2 error of pot+grad against an FP64 direct
+sum at 1e4 sampled sources, and the reference library timed on the host cores
+on a bounded sample of the same distribution.
+
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--n 10000000]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
+STAGES = ["p2m", "list4", "m2m", "m2l", "l2l", "list3", "l2p+list3", "near", "tree+lists",
+          "sort+gather", "scatter"]
+
+
+def c3_points(n, seed=3):
+    """The clustered point set of the workload (deterministic in n, seed)."""
+    rng = np.random.default_rng(seed)
+    nb = n // 10
+    nc = n - nb
+    cen = rng.uniform(0.05, 0.95, (64, 2))
+    sig = np.exp(rng.uniform(np.log(1e-3), np.log(1e-2), 64))
+    lab = rng.integers(0, 64, nc)
+    pts = cen[lab] + sig[lab, None] * rng.standard_normal((nc, 2))
+    pts = np.concatenate([pts, rng.random((nb, 2))])
+    q = rng.standard_normal(n)
+    return pts, q  # pts (n,2) = Fortran (2,n)
+
+
+def cpu_baseline(n_sample, threads):
+    """Reference lfmm2d (oracle/_ref, built from the reference sources) on the
+    host cores, in a child process so its OpenMP runtime is its own."""
+    code = f"""
+import sys, time, json
+sys.path.insert(0, {ROOT!r})
+import numpy as np
+from bench import c3_points
+from oracle import ref_binding as R
+pts, q = c3_points({n_sample})
+src = np.ascontiguousarray(pts.T)
+R.lfmm2d(nd=1, eps=1e-6, sources=src[:, :1000], charges=q[:1000] + 0j, ifpgh=2)
+t = time.perf_counter()
+R.lfmm2d(nd=1, eps=1e-6, sources=src, charges=q + 0j, ifpgh=2)
+dt = time.perf_counter() - t
+print(json.dumps({{"seconds": dt}}))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=600)
+    if r.returncode != 0:
+        return None, r.stderr[-400:]
+    dt = json.loads(r.stdout.strip().splitlines()[-1])["seconds"]
+    return dt, None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nerr", type=int, default=10_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
+              "decomposition is described in DESIGN.md and not measured.", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes as C
+    import torch
+    from fmm2d_amd import _native as nat
+
+    L = nat.lib()
+    dev = nat.torch_device()
+    n = args.n
+    print(f"bench: generating {n} clustered points", file=sys.stderr, flush=True)
+    pts, q = c3_points(n)
+    src = torch.from_numpy(pts).to(dev)                    # (n,2) == Fortran (2,n)
+    chg = torch.from_numpy(q.astype(np.complex128)).to(dev)
+    pot = torch.zeros(n, dtype=torch.complex128, device=dev)
+    grad = torch.zeros((n, 2), dtype=torch.complex128, device=dev)  # Fortran (1,2,n)
+    stream = nat.stream_ptr()
+    opts = nat.opts_struct({"precision": args.precision})
+
+    def step(o):
+        ier = L.fmm2d_lfmm2d(1, 1e-6, n, nat.ptr(src), 1, nat.ptr(chg), 0, None, None, 0, 2,
+                             nat.ptr(pot), nat.ptr(grad), None, 0, None, 0, None, None, None,
+                             nat.byref_opts(o), stream)
+        if ier != 0:
+            raise RuntimeError(f"fmm2d_lfmm2d returned {ier}")
+
+    for _ in range(args.warmup):
+        step(opts)
+    torch.cuda.synchronize()
+    stage = np.zeros(16)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(opts)
+        stage += np.array(opts.timeinfo[:])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    stage /= args.steps
+    work = np.array(opts.work[:])
+    print(f"bench: {dt*1e3:.1f} ms/step, levels {opts.nlevels}, boxes {opts.nboxes}, "
+          f"near pairs {opts.near_pairs}, m2l pairs {opts.m2l_pairs}", file=sys.stderr, flush=True)
+    for k, name in enumerate(STAGES):
+        print(f"bench:   {name:12s} {stage[k]*1e3:9.3f} ms", file=sys.stderr)
+
+    # ---- accuracy: FP64 direct sum at sampled sources
+    rng = np.random.default_rng(12345)
+    idx = np.sort(rng.choice(n, size=min(args.nerr, n), replace=False))
+    m = len(idx)
+    tsel = src[torch.from_numpy(idx).to(dev)].contiguous()
+    pex = torch.zeros(m, dtype=torch.complex128, device=dev)
+    gex = torch.zeros((m, 2), dtype=torch.complex128, device=dev)
+    ier = L.fmm2d_l2d_direct(1, nat.ptr(src), n, 1, nat.ptr(chg), 0, None, None, nat.ptr(tsel),
+                             m, 2, nat.ptr(pex), nat.ptr(gex), None, 1e-16, stream)
+    assert ier == 0
+    it = torch.from_numpy(idx).to(dev)
+    dp = pot[it] - pex
+    dg = grad[it] - gex
+    num = (dp.abs() ** 2).sum() + (dg.abs() ** 2).sum()
+    den = (pex.abs() ** 2).sum() + (gex.abs() ** 2).sum()
+    relerr = float(torch.sqrt(num / den))
+    print(f"bench: rel L2 error (pot+grad, {m} sampled sources) {relerr:.3e}", file=sys.stderr)
+
+    # ---- roofline of the dominant stage (algorithmic flops / event time)
+    k_dom = int(np.argmax(stage[:8]))
+    achieved = work[k_dom] / stage[k_dom] / 1e12 if stage[k_dom] > 0 else 0.0
+    peak = PEAK["fp64_tflops"] if args.precision == "fp64" else PEAK["fp32_tflops"]
+    roofline = {"bound": "mfma", "kernel": STAGES[k_dom], "achieved": round(achieved, 3),
+                "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                "traffic": None}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        print(f"bench: reference CPU baseline on {args.cpu_sample} points, {threads} threads",
+              file=sys.stderr, flush=True)
+        secs, errmsg = cpu_baseline(args.cpu_sample, threads)
+        if secs is not None:
+            cpu = {"value": round(args.cpu_sample / secs, 1), "unit": "points/s",
+                   "cores": threads, "kind": "reference",
+                   "sample": f"reference lfmm2d (oracle/_ref, flang -O3) on the first "
+                             f"{args.cpu_sample} points of the same generator, eps 1e-6, pg 2; "
+                             f"{secs:.2f} s"}
+        else:
+            print("bench: cpu baseline failed: " + str(errmsg), file=sys.stderr)
+
+    line = {
+        "metric": "FMM pts/sec at N=1e7, eps 1e-6: mixed-prec+learned vs FP64 GPU; rel L2 err",
+        "value": round(n / dt, 1),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "lfmm2d N=1e7 clustered (64 Gaussians + 10% uniform), "
+                               "real charges, eps=1e-6, pot+grad at sources",
+                   "n": n, "eps": 1e-6, "pg": 2, "precision": args.precision,
+                   "nlevels": int(opts.nlevels), "nboxes": int(opts.nboxes)},
+        "rel_l2_err": relerr,
+        "stage_ms": {name: round(stage[k] * 1e3, 4) for k, name in enumerate(STAGES)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -49,7 +49,8 @@ class FmmOpts(C.Structure):
                 ("precision", C.c_int32), ("n_nterms", C.c_int32), ("pad0", C.c_int32),
                 ("nterms", C.POINTER(C.c_int32)), ("timeinfo", C.c_double * 16),
                 ("nlevels", C.c_int32), ("nboxes", C.c_int32), ("pad1", C.c_int32),
-                ("pad2", C.c_int32)]
+                ("pad2", C.c_int32), ("work", C.c_double * 16), ("near_pairs", C.c_int64),
+                ("m2l_pairs", C.c_int64)]
 
     def timeinfo_array(self):
         return np.array(self.timeinfo[:], dtype=np.float64)

@@ -410,6 +410,49 @@ __global__ void k_near(int64_t n, int nd, int d0, const double* xy, const int* l
   }
 }
 
+// ---------------------------------------------------------- work counters
+// cnt[0] near pairs, [1] list-2 pairs, [2] M2M edges, [3] L2L edges,
+// [4] list-3 (point, box) pairs, [5] list-4 (source, box) pairs
+__global__ void k_counts(int64_t n, const int* ilevel, const int* iparent, const int* nchild,
+                         const int* ichild, const int* sse, const int* tse, int ifpgh,
+                         int ifpght, const int* n1, const int* l1, const int* n2, const int* n3,
+                         const int* n4, const int* l4, unsigned long long* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)i;
+  const int lev = ilevel[b];
+  const int ne = npts_eval(b, sse, tse, ifpgh, ifpght);
+  unsigned long long near = 0, l3 = 0, l4c = 0, m2m = 0, l2l = 0, m2l = 0;
+  if (nchild[b] == 0) {
+    unsigned long long s = 0;
+    for (int k = 0; k < n1[b]; ++k) {
+      const int c = l1[(int64_t)kMaxList1 * b + k];
+      s += sse[2 * c + 1] - sse[2 * c];
+    }
+    near = s * (unsigned long long)ne;
+    l3 = (unsigned long long)n3[b] * ne;
+  } else if (lev >= 1) {
+    for (int q = 0; q < 4; ++q) {
+      const int c = ichild[4 * b + q];
+      if (sse[2 * c + 1] > sse[2 * c]) ++m2m;
+    }
+  }
+  if (lev >= 2 && ne > 0) {
+    m2l = n2[b];
+    for (int k = 0; k < n4[b]; ++k) {
+      const int c = l4[(int64_t)kMaxList4 * b + k];
+      l4c += sse[2 * c + 1] - sse[2 * c];
+    }
+    if (npts_eval(iparent[b], sse, tse, ifpgh, ifpght) > 0) l2l = 1;
+  }
+  if (near) atomicAdd(&cnt[0], near);
+  if (m2l) atomicAdd(&cnt[1], m2l);
+  if (m2m) atomicAdd(&cnt[2], m2m);
+  if (l2l) atomicAdd(&cnt[3], l2l);
+  if (l3) atomicAdd(&cnt[4], l3);
+  if (l4c) atomicAdd(&cnt[5], l4c);
+}
+
 // ---------------------------------------------------------- host side
 using lcplx = std::complex<long double>;
 
@@ -719,6 +762,14 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
                  a.gradtarg, a.hesstarg);
     }
     tm.mark(10);
+    unsigned long long hc[6] = {0, 0, 0, 0, 0, 0};
+    if (o) {
+      DBuf<unsigned long long> dc(6, st);
+      dc.zero();
+      launch1d(k_counts, nb, st, t.ilevel.p, t.iparent.p, t.nchild.p, t.ichild.p, sse, tse, pg,
+               pgt, L.n1.p, L.l1.p, L.n2.p, L.n3.p, L.n4.p, L.l4.p, dc.p);
+      hip_throw(hipMemcpyAsync(hc, dc.p, sizeof(hc), hipMemcpyDeviceToHost, st), "copy");
+    }
     hip_throw(hipStreamSynchronize(st), "sync");
     if (o) {
       fmm2d_opts* ow = const_cast<fmm2d_opts*>(o);
@@ -726,6 +777,22 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       for (int k = 0; k < 16; ++k) ow->timeinfo[k] = tinfo[k];
       ow->nlevels = nl;
       ow->nboxes = nb;
+      const double Pd = (double)P, ndd = (double)nd;
+      const int64_t neval = (pg > 0 ? ns : 0) + (pgt > 0 ? nt : 0);
+      const double gfac = (std::max(pg, pgt) >= 2) ? 2.0 : 1.0;
+      double w[16] = {0};
+      w[0] = 8.0 * Pd * ndd * (double)ns;
+      w[1] = 8.0 * Pd * ndd * (double)hc[5];
+      w[2] = 8.0 * Pd * Pd * ndd * (double)hc[2];
+      w[3] = 8.0 * Pd * Pd * ndd * (double)hc[1];
+      w[4] = 8.0 * Pd * Pd * ndd * (double)hc[3];
+      w[5] = 0.0;  // list 3 is fused with L2P: counted in [6]
+      w[6] = 8.0 * Pd * ndd * gfac * ((double)neval + (double)hc[4]);
+      w[7] = (12.0 + 12.0 * ndd) * (double)hc[0];
+      w[8] = 40.0 * (double)(ns + nt) + 80.0 * (double)nb;
+      for (int k = 0; k < 16; ++k) ow->work[k] = w[k];
+      ow->near_pairs = (int64_t)hc[0];
+      ow->m2l_pairs = (int64_t)hc[1];
     }
     return FMM2D_OK;
   } catch (const HipError& e) {

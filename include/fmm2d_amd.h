@@ -58,6 +58,13 @@ typedef struct {
   double timeinfo[16];
   int32_t nlevels, nboxes; /* out: tree depth and box count */
   int32_t pad1, pad2;
+  /* out: algorithmic work per stage, same slots as timeinfo: flops for
+   * [0..7] (P2P 12+12*nd per Cauchy pair for pot+grad, M2L 8(p+1)^2 nd per
+   * list-2 pair, 8(p+1) nd per point for P2M/L2P, 8(p+1)^2 nd per M2M/L2L
+   * edge), bytes for [8] (40 per point + 80 per box); 0 when not counted. */
+  double work[16];
+  int64_t near_pairs;      /* out: source-target pairs in the near field */
+  int64_t m2l_pairs;       /* out: list-2 translations performed */
 } fmm2d_opts;
 
 /* Library ABI version (FMM2D_ABI_VERSION).  No reference counterpart. */
