@@ -33,7 +33,7 @@ constexpr double kColl = (double)1.05f;
 
 struct Boxes {
   DBuf<double> centers;
-  DBuf<int> ilevel, iparent, nchild, ichild, ncoll, coll, cs, ct, iflag;
+  DBuf<int> ilevel, iparent, nchild, ichild, ncoll, coll, ses, set, iflag;
   size_t cap = 0;
   void alloc(size_t c, hipStream_t st) {
     cap = c;
@@ -44,8 +44,8 @@ struct Boxes {
     ichild.alloc(4 * c, st);
     ncoll.alloc(c, st);
     coll.alloc(9 * c, st);
-    cs.alloc(c, st);
-    ct.alloc(c, st);
+    ses.alloc(2 * c, st);
+    set.alloc(2 * c, st);
     iflag.alloc(c, st);
   }
   void grow(size_t need, size_t keep) {
@@ -58,8 +58,8 @@ struct Boxes {
     ichild.grow(4 * c, 4 * keep);
     ncoll.grow(c, keep);
     coll.grow(9 * c, 9 * keep);
-    cs.grow(c, keep);
-    ct.grow(c, keep);
+    ses.grow(2 * c, 2 * keep);
+    set.grow(2 * c, 2 * keep);
     iflag.grow(c, keep);
     cap = c;
   }
@@ -117,15 +117,86 @@ __global__ void k_bbox_final(const double* part, int nb, double* out) {
   }
 }
 
+// ------------------------------------------------------------ point paths
+// Each point's path through the tree is the sequence of its quadrants,
+// computed with the reference's own centre arithmetic: child centre = parent
+// centre +- bs_child/2 (one rounding), point below/left of the centre when
+// y - cy < 0 / x - cx < 0.  The path does not depend on which boxes get
+// refined, so it is computed once per point; digits are child indices 0..3 in
+// the reference's order (1=(-,-), 2=(+,-), 3=(-,+), 4=(+,+)), most significant
+// digit = level 1.  Levels 1..31 go to key1, levels 32..62 to key2 (only
+// computed when the tree is that deep).
+constexpr int kD1 = 31;
+
+__global__ void k_path_keys(int64_t n, const double* __restrict__ xy, double cx0, double cy0,
+                            const double* __restrict__ half, int d1, int d2,
+                            unsigned long long* key1, unsigned long long* key2, int* iota) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = xy[2 * i], y = xy[2 * i + 1];
+  double cx = cx0, cy = cy0;
+  unsigned long long k = 0;
+  for (int l = 1; l <= d1; ++l) {
+    const int qx = (x - cx < 0.0) ? 0 : 1, qy = (y - cy < 0.0) ? 0 : 1;
+    k = (k << 2) | (unsigned long long)(2 * qy + qx);
+    const double h = half[l];
+    cx += qx ? h : -h;
+    cy += qy ? h : -h;
+  }
+  key1[i] = k;
+  if (d2 > 0) {
+    k = 0;
+    for (int l = d1 + 1; l <= d1 + d2; ++l) {
+      const int qx = (x - cx < 0.0) ? 0 : 1, qy = (y - cy < 0.0) ? 0 : 1;
+      k = (k << 2) | (unsigned long long)(2 * qy + qx);
+      const double h = half[l];
+      cx += qx ? h : -h;
+      cy += qy ? h : -h;
+    }
+    key2[i] = k;
+  }
+  if (iota) iota[i] = (int)i;
+}
+
+template <typename T>
+__global__ void k_gather_by(int64_t n, const int* idx, const T* in, T* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = in[idx[i]];
+}
+
+struct Keys {
+  const unsigned long long* k1;
+  const unsigned long long* k2;  // null unless the tree is deeper than kD1
+  int d1, d2;
+};
+
+__device__ __forceinline__ int digit_at(const Keys& K, int64_t i, int lev) {
+  if (lev <= K.d1) return (int)((K.k1[i] >> (2 * (K.d1 - lev))) & 3ull);
+  return (int)((K.k2[i] >> (2 * (K.d2 - (lev - K.d1)))) & 3ull);
+}
+
+// first i in [lo, hi) whose digit at level lev is >= q (digits are sorted
+// there because all keys in a box's range share the path above lev)
+__device__ int64_t digit_bound(const Keys& K, int64_t lo, int64_t hi, int lev, int q) {
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (digit_at(K, mid, lev) < q) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // ------------------------------------------------------------ refinement
-__global__ void k_refine_flags(int64_t n, int lo, const int* cs, const int* ct, int idivflag,
+__global__ void k_refine_flags(int64_t n, int lo, const int* ses, const int* set, int idivflag,
                                int ndiv, int* flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = lo + (int)i;
-  int nn = cs[b];
-  if (idivflag == 1) nn = ct[b];
-  if (idivflag == 2) nn = max(cs[b], ct[b]);
+  const int cs = ses[2 * b + 1] - ses[2 * b], ct = set[2 * b + 1] - set[2 * b];
+  int nn = cs;
+  if (idivflag == 1) nn = ct;
+  if (idivflag == 2) nn = max(cs, ct);
   flag[i] = nn > ndiv ? 1 : 0;
 }
 
@@ -142,12 +213,13 @@ __global__ void k_total(int n, const int* flag, const int* offs, int* total) {
 
 // Children of the flagged boxes lo..lo+n-1 are numbered base + 4*offs + q, q =
 // 0..3 for (-,-), (+,-), (-,+), (+,+) around the parent centre
-// (tree_refine_boxes, tree_routs2d.f:25-52).  mode 0: first refinement (child
-// counts zeroed); mode 1: level-restriction refinement (child iflag = 3 when
-// the parent's flag is 1, else 0).
+// (tree_refine_boxes, tree_routs2d.f:25-52).  Their point ranges split the
+// parent's by the path digit at level lev.  mode 1 (level-restriction
+// refinement): child iflag = 3 when the parent's flag is 1, else 0.
 __global__ void k_make_children(int64_t n, int lo, const int* flag, const int* offs, int base,
                                 double half, int lev, double* centers, int* ilevel, int* iparent,
-                                int* nchild, int* ichild, int* cs, int* ct, int* iflag, int mode) {
+                                int* nchild, int* ichild, int* ses, int* set, int* iflag, int mode,
+                                Keys ks, Keys kt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || !flag[i]) return;
   const int b = lo + (int)i;
@@ -155,6 +227,15 @@ __global__ void k_make_children(int64_t n, int lo, const int* flag, const int* o
   const double cx = centers[2 * b], cy = centers[2 * b + 1];
   nchild[b] = 4;
   const int pf = (mode == 1) ? iflag[b] : 0;
+  int bs_[5], bt_[5];
+  bs_[0] = ses[2 * b];
+  bs_[4] = ses[2 * b + 1];
+  bt_[0] = set[2 * b];
+  bt_[4] = set[2 * b + 1];
+  for (int q = 1; q < 4; ++q) {
+    bs_[q] = (int)digit_bound(ks, bs_[q - 1], bs_[4], lev, q);
+    bt_[q] = (int)digit_bound(kt, bt_[q - 1], bt_[4], lev, q);
+  }
   for (int q = 0; q < 4; ++q) {
     const int c = c0 + q;
     centers[2 * c] = cx + ((q & 1) ? half : -half);
@@ -164,24 +245,12 @@ __global__ void k_make_children(int64_t n, int lo, const int* flag, const int* o
     ichild[4 * c] = ichild[4 * c + 1] = ichild[4 * c + 2] = ichild[4 * c + 3] = -1;
     ichild[4 * b + q] = c;
     ilevel[c] = lev;
-    cs[c] = 0;
-    ct[c] = 0;
+    ses[2 * c] = bs_[q];
+    ses[2 * c + 1] = bs_[q + 1];
+    set[2 * c] = bt_[q];
+    set[2 * c + 1] = bt_[q + 1];
     iflag[c] = (mode == 1 && pf == 1) ? 3 : 0;
   }
-}
-
-// Move each point of a just-refined box into its child and count it.
-__global__ void k_classify(int64_t n, const double* __restrict__ xy, int* pbox, const int* nchild,
-                           const int* ichild, const double* centers, int* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = pbox[i];
-  if (nchild[b] == 0) return;
-  const double x = xy[2 * i], y = xy[2 * i + 1];
-  const int q = ((y - centers[2 * b + 1] < 0.0) ? 0 : 2) + ((x - centers[2 * b] < 0.0) ? 0 : 1);
-  const int c = ichild[4 * b + q];
-  pbox[i] = c;
-  atomicAdd(&cnt[c], 1);
 }
 
 // ------------------------------------------------------------ colleagues
@@ -295,7 +364,8 @@ __global__ void k_map_range(int64_t n, int from, int to, int* map) {
 
 __global__ void k_reorg_apply(int64_t n, const int* map, const double* c0, const int* lev0,
                               const int* par0, const int* nch0, const int* ich0, const int* fl0,
-                              double* c1, int* lev1, int* par1, int* nch1, int* ich1, int* fl1) {
+                              const int* s0, const int* t0, double* c1, int* lev1, int* par1,
+                              int* nch1, int* ich1, int* fl1, int* s1, int* t1) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int m = map[i];
@@ -304,77 +374,84 @@ __global__ void k_reorg_apply(int64_t n, const int* map, const double* c0, const
   lev1[m] = lev0[i];
   nch1[m] = nch0[i];
   fl1[m] = fl0[i];
+  s1[2 * m] = s0[2 * i];
+  s1[2 * m + 1] = s0[2 * i + 1];
+  t1[2 * m] = t0[2 * i];
+  t1[2 * m + 1] = t0[2 * i + 1];
   par1[m] = par0[i] < 0 ? -1 : map[par0[i]];
   for (int q = 0; q < 4; ++q) ich1[4 * m + q] = ich0[4 * i + q] < 0 ? -1 : map[ich0[4 * i + q]];
 }
 
-__global__ void k_remap(int64_t n, const int* map, int* v) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  v[i] = map[v[i]];
-}
-
 // ------------------------------------------------------------ final point order
-// Descend to the leaf of the final tree and count points per leaf.
-__global__ void k_descend(int64_t n, const double* __restrict__ xy, int* pbox, const int* nchild,
-                          const int* ichild, const double* centers, int* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int b = pbox[i];
-  const double x = xy[2 * i], y = xy[2 * i + 1];
-  while (nchild[b] > 0) {
-    const int q = ((y - centers[2 * b + 1] < 0.0) ? 0 : 2) + ((x - centers[2 * b] < 0.0) ? 0 : 1);
-    b = ichild[4 * b + q];
-  }
-  pbox[i] = b;
-  atomicAdd(&cnt[b], 1);
+// Positions in path order are the reference's point positions up to the order
+// inside each leaf, which the reference keeps in input order: label every
+// point (in input order) with its leaf's first position, then stable-sort.
+__global__ void k_leaf_label(int64_t n, const int* nchild, const int* se, const int* perm,
+                             int* label) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n || nchild[b] != 0) return;
+  const int s = se[2 * b];
+  for (int i = s; i < se[2 * b + 1]; ++i) label[perm[i]] = s;
 }
 
-__global__ void k_sum_children(int64_t n, int lo, const int* nchild, const int* ichild, int* cnt) {
+__global__ void k_iota(int64_t n, int* v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = lo + (int)i;
-  if (nchild[b] == 0) return;
-  int s = 0;
-  for (int q = 0; q < 4; ++q) s += cnt[ichild[4 * b + q]];
-  cnt[b] = s;
-}
-
-// se[2b] = first sorted position of box b, se[2b+1] = one past its last.
-__global__ void k_starts(int64_t n, int lo, const int* nchild, const int* ichild, const int* cnt,
-                         int* se) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = lo + (int)i;
-  if (nchild[b] == 0) return;
-  int s = se[2 * b];
-  for (int q = 0; q < 4; ++q) {
-    const int c = ichild[4 * b + q];
-    se[2 * c] = s;
-    se[2 * c + 1] = s + cnt[c];
-    s += cnt[c];
-  }
-}
-
-__global__ void k_root_se(int* se, int n) {
-  se[0] = 0;
-  se[1] = n;
-}
-
-__global__ void k_keys(int64_t n, const int* pbox, const int* se, int* key, int* val) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  key[i] = se[2 * pbox[i]];
-  val[i] = (int)i;
+  if (i < n) v[i] = (int)i;
 }
 
 // ------------------------------------------------------------ host helpers
+template <typename K, typename V>
+void sort_pairs(const K* kin, K* kout, const V* vin, V* vout, int64_t n, int bits,
+                hipStream_t st) {
+  size_t bytes = 0;
+  hip_throw(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0,
+                                               bits, st), "sort");
+  DBuf<char> tmp(bytes, st);
+  hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kin, kout, vin, vout, (int)n, 0,
+                                               bits, st), "sort");
+}
+
+// Points of one set (sources or targets) sorted by path key.
+struct PathKeys {
+  int64_t n = 0;
+  int d1 = 0, d2 = 0;
+  DBuf<unsigned long long> k1, k2;  // sorted
+  DBuf<int> perm;                   // sorted position -> input index
+  Keys view() const { return Keys{k1.p, k2.p, d1, d2}; }
+
+  void build(const double* xy, int64_t np, double cx, double cy, const double* half, int dd1,
+             int dd2, hipStream_t st) {
+    n = np;
+    d1 = dd1;
+    d2 = dd2;
+    perm.alloc(std::max<int64_t>(n, 1), st);
+    k1.alloc(std::max<int64_t>(n, 1), st);
+    k2.alloc(d2 > 0 ? std::max<int64_t>(n, 1) : 0, st);
+    if (n == 0) return;
+    DBuf<unsigned long long> u1(n, st), u2(d2 > 0 ? n : 0, st);
+    DBuf<int> iota(n, st);
+    launch1d(k_path_keys, n, st, xy, cx, cy, half, d1, d2, u1.p, u2.p, iota.p);
+    if (d2 == 0) {
+      sort_pairs(u1.p, k1.p, iota.p, perm.p, n, 2 * d1, st);
+    } else {
+      // lexicographic (key1, key2): stable LSD passes, key2 first
+      DBuf<unsigned long long> t2(n, st), t1(n, st);
+      DBuf<int> p1(n, st);
+      sort_pairs(u2.p, t2.p, iota.p, p1.p, n, 2 * d2, st);
+      launch1d(k_gather_by<unsigned long long>, n, st, p1.p, u1.p, t1.p);
+      sort_pairs(t1.p, k1.p, p1.p, perm.p, n, 2 * d1, st);
+      launch1d(k_gather_by<unsigned long long>, n, st, perm.p, u2.p, k2.p);
+    }
+  }
+};
+
 struct Builder {
   hipStream_t st;
   Boxes B;
   int nboxes = 0;
   DBuf<int> flag, offs, total;
   int* h_total = nullptr;
+  Keys ks{}, kt{};
 
   explicit Builder(hipStream_t s) : st(s) {
     total.alloc(1, s);
@@ -408,8 +485,8 @@ struct Builder {
     if (nref == 0) return 0;
     B.grow(nboxes + 4 * (size_t)nref, nboxes);
     launch1d(k_make_children, n, st, lo, flag.p, offs.p, nboxes, 0.5 * bs_child, lev,
-             B.centers.p, B.ilevel.p, B.iparent.p, B.nchild.p, B.ichild.p, B.cs.p, B.ct.p,
-             B.iflag.p, mode);
+             B.centers.p, B.ilevel.p, B.iparent.p, B.nchild.p, B.ichild.p, B.ses.p, B.set.p,
+             B.iflag.p, mode, ks, kt);
     nboxes += 4 * nref;
     return nref;
   }
@@ -422,7 +499,7 @@ struct Builder {
   }
   // pts_tree_reorg: per level >= 2, original boxes then the tail boxes.
   void reorg(std::vector<int>& ls, std::vector<int>& le, const std::vector<int>& ts,
-             const std::vector<int>& te, int nlevels, DBuf<int>* pts[2], int64_t npts[2]) {
+             const std::vector<int>& te, int nlevels) {
     DBuf<int> map(nboxes, st);
     for (int l = 0; l <= std::min(1, nlevels); ++l)
       launch1d(k_map_range, le[l] - ls[l], st, ls[l], ls[l], map.p);
@@ -439,16 +516,16 @@ struct Builder {
     Boxes N;
     N.alloc(B.cap, st);
     launch1d(k_reorg_apply, nboxes, st, map.p, B.centers.p, B.ilevel.p, B.iparent.p, B.nchild.p,
-             B.ichild.p, B.iflag.p, N.centers.p, N.ilevel.p, N.iparent.p, N.nchild.p,
-             N.ichild.p, N.iflag.p);
+             B.ichild.p, B.iflag.p, B.ses.p, B.set.p, N.centers.p, N.ilevel.p, N.iparent.p,
+             N.nchild.p, N.ichild.p, N.iflag.p, N.ses.p, N.set.p);
     B.centers.swap(N.centers);
     B.ilevel.swap(N.ilevel);
     B.iparent.swap(N.iparent);
     B.nchild.swap(N.nchild);
     B.ichild.swap(N.ichild);
     B.iflag.swap(N.iflag);
-    for (int k = 0; k < 2; ++k)
-      if (npts[k] > 0) launch1d(k_remap, npts[k], st, map.p, pts[k]->p);
+    B.ses.swap(N.ses);
+    B.set.swap(N.set);
   }
 };
 
@@ -469,11 +546,27 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
     hip_throw(hipMemcpyAsync(bb, out.p, sizeof(bb), hipMemcpyDeviceToHost, st), "copy");
     hip_throw(hipStreamSynchronize(st), "sync");
   }
-  std::vector<double> bs(1);
-  bs[0] = bb[1] - bb[0];
+  // box sizes of every possible level (boxsize(l+1) = boxsize(l)/2)
+  std::vector<double> bsall(nlmax + 2);
+  bsall[0] = bb[1] - bb[0];
   const double sizey = bb[3] - bb[2];
-  if (sizey > bs[0]) bs[0] = sizey;
+  if (sizey > bsall[0]) bsall[0] = sizey;
+  for (int l = 1; l < (int)bsall.size(); ++l) bsall[l] = bsall[l - 1] / 2;
   const double root[2] = {(bb[0] + bb[1]) / 2, (bb[2] + bb[3]) / 2};
+
+  // ---- point paths, sorted (sources and targets separately)
+  std::vector<double> half(bsall.size());
+  for (size_t l = 0; l < half.size(); ++l) half[l] = 0.5 * bsall[l];
+  DBuf<double> dhalf(half.size(), st);
+  hip_throw(hipMemcpyAsync(dhalf.p, half.data(), sizeof(double) * half.size(),
+                           hipMemcpyHostToDevice, st), "copy");
+  hip_throw(hipStreamSynchronize(st), "sync");
+  const int d1 = std::min(kD1, nlmax);
+  PathKeys pk[2];
+  pk[0].build(src, ns, root[0], root[1], dhalf.p, d1, 0, st);
+  pk[1].build(trg, nt, root[0], root[1], dhalf.p, d1, 0, st);
+  bld.ks = pk[0].view();
+  bld.kt = pk[1].view();
 
   size_t cap0 = std::max<size_t>(1024, (size_t)(8 * (ns + nt) / std::max(ndiv, 1)));
   bld.B.alloc(cap0, st);
@@ -481,30 +574,39 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   {
     const int none4[4] = {-1, -1, -1, -1};
     const int zero = 0, mone = -1;
-    const int cnt[2] = {(int)ns, (int)nt};
+    const int se[4] = {0, (int)ns, 0, (int)nt};
     hip_throw(hipMemcpyAsync(bld.B.centers.p, root, sizeof(root), hipMemcpyHostToDevice, st), "c");
     hip_throw(hipMemcpyAsync(bld.B.ilevel.p, &zero, sizeof(int), hipMemcpyHostToDevice, st), "c");
     hip_throw(hipMemcpyAsync(bld.B.iparent.p, &mone, sizeof(int), hipMemcpyHostToDevice, st), "c");
     hip_throw(hipMemcpyAsync(bld.B.nchild.p, &zero, sizeof(int), hipMemcpyHostToDevice, st), "c");
     hip_throw(hipMemcpyAsync(bld.B.ichild.p, none4, sizeof(none4), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipMemcpyAsync(bld.B.cs.p, &cnt[0], sizeof(int), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipMemcpyAsync(bld.B.ct.p, &cnt[1], sizeof(int), hipMemcpyHostToDevice, st), "c");
+    hip_throw(hipMemcpyAsync(bld.B.ses.p, &se[0], 2 * sizeof(int), hipMemcpyHostToDevice, st), "c");
+    hip_throw(hipMemcpyAsync(bld.B.set.p, &se[2], 2 * sizeof(int), hipMemcpyHostToDevice, st), "c");
     hip_throw(hipMemcpyAsync(bld.B.iflag.p, &zero, sizeof(int), hipMemcpyHostToDevice, st), "c");
     hip_throw(hipStreamSynchronize(st), "sync");
   }
-  DBuf<int> pbs(std::max<int64_t>(ns, 1), st), pbt(std::max<int64_t>(nt, 1), st);
-  pbs.zero();
-  pbt.zero();
 
   // ---- level-by-level refinement (pts_tree2d.f:551-631)
   std::vector<int> ls{0}, le{1};
+  std::vector<double> bs{bsall[0]};
   int nlevels = nlmax;
   for (int l = 0; l < nlmax; ++l) {
     const int lo = ls[l], n = le[l] - ls[l];
     bld.ensure_flag(n);
-    launch1d(k_refine_flags, n, st, lo, bld.B.cs.p, bld.B.ct.p, idivflag, ndiv, bld.flag.p);
+    launch1d(k_refine_flags, n, st, lo, bld.B.ses.p, bld.B.set.p, idivflag, ndiv, bld.flag.p);
+    if (l + 1 > d1 && pk[0].d2 == 0) {
+      // a box at level d1 may need splitting: extend the paths (rare: only
+      // for points closer than 2^-31 of the domain)
+      if (bld.scan(n) > 0) {
+        const int d2 = std::min(62, nlmax) - d1;
+        pk[0].build(src, ns, root[0], root[1], dhalf.p, d1, d2, st);
+        pk[1].build(trg, nt, root[0], root[1], dhalf.p, d1, d2, st);
+        bld.ks = pk[0].view();
+        bld.kt = pk[1].view();
+      }
+    }
     const int base = bld.nboxes;
-    bs.push_back(bs[l] / 2);
+    bs.push_back(bsall[l + 1]);
     const int nref = bld.refine(lo, n, bs[l + 1], l + 1, 0);
     if (nref == 0) {
       bs.pop_back();
@@ -513,18 +615,12 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
     }
     ls.push_back(base);
     le.push_back(bld.nboxes);
-    launch1d(k_classify, ns, st, src, pbs.p, bld.B.nchild.p, bld.B.ichild.p, bld.B.centers.p,
-             bld.B.cs.p);
-    launch1d(k_classify, nt, st, trg, pbt.p, bld.B.nchild.p, bld.B.ichild.p, bld.B.centers.p,
-             bld.B.ct.p);
   }
   ls.resize(nlevels + 1);
   le.resize(nlevels + 1);
   bs.resize(nlevels + 1);
   bld.computecoll(ls, le, bs, nlevels);
 
-  DBuf<int>* pts[2] = {&pbs, &pbt};
-  int64_t npts[2] = {ns, nt};
   // ---- level restriction (pts_tree_fix_lr, pts_tree2d.f:758-1061)
   if (nlevels >= 2) {
     int* iflag = bld.B.iflag.p;
@@ -548,7 +644,7 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
       bld.refine(ls[l], n, bs[l + 1], l + 1, 1);
       te[l + 1] = bld.nboxes;
     }
-    bld.reorg(ls, le, ts, te, nlevels, pts, npts);
+    bld.reorg(ls, le, ts, te, nlevels);
     bld.computecoll(ls, le, bs, nlevels);
     launch1d(k_keep3, bld.nboxes, st, bld.B.iflag.p);
     std::fill(ts.begin(), ts.end(), 0);
@@ -576,7 +672,7 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
       launch1d(k_coll, te[l + 1] - ts[l + 1], st, ts[l + 1], bs[l + 1], bld.B.iparent.p,
                bld.B.ichild.p, bld.B.centers.p, bld.B.ncoll.p, bld.B.coll.p);
     }
-    bld.reorg(ls, le, ts, te, nlevels, pts, npts);
+    bld.reorg(ls, le, ts, te, nlevels);
     bld.computecoll(ls, le, bs, nlevels);
   }
 
@@ -587,36 +683,23 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   t.lstart = ls;
   t.lend = le;
   t.boxsize = bs;
-  t.isrcse.alloc(2 * (size_t)nb, st);
-  t.itargse.alloc(2 * (size_t)nb, st);
   t.isrc.alloc(std::max<int64_t>(ns, 1), st);
   t.itarg.alloc(std::max<int64_t>(nt, 1), st);
-  DBuf<int>* se[2] = {&t.isrcse, &t.itargse};
   DBuf<int>* perm[2] = {&t.isrc, &t.itarg};
-  const double* xy[2] = {src, trg};
+  const int* se[2] = {bld.B.ses.p, bld.B.set.p};
+  const int64_t npts[2] = {ns, nt};
   for (int k = 0; k < 2; ++k) {
-    DBuf<int> cnt(nb, st);
-    cnt.zero();
-    se[k]->zero();
-    launch1d(k_descend, npts[k], st, xy[k], pts[k]->p, bld.B.nchild.p, bld.B.ichild.p,
-             bld.B.centers.p, cnt.p);
-    for (int l = nlevels - 1; l >= 0; --l)
-      launch1d(k_sum_children, le[l] - ls[l], st, ls[l], bld.B.nchild.p, bld.B.ichild.p, cnt.p);
-    hipLaunchKernelGGL(k_root_se, dim3(1), dim3(1), 0, st, se[k]->p, (int)npts[k]);
-    for (int l = 0; l < nlevels; ++l)
-      launch1d(k_starts, le[l] - ls[l], st, ls[l], bld.B.nchild.p, bld.B.ichild.p, cnt.p, se[k]->p);
     if (npts[k] == 0) continue;
-    DBuf<int> keys(npts[k], st), vals(npts[k], st), keys2(npts[k], st);
-    launch1d(k_keys, npts[k], st, pts[k]->p, se[k]->p, keys.p, vals.p);
+    DBuf<int> label(npts[k], st), iota(npts[k], st), lsorted(npts[k], st);
+    launch1d(k_leaf_label, nb, st, bld.B.nchild.p, se[k], pk[k].perm.p, label.p);
     int bits = 1;
     while ((int64_t(1) << bits) <= npts[k]) ++bits;
-    size_t bytes = 0;
-    hip_throw(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys.p, keys2.p, vals.p,
-                                                 perm[k]->p, (int)npts[k], 0, bits, st), "sort");
-    DBuf<char> tmp(bytes, st);
-    hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, keys.p, keys2.p, vals.p,
-                                                 perm[k]->p, (int)npts[k], 0, bits, st), "sort");
+    // values: input indices 0..n-1 in input order
+    launch1d(k_iota, npts[k], st, iota.p);
+    sort_pairs(label.p, lsorted.p, iota.p, perm[k]->p, npts[k], bits, st);
   }
+  t.isrcse = std::move(bld.B.ses);
+  t.itargse = std::move(bld.B.set);
   t.centers = std::move(bld.B.centers);
   t.ilevel = std::move(bld.B.ilevel);
   t.iparent = std::move(bld.B.iparent);

@@ -91,6 +91,10 @@ def point_sets():
     r = 1 + 0.3 * np.cos(5 * t)
     curve = np.stack([r * np.cos(t), r * np.sin(t)]) + 1e-4 * rng.standard_normal((2, 2500))
     sets["curve"] = (curve, None)
+    # exact duplicates (more than ndiv of them): refinement runs to nlmax = 51
+    d = [np.full((2, 12), 0.3141592653589793), 0.77 + 1e-11 * rng.standard_normal((2, 30)),
+         rng.random((2, 300))]
+    sets["deep"] = (np.concatenate(d, axis=1), None)
     return sets
 
 
@@ -160,9 +164,14 @@ def fmm_fixtures():
 
 
 if __name__ == "__main__":
-    direct_fixtures()
-    hankel_fixture()
-    tree_fixtures()
-    fmm_fixtures()
+    which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm"]
+    if "direct" in which:
+        direct_fixtures()
+    if "hankel" in which:
+        hankel_fixture()
+    if "tree" in which:
+        tree_fixtures()
+    if "fmm" in which:
+        fmm_fixtures()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))

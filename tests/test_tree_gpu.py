@@ -3,7 +3,8 @@
 tests/golden/tree_ref.npz holds the reference's pts_tree_build /
 pts_tree_sort / computelists output for three point sets (uniform with
 separate targets; Gaussian clusters down to sigma = 3e-4 that force the
-level-restriction passes; a perturbed closed curve) at ndiv 5 and 15.  Every
+level-restriction passes; a perturbed closed curve; exact duplicates that
+drive refinement to the 51-level limit) at ndiv 5 and 15.  Every
 integer array must match exactly (box numbering, parents, children,
 colleagues, point permutation and ranges, lists as sets) and the centres and
 box sizes bit for bit.
@@ -18,7 +19,7 @@ KEYS = ("laddr", "ilevel", "iparent", "nchild", "ichild", "ncoll", "coll", "isrc
 
 
 @pytest.mark.parametrize("ndiv", [5, 15])
-@pytest.mark.parametrize("name", ["uniform", "cluster", "curve"])
+@pytest.mark.parametrize("name", ["uniform", "cluster", "curve", "deep"])
 def test_tree_bit_exact(golden_dir, name, ndiv):
     from fmm2d_amd.tree import build_tree
     g = np.load(f"{golden_dir}/tree_ref.npz")
