@@ -20,6 +20,8 @@
 //   near field one thread per evaluation point over its leaf's list 1
 // All kernels read and write the box expansions in place; there are no
 // atomics in the FMM itself (pull form, as the reference's loops).
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <complex>
 #include <vector>
@@ -247,6 +249,124 @@ __global__ __launch_bounds__(kW) void k_m2l(int nd, int P, const int* ilevel, co
       loc[((int64_t)b * nd + d) * P + nn] += acc;
     }
   }
+}
+
+// ---------------------------------------------------------- M2L on MFMA
+// The M2L of one target box is sum over its list-2 sources of T_o M_c, with o
+// the source's offset.  For a target in child position q of its parent the
+// possible offsets form a fixed set of 27 (the 6x6 children of the parent's
+// 3x3 neighbourhood minus the 3x3 adjacent ones), so targets of the same q are
+// processed together: a wavefront owns 16 (target, density) columns and, for
+// each of the 27 offset slots, multiplies the real 2P x 2P form of T_o
+// (interleaved re/im, held as pre-swizzled A fragments) into the gathered
+// source multipoles (B, zero where the slot is empty) with
+// v_mfma_f64_16x16x4f64, accumulating the local expansions in registers.
+typedef double d4v __attribute__((ext_vector_type(4)));
+constexpr int kSlots = 27;
+
+// slot of offset (dx,dy) for child position q, or -1 (adjacent / impossible)
+FMM2D_HD int m2l_slot(int q, int dx, int dy) {
+  const int bx = (q & 1) ? -3 : -2, by = (q & 2) ? -3 : -2;
+  const int ix = dx - bx, iy = dy - by;
+  if (ix < 0 || ix > 5 || iy < 0 || iy > 5) return -1;
+  if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) return -1;
+  int s = 0;
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) {
+      const int ex = bx + a, ey = by + b;
+      if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1) continue;
+      if (a == ix && b == iy) return s;
+      ++s;
+    }
+  return -1;
+}
+
+// class key of every box: child position (0..3) of an M2L target, 4 otherwise
+__global__ void k_m2l_class(int64_t n, const int* ilevel, const int* iparent, const int* ichild,
+                            const int* n2, const int* sse, const int* tse, int ifpgh,
+                            int ifpght, int* key, int* val) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)i;
+  int k = 4;
+  if (ilevel[b] >= 2 && n2[b] > 0 && npts_eval(b, sse, tse, ifpgh, ifpght) > 0) {
+    const int dad = iparent[b];
+    k = 0;
+    while (k < 3 && ichild[4 * dad + k] != b) ++k;
+  }
+  key[i] = k;
+  val[i] = b;
+}
+
+// slots[pos][s] = source box of slot s of target tlist[pos] (or -1)
+__global__ void k_m2l_slots(int64_t n, const int* tlist, const int* tkey, const double* centers,
+                            const double* rsc, const int* ilevel, const int* n2, const int* l2,
+                            int* slots) {
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= n) return;
+  const int b = tlist[pos], q = tkey[pos];
+  int* row = slots + pos * kSlots;
+  for (int s = 0; s < kSlots; ++s) row[s] = -1;
+  const double rinv = 1.0 / rsc[ilevel[b]];
+  const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  for (int k = 0; k < n2[b]; ++k) {
+    const int c = l2[(int64_t)kMaxList2 * b + k];
+    const int ox = (int)rint((centers[2 * c] - cx) * rinv);
+    const int oy = (int)rint((centers[2 * c + 1] - cy) * rinv);
+    const int s = m2l_slot(q, ox, oy);
+    if (s >= 0) row[s] = c;
+  }
+}
+
+template <int RT>
+__global__ __launch_bounds__(256) void k_m2l_mfma(int nd, int P, int KS, int64_t nwaves,
+                                                  const int* tlist, const int* slots,
+                                                  const int* seg, const int* tile0,
+                                                  const int* slot_off, const double* Afrag,
+                                                  const int* ilevel, const double* logr,
+                                                  const cplx* mp, cplx* loc) {
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wave >= nwaves) return;
+  const int lane = threadIdx.x & 63;
+  int q = 0;
+  while (q < 3 && wave >= tile0[q + 1]) ++q;
+  const int64_t col = (wave - tile0[q]) * 16 + (lane & 15);
+  const int64_t tpos = seg[q] + col / nd;
+  const int d = (int)(col % nd);
+  const bool valid = tpos < seg[q + 1];
+  const int tb = valid ? tlist[tpos] : -1;
+  const int K2 = 2 * P;
+  const int kr = lane >> 4;
+  d4v acc[RT];
+#pragma unroll
+  for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
+  double m0 = 0.0;
+  for (int s = 0; s < kSlots; ++s) {
+    const int c = valid ? slots[tpos * kSlots + s] : -1;
+    const double* A = Afrag + (int64_t)slot_off[q * kSlots + s] * RT * KS * 64;
+    const double* M = (c >= 0) ? reinterpret_cast<const double*>(mp + ((int64_t)c * nd + d) * P)
+                               : nullptr;
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k = 4 * ks + kr;
+      const double b = (M != nullptr && k < K2) ? M[k] : 0.0;
+      if (ks == 0 && lane < 32) m0 += b;
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+        acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[((int64_t)r * KS + ks) * 64 + lane], b,
+                                                      acc[r], 0, 0, 0);
+    }
+  }
+  if (!valid) return;
+  // level-dependent part of L_0: log(boxsize) * sum of the sources' M_0
+  if (lane < 32) acc[0][0] += logr[ilevel[tb]] * m0;
+  double* L = reinterpret_cast<double*>(loc + ((int64_t)tb * nd + d) * P);
+#pragma unroll
+  for (int r = 0; r < RT; ++r)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 16 * r + kr + 4 * i;
+      if (row < K2) L[row] += acc[r][i];
+    }
 }
 
 // ---------------------------------------------------------- list 4 (P2L)
@@ -508,6 +628,40 @@ void make_matrices(int P, std::vector<cplx>& A, std::vector<cplx>& B, std::vecto
     }
 }
 
+// Real interleaved form of each offset's translation, swizzled into the A
+// operand layout of v_mfma_f64_16x16x4f64 (lane l holds row l%16, column
+// l/16 of a 16x4 block): Afrag[o][row tile][k step][lane].
+std::vector<double> make_afrag(int P, int RT, int KS, const std::vector<cplx>& T) {
+  const int K2 = 2 * P;
+  std::vector<double> out((size_t)kOff * kOff * RT * KS * 64, 0.0);
+  for (int o = 0; o < kOff * kOff; ++o)
+    for (int r = 0; r < RT; ++r)
+      for (int ks = 0; ks < KS; ++ks)
+        for (int l = 0; l < 64; ++l) {
+          const int row = 16 * r + (l & 15), col = 4 * ks + (l >> 4);
+          double v = 0.0;
+          if (row < K2 && col < K2) {
+            const cplx t = T[((size_t)o * P + col / 2) * P + row / 2];
+            const int a = row & 1, b = col & 1;
+            v = (a == b) ? t.re : (a == 0 ? -t.im : t.im);
+          }
+          out[(((size_t)o * RT + r) * KS + ks) * 64 + l] = v;
+        }
+  return out;
+}
+
+__global__ void k_class_bounds(const int* keys, int n, int* seg) {
+  for (int q = 0; q <= 4; ++q) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (keys[mid] < q) lo = mid + 1;
+      else hi = mid;
+    }
+    seg[q] = lo;
+  }
+}
+
 template <typename T> DBuf<T> upload(const std::vector<T>& v, hipStream_t st) {
   DBuf<T> d(v.size(), st);
   if (!v.empty())
@@ -710,10 +864,65 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     hip_throw(hipGetLastError(), "m2m");
     tm.mark(2);
     // ---- step 4: M2L
-    hipLaunchKernelGGL(k_m2l, dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p, d_rsc.p,
-                       d_logr.p, t.centers.p, L.n2.p, L.l2.p, sse, tse, pg, pgt, d_T.p, mp.p,
-                       loc.p);
-    hip_throw(hipGetLastError(), "m2l");
+    const int RT = (2 * P + 15) / 16, KS = (2 * P + 3) / 4;
+    if (RT <= 4) {
+      // targets grouped by child position, in box order within a group
+      DBuf<int> key(nb, st), val(nb, st), skey(nb, st), tlist(nb, st), seg(5, st);
+      launch1d(k_m2l_class, nb, st, t.ilevel.p, t.iparent.p, t.ichild.p, L.n2.p, sse, tse, pg,
+               pgt, key.p, val.p);
+      size_t bytes = 0;
+      hip_throw(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, skey.p, val.p,
+                                                   tlist.p, nb, 0, 3, st), "sort");
+      DBuf<char> tmp(bytes, st);
+      hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, skey.p, val.p, tlist.p,
+                                                   nb, 0, 3, st), "sort");
+      hipLaunchKernelGGL(k_class_bounds, dim3(1), dim3(1), 0, st, skey.p, nb, seg.p);
+      int hseg[5];
+      hip_throw(hipMemcpyAsync(hseg, seg.p, sizeof(hseg), hipMemcpyDeviceToHost, st), "copy");
+      hip_throw(hipStreamSynchronize(st), "sync");
+      std::vector<int> tile0(5, 0);
+      for (int q = 0; q < 4; ++q)
+        tile0[q + 1] = tile0[q] + (int)(((int64_t)(hseg[q + 1] - hseg[q]) * nd + 15) / 16);
+      const int ntg = hseg[4];
+      if (ntg > 0) {
+        DBuf<int> slots((size_t)ntg * kSlots, st);
+        launch1d(k_m2l_slots, ntg, st, tlist.p, skey.p, t.centers.p, d_rsc.p, t.ilevel.p, L.n2.p,
+                 L.l2.p, slots.p);
+        std::vector<int> soff(4 * kSlots, 0);
+        for (int q = 0; q < 4; ++q)
+          for (int dx = -3; dx <= 3; ++dx)
+            for (int dy = -3; dy <= 3; ++dy) {
+              const int sl = m2l_slot(q, dx, dy);
+              if (sl >= 0) soff[q * kSlots + sl] = (dx + 3) * kOff + (dy + 3);
+            }
+        DBuf<int> d_tile0 = upload(tile0, st), d_soff = upload(soff, st);
+        DBuf<double> d_af = upload(make_afrag(P, RT, KS, T), st);
+        const int64_t nwaves = tile0[4];
+        const dim3 grid((unsigned)((nwaves + 3) / 4));
+        if (RT == 1)
+          hipLaunchKernelGGL(k_m2l_mfma<1>, grid, dim3(256), 0, st, nd, P, KS, nwaves, tlist.p,
+                             slots.p, seg.p, d_tile0.p, d_soff.p, d_af.p, t.ilevel.p, d_logr.p,
+                             mp.p, loc.p);
+        else if (RT == 2)
+          hipLaunchKernelGGL(k_m2l_mfma<2>, grid, dim3(256), 0, st, nd, P, KS, nwaves, tlist.p,
+                             slots.p, seg.p, d_tile0.p, d_soff.p, d_af.p, t.ilevel.p, d_logr.p,
+                             mp.p, loc.p);
+        else if (RT == 3)
+          hipLaunchKernelGGL(k_m2l_mfma<3>, grid, dim3(256), 0, st, nd, P, KS, nwaves, tlist.p,
+                             slots.p, seg.p, d_tile0.p, d_soff.p, d_af.p, t.ilevel.p, d_logr.p,
+                             mp.p, loc.p);
+        else
+          hipLaunchKernelGGL(k_m2l_mfma<4>, grid, dim3(256), 0, st, nd, P, KS, nwaves, tlist.p,
+                             slots.p, seg.p, d_tile0.p, d_soff.p, d_af.p, t.ilevel.p, d_logr.p,
+                             mp.p, loc.p);
+        hip_throw(hipGetLastError(), "m2l");
+      }
+    } else {
+      hipLaunchKernelGGL(k_m2l, dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p,
+                         d_rsc.p, d_logr.p, t.centers.p, L.n2.p, L.l2.p, sse, tse, pg, pgt, d_T.p,
+                         mp.p, loc.p);
+      hip_throw(hipGetLastError(), "m2l");
+    }
     tm.mark(3);
     // ---- step 5: L2L, coarsest first
     for (int l = 1; l <= nl - 1; ++l) {
