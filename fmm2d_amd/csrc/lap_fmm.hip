@@ -171,20 +171,29 @@ __global__ __launch_bounds__(kW) void k_p2m(int nd, int P, const int* ilevel, co
   const int p = nterms[lev];
   const double r = rsc[lev], rinv = 1.0 / r;
   const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  constexpr int DC = 4;  // densities per pass
   for (int nn = threadIdx.x; nn <= p; nn += kW) {
-    for (int d = 0; d < nd; ++d) {
-      cplx acc = mkc(0.0);
+    for (int d0 = 0; d0 < nd; d0 += DC) {
+      cplx acc[DC];
+#pragma unroll
+      for (int dd = 0; dd < DC; ++dd) acc[dd] = mkc(0.0);
       for (int j = s0; j < s1; ++j) {
         const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
-        if (nn == 0) {
-          if (IFC) acc += chs[(int64_t)j * nd + d];
-        } else {
-          const cplx pw = cpowi(w, nn - 1);  // (z/r)^(n-1)
-          if (IFC) acc -= (pw * w) * chs[(int64_t)j * nd + d] * (1.0 / nn);
-          if (IFD) acc += pw * dps[(int64_t)j * nd + d] * rinv;
+        const cplx pw = cpowi(w, nn > 0 ? nn - 1 : 0);  // (z/r)^(n-1)
+        const cplx fc = (nn == 0) ? mkc(1.0) : (pw * w) * (-1.0 / nn);
+        const cplx fd = pw * rinv;
+#pragma unroll
+        for (int dd = 0; dd < DC; ++dd) {
+          if (d0 + dd >= nd) break;
+          if (IFC) acc[dd] += fc * chs[(int64_t)j * nd + d0 + dd];
+          if (IFD && nn > 0) acc[dd] += fd * dps[(int64_t)j * nd + d0 + dd];
         }
       }
-      mp[((int64_t)b * nd + d) * P + nn] += acc;
+#pragma unroll
+      for (int dd = 0; dd < DC; ++dd) {
+        if (d0 + dd >= nd) break;
+        mp[((int64_t)b * nd + d0 + dd) * P + nn] += acc[dd];
+      }
     }
   }
 }
@@ -318,55 +327,148 @@ __global__ void k_m2l_slots(int64_t n, const int* tlist, const int* tkey, const 
   }
 }
 
-template <int RT>
-__global__ __launch_bounds__(256) void k_m2l_mfma(int nd, int P, int KS, int64_t nwaves,
-                                                  const int* tlist, const int* slots,
-                                                  const int* seg, const int* tile0,
-                                                  const int* slot_off, const double* Afrag,
-                                                  const int* ilevel, const double* logr,
-                                                  const cplx* mp, cplx* loc) {
+// Work partition of a translate launch: targets tlist[seg[q]..seg[q+1]) form
+// class q (a child position for M2L, a single class otherwise), and waves
+// [tile0[q], tile0[q+1]) cover class q in tiles of 16 (target, density) columns.
+struct Part {
+  int seg[5];
+  int tile0[5];
+};
+
+// Translations on MFMA.  MODE 0 (gather): dst[t] += sum_s A_op(q,s) src[slot(t,s)]
+// (M2L over 27 offset slots, M2M over 4 children).  MODE 1 (scatter): for
+// each slot s, dst[slot(t,s)] += A_op(q,s) src[t] (L2L to the 4 children).
+// Coefficient rows/columns beyond a level's own order are neither read nor
+// written, so per-level orders share one padded operator size.
+template <int RT, int MODE, int LOGT>
+__global__ __launch_bounds__(256) void k_translate_mfma(
+    int nd, int P, int KS, int nslots, int64_t nwaves, Part part, const int* tlist,
+    const int* slots, const int* slot_op, const double* Afrag, const int* ilevel,
+    const int* nterms, const double* logr, const cplx* src, cplx* dst) {
   const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (wave >= nwaves) return;
   const int lane = threadIdx.x & 63;
   int q = 0;
-  while (q < 3 && wave >= tile0[q + 1]) ++q;
-  const int64_t col = (wave - tile0[q]) * 16 + (lane & 15);
-  const int64_t tpos = seg[q] + col / nd;
+  while (q < 3 && wave >= part.tile0[q + 1]) ++q;
+  const int64_t col = (wave - part.tile0[q]) * 16 + (lane & 15);
+  const int64_t tpos = part.seg[q] + col / nd;
   const int d = (int)(col % nd);
-  const bool valid = tpos < seg[q + 1];
+  const bool valid = tpos < part.seg[q + 1];
   const int tb = valid ? tlist[tpos] : -1;
-  const int K2 = 2 * P;
   const int kr = lane >> 4;
-  d4v acc[RT];
+  const int64_t afs = (int64_t)RT * KS * 64;
+  if (MODE == 0) {
+    d4v acc[RT];
 #pragma unroll
-  for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
-  double m0 = 0.0;
-  for (int s = 0; s < kSlots; ++s) {
-    const int c = valid ? slots[tpos * kSlots + s] : -1;
-    const double* A = Afrag + (int64_t)slot_off[q * kSlots + s] * RT * KS * 64;
-    const double* M = (c >= 0) ? reinterpret_cast<const double*>(mp + ((int64_t)c * nd + d) * P)
-                               : nullptr;
-    for (int ks = 0; ks < KS; ++ks) {
+    for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
+    double m0 = 0.0;
+    for (int s = 0; s < nslots; ++s) {
+      const int c = valid ? slots[tpos * nslots + s] : -1;
+      const double* A = Afrag + (int64_t)slot_op[q * nslots + s] * afs;
+      const double* M = nullptr;
+      int kmax = 0;
+      if (c >= 0) {
+        M = reinterpret_cast<const double*>(src + ((int64_t)c * nd + d) * P);
+        kmax = 2 * (nterms[ilevel[c]] + 1);
+      }
+#pragma unroll
+      for (int ks = 0; ks < 16; ++ks) {
+        if (ks < KS) {
+          const int k = 4 * ks + kr;
+          const double b = (k < kmax) ? M[k] : 0.0;
+          if (LOGT && ks == 0 && lane < 32) m0 += b;
+#pragma unroll
+          for (int r = 0; r < RT; ++r)
+            acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[((int64_t)r * KS + ks) * 64 + lane],
+                                                          b, acc[r], 0, 0, 0);
+        }
+      }
+    }
+    if (!valid) return;
+    // level-dependent part of L_0: log(boxsize) * sum of the sources' M_0
+    if (LOGT && lane < 32) acc[0][0] += logr[ilevel[tb]] * m0;
+    const int rmax = 2 * (nterms[ilevel[tb]] + 1);
+    double* L = reinterpret_cast<double*>(dst + ((int64_t)tb * nd + d) * P);
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = 16 * r + kr + 4 * i;
+        if (row < rmax) L[row] += acc[r][i];
+      }
+  } else {
+    double bv[16];
+    const double* M = valid ? reinterpret_cast<const double*>(src + ((int64_t)tb * nd + d) * P)
+                            : nullptr;
+    const int kmax = valid ? 2 * (nterms[ilevel[tb]] + 1) : 0;
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
       const int k = 4 * ks + kr;
-      const double b = (M != nullptr && k < K2) ? M[k] : 0.0;
-      if (ks == 0 && lane < 32) m0 += b;
+      bv[ks] = (ks < KS && k < kmax) ? M[k] : 0.0;
+    }
+    for (int s = 0; s < nslots; ++s) {
+      const int c = valid ? slots[tpos * nslots + s] : -1;
+      const double* A = Afrag + (int64_t)slot_op[q * nslots + s] * afs;
+      d4v acc[RT];
 #pragma unroll
-      for (int r = 0; r < RT; ++r)
-        acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[((int64_t)r * KS + ks) * 64 + lane], b,
-                                                      acc[r], 0, 0, 0);
+      for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < 16; ++ks) {
+        if (ks < KS) {
+#pragma unroll
+          for (int r = 0; r < RT; ++r)
+            acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[((int64_t)r * KS + ks) * 64 + lane],
+                                                          bv[ks], acc[r], 0, 0, 0);
+        }
+      }
+      if (c >= 0) {
+        const int rmax = 2 * (nterms[ilevel[c]] + 1);
+        double* L = reinterpret_cast<double*>(dst + ((int64_t)c * nd + d) * P);
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = 16 * r + kr + 4 * i;
+            if (row < rmax) L[row] += acc[r][i];
+          }
+      }
     }
   }
-  if (!valid) return;
-  // level-dependent part of L_0: log(boxsize) * sum of the sources' M_0
-  if (lane < 32) acc[0][0] += logr[ilevel[tb]] * m0;
-  double* L = reinterpret_cast<double*>(loc + ((int64_t)tb * nd + d) * P);
-#pragma unroll
-  for (int r = 0; r < RT; ++r)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = 16 * r + kr + 4 * i;
-      if (row < K2) L[row] += acc[r][i];
+}
+
+// Non-leaf boxes (in box order, hence level order) and their 4-child slots for
+// M2M (a child without sources is skipped, laprouts2d.f via cfmm2d.f:962-983)
+// and L2L (all children when the parent has points to evaluate, cfmm2d.f:1042-1076).
+__global__ void k_nonleaf_flag(int64_t n, const int* nchild, int* flag, int* iota) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = nchild[i] > 0 ? 1 : 0;
+  iota[i] = (int)i;
+}
+
+__global__ void k_child_slots(int64_t n, const int* list, const int* ichild, const int* sse,
+                              const int* tse, int ifpgh, int ifpght, int* up, int* down) {
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= n) return;
+  const int b = list[pos];
+  const bool ev = npts_eval(b, sse, tse, ifpgh, ifpght) > 0;
+  for (int qq = 0; qq < 4; ++qq) {
+    const int c = ichild[4 * b + qq];
+    up[pos * 4 + qq] = (sse[2 * c + 1] > sse[2 * c]) ? c : -1;
+    down[pos * 4 + qq] = ev ? c : -1;
+  }
+}
+
+__global__ void k_level_bounds(const int* list, int n, const int* lstart, int nl, int* out) {
+  for (int l = 0; l <= nl + 1; ++l) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (list[mid] < lstart[l]) lo = mid + 1;
+      else hi = mid;
     }
+    out[l] = lo;
+  }
 }
 
 // ---------------------------------------------------------- list 4 (P2L)
@@ -385,27 +487,37 @@ __global__ __launch_bounds__(kW) void k_p2l(int nd, int P, const int* ilevel, co
   const int p = nterms[lev];
   const double r = rsc[lev];
   const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  constexpr int DC = 4;  // densities per pass
   for (int nn = threadIdx.x; nn <= p; nn += kW) {
-    for (int d = 0; d < nd; ++d) {
-      cplx acc = mkc(0.0);
+    for (int d0 = 0; d0 < nd; d0 += DC) {
+      cplx acc[DC];
+#pragma unroll
+      for (int dd = 0; dd < DC; ++dd) acc[dd] = mkc(0.0);
       for (int k = 0; k < cnt; ++k) {
         const int c = l4[(int64_t)kMaxList4 * b + k];
         for (int j = sse[2 * c]; j < sse[2 * c + 1]; ++j) {
           const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
           const cplx zi = invc(z0);
           const cplx wn = cpowi(r * zi, nn);  // (r/z0)^n
-          if (IFC) {
-            const cplx c0 = chs[(int64_t)j * nd + d];
-            if (nn == 0) acc += log(absc(z0)) * c0;
-            else acc -= (wn * c0) * (1.0 / nn);
+          const cplx fc = (nn == 0) ? mkc(log(absc(z0))) : wn * (-1.0 / nn);
+          const cplx fd = -(zi * wn);
+#pragma unroll
+          for (int dd = 0; dd < DC; ++dd) {
+            if (d0 + dd >= nd) break;
+            if (IFC) acc[dd] += fc * chs[(int64_t)j * nd + d0 + dd];
+            if (IFD) acc[dd] += fd * dps[(int64_t)j * nd + d0 + dd];
           }
-          if (IFD) acc -= (zi * wn) * dps[(int64_t)j * nd + d];
         }
       }
-      loc[((int64_t)b * nd + d) * P + nn] += acc;
+#pragma unroll
+      for (int dd = 0; dd < DC; ++dd) {
+        if (d0 + dd >= nd) break;
+        loc[((int64_t)b * nd + d0 + dd) * P + nn] += acc[dd];
+      }
     }
   }
 }
+
 
 // ---------------------------------------------------------- L2L
 // l2dlocloc (laprouts2d.f:1020-1123): children at one level from their parent.
@@ -631,10 +743,10 @@ void make_matrices(int P, std::vector<cplx>& A, std::vector<cplx>& B, std::vecto
 // Real interleaved form of each offset's translation, swizzled into the A
 // operand layout of v_mfma_f64_16x16x4f64 (lane l holds row l%16, column
 // l/16 of a 16x4 block): Afrag[o][row tile][k step][lane].
-std::vector<double> make_afrag(int P, int RT, int KS, const std::vector<cplx>& T) {
+std::vector<double> make_afrag_ops(int P, int RT, int KS, const std::vector<cplx>& T, int nops) {
   const int K2 = 2 * P;
-  std::vector<double> out((size_t)kOff * kOff * RT * KS * 64, 0.0);
-  for (int o = 0; o < kOff * kOff; ++o)
+  std::vector<double> out((size_t)nops * RT * KS * 64, 0.0);
+  for (int o = 0; o < nops; ++o)
     for (int r = 0; r < RT; ++r)
       for (int ks = 0; ks < KS; ++ks)
         for (int l = 0; l < 64; ++l) {
@@ -854,18 +966,99 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
                          dps.p, loc.p);
     hip_throw(hipGetLastError(), "p2l");
     tm.mark(1);
+    // ---- translation operators as MFMA fragments: 49 M2L offsets, 4 M2M, 4 L2L
+    const int RT = (2 * P + 15) / 16, KS = (2 * P + 3) / 4;
+    const bool use_mfma = RT <= 4;
+    const int opM2M = kOff * kOff, opL2L = opM2M + 4;
+    DBuf<double> d_af;
+    if (use_mfma) {
+      std::vector<cplx> ops(T);
+      ops.insert(ops.end(), A.begin(), A.end());
+      ops.insert(ops.end(), B.begin(), B.end());
+      d_af = upload(make_afrag_ops(P, RT, KS, ops, opL2L + 4), st);
+    }
+    auto launch_tr = [&](int mode, int logt, int nslots, const Part& part, const int* tlist,
+                         const int* slots, const int* slot_op, const cplx* src, cplx* dst) {
+      const int64_t nwaves = part.tile0[4];
+      if (nwaves == 0) return;
+      const dim3 grid((unsigned)((nwaves + 3) / 4));
+#define FMM2D_TR(R, M, LG)                                                                    \
+  hipLaunchKernelGGL((k_translate_mfma<R, M, LG>), grid, dim3(256), 0, st, nd, P, KS, nslots, \
+                     nwaves, part, tlist, slots, slot_op, d_af.p, t.ilevel.p, d_nterms.p,     \
+                     d_logr.p, src, dst)
+      if (mode == 0 && logt) {
+        if (RT == 1) FMM2D_TR(1, 0, 1); else if (RT == 2) FMM2D_TR(2, 0, 1);
+        else if (RT == 3) FMM2D_TR(3, 0, 1); else FMM2D_TR(4, 0, 1);
+      } else if (mode == 0) {
+        if (RT == 1) FMM2D_TR(1, 0, 0); else if (RT == 2) FMM2D_TR(2, 0, 0);
+        else if (RT == 3) FMM2D_TR(3, 0, 0); else FMM2D_TR(4, 0, 0);
+      } else {
+        if (RT == 1) FMM2D_TR(1, 1, 0); else if (RT == 2) FMM2D_TR(2, 1, 0);
+        else if (RT == 3) FMM2D_TR(3, 1, 0); else FMM2D_TR(4, 1, 0);
+      }
+#undef FMM2D_TR
+      hip_throw(hipGetLastError(), "translate");
+    };
+    auto one_class = [&](int s0, int s1) {
+      Part pt;
+      pt.seg[0] = s0;
+      for (int k = 1; k < 5; ++k) pt.seg[k] = s1;
+      pt.tile0[0] = 0;
+      const int nt16 = (int)(((int64_t)(s1 - s0) * nd + 15) / 16);
+      for (int k = 1; k < 5; ++k) pt.tile0[k] = nt16;
+      return pt;
+    };
+    // non-leaf boxes, level by level, with their child slots
+    DBuf<int> nl_list(nb, st), up_slots, down_slots;
+    std::vector<int> lev_pos(nl + 2, 0);
+    {
+      DBuf<int> flag(nb, st), iota(nb, st), cnt(1, st), dls(nl + 2, st), dlb(nl + 2, st);
+      launch1d(k_nonleaf_flag, nb, st, t.nchild.p, flag.p, iota.p);
+      size_t bytes = 0;
+      hip_throw(hipcub::DeviceSelect::Flagged(nullptr, bytes, iota.p, flag.p, nl_list.p, cnt.p,
+                                              nb, st), "select");
+      DBuf<char> tmp(bytes, st);
+      hip_throw(hipcub::DeviceSelect::Flagged(tmp.p, bytes, iota.p, flag.p, nl_list.p, cnt.p, nb,
+                                              st), "select");
+      int nnl = 0;
+      hip_throw(hipMemcpyAsync(&nnl, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
+      hip_throw(hipStreamSynchronize(st), "sync");
+      std::vector<int> lsx(t.lstart);
+      lsx.push_back(nb);
+      DBuf<int> d_lsx = upload(lsx, st);
+      hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(1), 0, st, nl_list.p, nnl, d_lsx.p, nl,
+                         dlb.p);
+      hip_throw(hipMemcpyAsync(lev_pos.data(), dlb.p, sizeof(int) * (nl + 2),
+                               hipMemcpyDeviceToHost, st), "copy");
+      up_slots.alloc(4 * (size_t)std::max(nnl, 1), st);
+      down_slots.alloc(4 * (size_t)std::max(nnl, 1), st);
+      launch1d(k_child_slots, nnl, st, nl_list.p, t.ichild.p, sse, tse, pg, pgt, up_slots.p,
+               down_slots.p);
+      hip_throw(hipStreamSynchronize(st), "sync");
+    }
+    std::vector<int> op_up(4 * 4), op_down(4 * 4);
+    for (int q = 0; q < 4; ++q)
+      for (int k = 0; k < 4; ++k) {
+        op_up[q * 4 + k] = opM2M + k;
+        op_down[q * 4 + k] = opL2L + k;
+      }
+    DBuf<int> d_op_up = upload(op_up, st), d_op_down = upload(op_down, st);
     // ---- step 3: M2M, finest parents first
     for (int l = nl - 1; l >= 1; --l) {
-      const int nlev = t.lend[l] - t.lstart[l];
-      if (nlev > 0)
-        hipLaunchKernelGGL(k_m2m, dim3(nlev), dim3(kW), 0, st, t.lstart[l], nd, P, nterms[l + 1],
-                           nterms[l], t.nchild.p, t.ichild.p, sse, d_A.p, mp.p);
+      if (use_mfma) {
+        launch_tr(0, 0, 4, one_class(lev_pos[l], lev_pos[l + 1]), nl_list.p, up_slots.p,
+                  d_op_up.p, mp.p, mp.p);
+      } else {
+        const int nlev = t.lend[l] - t.lstart[l];
+        if (nlev > 0)
+          hipLaunchKernelGGL(k_m2m, dim3(nlev), dim3(kW), 0, st, t.lstart[l], nd, P,
+                             nterms[l + 1], nterms[l], t.nchild.p, t.ichild.p, sse, d_A.p, mp.p);
+      }
     }
     hip_throw(hipGetLastError(), "m2m");
     tm.mark(2);
     // ---- step 4: M2L
-    const int RT = (2 * P + 15) / 16, KS = (2 * P + 3) / 4;
-    if (RT <= 4) {
+    if (use_mfma) {
       // targets grouped by child position, in box order within a group
       DBuf<int> key(nb, st), val(nb, st), skey(nb, st), tlist(nb, st), seg(5, st);
       launch1d(k_m2l_class, nb, st, t.ilevel.p, t.iparent.p, t.ichild.p, L.n2.p, sse, tse, pg,
@@ -877,13 +1070,15 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, skey.p, val.p, tlist.p,
                                                    nb, 0, 3, st), "sort");
       hipLaunchKernelGGL(k_class_bounds, dim3(1), dim3(1), 0, st, skey.p, nb, seg.p);
-      int hseg[5];
-      hip_throw(hipMemcpyAsync(hseg, seg.p, sizeof(hseg), hipMemcpyDeviceToHost, st), "copy");
+      Part part;
+      hip_throw(hipMemcpyAsync(part.seg, seg.p, sizeof(part.seg), hipMemcpyDeviceToHost, st),
+                "copy");
       hip_throw(hipStreamSynchronize(st), "sync");
-      std::vector<int> tile0(5, 0);
+      part.tile0[0] = 0;
       for (int q = 0; q < 4; ++q)
-        tile0[q + 1] = tile0[q] + (int)(((int64_t)(hseg[q + 1] - hseg[q]) * nd + 15) / 16);
-      const int ntg = hseg[4];
+        part.tile0[q + 1] =
+            part.tile0[q] + (int)(((int64_t)(part.seg[q + 1] - part.seg[q]) * nd + 15) / 16);
+      const int ntg = part.seg[4];
       if (ntg > 0) {
         DBuf<int> slots((size_t)ntg * kSlots, st);
         launch1d(k_m2l_slots, ntg, st, tlist.p, skey.p, t.centers.p, d_rsc.p, t.ilevel.p, L.n2.p,
@@ -895,27 +1090,8 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
               const int sl = m2l_slot(q, dx, dy);
               if (sl >= 0) soff[q * kSlots + sl] = (dx + 3) * kOff + (dy + 3);
             }
-        DBuf<int> d_tile0 = upload(tile0, st), d_soff = upload(soff, st);
-        DBuf<double> d_af = upload(make_afrag(P, RT, KS, T), st);
-        const int64_t nwaves = tile0[4];
-        const dim3 grid((unsigned)((nwaves + 3) / 4));
-        if (RT == 1)
-          hipLaunchKernelGGL(k_m2l_mfma<1>, grid, dim3(256), 0, st, nd, P, KS, nwaves, tlist.p,
-                             slots.p, seg.p, d_tile0.p, d_soff.p, d_af.p, t.ilevel.p, d_logr.p,
-                             mp.p, loc.p);
-        else if (RT == 2)
-          hipLaunchKernelGGL(k_m2l_mfma<2>, grid, dim3(256), 0, st, nd, P, KS, nwaves, tlist.p,
-                             slots.p, seg.p, d_tile0.p, d_soff.p, d_af.p, t.ilevel.p, d_logr.p,
-                             mp.p, loc.p);
-        else if (RT == 3)
-          hipLaunchKernelGGL(k_m2l_mfma<3>, grid, dim3(256), 0, st, nd, P, KS, nwaves, tlist.p,
-                             slots.p, seg.p, d_tile0.p, d_soff.p, d_af.p, t.ilevel.p, d_logr.p,
-                             mp.p, loc.p);
-        else
-          hipLaunchKernelGGL(k_m2l_mfma<4>, grid, dim3(256), 0, st, nd, P, KS, nwaves, tlist.p,
-                             slots.p, seg.p, d_tile0.p, d_soff.p, d_af.p, t.ilevel.p, d_logr.p,
-                             mp.p, loc.p);
-        hip_throw(hipGetLastError(), "m2l");
+        DBuf<int> d_soff = upload(soff, st);
+        launch_tr(0, 1, kSlots, part, tlist.p, slots.p, d_soff.p, mp.p, loc.p);
       }
     } else {
       hipLaunchKernelGGL(k_m2l, dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p,
@@ -926,11 +1102,16 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     tm.mark(3);
     // ---- step 5: L2L, coarsest first
     for (int l = 1; l <= nl - 1; ++l) {
-      const int nlev = t.lend[l + 1] - t.lstart[l + 1];
-      if (nlev > 0)
-        hipLaunchKernelGGL(k_l2l, dim3(nlev), dim3(kW), 0, st, t.lstart[l + 1], nd, P,
-                           nterms[l], nterms[l + 1], t.iparent.p, t.ichild.p, sse, tse, pg, pgt,
-                           d_B.p, loc.p);
+      if (use_mfma) {
+        launch_tr(1, 0, 4, one_class(lev_pos[l], lev_pos[l + 1]), nl_list.p, down_slots.p,
+                  d_op_down.p, loc.p, loc.p);
+      } else {
+        const int nlev = t.lend[l + 1] - t.lstart[l + 1];
+        if (nlev > 0)
+          hipLaunchKernelGGL(k_l2l, dim3(nlev), dim3(kW), 0, st, t.lstart[l + 1], nd, P,
+                             nterms[l], nterms[l + 1], t.iparent.p, t.ichild.p, sse, tse, pg,
+                             pgt, d_B.p, loc.p);
+      }
     }
     hip_throw(hipGetLastError(), "l2l");
     tm.mark(4);
