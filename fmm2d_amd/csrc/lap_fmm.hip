@@ -29,6 +29,7 @@
 #include "devbuf.h"
 #include "lap_ops.h"
 #include "p2p_math.h"
+#include "translate.h"
 #include "tree.h"
 
 namespace fmm2d {
@@ -270,7 +271,6 @@ __global__ __launch_bounds__(kW) void k_m2l(int nd, int P, const int* ilevel, co
 // (interleaved re/im, held as pre-swizzled A fragments) into the gathered
 // source multipoles (B, zero where the slot is empty) with
 // v_mfma_f64_16x16x4f64, accumulating the local expansions in registers.
-typedef double d4v __attribute__((ext_vector_type(4)));
 constexpr int kSlots = 27;
 
 // slot of offset (dx,dy) for child position q, or -1 (adjacent / impossible)
@@ -324,115 +324,6 @@ __global__ void k_m2l_slots(int64_t n, const int* tlist, const int* tkey, const 
     const int oy = (int)rint((centers[2 * c + 1] - cy) * rinv);
     const int s = m2l_slot(q, ox, oy);
     if (s >= 0) row[s] = c;
-  }
-}
-
-// Work partition of a translate launch: targets tlist[seg[q]..seg[q+1]) form
-// class q (a child position for M2L, a single class otherwise), and waves
-// [tile0[q], tile0[q+1]) cover class q in tiles of 16 (target, density) columns.
-struct Part {
-  int seg[5];
-  int tile0[5];
-};
-
-// Translations on MFMA.  MODE 0 (gather): dst[t] += sum_s A_op(q,s) src[slot(t,s)]
-// (M2L over 27 offset slots, M2M over 4 children).  MODE 1 (scatter): for
-// each slot s, dst[slot(t,s)] += A_op(q,s) src[t] (L2L to the 4 children).
-// Coefficient rows/columns beyond a level's own order are neither read nor
-// written, so per-level orders share one padded operator size.
-template <int RT, int MODE, int LOGT>
-__global__ __launch_bounds__(256) void k_translate_mfma(
-    int nd, int P, int KS, int nslots, int64_t nwaves, Part part, const int* tlist,
-    const int* slots, const int* slot_op, const double* Afrag, const int* ilevel,
-    const int* nterms, const double* logr, const cplx* src, cplx* dst) {
-  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (wave >= nwaves) return;
-  const int lane = threadIdx.x & 63;
-  int q = 0;
-  while (q < 3 && wave >= part.tile0[q + 1]) ++q;
-  const int64_t col = (wave - part.tile0[q]) * 16 + (lane & 15);
-  const int64_t tpos = part.seg[q] + col / nd;
-  const int d = (int)(col % nd);
-  const bool valid = tpos < part.seg[q + 1];
-  const int tb = valid ? tlist[tpos] : -1;
-  const int kr = lane >> 4;
-  const int64_t afs = (int64_t)RT * KS * 64;
-  if (MODE == 0) {
-    d4v acc[RT];
-#pragma unroll
-    for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
-    double m0 = 0.0;
-    for (int s = 0; s < nslots; ++s) {
-      const int c = valid ? slots[tpos * nslots + s] : -1;
-      const double* A = Afrag + (int64_t)slot_op[q * nslots + s] * afs;
-      const double* M = nullptr;
-      int kmax = 0;
-      if (c >= 0) {
-        M = reinterpret_cast<const double*>(src + ((int64_t)c * nd + d) * P);
-        kmax = 2 * (nterms[ilevel[c]] + 1);
-      }
-#pragma unroll
-      for (int ks = 0; ks < 16; ++ks) {
-        if (ks < KS) {
-          const int k = 4 * ks + kr;
-          const double b = (k < kmax) ? M[k] : 0.0;
-          if (LOGT && ks == 0 && lane < 32) m0 += b;
-#pragma unroll
-          for (int r = 0; r < RT; ++r)
-            acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[((int64_t)r * KS + ks) * 64 + lane],
-                                                          b, acc[r], 0, 0, 0);
-        }
-      }
-    }
-    if (!valid) return;
-    // level-dependent part of L_0: log(boxsize) * sum of the sources' M_0
-    if (LOGT && lane < 32) acc[0][0] += logr[ilevel[tb]] * m0;
-    const int rmax = 2 * (nterms[ilevel[tb]] + 1);
-    double* L = reinterpret_cast<double*>(dst + ((int64_t)tb * nd + d) * P);
-#pragma unroll
-    for (int r = 0; r < RT; ++r)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = 16 * r + kr + 4 * i;
-        if (row < rmax) L[row] += acc[r][i];
-      }
-  } else {
-    double bv[16];
-    const double* M = valid ? reinterpret_cast<const double*>(src + ((int64_t)tb * nd + d) * P)
-                            : nullptr;
-    const int kmax = valid ? 2 * (nterms[ilevel[tb]] + 1) : 0;
-#pragma unroll
-    for (int ks = 0; ks < 16; ++ks) {
-      const int k = 4 * ks + kr;
-      bv[ks] = (ks < KS && k < kmax) ? M[k] : 0.0;
-    }
-    for (int s = 0; s < nslots; ++s) {
-      const int c = valid ? slots[tpos * nslots + s] : -1;
-      const double* A = Afrag + (int64_t)slot_op[q * nslots + s] * afs;
-      d4v acc[RT];
-#pragma unroll
-      for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int ks = 0; ks < 16; ++ks) {
-        if (ks < KS) {
-#pragma unroll
-          for (int r = 0; r < RT; ++r)
-            acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[((int64_t)r * KS + ks) * 64 + lane],
-                                                          bv[ks], acc[r], 0, 0, 0);
-        }
-      }
-      if (c >= 0) {
-        const int rmax = 2 * (nterms[ilevel[c]] + 1);
-        double* L = reinterpret_cast<double*>(dst + ((int64_t)c * nd + d) * P);
-#pragma unroll
-        for (int r = 0; r < RT; ++r)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int row = 16 * r + kr + 4 * i;
-            if (row < rmax) L[row] += acc[r][i];
-          }
-      }
-    }
   }
 }
 
@@ -740,28 +631,6 @@ void make_matrices(int P, std::vector<cplx>& A, std::vector<cplx>& B, std::vecto
     }
 }
 
-// Real interleaved form of each offset's translation, swizzled into the A
-// operand layout of v_mfma_f64_16x16x4f64 (lane l holds row l%16, column
-// l/16 of a 16x4 block): Afrag[o][row tile][k step][lane].
-std::vector<double> make_afrag_ops(int P, int RT, int KS, const std::vector<cplx>& T, int nops) {
-  const int K2 = 2 * P;
-  std::vector<double> out((size_t)nops * RT * KS * 64, 0.0);
-  for (int o = 0; o < nops; ++o)
-    for (int r = 0; r < RT; ++r)
-      for (int ks = 0; ks < KS; ++ks)
-        for (int l = 0; l < 64; ++l) {
-          const int row = 16 * r + (l & 15), col = 4 * ks + (l >> 4);
-          double v = 0.0;
-          if (row < K2 && col < K2) {
-            const cplx t = T[((size_t)o * P + col / 2) * P + row / 2];
-            const int a = row & 1, b = col & 1;
-            v = (a == b) ? t.re : (a == 0 ? -t.im : t.im);
-          }
-          out[(((size_t)o * RT + r) * KS + ks) * 64 + l] = v;
-        }
-  return out;
-}
-
 __global__ void k_class_bounds(const int* keys, int n, int* seg) {
   for (int q = 0; q <= 4; ++q) {
     int lo = 0, hi = n;
@@ -967,52 +836,22 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     hip_throw(hipGetLastError(), "p2l");
     tm.mark(1);
     // ---- translation operators as MFMA fragments: 49 M2L offsets, 4 M2M, 4 L2L
-    const int RT = (2 * P + 15) / 16, KS = (2 * P + 3) / 4;
-    const bool use_mfma = RT <= 4;
     const int opM2M = kOff * kOff, opL2L = opM2M + 4;
     DBuf<double> d_af;
-    if (use_mfma) {
+    {
       std::vector<cplx> ops(T);
       ops.insert(ops.end(), A.begin(), A.end());
       ops.insert(ops.end(), B.begin(), B.end());
-      d_af = upload(make_afrag_ops(P, RT, KS, ops, opL2L + 4), st);
+      d_af = upload(swizzle_ops(P, ops, opL2L + 4), st);
     }
-    auto launch_tr = [&](int mode, int logt, int nslots, const Part& part, const int* tlist,
-                         const int* slots, const int* slot_op, const cplx* src, cplx* dst) {
-      const int64_t nwaves = part.tile0[4];
-      if (nwaves == 0) return;
-      const dim3 grid((unsigned)((nwaves + 3) / 4));
-#define FMM2D_TR(R, M, LG)                                                                    \
-  hipLaunchKernelGGL((k_translate_mfma<R, M, LG>), grid, dim3(256), 0, st, nd, P, KS, nslots, \
-                     nwaves, part, tlist, slots, slot_op, d_af.p, t.ilevel.p, d_nterms.p,     \
-                     d_logr.p, src, dst)
-      if (mode == 0 && logt) {
-        if (RT == 1) FMM2D_TR(1, 0, 1); else if (RT == 2) FMM2D_TR(2, 0, 1);
-        else if (RT == 3) FMM2D_TR(3, 0, 1); else FMM2D_TR(4, 0, 1);
-      } else if (mode == 0) {
-        if (RT == 1) FMM2D_TR(1, 0, 0); else if (RT == 2) FMM2D_TR(2, 0, 0);
-        else if (RT == 3) FMM2D_TR(3, 0, 0); else FMM2D_TR(4, 0, 0);
-      } else {
-        if (RT == 1) FMM2D_TR(1, 1, 0); else if (RT == 2) FMM2D_TR(2, 1, 0);
-        else if (RT == 3) FMM2D_TR(3, 1, 0); else FMM2D_TR(4, 1, 0);
-      }
-#undef FMM2D_TR
-      hip_throw(hipGetLastError(), "translate");
-    };
-    auto one_class = [&](int s0, int s1) {
-      Part pt;
-      pt.seg[0] = s0;
-      for (int k = 1; k < 5; ++k) pt.seg[k] = s1;
-      pt.tile0[0] = 0;
-      const int nt16 = (int)(((int64_t)(s1 - s0) * nd + 15) / 16);
-      for (int k = 1; k < 5; ++k) pt.tile0[k] = nt16;
-      return pt;
-    };
+    std::vector<int2> band(nl + 1);
+    for (int l = 0; l <= nl; ++l) band[l] = int2{0, 2 * (nterms[l] + 1)};
+    DBuf<int2> d_band = upload(band, st);
     // non-leaf boxes, level by level, with their child slots
     DBuf<int> nl_list(nb, st), up_slots, down_slots;
     std::vector<int> lev_pos(nl + 2, 0);
     {
-      DBuf<int> flag(nb, st), iota(nb, st), cnt(1, st), dls(nl + 2, st), dlb(nl + 2, st);
+      DBuf<int> flag(nb, st), iota(nb, st), cnt(1, st), dlb(nl + 2, st);
       launch1d(k_nonleaf_flag, nb, st, t.nchild.p, flag.p, iota.p);
       size_t bytes = 0;
       hip_throw(hipcub::DeviceSelect::Flagged(nullptr, bytes, iota.p, flag.p, nl_list.p, cnt.p,
@@ -1036,30 +875,15 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
                down_slots.p);
       hip_throw(hipStreamSynchronize(st), "sync");
     }
-    std::vector<int> op_up(4 * 4), op_down(4 * 4);
-    for (int q = 0; q < 4; ++q)
-      for (int k = 0; k < 4; ++k) {
-        op_up[q * 4 + k] = opM2M + k;
-        op_down[q * 4 + k] = opL2L + k;
-      }
-    DBuf<int> d_op_up = upload(op_up, st), d_op_down = upload(op_down, st);
+    DBuf<int> d_op4 = upload(std::vector<int>{0, 1, 2, 3}, st);
     // ---- step 3: M2M, finest parents first
-    for (int l = nl - 1; l >= 1; --l) {
-      if (use_mfma) {
-        launch_tr(0, 0, 4, one_class(lev_pos[l], lev_pos[l + 1]), nl_list.p, up_slots.p,
-                  d_op_up.p, mp.p, mp.p);
-      } else {
-        const int nlev = t.lend[l] - t.lstart[l];
-        if (nlev > 0)
-          hipLaunchKernelGGL(k_m2m, dim3(nlev), dim3(kW), 0, st, t.lstart[l], nd, P,
-                             nterms[l + 1], nterms[l], t.nchild.p, t.ichild.p, sse, d_A.p, mp.p);
-      }
-    }
-    hip_throw(hipGetLastError(), "m2m");
+    for (int l = nl - 1; l >= 1; --l)
+      launch_translate(0, 0, nd, P, 4, {TrGroup{lev_pos[l], lev_pos[l + 1], opM2M, 0}},
+                       nl_list.p, up_slots.p, d_op4.p, d_af.p, t.ilevel.p, d_band.p, d_logr.p,
+                       mp.p, mp.p, st);
     tm.mark(2);
-    // ---- step 4: M2L
-    if (use_mfma) {
-      // targets grouped by child position, in box order within a group
+    // ---- step 4: M2L, targets grouped by child position (box order within a group)
+    {
       DBuf<int> key(nb, st), val(nb, st), skey(nb, st), tlist(nb, st), seg(5, st);
       launch1d(k_m2l_class, nb, st, t.ilevel.p, t.iparent.p, t.ichild.p, L.n2.p, sse, tse, pg,
                pgt, key.p, val.p);
@@ -1070,15 +894,10 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, skey.p, val.p, tlist.p,
                                                    nb, 0, 3, st), "sort");
       hipLaunchKernelGGL(k_class_bounds, dim3(1), dim3(1), 0, st, skey.p, nb, seg.p);
-      Part part;
-      hip_throw(hipMemcpyAsync(part.seg, seg.p, sizeof(part.seg), hipMemcpyDeviceToHost, st),
-                "copy");
+      int hseg[5];
+      hip_throw(hipMemcpyAsync(hseg, seg.p, sizeof(hseg), hipMemcpyDeviceToHost, st), "copy");
       hip_throw(hipStreamSynchronize(st), "sync");
-      part.tile0[0] = 0;
-      for (int q = 0; q < 4; ++q)
-        part.tile0[q + 1] =
-            part.tile0[q] + (int)(((int64_t)(part.seg[q + 1] - part.seg[q]) * nd + 15) / 16);
-      const int ntg = part.seg[4];
+      const int ntg = hseg[4];
       if (ntg > 0) {
         DBuf<int> slots((size_t)ntg * kSlots, st);
         launch1d(k_m2l_slots, ntg, st, tlist.p, skey.p, t.centers.p, d_rsc.p, t.ilevel.p, L.n2.p,
@@ -1091,29 +910,18 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
               if (sl >= 0) soff[q * kSlots + sl] = (dx + 3) * kOff + (dy + 3);
             }
         DBuf<int> d_soff = upload(soff, st);
-        launch_tr(0, 1, kSlots, part, tlist.p, slots.p, d_soff.p, mp.p, loc.p);
+        std::vector<TrGroup> groups;
+        for (int q = 0; q < 4; ++q) groups.push_back(TrGroup{hseg[q], hseg[q + 1], 0, q});
+        launch_translate(0, 1, nd, P, kSlots, groups, tlist.p, slots.p, d_soff.p, d_af.p,
+                         t.ilevel.p, d_band.p, d_logr.p, mp.p, loc.p, st);
       }
-    } else {
-      hipLaunchKernelGGL(k_m2l, dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p,
-                         d_rsc.p, d_logr.p, t.centers.p, L.n2.p, L.l2.p, sse, tse, pg, pgt, d_T.p,
-                         mp.p, loc.p);
-      hip_throw(hipGetLastError(), "m2l");
     }
     tm.mark(3);
     // ---- step 5: L2L, coarsest first
-    for (int l = 1; l <= nl - 1; ++l) {
-      if (use_mfma) {
-        launch_tr(1, 0, 4, one_class(lev_pos[l], lev_pos[l + 1]), nl_list.p, down_slots.p,
-                  d_op_down.p, loc.p, loc.p);
-      } else {
-        const int nlev = t.lend[l + 1] - t.lstart[l + 1];
-        if (nlev > 0)
-          hipLaunchKernelGGL(k_l2l, dim3(nlev), dim3(kW), 0, st, t.lstart[l + 1], nd, P,
-                             nterms[l], nterms[l + 1], t.iparent.p, t.ichild.p, sse, tse, pg,
-                             pgt, d_B.p, loc.p);
-      }
-    }
-    hip_throw(hipGetLastError(), "l2l");
+    for (int l = 1; l <= nl - 1; ++l)
+      launch_translate(1, 0, nd, P, 4, {TrGroup{lev_pos[l], lev_pos[l + 1], opL2L, 0}},
+                       nl_list.p, down_slots.p, d_op4.p, d_af.p, t.ilevel.p, d_band.p, d_logr.p,
+                       loc.p, loc.p, st);
     tm.mark(4);
     // ---- steps 6+7: list 3 and L2P
     if (pg > 0)

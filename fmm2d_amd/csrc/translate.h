@@ -1,0 +1,250 @@
+// Expansion-to-expansion translations (M2M, M2L, L2L) on MFMA, shared by the
+// Laplace/Cauchy and Helmholtz drivers.
+//
+// A translation is y = T x with T a complex P x P operator and x, y box
+// expansions.  In real form T becomes a 2P x 2P matrix over interleaved
+// (re, im) coefficients; operators are pre-swizzled into the A-operand layout
+// of v_mfma_f64_16x16x4f64 (lane l holds row l%16, column l/16 of a 16x4
+// block; the result tile holds row l/16 + 4 i, column l%16 in element i).
+//
+// Work is split into groups: group g owns targets tlist[seg0[g]..seg1[g])
+// and waves [tile0[g], tile0[g+1]); within a group every target has the same
+// slot -> operator map (opbase[g] + slot_op[cls[g]*nslots + s]), so the A
+// operand is uniform across a wavefront, whose 16 columns are (target,
+// density) pairs.  MODE 0 gathers: dst[t] += sum_s T_s src[slot(t,s)]; MODE 1
+// scatters: dst[slot(t,s)] += T_s src[t].  band[level] = [lo, hi) are the
+// interleaved coefficient rows a level uses (the rest stay zero).  With LOGT
+// the Laplace M2L term log(boxsize) M_0 is added to L_0.
+#pragma once
+#include <vector>
+
+#include "common.h"
+#include "devbuf.h"
+
+namespace fmm2d {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+// Group table of one launch, passed by value: group g owns targets
+// tlist[seg0[g]..seg1[g]) and waves [tile0[g], tile0[g+1]).
+constexpr int kTrMaxGroups = 8;
+struct TrTable {
+  int ng;
+  int tile0[kTrMaxGroups + 1];
+  int seg0[kTrMaxGroups], seg1[kTrMaxGroups];
+  int opbase[kTrMaxGroups], cls[kTrMaxGroups];
+};
+
+// RT row tiles are accumulated per pass over the k steps; with KSU > 0 the k
+// loop is unrolled to KSU steps (operators of at most 16 RT rows and 4 KSU
+// columns, the whole operator in one pass), with KSU == 0 it is a runtime
+// loop and the row tiles are covered in chunks of RT.
+template <int RT, int KSU, int MODE, int LOGT>
+__global__ __launch_bounds__(256) void k_translate(
+    int nd, int P, int nslots, int64_t nwaves, TrTable tab, const int* tlist, const int* slots,
+    const int* slot_op,
+    const double* Afrag, int64_t afs, const int* ilevel, const int2* band, const double* logr,
+    const cplx* src, cplx* dst) {
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wave >= nwaves) return;
+  const int lane = threadIdx.x & 63;
+  int g = 0;
+  while (g + 1 < tab.ng && wave >= tab.tile0[g + 1]) ++g;
+  const int q = tab.cls[g];
+  const int64_t col = (wave - tab.tile0[g]) * 16 + (lane & 15);
+  const int64_t tpos = tab.seg0[g] + col / nd;
+  const int d = (int)(col % nd);
+  const bool valid = tpos < tab.seg1[g];
+  const int tb = valid ? tlist[tpos] : -1;
+  const int kr = lane >> 4;
+  const int K2 = 2 * P, RTT = (K2 + 15) / 16, KS = (K2 + 3) / 4;
+  const int ob = tab.opbase[g];
+  for (int rc = 0; rc < RTT; rc += RT) {
+    if (MODE == 0) {
+      d4v acc[RT];
+#pragma unroll
+      for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
+      double m0 = 0.0;
+      for (int s = 0; s < nslots; ++s) {
+        const int c = valid ? slots[tpos * nslots + s] : -1;
+        const double* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
+        const double* M = nullptr;
+        int klo = 0, khi = 0;
+        if (c >= 0) {
+          M = reinterpret_cast<const double*>(src + ((int64_t)c * nd + d) * P);
+          const int2 bd = band[ilevel[c]];
+          klo = bd.x;
+          khi = bd.y;
+        }
+        auto step = [&](int ks) {
+          const int k = 4 * ks + kr;
+          const double b = (k >= klo && k < khi) ? M[k] : 0.0;
+          if (LOGT && rc == 0 && ks == 0 && lane < 32) m0 += b;
+#pragma unroll
+          for (int r = 0; r < RT; ++r)
+            if (RT == 1 || KSU > 0 || rc + r < RTT)
+              acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                  A[((int64_t)(rc + r) * KS + ks) * 64 + lane], b, acc[r], 0, 0, 0);
+        };
+        if (KSU > 0) {
+#pragma unroll
+          for (int ks = 0; ks < KSU; ++ks)
+            if (ks < KS) step(ks);
+        } else {
+          for (int ks = 0; ks < KS; ++ks) step(ks);
+        }
+      }
+      if (valid) {
+        if (LOGT && rc == 0 && lane < 32) acc[0][0] += logr[ilevel[tb]] * m0;
+        const int2 bd = band[ilevel[tb]];
+        double* L = reinterpret_cast<double*>(dst + ((int64_t)tb * nd + d) * P);
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = 16 * (rc + r) + kr + 4 * i;
+            if (rc + r < RTT && row >= bd.x && row < bd.y) L[row] += acc[r][i];
+          }
+      }
+    } else {
+      const double* M = valid ? reinterpret_cast<const double*>(src + ((int64_t)tb * nd + d) * P)
+                              : nullptr;
+      int klo = 0, khi = 0;
+      if (valid) {
+        const int2 bd = band[ilevel[tb]];
+        klo = bd.x;
+        khi = bd.y;
+      }
+      // the source coefficients are reused by every slot: keep them in
+      // registers when the k loop is unrolled
+      double bv[KSU > 0 ? KSU : 1];
+      if (KSU > 0) {
+#pragma unroll
+        for (int ks = 0; ks < (KSU > 0 ? KSU : 1); ++ks) {
+          const int k = 4 * ks + kr;
+          bv[ks] = (ks < KS && k >= klo && k < khi) ? M[k] : 0.0;
+        }
+      }
+      for (int s = 0; s < nslots; ++s) {
+        const int c = valid ? slots[tpos * nslots + s] : -1;
+        const double* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
+        d4v acc[RT];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
+        auto step = [&](int ks, double b) {
+#pragma unroll
+          for (int r = 0; r < RT; ++r)
+            if (RT == 1 || KSU > 0 || rc + r < RTT)
+              acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                  A[((int64_t)(rc + r) * KS + ks) * 64 + lane], b, acc[r], 0, 0, 0);
+        };
+        if (KSU > 0) {
+#pragma unroll
+          for (int ks = 0; ks < (KSU > 0 ? KSU : 1); ++ks)
+            if (ks < KS) step(ks, bv[ks]);
+        } else {
+          for (int ks = 0; ks < KS; ++ks) {
+            const int k = 4 * ks + kr;
+            step(ks, (k >= klo && k < khi) ? M[k] : 0.0);
+          }
+        }
+        if (c >= 0) {
+          const int2 bd = band[ilevel[c]];
+          double* L = reinterpret_cast<double*>(dst + ((int64_t)c * nd + d) * P);
+#pragma unroll
+          for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int row = 16 * (rc + r) + kr + 4 * i;
+              if (rc + r < RTT && row >= bd.x && row < bd.y) L[row] += acc[r][i];
+            }
+        }
+      }
+    }
+  }
+}
+
+// Real interleaved form of complex operators ops[o][j][n] (input j, output n),
+// swizzled as Afrag[o][row tile][k step][lane].
+inline std::vector<double> swizzle_ops(int P, const std::vector<cplx>& ops, int nops) {
+  const int K2 = 2 * P, RTT = (K2 + 15) / 16, KS = (K2 + 3) / 4;
+  std::vector<double> out((size_t)nops * RTT * KS * 64, 0.0);
+  for (int o = 0; o < nops; ++o)
+    for (int r = 0; r < RTT; ++r)
+      for (int ks = 0; ks < KS; ++ks)
+        for (int l = 0; l < 64; ++l) {
+          const int row = 16 * r + (l & 15), col = 4 * ks + (l >> 4);
+          double v = 0.0;
+          if (row < K2 && col < K2) {
+            const cplx t = ops[((size_t)o * P + col / 2) * P + row / 2];
+            const int a = row & 1, b = col & 1;
+            v = (a == b) ? t.re : (a == 0 ? -t.im : t.im);
+          }
+          out[(((size_t)o * RTT + r) * KS + ks) * 64 + l] = v;
+        }
+  return out;
+}
+
+inline int64_t afrag_stride(int P) {
+  const int K2 = 2 * P;
+  return (int64_t)((K2 + 15) / 16) * ((K2 + 3) / 4) * 64;
+}
+
+struct TrGroup {
+  int seg0, seg1;  // targets tlist[seg0..seg1)
+  int opbase, cls;
+};
+
+// Launch translate passes over the given groups (at most kTrMaxGroups per
+// launch; launches are stream-ordered, so groups must not depend on each other).
+inline void launch_translate(int mode, int logt, int nd, int P, int nslots,
+                             const std::vector<TrGroup>& groups, const int* tlist,
+                             const int* slots, const int* slot_op, const double* Afrag,
+                             const int* ilevel, const int2* band, const double* logr,
+                             const cplx* src, cplx* dst, hipStream_t st) {
+  const int64_t afs = afrag_stride(P);
+  const int RTT = (2 * P + 15) / 16;
+  size_t gi = 0;
+  while (gi < groups.size()) {
+    TrTable tab{};
+    int tiles = 0;
+    for (; gi < groups.size() && tab.ng < kTrMaxGroups; ++gi) {
+      const TrGroup& g = groups[gi];
+      if (g.seg1 <= g.seg0) continue;
+      tab.tile0[tab.ng] = tiles;
+      tab.seg0[tab.ng] = g.seg0;
+      tab.seg1[tab.ng] = g.seg1;
+      tab.opbase[tab.ng] = g.opbase;
+      tab.cls[tab.ng] = g.cls;
+      ++tab.ng;
+      tiles += (int)(((int64_t)(g.seg1 - g.seg0) * nd + 15) / 16);
+    }
+    if (tab.ng == 0) continue;
+    tab.tile0[tab.ng] = tiles;
+    const dim3 grid((unsigned)((tiles + 3) / 4));
+#define FMM2D_TR(RT, KSU, MODE, LOGT)                                                           \
+  hipLaunchKernelGGL((k_translate<RT, KSU, MODE, LOGT>), grid, dim3(256), 0, st, nd, P, nslots, \
+                     (int64_t)tiles, tab, tlist, slots, slot_op, Afrag, afs, ilevel, band, logr, \
+                     src, dst)
+#define FMM2D_TR_RT(MODE, LOGT)                 \
+  switch (RTT) {                                \
+    case 1: FMM2D_TR(1, 4, MODE, LOGT); break;  \
+    case 2: FMM2D_TR(2, 8, MODE, LOGT); break;  \
+    case 3: FMM2D_TR(3, 12, MODE, LOGT); break; \
+    case 4: FMM2D_TR(4, 16, MODE, LOGT); break; \
+    default: FMM2D_TR(4, 0, MODE, LOGT); break; \
+  }
+    if (mode == 0 && logt) {
+      FMM2D_TR_RT(0, 1)
+    } else if (mode == 0) {
+      FMM2D_TR_RT(0, 0)
+    } else {
+      FMM2D_TR_RT(1, 0)
+    }
+#undef FMM2D_TR_RT
+#undef FMM2D_TR
+    hip_throw(hipGetLastError(), "translate");
+  }
+}
+
+}  // namespace fmm2d
