@@ -1,0 +1,390 @@
+// Pieces shared by the FMM drivers (Laplace/Cauchy and Helmholtz): point
+// gather/scatter, leaf lookup, the child and list-2 work plans that feed the
+// MFMA translate kernel, work counters, upload and stage timing.
+//
+// Everything here has internal linkage (anonymous namespace), so each driver
+// translation unit carries its own copy of the kernels.
+#pragma once
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "devbuf.h"
+#include "tree.h"
+
+namespace fmm2d {
+namespace {
+
+constexpr int kW = 64;  // wavefront size
+constexpr int kOff = 7;  // offsets -3..3 per axis
+
+FMM2D_HD cplx cpowi(cplx w, int k) {
+  cplx r = mkc(1.0), b = w;
+  while (k) {
+    if (k & 1) r = r * b;
+    b = b * b;
+    k >>= 1;
+  }
+  return r;
+}
+
+__device__ __forceinline__ int npts_eval(int b, const int* sse, const int* tse, int ifpgh,
+                                         int ifpght) {
+  int n = 0;
+  if (ifpgh > 0) n += sse[2 * b + 1] - sse[2 * b];
+  if (ifpght > 0) n += tse[2 * b + 1] - tse[2 * b];
+  return n;
+}
+
+
+// ---------------------------------------------------------- gather / scatter
+__global__ void k_gather_xy(int64_t n, const int* perm, const double* xy, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = perm[i];
+  out[2 * i] = xy[2 * (int64_t)j];
+  out[2 * i + 1] = xy[2 * (int64_t)j + 1];
+}
+
+// cfmm2d: densities copied as they are.
+__global__ void k_gather_dens(int64_t n, int nd, const int* perm, const cplx* in, cplx* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = perm[i];
+  for (int d = 0; d < nd; ++d) out[i * nd + d] = in[j * nd + d];
+}
+
+
+// out[perm[i]] = in[i] for m values per point
+__global__ void k_scatter(int64_t n, int m, const int* perm, const cplx* in, cplx* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = perm[i];
+  for (int k = 0; k < m; ++k) out[j * m + k] = in[i * m + k];
+}
+
+
+__global__ void k_leaf_of(int64_t n, const int* nchild, const int* se, int* leaf) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n || nchild[b] != 0) return;
+  for (int i = se[2 * b]; i < se[2 * b + 1]; ++i) leaf[i] = (int)b;
+}
+
+
+// ---------------------------------------------------------- list-2 plan
+// The M2L of one target box is sum over its list-2 sources of T_o M_c, with o
+// the source's offset.  For a target in child position q of its parent the
+// possible offsets form a fixed set of 27 (the 6x6 children of the parent's
+// 3x3 neighbourhood minus the 3x3 adjacent ones), so targets of the same q are
+// processed together by the translate kernel, one offset slot after another.
+constexpr int kSlots = 27;
+
+// slot of offset (dx,dy) for child position q, or -1 (adjacent / impossible)
+FMM2D_HD int m2l_slot(int q, int dx, int dy) {
+  const int bx = (q & 1) ? -3 : -2, by = (q & 2) ? -3 : -2;
+  const int ix = dx - bx, iy = dy - by;
+  if (ix < 0 || ix > 5 || iy < 0 || iy > 5) return -1;
+  if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) return -1;
+  int s = 0;
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) {
+      const int ex = bx + a, ey = by + b;
+      if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1) continue;
+      if (a == ix && b == iy) return s;
+      ++s;
+    }
+  return -1;
+}
+
+// class key of every box: child position (0..3) of an M2L target, 4 otherwise
+__global__ void k_m2l_class(int64_t n, const int* ilevel, const int* iparent, const int* ichild,
+                            const int* n2, const int* sse, const int* tse, int ifpgh,
+                            int ifpght, int* key, int* val) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)i;
+  int k = 4;
+  if (ilevel[b] >= 2 && n2[b] > 0 && npts_eval(b, sse, tse, ifpgh, ifpght) > 0) {
+    const int dad = iparent[b];
+    k = 0;
+    while (k < 3 && ichild[4 * dad + k] != b) ++k;
+  }
+  key[i] = k;
+  val[i] = b;
+}
+
+// slots[pos][s] = source box of slot s of target tlist[pos] (or -1)
+__global__ void k_m2l_slots(int64_t n, const int* tlist, const int* tkey, const double* centers,
+                            const double* bsz, const int* ilevel, const int* n2, const int* l2,
+                            int* slots) {
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= n) return;
+  const int b = tlist[pos], q = tkey[pos];
+  int* row = slots + pos * kSlots;
+  for (int s = 0; s < kSlots; ++s) row[s] = -1;
+  const double rinv = 1.0 / bsz[ilevel[b]];
+  const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  for (int k = 0; k < n2[b]; ++k) {
+    const int c = l2[(int64_t)kMaxList2 * b + k];
+    const int ox = (int)rint((centers[2 * c] - cx) * rinv);
+    const int oy = (int)rint((centers[2 * c + 1] - cy) * rinv);
+    const int s = m2l_slot(q, ox, oy);
+    if (s >= 0) row[s] = c;
+  }
+}
+
+// Non-leaf boxes (in box order, hence level order) and their 4-child slots for
+// M2M (a child without sources is skipped, laprouts2d.f via cfmm2d.f:962-983)
+// and L2L (all children when the parent has points to evaluate, cfmm2d.f:1042-1076).
+__global__ void k_nonleaf_flag(int64_t n, const int* nchild, int* flag, int* iota) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = nchild[i] > 0 ? 1 : 0;
+  iota[i] = (int)i;
+}
+
+__global__ void k_child_slots(int64_t n, const int* list, const int* ichild, const int* sse,
+                              const int* tse, int ifpgh, int ifpght, int* up, int* down) {
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= n) return;
+  const int b = list[pos];
+  const bool ev = npts_eval(b, sse, tse, ifpgh, ifpght) > 0;
+  for (int qq = 0; qq < 4; ++qq) {
+    const int c = ichild[4 * b + qq];
+    up[pos * 4 + qq] = (sse[2 * c + 1] > sse[2 * c]) ? c : -1;
+    down[pos * 4 + qq] = ev ? c : -1;
+  }
+}
+
+__global__ void k_level_bounds(const int* list, int n, const int* lstart, int nl, int* out) {
+  for (int l = 0; l <= nl + 1; ++l) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (list[mid] < lstart[l]) lo = mid + 1;
+      else hi = mid;
+    }
+    out[l] = lo;
+  }
+}
+
+
+// ---------------------------------------------------------- work counters
+// cnt[0] near pairs, [1] list-2 pairs, [2] M2M edges, [3] L2L edges,
+// [4] list-3 (point, box) pairs, [5] list-4 (source, box) pairs; M2M edges
+// count at the parent, L2L edges at the child
+__global__ void k_counts(int64_t n, const int* ilevel, const int* iparent, const int* nchild,
+                         const int* ichild, const int* sse, const int* tse, int ifpgh,
+                         int ifpght, const int* n1, const int* l1, const int* n2, const int* n3,
+                         const int* n4, const int* l4, int per_level,
+                         unsigned long long* cnt_all) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)i;
+  const int lev = ilevel[b];
+  // per_level: counters of box b's level at cnt_all[6 lev ..]
+  unsigned long long* cnt = cnt_all + (per_level ? 6 * lev : 0);
+  const int ne = npts_eval(b, sse, tse, ifpgh, ifpght);
+  unsigned long long near = 0, l3 = 0, l4c = 0, m2m = 0, l2l = 0, m2l = 0;
+  if (nchild[b] == 0) {
+    unsigned long long s = 0;
+    for (int k = 0; k < n1[b]; ++k) {
+      const int c = l1[(int64_t)kMaxList1 * b + k];
+      s += sse[2 * c + 1] - sse[2 * c];
+    }
+    near = s * (unsigned long long)ne;
+    l3 = (unsigned long long)n3[b] * ne;
+  } else if (lev >= 1) {
+    for (int q = 0; q < 4; ++q) {
+      const int c = ichild[4 * b + q];
+      if (sse[2 * c + 1] > sse[2 * c]) ++m2m;
+    }
+  }
+  if (lev >= 2 && ne > 0) {
+    m2l = n2[b];
+    for (int k = 0; k < n4[b]; ++k) {
+      const int c = l4[(int64_t)kMaxList4 * b + k];
+      l4c += sse[2 * c + 1] - sse[2 * c];
+    }
+    if (npts_eval(iparent[b], sse, tse, ifpgh, ifpght) > 0) l2l = 1;
+  }
+  if (near) atomicAdd(&cnt[0], near);
+  if (m2l) atomicAdd(&cnt[1], m2l);
+  if (m2m) atomicAdd(&cnt[2], m2m);
+  if (l2l) atomicAdd(&cnt[3], l2l);
+  if (l3) atomicAdd(&cnt[4], l3);
+  if (l4c) atomicAdd(&cnt[5], l4c);
+}
+
+
+__global__ void k_class_bounds(const int* keys, int n, int* seg) {
+  for (int q = 0; q <= 4; ++q) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (keys[mid] < q) lo = mid + 1;
+      else hi = mid;
+    }
+    seg[q] = lo;
+  }
+}
+
+template <typename T> DBuf<T> upload(const std::vector<T>& v, hipStream_t st) {
+  DBuf<T> d(v.size(), st);
+  if (!v.empty())
+    hip_throw(hipMemcpyAsync(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st),
+              "upload");
+  hip_throw(hipStreamSynchronize(st), "sync");  // v may die after return
+  return d;
+}
+
+
+struct Timer {
+  bool on;
+  hipStream_t st;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> slot;
+  Timer(bool enable, hipStream_t s) : on(enable), st(s) { mark(-1); }
+  void mark(int s) {
+    if (!on) return;
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, st);
+    ev.push_back(e);
+    slot.push_back(s);
+  }
+  // after a stream sync: accumulate interval k (ending at event k) into slot
+  void collect(double* out) {
+    if (!on) return;
+    for (size_t k = 1; k < ev.size(); ++k) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[k - 1], ev[k]);
+      if (slot[k] >= 0) out[slot[k]] += 1e-3 * ms;
+    }
+    float tot = 0.f;
+    (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
+    out[11] = 1e-3 * tot;
+  }
+  ~Timer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+
+// Non-leaf boxes level by level with their child slots: up[4 pos + q] is
+// child q if it holds sources (M2M input), down[4 pos + q] is child q if the
+// parent has points to evaluate (L2L output).  list[lev_pos[l] .. lev_pos[l+1])
+// are the non-leaf boxes of level l.
+struct ChildPlan {
+  DBuf<int> list, up, down;
+  std::vector<int> lev_pos;
+};
+
+void plan_children(const Tree& t, int pg, int pgt, ChildPlan& cp, hipStream_t st) {
+  const int nb = t.nboxes, nl = t.nlevels;
+  cp.list.alloc(nb, st);
+  cp.lev_pos.assign(nl + 2, 0);
+  DBuf<int> flag(nb, st), iota(nb, st), cnt(1, st), dlb(nl + 2, st);
+  launch1d(k_nonleaf_flag, nb, st, t.nchild.p, flag.p, iota.p);
+  size_t bytes = 0;
+  hip_throw(hipcub::DeviceSelect::Flagged(nullptr, bytes, iota.p, flag.p, cp.list.p, cnt.p, nb,
+                                          st), "select");
+  DBuf<char> tmp(bytes, st);
+  hip_throw(hipcub::DeviceSelect::Flagged(tmp.p, bytes, iota.p, flag.p, cp.list.p, cnt.p, nb,
+                                          st), "select");
+  int nnl = 0;
+  hip_throw(hipMemcpyAsync(&nnl, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
+  hip_throw(hipStreamSynchronize(st), "sync");
+  std::vector<int> lsx(t.lstart);
+  lsx.push_back(nb);
+  DBuf<int> d_lsx = upload(lsx, st);
+  hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(1), 0, st, cp.list.p, nnl, d_lsx.p, nl,
+                     dlb.p);
+  hip_throw(hipMemcpyAsync(cp.lev_pos.data(), dlb.p, sizeof(int) * (nl + 2),
+                           hipMemcpyDeviceToHost, st), "copy");
+  cp.up.alloc(4 * (size_t)std::max(nnl, 1), st);
+  cp.down.alloc(4 * (size_t)std::max(nnl, 1), st);
+  launch1d(k_child_slots, nnl, st, cp.list.p, t.ichild.p, t.isrcse.p, t.itargse.p, pg, pgt,
+           cp.up.p, cp.down.p);
+  hip_throw(hipStreamSynchronize(st), "sync");
+}
+
+// seg[q * (nl + 2) + l]: first position in tlist of class q at level >= l
+__global__ void k_class_level_bounds(const int* tlist, const int* cseg, const int* lstart,
+                                     int nl, int* seg) {
+  const int q = threadIdx.x;
+  if (q >= 4) return;
+  for (int l = 0; l <= nl + 1; ++l) {
+    int lo = cseg[q], hi = cseg[q + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (tlist[mid] < lstart[l]) lo = mid + 1;
+      else hi = mid;
+    }
+    seg[q * (nl + 2) + l] = lo;
+  }
+}
+
+// M2L targets (level >= 2, non-empty list 2, points to evaluate) sorted by
+// child position q, box order within a class; slots[pos * 27 + s] = source
+// box of offset slot s (or -1).  cls_seg[q] .. cls_seg[q+1] is class q, and
+// lev_seg[q][l] .. lev_seg[q][l+1] its part at level l.
+struct M2LPlan {
+  DBuf<int> tlist, skey, slots;
+  int cls_seg[5] = {0, 0, 0, 0, 0};
+  std::vector<int> lev_seg;  // 4 x (nl + 2)
+  int ntg = 0;
+  int lev(int q, int l, int nl) const { return lev_seg[(size_t)q * (nl + 2) + l]; }
+};
+
+void plan_m2l(const Tree& t, const Lists& L, int pg, int pgt, M2LPlan& mp, hipStream_t st) {
+  const int nb = t.nboxes, nl = t.nlevels;
+  DBuf<int> key(nb, st), val(nb, st), seg(5, st);
+  mp.skey.alloc(nb, st);
+  mp.tlist.alloc(nb, st);
+  launch1d(k_m2l_class, nb, st, t.ilevel.p, t.iparent.p, t.ichild.p, L.n2.p, t.isrcse.p,
+           t.itargse.p, pg, pgt, key.p, val.p);
+  size_t bytes = 0;
+  hip_throw(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, mp.skey.p, val.p,
+                                               mp.tlist.p, nb, 0, 3, st), "sort");
+  DBuf<char> tmp(bytes, st);
+  hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, mp.skey.p, val.p,
+                                               mp.tlist.p, nb, 0, 3, st), "sort");
+  hipLaunchKernelGGL(k_class_bounds, dim3(1), dim3(1), 0, st, mp.skey.p, nb, seg.p);
+  std::vector<int> lsx(t.lstart);
+  lsx.push_back(nb);
+  DBuf<int> d_lsx = upload(lsx, st);
+  DBuf<int> dls(4 * (nl + 2), st);
+  hipLaunchKernelGGL(k_class_level_bounds, dim3(1), dim3(4), 0, st, mp.tlist.p, seg.p, d_lsx.p,
+                     nl, dls.p);
+  mp.lev_seg.assign(4 * (size_t)(nl + 2), 0);
+  hip_throw(hipMemcpyAsync(mp.cls_seg, seg.p, sizeof(mp.cls_seg), hipMemcpyDeviceToHost, st),
+            "copy");
+  hip_throw(hipMemcpyAsync(mp.lev_seg.data(), dls.p, sizeof(int) * mp.lev_seg.size(),
+                           hipMemcpyDeviceToHost, st), "copy");
+  hip_throw(hipStreamSynchronize(st), "sync");
+  mp.ntg = mp.cls_seg[4];
+  if (mp.ntg > 0) {
+    DBuf<double> d_bs = upload(t.boxsize, st);
+    mp.slots.alloc((size_t)mp.ntg * kSlots, st);
+    launch1d(k_m2l_slots, mp.ntg, st, mp.tlist.p, mp.skey.p, t.centers.p, d_bs.p,
+             t.ilevel.p, L.n2.p, L.l2.p, mp.slots.p);
+    hip_throw(hipStreamSynchronize(st), "sync");  // d_bs dies here
+  }
+}
+
+// slot -> offset index (dx+3)*7 + (dy+3) for each child position q
+std::vector<int> m2l_slot_offsets() {
+  std::vector<int> soff(4 * kSlots, 0);
+  for (int q = 0; q < 4; ++q)
+    for (int dx = -3; dx <= 3; ++dx)
+      for (int dy = -3; dy <= 3; ++dy) {
+        const int sl = m2l_slot(q, dx, dy);
+        if (sl >= 0) soff[q * kSlots + sl] = (dx + 3) * kOff + (dy + 3);
+      }
+  return soff;
+}
+
+}  // namespace
+}  // namespace fmm2d

@@ -1,6 +1,9 @@
 // Helmholtz FMM (hfmm2d) on the device.
 #pragma once
 #include "../../include/fmm2d_amd.h"
+#include <complex>
+#include <vector>
+
 #include "common.h"
 
 namespace fmm2d {
@@ -27,5 +30,9 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st);
 // h2dterms (helmholtz/h2dterms.f:35-96); *ier = 13 when no order up to 50000
 // meets the criterion.
 int h2dterms(double bsize, cplx zk, double eps, int* ier);
+
+// J_0(z) .. J_n(z) in long double (Miller's backward recurrence, normalised by
+// the Jacobi-Anger identity), n >= 1.
+std::vector<std::complex<long double>> bessel_j(std::complex<long double> z, int n);
 
 }  // namespace fmm2d

@@ -18,7 +18,6 @@
 
 namespace fmm2d {
 
-namespace {
 using lc = std::complex<long double>;
 
 // J_0..J_n(z) for n >= 1
@@ -64,7 +63,6 @@ std::vector<lc> bessel_j(lc z, int n) {
   for (int i = 0; i <= n; ++i) out[i] = f[i] * sc;
   return out;
 }
-}  // namespace
 
 int h2dterms(double bsize, cplx zk, double eps, int* ier) {
   *ier = 0;

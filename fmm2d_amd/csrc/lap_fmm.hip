@@ -20,13 +20,12 @@
 //   near field one thread per evaluation point over its leaf's list 1
 // All kernels read and write the box expansions in place; there are no
 // atomics in the FMM itself (pull form, as the reference's loops).
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <complex>
 #include <vector>
 
 #include "devbuf.h"
+#include "fmm_common.h"
 #include "lap_ops.h"
 #include "p2p_math.h"
 #include "translate.h"
@@ -57,44 +56,6 @@ int l2dterms(double eps) {
 
 namespace {
 
-constexpr int kW = 64;  // wavefront size
-constexpr int kOff = 7;  // offsets -3..3 per axis
-
-FMM2D_HD cplx cpowi(cplx w, int k) {
-  cplx r = mkc(1.0), b = w;
-  while (k) {
-    if (k & 1) r = r * b;
-    b = b * b;
-    k >>= 1;
-  }
-  return r;
-}
-
-__device__ __forceinline__ int npts_eval(int b, const int* sse, const int* tse, int ifpgh,
-                                         int ifpght) {
-  int n = 0;
-  if (ifpgh > 0) n += sse[2 * b + 1] - sse[2 * b];
-  if (ifpght > 0) n += tse[2 * b + 1] - tse[2 * b];
-  return n;
-}
-
-// ---------------------------------------------------------- gather / scatter
-__global__ void k_gather_xy(int64_t n, const int* perm, const double* xy, double* out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int j = perm[i];
-  out[2 * i] = xy[2 * (int64_t)j];
-  out[2 * i + 1] = xy[2 * (int64_t)j + 1];
-}
-
-// cfmm2d: densities copied as they are.
-__global__ void k_gather_dens(int64_t n, int nd, const int* perm, const cplx* in, cplx* out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t j = perm[i];
-  for (int d = 0; d < nd; ++d) out[i * nd + d] = in[j * nd + d];
-}
-
 // lfmm2d split (laplace/lfmm2d.f:139-156): density d of the Laplace problem
 // becomes Cauchy densities 2d (real part) and 2d+1 (imaginary part); a dipole
 // (d, v) becomes -(v1 + i v2) times each part.
@@ -116,14 +77,6 @@ __global__ void k_gather_split(int64_t n, int nd, const int* perm, const cplx* c
       ods[i * 2 * nd + 2 * d + 1] = s.im * v;
     }
   }
-}
-
-// out[perm[i]] = in[i] for m values per point
-__global__ void k_scatter(int64_t n, int m, const int* perm, const cplx* in, cplx* out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t j = perm[i];
-  for (int k = 0; k < m; ++k) out[j * m + k] = in[i * m + k];
 }
 
 // lfmm2d unpack (laplace/lfmm2d.f:169-223).  in: Cauchy outputs of 2nd
@@ -148,12 +101,6 @@ __global__ void k_scatter_unpack(int64_t n, int nd, int pg, const int* perm, con
       hess[nd * (3 * j + 2) + d] = -hxx;
     }
   }
-}
-
-__global__ void k_leaf_of(int64_t n, const int* nchild, const int* se, int* leaf) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n || nchild[b] != 0) return;
-  for (int i = se[2 * b]; i < se[2 * b + 1]; ++i) leaf[i] = (int)b;
 }
 
 // ---------------------------------------------------------- P2M
@@ -258,107 +205,6 @@ __global__ __launch_bounds__(kW) void k_m2l(int nd, int P, const int* ilevel, co
       }
       loc[((int64_t)b * nd + d) * P + nn] += acc;
     }
-  }
-}
-
-// ---------------------------------------------------------- M2L on MFMA
-// The M2L of one target box is sum over its list-2 sources of T_o M_c, with o
-// the source's offset.  For a target in child position q of its parent the
-// possible offsets form a fixed set of 27 (the 6x6 children of the parent's
-// 3x3 neighbourhood minus the 3x3 adjacent ones), so targets of the same q are
-// processed together: a wavefront owns 16 (target, density) columns and, for
-// each of the 27 offset slots, multiplies the real 2P x 2P form of T_o
-// (interleaved re/im, held as pre-swizzled A fragments) into the gathered
-// source multipoles (B, zero where the slot is empty) with
-// v_mfma_f64_16x16x4f64, accumulating the local expansions in registers.
-constexpr int kSlots = 27;
-
-// slot of offset (dx,dy) for child position q, or -1 (adjacent / impossible)
-FMM2D_HD int m2l_slot(int q, int dx, int dy) {
-  const int bx = (q & 1) ? -3 : -2, by = (q & 2) ? -3 : -2;
-  const int ix = dx - bx, iy = dy - by;
-  if (ix < 0 || ix > 5 || iy < 0 || iy > 5) return -1;
-  if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) return -1;
-  int s = 0;
-  for (int a = 0; a < 6; ++a)
-    for (int b = 0; b < 6; ++b) {
-      const int ex = bx + a, ey = by + b;
-      if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1) continue;
-      if (a == ix && b == iy) return s;
-      ++s;
-    }
-  return -1;
-}
-
-// class key of every box: child position (0..3) of an M2L target, 4 otherwise
-__global__ void k_m2l_class(int64_t n, const int* ilevel, const int* iparent, const int* ichild,
-                            const int* n2, const int* sse, const int* tse, int ifpgh,
-                            int ifpght, int* key, int* val) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = (int)i;
-  int k = 4;
-  if (ilevel[b] >= 2 && n2[b] > 0 && npts_eval(b, sse, tse, ifpgh, ifpght) > 0) {
-    const int dad = iparent[b];
-    k = 0;
-    while (k < 3 && ichild[4 * dad + k] != b) ++k;
-  }
-  key[i] = k;
-  val[i] = b;
-}
-
-// slots[pos][s] = source box of slot s of target tlist[pos] (or -1)
-__global__ void k_m2l_slots(int64_t n, const int* tlist, const int* tkey, const double* centers,
-                            const double* rsc, const int* ilevel, const int* n2, const int* l2,
-                            int* slots) {
-  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= n) return;
-  const int b = tlist[pos], q = tkey[pos];
-  int* row = slots + pos * kSlots;
-  for (int s = 0; s < kSlots; ++s) row[s] = -1;
-  const double rinv = 1.0 / rsc[ilevel[b]];
-  const double cx = centers[2 * b], cy = centers[2 * b + 1];
-  for (int k = 0; k < n2[b]; ++k) {
-    const int c = l2[(int64_t)kMaxList2 * b + k];
-    const int ox = (int)rint((centers[2 * c] - cx) * rinv);
-    const int oy = (int)rint((centers[2 * c + 1] - cy) * rinv);
-    const int s = m2l_slot(q, ox, oy);
-    if (s >= 0) row[s] = c;
-  }
-}
-
-// Non-leaf boxes (in box order, hence level order) and their 4-child slots for
-// M2M (a child without sources is skipped, laprouts2d.f via cfmm2d.f:962-983)
-// and L2L (all children when the parent has points to evaluate, cfmm2d.f:1042-1076).
-__global__ void k_nonleaf_flag(int64_t n, const int* nchild, int* flag, int* iota) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  flag[i] = nchild[i] > 0 ? 1 : 0;
-  iota[i] = (int)i;
-}
-
-__global__ void k_child_slots(int64_t n, const int* list, const int* ichild, const int* sse,
-                              const int* tse, int ifpgh, int ifpght, int* up, int* down) {
-  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= n) return;
-  const int b = list[pos];
-  const bool ev = npts_eval(b, sse, tse, ifpgh, ifpght) > 0;
-  for (int qq = 0; qq < 4; ++qq) {
-    const int c = ichild[4 * b + qq];
-    up[pos * 4 + qq] = (sse[2 * c + 1] > sse[2 * c]) ? c : -1;
-    down[pos * 4 + qq] = ev ? c : -1;
-  }
-}
-
-__global__ void k_level_bounds(const int* list, int n, const int* lstart, int nl, int* out) {
-  for (int l = 0; l <= nl + 1; ++l) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-      const int mid = (lo + hi) / 2;
-      if (list[mid] < lstart[l]) lo = mid + 1;
-      else hi = mid;
-    }
-    out[l] = lo;
   }
 }
 
@@ -533,49 +379,6 @@ __global__ void k_near(int64_t n, int nd, int d0, const double* xy, const int* l
   }
 }
 
-// ---------------------------------------------------------- work counters
-// cnt[0] near pairs, [1] list-2 pairs, [2] M2M edges, [3] L2L edges,
-// [4] list-3 (point, box) pairs, [5] list-4 (source, box) pairs
-__global__ void k_counts(int64_t n, const int* ilevel, const int* iparent, const int* nchild,
-                         const int* ichild, const int* sse, const int* tse, int ifpgh,
-                         int ifpght, const int* n1, const int* l1, const int* n2, const int* n3,
-                         const int* n4, const int* l4, unsigned long long* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = (int)i;
-  const int lev = ilevel[b];
-  const int ne = npts_eval(b, sse, tse, ifpgh, ifpght);
-  unsigned long long near = 0, l3 = 0, l4c = 0, m2m = 0, l2l = 0, m2l = 0;
-  if (nchild[b] == 0) {
-    unsigned long long s = 0;
-    for (int k = 0; k < n1[b]; ++k) {
-      const int c = l1[(int64_t)kMaxList1 * b + k];
-      s += sse[2 * c + 1] - sse[2 * c];
-    }
-    near = s * (unsigned long long)ne;
-    l3 = (unsigned long long)n3[b] * ne;
-  } else if (lev >= 1) {
-    for (int q = 0; q < 4; ++q) {
-      const int c = ichild[4 * b + q];
-      if (sse[2 * c + 1] > sse[2 * c]) ++m2m;
-    }
-  }
-  if (lev >= 2 && ne > 0) {
-    m2l = n2[b];
-    for (int k = 0; k < n4[b]; ++k) {
-      const int c = l4[(int64_t)kMaxList4 * b + k];
-      l4c += sse[2 * c + 1] - sse[2 * c];
-    }
-    if (npts_eval(iparent[b], sse, tse, ifpgh, ifpght) > 0) l2l = 1;
-  }
-  if (near) atomicAdd(&cnt[0], near);
-  if (m2l) atomicAdd(&cnt[1], m2l);
-  if (m2m) atomicAdd(&cnt[2], m2m);
-  if (l2l) atomicAdd(&cnt[3], l2l);
-  if (l3) atomicAdd(&cnt[4], l3);
-  if (l4c) atomicAdd(&cnt[5], l4c);
-}
-
 // ---------------------------------------------------------- host side
 using lcplx = std::complex<long double>;
 
@@ -631,27 +434,6 @@ void make_matrices(int P, std::vector<cplx>& A, std::vector<cplx>& B, std::vecto
     }
 }
 
-__global__ void k_class_bounds(const int* keys, int n, int* seg) {
-  for (int q = 0; q <= 4; ++q) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-      const int mid = (lo + hi) / 2;
-      if (keys[mid] < q) lo = mid + 1;
-      else hi = mid;
-    }
-    seg[q] = lo;
-  }
-}
-
-template <typename T> DBuf<T> upload(const std::vector<T>& v, hipStream_t st) {
-  DBuf<T> d(v.size(), st);
-  if (!v.empty())
-    hip_throw(hipMemcpyAsync(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st),
-              "upload");
-  hip_throw(hipStreamSynchronize(st), "sync");  // v may die after return
-  return d;
-}
-
 template <int PG, int NDT>
 void launch_near_pg(int ifc, int ifd, int64_t n, hipStream_t st, int nd, int d0, const double* xy,
                     const int* leaf, const Lists& L, const int* sse, const double* xs,
@@ -697,37 +479,6 @@ void launch_eval(int pg, int64_t n, hipStream_t st, int nd, int P, const double*
     launch1d(k_eval<3>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
              L.l3.p, mp, loc, p, g, h);
 }
-
-struct Timer {
-  bool on;
-  hipStream_t st;
-  std::vector<hipEvent_t> ev;
-  std::vector<int> slot;
-  Timer(bool enable, hipStream_t s) : on(enable), st(s) { mark(-1); }
-  void mark(int s) {
-    if (!on) return;
-    hipEvent_t e;
-    (void)hipEventCreate(&e);
-    (void)hipEventRecord(e, st);
-    ev.push_back(e);
-    slot.push_back(s);
-  }
-  // after a stream sync: accumulate interval k (ending at event k) into slot
-  void collect(double* out) {
-    if (!on) return;
-    for (size_t k = 1; k < ev.size(); ++k) {
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, ev[k - 1], ev[k]);
-      if (slot[k] >= 0) out[slot[k]] += 1e-3 * ms;
-    }
-    float tot = 0.f;
-    (void)hipEventElapsedTime(&tot, ev.front(), ev.back());
-    out[11] = 1e-3 * tot;
-  }
-  ~Timer() {
-    for (auto e : ev) (void)hipEventDestroy(e);
-  }
-};
 
 }  // namespace
 
@@ -847,81 +598,35 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     std::vector<int2> band(nl + 1);
     for (int l = 0; l <= nl; ++l) band[l] = int2{0, 2 * (nterms[l] + 1)};
     DBuf<int2> d_band = upload(band, st);
-    // non-leaf boxes, level by level, with their child slots
-    DBuf<int> nl_list(nb, st), up_slots, down_slots;
-    std::vector<int> lev_pos(nl + 2, 0);
-    {
-      DBuf<int> flag(nb, st), iota(nb, st), cnt(1, st), dlb(nl + 2, st);
-      launch1d(k_nonleaf_flag, nb, st, t.nchild.p, flag.p, iota.p);
-      size_t bytes = 0;
-      hip_throw(hipcub::DeviceSelect::Flagged(nullptr, bytes, iota.p, flag.p, nl_list.p, cnt.p,
-                                              nb, st), "select");
-      DBuf<char> tmp(bytes, st);
-      hip_throw(hipcub::DeviceSelect::Flagged(tmp.p, bytes, iota.p, flag.p, nl_list.p, cnt.p, nb,
-                                              st), "select");
-      int nnl = 0;
-      hip_throw(hipMemcpyAsync(&nnl, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
-      hip_throw(hipStreamSynchronize(st), "sync");
-      std::vector<int> lsx(t.lstart);
-      lsx.push_back(nb);
-      DBuf<int> d_lsx = upload(lsx, st);
-      hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(1), 0, st, nl_list.p, nnl, d_lsx.p, nl,
-                         dlb.p);
-      hip_throw(hipMemcpyAsync(lev_pos.data(), dlb.p, sizeof(int) * (nl + 2),
-                               hipMemcpyDeviceToHost, st), "copy");
-      up_slots.alloc(4 * (size_t)std::max(nnl, 1), st);
-      down_slots.alloc(4 * (size_t)std::max(nnl, 1), st);
-      launch1d(k_child_slots, nnl, st, nl_list.p, t.ichild.p, sse, tse, pg, pgt, up_slots.p,
-               down_slots.p);
-      hip_throw(hipStreamSynchronize(st), "sync");
-    }
+    ChildPlan cp;
+    plan_children(t, pg, pgt, cp, st);
     DBuf<int> d_op4 = upload(std::vector<int>{0, 1, 2, 3}, st);
     // ---- step 3: M2M, finest parents first
     for (int l = nl - 1; l >= 1; --l)
-      launch_translate(0, 0, nd, P, 4, {TrGroup{lev_pos[l], lev_pos[l + 1], opM2M, 0}},
-                       nl_list.p, up_slots.p, d_op4.p, d_af.p, t.ilevel.p, d_band.p, d_logr.p,
-                       mp.p, mp.p, st);
+      launch_translate(0, 0, nd, TrBufs{mp.p, 0, P, mp.p, 0, P}, 4,
+                       {TrGroup{cp.lev_pos[l], cp.lev_pos[l + 1], opM2M, 0}}, cp.list.p, cp.up.p,
+                       d_op4.p, d_af.p, t.ilevel.p, d_band.p, d_logr.p, st);
     tm.mark(2);
     // ---- step 4: M2L, targets grouped by child position (box order within a group)
     {
-      DBuf<int> key(nb, st), val(nb, st), skey(nb, st), tlist(nb, st), seg(5, st);
-      launch1d(k_m2l_class, nb, st, t.ilevel.p, t.iparent.p, t.ichild.p, L.n2.p, sse, tse, pg,
-               pgt, key.p, val.p);
-      size_t bytes = 0;
-      hip_throw(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, skey.p, val.p,
-                                                   tlist.p, nb, 0, 3, st), "sort");
-      DBuf<char> tmp(bytes, st);
-      hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, skey.p, val.p, tlist.p,
-                                                   nb, 0, 3, st), "sort");
-      hipLaunchKernelGGL(k_class_bounds, dim3(1), dim3(1), 0, st, skey.p, nb, seg.p);
-      int hseg[5];
-      hip_throw(hipMemcpyAsync(hseg, seg.p, sizeof(hseg), hipMemcpyDeviceToHost, st), "copy");
-      hip_throw(hipStreamSynchronize(st), "sync");
-      const int ntg = hseg[4];
-      if (ntg > 0) {
-        DBuf<int> slots((size_t)ntg * kSlots, st);
-        launch1d(k_m2l_slots, ntg, st, tlist.p, skey.p, t.centers.p, d_rsc.p, t.ilevel.p, L.n2.p,
-                 L.l2.p, slots.p);
-        std::vector<int> soff(4 * kSlots, 0);
-        for (int q = 0; q < 4; ++q)
-          for (int dx = -3; dx <= 3; ++dx)
-            for (int dy = -3; dy <= 3; ++dy) {
-              const int sl = m2l_slot(q, dx, dy);
-              if (sl >= 0) soff[q * kSlots + sl] = (dx + 3) * kOff + (dy + 3);
-            }
-        DBuf<int> d_soff = upload(soff, st);
+      M2LPlan m2l;
+      plan_m2l(t, L, pg, pgt, m2l, st);
+      if (m2l.ntg > 0) {
+        DBuf<int> d_soff = upload(m2l_slot_offsets(), st);
         std::vector<TrGroup> groups;
-        for (int q = 0; q < 4; ++q) groups.push_back(TrGroup{hseg[q], hseg[q + 1], 0, q});
-        launch_translate(0, 1, nd, P, kSlots, groups, tlist.p, slots.p, d_soff.p, d_af.p,
-                         t.ilevel.p, d_band.p, d_logr.p, mp.p, loc.p, st);
+        for (int q = 0; q < 4; ++q)
+          groups.push_back(TrGroup{m2l.cls_seg[q], m2l.cls_seg[q + 1], 0, q});
+        launch_translate(0, 1, nd, TrBufs{mp.p, 0, P, loc.p, 0, P}, kSlots, groups,
+                         m2l.tlist.p, m2l.slots.p, d_soff.p, d_af.p, t.ilevel.p, d_band.p,
+                         d_logr.p, st);
       }
     }
     tm.mark(3);
     // ---- step 5: L2L, coarsest first
     for (int l = 1; l <= nl - 1; ++l)
-      launch_translate(1, 0, nd, P, 4, {TrGroup{lev_pos[l], lev_pos[l + 1], opL2L, 0}},
-                       nl_list.p, down_slots.p, d_op4.p, d_af.p, t.ilevel.p, d_band.p, d_logr.p,
-                       loc.p, loc.p, st);
+      launch_translate(1, 0, nd, TrBufs{loc.p, 0, P, loc.p, 0, P}, 4,
+                       {TrGroup{cp.lev_pos[l], cp.lev_pos[l + 1], opL2L, 0}}, cp.list.p,
+                       cp.down.p, d_op4.p, d_af.p, t.ilevel.p, d_band.p, d_logr.p, st);
     tm.mark(4);
     // ---- steps 6+7: list 3 and L2P
     if (pg > 0)
@@ -965,7 +670,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       DBuf<unsigned long long> dc(6, st);
       dc.zero();
       launch1d(k_counts, nb, st, t.ilevel.p, t.iparent.p, t.nchild.p, t.ichild.p, sse, tse, pg,
-               pgt, L.n1.p, L.l1.p, L.n2.p, L.n3.p, L.n4.p, L.l4.p, dc.p);
+               pgt, L.n1.p, L.l1.p, L.n2.p, L.n3.p, L.n4.p, L.l4.p, 0, dc.p);
       hip_throw(hipMemcpyAsync(hc, dc.p, sizeof(hc), hipMemcpyDeviceToHost, st), "copy");
     }
     hip_throw(hipStreamSynchronize(st), "sync");

@@ -1,8 +1,8 @@
 // Expansion-to-expansion translations (M2M, M2L, L2L) on MFMA, shared by the
 // Laplace/Cauchy and Helmholtz drivers.
 //
-// A translation is y = T x with T a complex P x P operator and x, y box
-// expansions.  In real form T becomes a 2P x 2P matrix over interleaved
+// A translation is y = T x with T a complex Pout x Pin operator and x, y box
+// expansions.  In real form T becomes a 2Pout x 2Pin matrix over interleaved
 // (re, im) coefficients; operators are pre-swizzled into the A-operand layout
 // of v_mfma_f64_16x16x4f64 (lane l holds row l%16, column l/16 of a 16x4
 // block; the result tile holds row l/16 + 4 i, column l%16 in element i).
@@ -12,9 +12,12 @@
 // slot -> operator map (opbase[g] + slot_op[cls[g]*nslots + s]), so the A
 // operand is uniform across a wavefront, whose 16 columns are (target,
 // density) pairs.  MODE 0 gathers: dst[t] += sum_s T_s src[slot(t,s)]; MODE 1
-// scatters: dst[slot(t,s)] += T_s src[t].  band[level] = [lo, hi) are the
-// interleaved coefficient rows a level uses (the rest stay zero).  With LOGT
-// the Laplace M2L term log(boxsize) M_0 is added to L_0.
+// scatters: dst[slot(t,s)] += T_s src[t].  Box b's expansion for density d
+// starts at ((b - box0) nd + d) P of its buffer (box0 = first box of the
+// buffer's level, 0 for a buffer holding all levels).  With a band table,
+// band[level] = [lo, hi) are the interleaved coefficient rows that level uses
+// (the rest stay zero); without one every row is used.  With LOGT the Laplace
+// M2L term log(boxsize) M_0 is added to L_0.
 #pragma once
 #include <vector>
 
@@ -41,10 +44,9 @@ struct TrTable {
 // loop and the row tiles are covered in chunks of RT.
 template <int RT, int KSU, int MODE, int LOGT>
 __global__ __launch_bounds__(256) void k_translate(
-    int nd, int P, int nslots, int64_t nwaves, TrTable tab, const int* tlist, const int* slots,
-    const int* slot_op,
-    const double* Afrag, int64_t afs, const int* ilevel, const int2* band, const double* logr,
-    const cplx* src, cplx* dst) {
+    int nd, int Pin, int Pout, int nslots, int64_t nwaves, TrTable tab, const int* tlist,
+    const int* slots, const int* slot_op, const double* Afrag, int64_t afs, int sbox0, int dbox0,
+    const int* ilevel, const int2* band, const double* logr, const cplx* src, cplx* dst) {
   const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (wave >= nwaves) return;
   const int lane = threadIdx.x & 63;
@@ -57,7 +59,8 @@ __global__ __launch_bounds__(256) void k_translate(
   const bool valid = tpos < tab.seg1[g];
   const int tb = valid ? tlist[tpos] : -1;
   const int kr = lane >> 4;
-  const int K2 = 2 * P, RTT = (K2 + 15) / 16, KS = (K2 + 3) / 4;
+  const int RTT = (2 * Pout + 15) / 16, KS = (2 * Pin + 3) / 4;
+  const int2 full_in = int2{0, 2 * Pin}, full_out = int2{0, 2 * Pout};
   const int ob = tab.opbase[g];
   for (int rc = 0; rc < RTT; rc += RT) {
     if (MODE == 0) {
@@ -71,8 +74,8 @@ __global__ __launch_bounds__(256) void k_translate(
         const double* M = nullptr;
         int klo = 0, khi = 0;
         if (c >= 0) {
-          M = reinterpret_cast<const double*>(src + ((int64_t)c * nd + d) * P);
-          const int2 bd = band[ilevel[c]];
+          M = reinterpret_cast<const double*>(src + ((int64_t)(c - sbox0) * nd + d) * Pin);
+          const int2 bd = band ? band[ilevel[c]] : full_in;
           klo = bd.x;
           khi = bd.y;
         }
@@ -96,8 +99,8 @@ __global__ __launch_bounds__(256) void k_translate(
       }
       if (valid) {
         if (LOGT && rc == 0 && lane < 32) acc[0][0] += logr[ilevel[tb]] * m0;
-        const int2 bd = band[ilevel[tb]];
-        double* L = reinterpret_cast<double*>(dst + ((int64_t)tb * nd + d) * P);
+        const int2 bd = band ? band[ilevel[tb]] : full_out;
+        double* L = reinterpret_cast<double*>(dst + ((int64_t)(tb - dbox0) * nd + d) * Pout);
 #pragma unroll
         for (int r = 0; r < RT; ++r)
 #pragma unroll
@@ -107,11 +110,12 @@ __global__ __launch_bounds__(256) void k_translate(
           }
       }
     } else {
-      const double* M = valid ? reinterpret_cast<const double*>(src + ((int64_t)tb * nd + d) * P)
-                              : nullptr;
+      const double* M =
+          valid ? reinterpret_cast<const double*>(src + ((int64_t)(tb - sbox0) * nd + d) * Pin)
+                : nullptr;
       int klo = 0, khi = 0;
       if (valid) {
-        const int2 bd = band[ilevel[tb]];
+        const int2 bd = band ? band[ilevel[tb]] : full_in;
         klo = bd.x;
         khi = bd.y;
       }
@@ -149,8 +153,8 @@ __global__ __launch_bounds__(256) void k_translate(
           }
         }
         if (c >= 0) {
-          const int2 bd = band[ilevel[c]];
-          double* L = reinterpret_cast<double*>(dst + ((int64_t)c * nd + d) * P);
+          const int2 bd = band ? band[ilevel[c]] : full_out;
+          double* L = reinterpret_cast<double*>(dst + ((int64_t)(c - dbox0) * nd + d) * Pout);
 #pragma unroll
           for (int r = 0; r < RT; ++r)
 #pragma unroll
@@ -164,8 +168,8 @@ __global__ __launch_bounds__(256) void k_translate(
   }
 }
 
-// Real interleaved form of complex operators ops[o][j][n] (input j, output n),
-// swizzled as Afrag[o][row tile][k step][lane].
+// Real interleaved form of square complex operators ops[o][j][n] (input j,
+// output n), swizzled as Afrag[o][row tile][k step][lane].
 inline std::vector<double> swizzle_ops(int P, const std::vector<cplx>& ops, int nops) {
   const int K2 = 2 * P, RTT = (K2 + 15) / 16, KS = (K2 + 3) / 4;
   std::vector<double> out((size_t)nops * RTT * KS * 64, 0.0);
@@ -185,10 +189,11 @@ inline std::vector<double> swizzle_ops(int P, const std::vector<cplx>& ops, int 
   return out;
 }
 
-inline int64_t afrag_stride(int P) {
-  const int K2 = 2 * P;
-  return (int64_t)((K2 + 15) / 16) * ((K2 + 3) / 4) * 64;
+// doubles per swizzled operator of Pout x Pin complex entries
+inline int64_t afrag_stride(int Pin, int Pout) {
+  return (int64_t)((2 * Pout + 15) / 16) * ((2 * Pin + 3) / 4) * 64;
 }
+inline int64_t afrag_stride(int P) { return afrag_stride(P, P); }
 
 struct TrGroup {
   int seg0, seg1;  // targets tlist[seg0..seg1)
@@ -197,13 +202,21 @@ struct TrGroup {
 
 // Launch translate passes over the given groups (at most kTrMaxGroups per
 // launch; launches are stream-ordered, so groups must not depend on each other).
-inline void launch_translate(int mode, int logt, int nd, int P, int nslots,
+struct TrBufs {
+  const cplx* src;
+  int sbox0, Pin;
+  cplx* dst;
+  int dbox0, Pout;
+};
+
+inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int nslots,
                              const std::vector<TrGroup>& groups, const int* tlist,
                              const int* slots, const int* slot_op, const double* Afrag,
                              const int* ilevel, const int2* band, const double* logr,
-                             const cplx* src, cplx* dst, hipStream_t st) {
-  const int64_t afs = afrag_stride(P);
-  const int RTT = (2 * P + 15) / 16;
+                             hipStream_t st) {
+  const int Pin = io.Pin, Pout = io.Pout;
+  const int64_t afs = afrag_stride(Pin, Pout);
+  const int RTT = (2 * Pout + 15) / 16, KS = (2 * Pin + 3) / 4;
   size_t gi = 0;
   while (gi < groups.size()) {
     TrTable tab{};
@@ -223,11 +236,11 @@ inline void launch_translate(int mode, int logt, int nd, int P, int nslots,
     tab.tile0[tab.ng] = tiles;
     const dim3 grid((unsigned)((tiles + 3) / 4));
 #define FMM2D_TR(RT, KSU, MODE, LOGT)                                                           \
-  hipLaunchKernelGGL((k_translate<RT, KSU, MODE, LOGT>), grid, dim3(256), 0, st, nd, P, nslots, \
-                     (int64_t)tiles, tab, tlist, slots, slot_op, Afrag, afs, ilevel, band, logr, \
-                     src, dst)
+  hipLaunchKernelGGL((k_translate<RT, KSU, MODE, LOGT>), grid, dim3(256), 0, st, nd, Pin, Pout, \
+                     nslots, (int64_t)tiles, tab, tlist, slots, slot_op, Afrag, afs, io.sbox0,   \
+                     io.dbox0, ilevel, band, logr, io.src, io.dst)
 #define FMM2D_TR_RT(MODE, LOGT)                 \
-  switch (RTT) {                                \
+  switch (KS <= 4 * RTT ? RTT : 0) {            \
     case 1: FMM2D_TR(1, 4, MODE, LOGT); break;  \
     case 2: FMM2D_TR(2, 8, MODE, LOGT); break;  \
     case 3: FMM2D_TR(3, 12, MODE, LOGT); break; \

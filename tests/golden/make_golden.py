@@ -163,8 +163,39 @@ def fmm_fixtures():
     np.savez_compressed(os.path.join(HERE, "fmm_ref.npz"), **out)
 
 
+# (zk, eps, nd, ifc, ifd, pg, pgt): hessians; complex zk; Im(zk) large enough
+# that the coarse levels drop out (hfmm2d.f:711); zk = 600, where levels 2 and
+# 3 are in the reference's high-frequency (FFT diagonal form) regime.
+HFMM_CASES = [
+    (12.0 + 0.0j, 1e-6, 1, 1, 1, 3, 3),
+    (12.0 + 0.0j, 1e-10, 1, 1, 1, 3, 3),
+    (12.0 + 3.0j, 1e-9, 1, 1, 0, 2, 2),
+    (12.0 + 3.0j, 1e-9, 1, 0, 1, 3, 3),
+    (5.0 + 100.0j, 1e-9, 1, 1, 1, 2, 2),
+    (600.0 + 0.0j, 1e-6, 1, 1, 0, 2, 2),
+    (600.0 + 0.0j, 1e-9, 2, 1, 1, 3, 3),
+]
+
+
+def hfmm_fixtures():
+    """hfmm2d outputs of the reference beyond fmm_ref.npz (same points)."""
+    base = np.load(os.path.join(HERE, "fmm_ref.npz"))
+    src, trg = base["src"], base["trg"]
+    out = {"src": src, "trg": trg}
+    for i, (zk, eps, nd, ifc, ifd, pg, pgt) in enumerate(HFMM_CASES):
+        ch, ds, dv = base[f"nd{nd}_charge"], base[f"nd{nd}_dipstr"], base[f"nd{nd}_dipvec"]
+        r = R.hfmm2d(zk=zk, nd=nd, eps=eps, sources=src, charges=ch if ifc else None,
+                     dipstr=ds if ifd else None, dipvec=dv if ifd else None,
+                     targets=trg if pgt else None, ifpgh=pg, ifpghtarg=pgt)
+        for k, v in r.items():
+            if k in ("pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"):
+                out[f"c{i}_{k}"] = v
+        out[f"c{i}_ier"] = np.array(r["ier"])
+    np.savez_compressed(os.path.join(HERE, "hfmm_ref.npz"), **out)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm"]
+    which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm", "hfmm"]
     if "direct" in which:
         direct_fixtures()
     if "hankel" in which:
@@ -173,5 +204,7 @@ if __name__ == "__main__":
         tree_fixtures()
     if "fmm" in which:
         fmm_fixtures()
+    if "hfmm" in which:
+        hfmm_fixtures()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -78,3 +78,90 @@ def _as_src(ex):
     o.grad = ex.gradtarg
     o.hess = ex.hesstarg
     return o
+
+
+CASES_H = [(1, c) for c in [(1, 0, 1, 0), (1, 0, 2, 2), (0, 1, 0, 2)]] + [(2, (1, 1, 2, 2))]
+
+
+@pytest.mark.parametrize("eps", [1e-6, 1e-10])
+@pytest.mark.parametrize("nd,case", CASES_H)
+def test_helmholtz_matches_reference(golden_dir, eps, nd, case):
+    """hfmm2d at zk = 12 against the reference's own FMM output."""
+    g = np.load(f"{golden_dir}/fmm_ref.npz")
+    ifc, ifd, pg, pgt = case
+    src, trg = g["src"], g["trg"]
+    ch, ds, dv = g[f"nd{nd}_charge"], g[f"nd{nd}_dipstr"], g[f"nd{nd}_dipvec"]
+    res, ier = F._fmm("h", eps, 12.0 + 0.0j, src, ch if ifc else None, ds if ifd else None,
+                      dv if ifd else None, trg if pgt else None, pg, pgt, nd)
+    assert ier == 0
+    tag = f"nd{nd}_h_{eps:.0e}_{ifc}{ifd}{pg}{pgt}_"
+    for k, v in res.items():
+        ref = g[tag + k]
+        if nd == 1:
+            ref = ref[0]
+        e = _rel(v, ref)
+        assert e < max(1e-3 * eps, 1e-13), (k, e)
+
+
+# the cases of tests/golden/make_golden.py:HFMM_CASES
+HFMM_CASES = [
+    (12.0 + 0.0j, 1e-6, 1, 1, 1, 3, 3),
+    (12.0 + 0.0j, 1e-10, 1, 1, 1, 3, 3),
+    (12.0 + 3.0j, 1e-9, 1, 1, 0, 2, 2),
+    (12.0 + 3.0j, 1e-9, 1, 0, 1, 3, 3),
+    (5.0 + 100.0j, 1e-9, 1, 1, 1, 2, 2),
+    (600.0 + 0.0j, 1e-6, 1, 1, 0, 2, 2),
+    (600.0 + 0.0j, 1e-9, 2, 1, 1, 3, 3),
+]
+
+
+@pytest.mark.parametrize("ci", range(len(HFMM_CASES)))
+def test_helmholtz_cases_match_reference(golden_dir, ci):
+    """Hessians, complex zk, the Im(zk) level cut-off and the high-frequency
+    regime (zk = 600: the reference's FFT diagonal forms at levels 2-3, dense
+    Graf operators here) against the reference's FMM output."""
+    g = np.load(f"{golden_dir}/hfmm_ref.npz")
+    base = np.load(f"{golden_dir}/fmm_ref.npz")
+    zk, eps, nd, ifc, ifd, pg, pgt = HFMM_CASES[ci]
+    src, trg = g["src"], g["trg"]
+    ch, ds, dv = base[f"nd{nd}_charge"], base[f"nd{nd}_dipstr"], base[f"nd{nd}_dipvec"]
+    res, ier = F._fmm("h", eps, zk, src, ch if ifc else None, ds if ifd else None,
+                      dv if ifd else None, trg if pgt else None, pg, pgt, nd)
+    assert ier == int(g[f"c{ci}_ier"]) == 0
+    for k, v in res.items():
+        ref = g[f"c{ci}_{k}"]
+        if nd == 1:
+            ref = ref[0]
+        e = _rel(v, ref)
+        assert e < max(1e-3 * eps, 1e-13), (k, e)
+
+
+def test_hfmm2d_public_api_vs_direct():
+    """fmm2dpy.hfmm2d (pg, pgt <= 2) against h2ddir, and the hessians of the
+    native driver against the direct kernel."""
+    rng = np.random.default_rng(5)
+    n, nt = 20000, 3000
+    zk = 30.0 + 1.0j
+    src = rng.random((2, n))
+    src[:, : n // 4] = 0.3 + 0.02 * rng.standard_normal((2, n // 4))
+    trg = rng.random((2, nt))
+    ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dv = rng.standard_normal((2, n))
+    ds = rng.standard_normal(n) + 0j
+    out = F.hfmm2d(eps=1e-8, zk=zk, sources=src, charges=ch, dipstr=ds, dipvec=dv,
+                   targets=trg, pg=2, pgt=2)
+    assert out.pot.shape == (n,) and out.grad.shape == (2, n)
+    ex = F.h2ddir(zk=zk, sources=src, targets=trg, charges=ch, dipstr=ds, dipvec=dv, pgt=2)
+    assert _rel(out.pottarg, ex.pottarg) < 1e-8
+    assert _rel(out.gradtarg, ex.gradtarg) < 1e-8
+    m = 300
+    ex2 = F.h2ddir(zk=zk, sources=src, targets=src[:, :m], charges=ch, dipstr=ds,
+                   dipvec=dv, pgt=2)
+    err = F.comperr(ntest=m, out=out, outex=_as_src(ex2), pg=2)
+    assert err < 1e-8, err
+    res, ier = F._fmm("h", 1e-8, zk, src, ch[None], ds[None], dv[None], trg, 0, 3, 1)
+    assert ier == 0
+    (p, g, h), _ = F._direct("h", zk, src, trg, ch[None], ds[None], dv[None], 3, 1,
+                             abs(zk) * 2.0 ** -51)
+    assert _rel(res["pottarg"], p[0]) < 1e-8
+    assert _rel(res["hesstarg"], h[0]) < 1e-8
