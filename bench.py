@@ -28,8 +28,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
-STAGES = ["p2m", "list4", "m2m", "m2l", "l2l", "list3", "l2p+list3", "near", "tree+lists",
-          "sort+gather", "scatter"]
+# timeinfo slots of fmm2d_opts (include/fmm2d_amd.h)
+STAGES = {0: "p2m", 1: "list4", 2: "m2m", 3: "m2l plan", 13: "m2l kernel", 4: "l2l",
+          6: "l2p+list3", 7: "near", 8: "tree+lists", 9: "sort+gather", 10: "scatter"}
+M2L_SLOT = 13
+# HBM bytes per M2L launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+# scaled by the counters' reading for a copy kernel of known size
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "m2l_lfmm2d_n1e7_traffic.txt")
+
+
+def read_traffic(precision):
+    """bytes_per_launch recorded for this precision in TRAFFIC_FILE, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                if key.strip() == f"bytes_per_launch_{precision}":
+                    return int(float(val.strip()))
+    except OSError:
+        pass
+    return None
 
 
 def c3_points(n, seed=3):
@@ -125,7 +143,7 @@ def main():
     work = np.array(opts.work[:])
     print(f"bench: {dt*1e3:.1f} ms/step, levels {opts.nlevels}, boxes {opts.nboxes}, "
           f"near pairs {opts.near_pairs}, m2l pairs {opts.m2l_pairs}", file=sys.stderr, flush=True)
-    for k, name in enumerate(STAGES):
+    for k, name in STAGES.items():
         print(f"bench:   {name:12s} {stage[k]*1e3:9.3f} ms", file=sys.stderr)
 
     # ---- accuracy: FP64 direct sum at sampled sources
@@ -146,13 +164,16 @@ def main():
     relerr = float(torch.sqrt(num / den))
     print(f"bench: rel L2 error (pot+grad, {m} sampled sources) {relerr:.3e}", file=sys.stderr)
 
-    # ---- roofline of the dominant stage (algorithmic flops / event time)
-    k_dom = int(np.argmax(stage[:8]))
-    achieved = work[k_dom] / stage[k_dom] / 1e12 if stage[k_dom] > 0 else 0.0
+    # ---- roofline of the M2L translate kernel: algorithmic flops of the
+    # list-2 translations (8 (p+1)^2 nd per pair) over the kernel's own time,
+    # from HIP events recorded around its launch on the call's stream
+    t_m2l = stage[M2L_SLOT]
+    achieved = work[3] / t_m2l / 1e12 if t_m2l > 0 else 0.0
     peak = PEAK["fp64_tflops"] if args.precision == "fp64" else PEAK["fp32_tflops"]
-    roofline = {"bound": "mfma", "kernel": STAGES[k_dom], "achieved": round(achieved, 3),
+    roofline = {"bound": "mfma", "kernel": "k_translate (M2L)", "achieved": round(achieved, 3),
                 "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-                "traffic": None}
+                "traffic": read_traffic(args.precision),
+                "launch_ms": round(t_m2l * 1e3, 4), "flops_per_launch": work[3]}
 
     cpu = None
     if not args.no_cpu:
@@ -187,7 +208,7 @@ def main():
                    "n": n, "eps": 1e-6, "pg": 2, "precision": args.precision,
                    "nlevels": int(opts.nlevels), "nboxes": int(opts.nboxes)},
         "rel_l2_err": relerr,
-        "stage_ms": {name: round(stage[k] * 1e3, 4) for k, name in enumerate(STAGES)},
+        "stage_ms": {name: round(stage[k] * 1e3, 4) for k, name in STAGES.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -893,6 +893,7 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
       plan_m2l(t, L, pg, pgt, m2l, st);
       if (m2l.ntg > 0) {
         DBuf<int> d_soff = upload(m2l_slot_offsets(), st);
+        tm.mark(3);  // planning
         for (int l = 2; l <= nl; ++l) {
           if (op_m2l[l] < 0) continue;
           std::vector<TrGroup> groups;
@@ -905,6 +906,7 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
                            kSlots, groups, m2l.tlist.p, m2l.slots.p, d_soff.p, opptr(op_m2l[l]),
                            t.ilevel.p, nullptr, nullptr, st);
         }
+        tm.mark(13);  // the M2L kernels alone
       }
     }
     tm.mark(3);

@@ -616,9 +616,11 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
         std::vector<TrGroup> groups;
         for (int q = 0; q < 4; ++q)
           groups.push_back(TrGroup{m2l.cls_seg[q], m2l.cls_seg[q + 1], 0, q});
+        tm.mark(3);  // planning
         launch_translate(0, 1, nd, TrBufs{mp.p, 0, P, loc.p, 0, P}, kSlots, groups,
                          m2l.tlist.p, m2l.slots.p, d_soff.p, d_af.p, t.ilevel.p, d_band.p,
                          d_logr.p, st);
+        tm.mark(13);  // the M2L kernel alone
       }
     }
     tm.mark(3);

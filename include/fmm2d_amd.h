@@ -54,7 +54,9 @@ typedef struct {
   /* out: seconds per stage.  [0..7] are the reference's timeinfo(1..8): P2M,
    * list-4 P2L, M2M, M2L, L2L, list-3 M2P, L2P, near field (list-3 and L2P
    * run fused and are reported together in [6]); [8] tree + lists, [9] sort
-   * and gather, [10] scatter, [11] whole call. */
+   * and gather, [10] scatter, [11] whole call, [12] translation operators
+   * (Helmholtz), [13] the M2L translate kernels alone ([3] then holds the
+   * M2L planning: target classes and source slots). */
   double timeinfo[16];
   int32_t nlevels, nboxes; /* out: tree depth and box count */
   int32_t pad1, pad2;
