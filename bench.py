@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nerr", type=int, default=10_000)
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first run one elementwise copy kernel over MIB MiB (reads and "
+                         "writes MIB MiB), the reference kernel for scaling HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -129,6 +132,12 @@ def main():
         if ier != 0:
             raise RuntimeError(f"fmm2d_lfmm2d returned {ier}")
 
+    if args.copy_calib > 0:
+        a = torch.ones(args.copy_calib * (1 << 20) // 8, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         step(opts)
     torch.cuda.synchronize()

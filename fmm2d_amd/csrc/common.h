@@ -83,3 +83,15 @@ enum {
   } while (0)
 
 static inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+#ifdef __HIPCC__
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  With gridDim.x a multiple of 8 this maps block b to logical block
+// (b % 8) * (gridDim.x / 8) + b / 8, so each XCD walks one contiguous range
+// of the work and neighbouring items share its L2.  Grids of fewer than 8
+// blocks keep their order.
+__device__ __forceinline__ int64_t xcd_block() {
+  const unsigned g8 = gridDim.x >> 3;
+  return g8 ? (int64_t)(blockIdx.x & 7) * g8 + (blockIdx.x >> 3) : (int64_t)blockIdx.x;
+}
+#endif

@@ -65,4 +65,18 @@ inline void launch1d(F kernel, int64_t n, hipStream_t st, A... args) {
   hip_throw(hipGetLastError(), "kernel launch");
 }
 
+// Grid size rounded up to a multiple of the 8 XCDs, for kernels that index
+// their work with xcd_block().
+inline unsigned xcd_grid(int64_t nblocks) {
+  return nblocks < 8 ? (unsigned)nblocks : (unsigned)(((nblocks + 7) / 8) * 8);
+}
+
+// launch1d for kernels whose thread index is xcd_block() * 256 + threadIdx.x
+template <typename F, typename... A>
+inline void launch1d_xcd(F kernel, int64_t n, hipStream_t st, A... args) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(kernel, dim3(xcd_grid(ceil_div(n, 256))), dim3(256), 0, st, n, args...);
+  hip_throw(hipGetLastError(), "kernel launch");
+}
+
 }  // namespace fmm2d

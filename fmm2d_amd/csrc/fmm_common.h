@@ -177,14 +177,13 @@ __global__ void k_level_bounds(const int* list, int n, const int* lstart, int nl
 __global__ void k_counts(int64_t n, const int* ilevel, const int* iparent, const int* nchild,
                          const int* ichild, const int* sse, const int* tse, int ifpgh,
                          int ifpght, const int* n1, const int* l1, const int* n2, const int* n3,
-                         const int* n4, const int* l4, int per_level,
-                         unsigned long long* cnt_all) {
+                         const int* n4, const int* l4, int b0, unsigned long long* cnt) {
+  // boxes b0 .. b0+n-1; cnt is the same for every lane, so the compiler
+  // combines each wavefront's atomics into one
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int b = (int)i;
+  const int b = b0 + (int)i;
   const int lev = ilevel[b];
-  // per_level: counters of box b's level at cnt_all[6 lev ..]
-  unsigned long long* cnt = cnt_all + (per_level ? 6 * lev : 0);
   const int ne = npts_eval(b, sse, tse, ifpgh, ifpght);
   unsigned long long near = 0, l3 = 0, l4c = 0, m2m = 0, l2l = 0, m2l = 0;
   if (nchild[b] == 0) {

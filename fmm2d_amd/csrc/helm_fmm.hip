@@ -414,7 +414,7 @@ __global__ void k_heval(int64_t n, int nd, cplx zk, const HLev* lev, const doubl
                         const int* leaf, const int* ilevel, const double* centers, const int* n3,
                         const int* l3, const cplx* mp, const cplx* loc, cplx* pot, cplx* grad,
                         cplx* hess) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = leaf[i];
   const double x = xy[2 * i], y = xy[2 * i + 1];
@@ -463,7 +463,7 @@ __global__ void k_hnear(int64_t n, int nd, int d0, cplx zk, const double* xy, co
                         const int* n1, const int* l1, const int* sse, const double* xs,
                         const cplx* chs, const cplx* dps, const double* dvs, double thresh,
                         cplx* pot, cplx* grad, cplx* hess) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = leaf[i];
   const double x = xy[2 * i], y = xy[2 * i + 1];
@@ -611,7 +611,7 @@ template <int PG>
 void launch_heval(int64_t n, hipStream_t st, int nd, cplx zk, const HLev* lev, const double* xy,
                   const int* leaf, const Tree& t, const Lists& L, const cplx* mp, const cplx* loc,
                   cplx* p, cplx* g, cplx* h) {
-  launch1d(k_heval<PG>, n, st, nd, zk, lev, xy, leaf, t.ilevel.p, t.centers.p, L.n3.p, L.l3.p, mp,
+  launch1d_xcd(k_heval<PG>, n, st, nd, zk, lev, xy, leaf, t.ilevel.p, t.centers.p, L.n3.p, L.l3.p, mp,
            loc, p, g, h);
 }
 
@@ -621,13 +621,13 @@ void launch_hnear_f(int pg, int64_t n, hipStream_t st, int nd, int d0, cplx zk, 
                     const cplx* chs, const cplx* dps, const double* dvs, double thresh, cplx* p,
                     cplx* g, cplx* h) {
   if (pg == 1)
-    launch1d(k_hnear<IFC, IFD, 1, NDT>, n, st, nd, d0, zk, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs,
+    launch1d_xcd(k_hnear<IFC, IFD, 1, NDT>, n, st, nd, d0, zk, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs,
              dps, dvs, thresh, p, g, h);
   else if (pg == 2)
-    launch1d(k_hnear<IFC, IFD, 2, NDT>, n, st, nd, d0, zk, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs,
+    launch1d_xcd(k_hnear<IFC, IFD, 2, NDT>, n, st, nd, d0, zk, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs,
              dps, dvs, thresh, p, g, h);
   else
-    launch1d(k_hnear<IFC, IFD, 3, NDT>, n, st, nd, d0, zk, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs,
+    launch1d_xcd(k_hnear<IFC, IFD, 3, NDT>, n, st, nd, d0, zk, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs,
              dps, dvs, thresh, p, g, h);
 }
 
@@ -956,8 +956,13 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     if (o) {
       DBuf<unsigned long long> dc(hc.size(), st);
       dc.zero();
-      launch1d(k_counts, nb, st, t.ilevel.p, t.iparent.p, t.nchild.p, t.ichild.p, sse, tse, pg,
-               pgt, L.n1.p, L.l1.p, L.n2.p, L.n3.p, L.n4.p, L.l4.p, 1, dc.p);
+      for (int l = 0; l <= nl; ++l) {
+        const int b0 = t.lstart[l], b1 = (l < nl) ? t.lstart[l + 1] : nb;
+        if (b1 > b0)
+          launch1d(k_counts, b1 - b0, st, t.ilevel.p, t.iparent.p, t.nchild.p, t.ichild.p, sse,
+                   tse, pg, pgt, L.n1.p, L.l1.p, L.n2.p, L.n3.p, L.n4.p, L.l4.p, b0,
+                   dc.p + 6 * l);
+      }
       hip_throw(hipMemcpyAsync(hc.data(), dc.p, sizeof(unsigned long long) * hc.size(),
                                hipMemcpyDeviceToHost, st), "copy");
     }

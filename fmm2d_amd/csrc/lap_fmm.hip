@@ -286,7 +286,7 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
                        const int* ilevel, const int* nterms, const double* rsc,
                        const double* centers, const int* n3, const int* l3, const cplx* mp,
                        const cplx* loc, cplx* pot, cplx* grad, cplx* hess) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = leaf[i];
   const double x = xy[2 * i], y = xy[2 * i + 1];
@@ -349,7 +349,7 @@ __global__ void k_near(int64_t n, int nd, int d0, const double* xy, const int* l
                        const int* n1, const int* l1, const int* sse, const double* xs,
                        const cplx* chs, const cplx* dps, double thresh, cplx* pot, cplx* grad,
                        cplx* hess) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = leaf[i];
   const double x = xy[2 * i], y = xy[2 * i + 1];
@@ -439,13 +439,13 @@ void launch_near_pg(int ifc, int ifd, int64_t n, hipStream_t st, int nd, int d0,
                     const int* leaf, const Lists& L, const int* sse, const double* xs,
                     const cplx* chs, const cplx* dps, double thresh, cplx* p, cplx* g, cplx* h) {
   if (ifc && !ifd)
-    launch1d(k_near<1, 0, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
+    launch1d_xcd(k_near<1, 0, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
              thresh, p, g, h);
   else if (!ifc && ifd)
-    launch1d(k_near<0, 1, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
+    launch1d_xcd(k_near<0, 1, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
              thresh, p, g, h);
   else
-    launch1d(k_near<1, 1, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
+    launch1d_xcd(k_near<1, 1, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
              thresh, p, g, h);
 }
 
@@ -470,13 +470,13 @@ void launch_eval(int pg, int64_t n, hipStream_t st, int nd, int P, const double*
                  const int* leaf, const Tree& t, const int* nterms, const double* rsc,
                  const Lists& L, const cplx* mp, const cplx* loc, cplx* p, cplx* g, cplx* h) {
   if (pg == 1)
-    launch1d(k_eval<1>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
+    launch1d_xcd(k_eval<1>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
              L.l3.p, mp, loc, p, g, h);
   else if (pg == 2)
-    launch1d(k_eval<2>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
+    launch1d_xcd(k_eval<2>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
              L.l3.p, mp, loc, p, g, h);
   else
-    launch1d(k_eval<3>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
+    launch1d_xcd(k_eval<3>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
              L.l3.p, mp, loc, p, g, h);
 }
 

@@ -44,10 +44,16 @@ struct TrTable {
 // loop and the row tiles are covered in chunks of RT.
 template <int RT, int KSU, int MODE, int LOGT>
 __global__ __launch_bounds__(256) void k_translate(
-    int nd, int Pin, int Pout, int nslots, int64_t nwaves, TrTable tab, const int* tlist,
+    int nd, int Pin, int Pout, int nslots, int64_t nwaves, int per_xcd, TrTable tab,
+    const int* tlist,
     const int* slots, const int* slot_op, const double* Afrag, int64_t afs, int sbox0, int dbox0,
     const int* ilevel, const int2* band, const double* logr, const cplx* src, cplx* dst) {
-  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // Workgroups are dispatched round-robin over the 8 XCDs; block b works on
+  // logical block (b % 8) * per_xcd + b / 8 so that each XCD walks a
+  // contiguous range of targets, whose neighbouring source boxes then stay
+  // in that XCD's L2.
+  const int64_t lblk = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  const int64_t wave = lblk * 4 + (threadIdx.x >> 6);
   if (wave >= nwaves) return;
   const int lane = threadIdx.x & 63;
   int g = 0;
@@ -234,10 +240,12 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     }
     if (tab.ng == 0) continue;
     tab.tile0[tab.ng] = tiles;
-    const dim3 grid((unsigned)((tiles + 3) / 4));
+    const int nblk = (tiles + 3) / 4, per_xcd = (nblk + 7) / 8;
+    const dim3 grid((unsigned)(8 * per_xcd));
 #define FMM2D_TR(RT, KSU, MODE, LOGT)                                                           \
   hipLaunchKernelGGL((k_translate<RT, KSU, MODE, LOGT>), grid, dim3(256), 0, st, nd, Pin, Pout, \
-                     nslots, (int64_t)tiles, tab, tlist, slots, slot_op, Afrag, afs, io.sbox0,   \
+                     nslots, (int64_t)tiles, per_xcd, tab, tlist, slots, slot_op, Afrag, afs,    \
+                     io.sbox0,                                                                   \
                      io.dbox0, ilevel, band, logr, io.src, io.dst)
 #define FMM2D_TR_RT(MODE, LOGT)                 \
   switch (KS <= 4 * RTT ? RTT : 0) {            \
