@@ -46,7 +46,7 @@ SIGNATURES["fmm2d_hfmm2d"] = (_i, [_i, _d, _d, _d] + _FMM_TAIL + [_vp] + _FMM_OU
 class FmmOpts(C.Structure):
     """struct fmm2d_opts of include/fmm2d_amd.h."""
     _fields_ = [("ndiv", C.c_int32), ("idivflag", C.c_int32), ("ifnear", C.c_int32),
-                ("precision", C.c_int32), ("n_nterms", C.c_int32), ("pad0", C.c_int32),
+                ("precision", C.c_int32), ("n_nterms", C.c_int32), ("count_work", C.c_int32),
                 ("nterms", C.POINTER(C.c_int32)), ("timeinfo", C.c_double * 16),
                 ("nlevels", C.c_int32), ("nboxes", C.c_int32), ("pad1", C.c_int32),
                 ("pad2", C.c_int32), ("work", C.c_double * 16), ("near_pairs", C.c_int64),
@@ -58,7 +58,8 @@ class FmmOpts(C.Structure):
 
 def opts_struct(opts=None):
     """Build an FmmOpts from a dict: ndiv, idivflag, ifnear, precision ("fp64" or
-    "mixed"), nterms (per-level list).  Missing keys take the reference defaults."""
+    "mixed"), nterms (per-level list), count_work (0/1).  Missing keys take the
+    reference defaults."""
     o = FmmOpts()
     opts = opts or {}
     o.ndiv = int(opts.get("ndiv", 0))
@@ -66,6 +67,7 @@ def opts_struct(opts=None):
     o.ifnear = int(opts.get("ifnear", 1))
     prec = opts.get("precision", "fp64")
     o.precision = {"fp64": 0, "mixed": 1}[prec] if isinstance(prec, str) else int(prec)
+    o.count_work = int(opts.get("count_work", 0))
     nt = opts.get("nterms")
     if nt is not None:
         arr = (C.c_int32 * len(nt))(*[int(v) for v in nt])

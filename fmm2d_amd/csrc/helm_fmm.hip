@@ -953,7 +953,8 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     if (pgt >= 3) launch1d(k_scatter, nt, st, 3 * nd, t.itarg.p, ht.p, a.hesstarg);
     tm.mark(10);
     std::vector<unsigned long long> hc(6 * (nl + 1), 0ull);
-    if (o) {
+    const bool counting = o && o->count_work;
+    if (counting) {
       DBuf<unsigned long long> dc(hc.size(), st);
       dc.zero();
       for (int l = 0; l <= nl; ++l) {
@@ -973,32 +974,34 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
       for (int kk = 0; kk < 16; ++kk) ow->timeinfo[kk] = tinfo[kk];
       ow->nlevels = nl;
       ow->nboxes = nb;
-      // algorithmic flops: complex multiply-add = 8 flops; translations are
-      // (2p+1)-square (M2L) or (2p_parent+1) x (2p_child+1) matrix-vector products
-      const double ndd = (double)nd;
-      double w[16] = {0};
-      int64_t near = 0, m2l_pairs = 0;
-      for (int l = 0; l <= nl; ++l) {
-        const double Pl = 2.0 * nterms[l] + 1.0;
-        const double Pc = (l < nl) ? 2.0 * nterms[l + 1] + 1.0 : Pl;
-        const double Pp = (l > 0) ? 2.0 * nterms[l - 1] + 1.0 : Pl;
-        const unsigned long long* c = &hc[6 * l];
-        w[1] += 8.0 * Pl * ndd * (double)c[5];
-        w[2] += 8.0 * Pl * Pc * ndd * (double)c[2];
-        w[3] += 8.0 * Pl * Pl * ndd * (double)c[1];
-        w[4] += 8.0 * Pl * Pp * ndd * (double)c[3];
-        w[6] += 8.0 * Pl * ndd * (double)c[4];
-        near += (int64_t)c[0];
-        m2l_pairs += (int64_t)c[1];
+      if (counting) {
+        // algorithmic flops: complex multiply-add = 8 flops; translations are
+        // (2p+1)-square (M2L) or (2p_parent+1) x (2p_child+1) matrix-vector products
+        const double ndd = (double)nd;
+        double w[16] = {0};
+        int64_t near = 0, m2l_pairs = 0;
+        for (int l = 0; l <= nl; ++l) {
+          const double Pl = 2.0 * nterms[l] + 1.0;
+          const double Pc = (l < nl) ? 2.0 * nterms[l + 1] + 1.0 : Pl;
+          const double Pp = (l > 0) ? 2.0 * nterms[l - 1] + 1.0 : Pl;
+          const unsigned long long* c = &hc[6 * l];
+          w[1] += 8.0 * Pl * ndd * (double)c[5];
+          w[2] += 8.0 * Pl * Pc * ndd * (double)c[2];
+          w[3] += 8.0 * Pl * Pl * ndd * (double)c[1];
+          w[4] += 8.0 * Pl * Pp * ndd * (double)c[3];
+          w[6] += 8.0 * Pl * ndd * (double)c[4];
+          near += (int64_t)c[0];
+          m2l_pairs += (int64_t)c[1];
+        }
+        // P2M and L2P at the finest order: a lower bound of their work
+        w[0] = 8.0 * (2.0 * nterms[nl] + 1.0) * ndd * (double)ns;
+        w[6] += 8.0 * (2.0 * nterms[nl] + 1.0) * ndd * (double)((pg > 0 ? ns : 0) + (pgt > 0 ? nt : 0));
+        w[7] = 40.0 * ndd * (double)near;
+        w[8] = 40.0 * (double)(ns + nt) + 80.0 * (double)nb;
+        for (int kk = 0; kk < 16; ++kk) ow->work[kk] = w[kk];
+        ow->near_pairs = near;
+        ow->m2l_pairs = m2l_pairs;
       }
-      // P2M and L2P at the finest order: a lower bound of their work
-      w[0] = 8.0 * (2.0 * nterms[nl] + 1.0) * ndd * (double)ns;
-      w[6] += 8.0 * (2.0 * nterms[nl] + 1.0) * ndd * (double)((pg > 0 ? ns : 0) + (pgt > 0 ? nt : 0));
-      w[7] = 40.0 * ndd * (double)near;
-      w[8] = 40.0 * (double)(ns + nt) + 80.0 * (double)nb;
-      for (int kk = 0; kk < 16; ++kk) ow->work[kk] = w[kk];
-      ow->near_pairs = near;
-      ow->m2l_pairs = m2l_pairs;
     }
     return FMM2D_OK;
   } catch (const HipError& e) {

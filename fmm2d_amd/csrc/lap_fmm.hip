@@ -104,13 +104,19 @@ __global__ void k_scatter_unpack(int64_t n, int nd, int pg, const int* perm, con
 }
 
 // ---------------------------------------------------------- P2M
-// l2dformmp{c,d,cd} (laprouts2d.f:84-282) on leaves at levels >= 2.
+// l2dformmp{c,d,cd} (laprouts2d.f:84-282) on leaves at levels >= 2.  One
+// wavefront per leaf; sources go through LDS in batches of up to 64: lane j builds
+// the power table w_j^n (w = (z_j - c)/r) by running products, then lane
+// (n, d) sums  M_n += sum_j [-w_j^n/n c_j + w_j^(n-1)/r d_j]  (M_0 += c_j).
 template <int IFC, int IFD>
 __global__ __launch_bounds__(kW) void k_p2m(int nd, int P, const int* ilevel, const int* nchild,
                                             const int* nterms, const double* rsc,
                                             const double* centers, const int* sse,
                                             const double* xs, const cplx* chs, const cplx* dps,
-                                            cplx* mp) {
+                                            int bat, cplx* mp) {
+  extern __shared__ cplx sh[];
+  cplx* W = sh;                                    // [bat][P]
+  int* J = reinterpret_cast<int*>(sh + bat * P);   // [bat]
   const int b = blockIdx.x;
   const int lev = ilevel[b];
   if (lev < 2 || nchild[b] != 0) return;
@@ -119,31 +125,125 @@ __global__ __launch_bounds__(kW) void k_p2m(int nd, int P, const int* ilevel, co
   const int p = nterms[lev];
   const double r = rsc[lev], rinv = 1.0 / r;
   const double cx = centers[2 * b], cy = centers[2 * b + 1];
-  constexpr int DC = 4;  // densities per pass
-  for (int nn = threadIdx.x; nn <= p; nn += kW) {
-    for (int d0 = 0; d0 < nd; d0 += DC) {
-      cplx acc[DC];
-#pragma unroll
-      for (int dd = 0; dd < DC; ++dd) acc[dd] = mkc(0.0);
-      for (int j = s0; j < s1; ++j) {
-        const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
-        const cplx pw = cpowi(w, nn > 0 ? nn - 1 : 0);  // (z/r)^(n-1)
-        const cplx fc = (nn == 0) ? mkc(1.0) : (pw * w) * (-1.0 / nn);
-        const cplx fd = pw * rinv;
-#pragma unroll
-        for (int dd = 0; dd < DC; ++dd) {
-          if (d0 + dd >= nd) break;
-          if (IFC) acc[dd] += fc * chs[(int64_t)j * nd + d0 + dd];
-          if (IFD && nn > 0) acc[dd] += fd * dps[(int64_t)j * nd + d0 + dd];
-        }
+  const int lane = threadIdx.x;
+  for (int j0 = s0; j0 < s1; j0 += bat) {
+    const int cnt = min(bat, s1 - j0);
+    if (lane < cnt) {
+      const int j = j0 + lane;
+      const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
+      cplx* row = W + lane * P;
+      cplx v = mkc(1.0);
+      row[0] = v;
+      for (int n = 1; n <= p; ++n) {
+        v = v * w;
+        row[n] = v;
       }
+      J[lane] = j;
+    }
+    __syncthreads();
+    for (int idx = lane; idx < (p + 1) * nd; idx += kW) {
+      const int n = idx / nd, d = idx - n * nd;
+      const double cn = (n == 0) ? 0.0 : -1.0 / n;
+      cplx acc = mkc(0.0);
+      for (int jj = 0; jj < cnt; ++jj) {
+        const int64_t j = J[jj];
+        if (IFC) {
+          const cplx c = chs[j * nd + d];
+          acc += (n == 0) ? c : cn * (W[jj * P + n] * c);
+        }
+        if (IFD && n > 0) acc += rinv * (W[jj * P + n - 1] * dps[j * nd + d]);
+      }
+      mp[((int64_t)b * nd + d) * P + n] += acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Register forms of P2M and list-4 P2L for expansions of at most PT
+// coefficients: one thread per (box, density) runs over the box's sources
+// with the running products w^n, accumulating all coefficients in registers
+// (same sums as k_p2m / k_p2l above, which serve longer expansions).
+template <int PT, int IFC, int IFD>
+__global__ void k_p2m_reg(int64_t n, int nd, int P, const int* ilevel, const int* nchild,
+                          const int* nterms, const double* rsc, const double* centers,
+                          const int* sse, const double* xs, const cplx* chs, const cplx* dps,
+                          cplx* mp) {
+  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)(i / nd), d = (int)(i - (int64_t)b * nd);
+  const int lev = ilevel[b];
+  if (lev < 2 || nchild[b] != 0) return;
+  const int s0 = sse[2 * b], s1 = sse[2 * b + 1];
+  if (s1 <= s0) return;
+  const int p = nterms[lev];
+  const double rinv = 1.0 / rsc[lev];
+  const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  cplx acc[PT];
 #pragma unroll
-      for (int dd = 0; dd < DC; ++dd) {
-        if (d0 + dd >= nd) break;
-        mp[((int64_t)b * nd + d0 + dd) * P + nn] += acc[dd];
+  for (int k = 0; k < PT; ++k) acc[k] = mkc(0.0);
+  for (int j = s0; j < s1; ++j) {
+    const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
+    const cplx c = IFC ? chs[(int64_t)j * nd + d] : mkc(0.0);
+    const cplx dp = IFD ? rinv * dps[(int64_t)j * nd + d] : mkc(0.0);
+    if (IFC) acc[0] += c;
+    cplx v = mkc(1.0);  // w^(k-1)
+#pragma unroll
+    for (int k = 1; k < PT; ++k) {
+      if (k <= p) {
+        if (IFD) acc[k] += v * dp;
+        v = v * w;
+        if (IFC) acc[k] += (-1.0 / k) * (v * c);
       }
     }
   }
+  cplx* out = mp + ((int64_t)b * nd + d) * P;
+#pragma unroll
+  for (int k = 0; k < PT; ++k)
+    if (k <= p) out[k] += acc[k];
+}
+
+template <int PT, int IFC, int IFD>
+__global__ void k_p2l_reg(int64_t n, int nd, int P, const int* ilevel, const int* nterms,
+                          const double* rsc, const double* centers, const int* n4, const int* l4,
+                          const int* sse, const int* tse, int ifpgh, int ifpght, const double* xs,
+                          const cplx* chs, const cplx* dps, cplx* loc) {
+  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)(i / nd), d = (int)(i - (int64_t)b * nd);
+  const int lev = ilevel[b];
+  const int nbox = n4[b];
+  if (lev < 2 || nbox == 0 || npts_eval(b, sse, tse, ifpgh, ifpght) == 0) return;
+  const int p = nterms[lev];
+  const double r = rsc[lev];
+  const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  cplx acc[PT];
+#pragma unroll
+  for (int k = 0; k < PT; ++k) acc[k] = mkc(0.0);
+  for (int kb = 0; kb < nbox; ++kb) {
+    const int c4 = l4[(int64_t)kMaxList4 * b + kb];
+    for (int j = sse[2 * c4]; j < sse[2 * c4 + 1]; ++j) {
+      const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
+      const cplx zi = invc(z0);
+      const cplx u = r * zi;
+      const cplx c = IFC ? chs[(int64_t)j * nd + d] : mkc(0.0);
+      const cplx dz = IFD ? -(zi * dps[(int64_t)j * nd + d]) : mkc(0.0);
+      if (IFC) acc[0] += log(absc(z0)) * c;
+      if (IFD) acc[0] += dz;
+      cplx v = mkc(1.0);  // u^k
+#pragma unroll
+      for (int k = 1; k < PT; ++k) {
+        if (k <= p) {
+          v = v * u;
+          if (IFC) acc[k] += (-1.0 / k) * (v * c);
+          if (IFD) acc[k] += v * dz;
+        }
+      }
+    }
+  }
+  cplx* out = loc + ((int64_t)b * nd + d) * P;
+#pragma unroll
+  for (int k = 0; k < PT; ++k)
+    if (k <= p) out[k] += acc[k];
 }
 
 // ---------------------------------------------------------- M2M
@@ -209,52 +309,80 @@ __global__ __launch_bounds__(kW) void k_m2l(int nd, int P, const int* ilevel, co
 }
 
 // ---------------------------------------------------------- list 4 (P2L)
-// l2dformta{c,d,cd} (laprouts2d.f:501-690): sources of coarser leaves in list 4.
+// l2dformta{c,d,cd} (laprouts2d.f:501-690): sources of the coarser leaves in
+// list 4 of a box.  One wavefront per box; the list-4 sources are taken as one
+// flat sequence in batches of up to 64: lane j stores u_j^n (u = r/z_j, z_j = source
+// - centre), 1/z_j and log|z_j| in LDS, then lane (n, d) sums
+//   L_0 += log|z_j| c_j,  L_n += -u_j^n/n c_j  (n >= 1),  L_n += -u_j^n/z_j d_j.
 template <int IFC, int IFD>
 __global__ __launch_bounds__(kW) void k_p2l(int nd, int P, const int* ilevel, const int* nterms,
                                             const double* rsc, const double* centers,
                                             const int* n4, const int* l4, const int* sse,
                                             const int* tse, int ifpgh, int ifpght,
                                             const double* xs, const cplx* chs, const cplx* dps,
-                                            cplx* loc) {
+                                            int bat, cplx* loc) {
+  extern __shared__ cplx sh[];
+  cplx* W = sh;                                        // [bat][P]
+  cplx* ZI = sh + bat * P;                             // [bat]
+  double* LG = reinterpret_cast<double*>(ZI + bat);    // [bat]
+  int* J = reinterpret_cast<int*>(LG + bat);           // [bat]
   const int b = blockIdx.x;
   const int lev = ilevel[b];
-  const int cnt = n4[b];
-  if (lev < 2 || cnt == 0 || npts_eval(b, sse, tse, ifpgh, ifpght) == 0) return;
+  const int nbox = n4[b];
+  if (lev < 2 || nbox == 0 || npts_eval(b, sse, tse, ifpgh, ifpght) == 0) return;
   const int p = nterms[lev];
   const double r = rsc[lev];
   const double cx = centers[2 * b], cy = centers[2 * b + 1];
-  constexpr int DC = 4;  // densities per pass
-  for (int nn = threadIdx.x; nn <= p; nn += kW) {
-    for (int d0 = 0; d0 < nd; d0 += DC) {
-      cplx acc[DC];
-#pragma unroll
-      for (int dd = 0; dd < DC; ++dd) acc[dd] = mkc(0.0);
-      for (int k = 0; k < cnt; ++k) {
-        const int c = l4[(int64_t)kMaxList4 * b + k];
-        for (int j = sse[2 * c]; j < sse[2 * c + 1]; ++j) {
-          const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
-          const cplx zi = invc(z0);
-          const cplx wn = cpowi(r * zi, nn);  // (r/z0)^n
-          const cplx fc = (nn == 0) ? mkc(log(absc(z0))) : wn * (-1.0 / nn);
-          const cplx fd = -(zi * wn);
-#pragma unroll
-          for (int dd = 0; dd < DC; ++dd) {
-            if (d0 + dd >= nd) break;
-            if (IFC) acc[dd] += fc * chs[(int64_t)j * nd + d0 + dd];
-            if (IFD) acc[dd] += fd * dps[(int64_t)j * nd + d0 + dd];
-          }
-        }
+  const int lane = threadIdx.x;
+  // flat numbering of the list-4 sources: box k holds [pre[k], pre[k+1])
+  int pre[kMaxList4 + 1], first[kMaxList4];
+  pre[0] = 0;
+  for (int k = 0; k < kMaxList4; ++k) {
+    int c = -1;
+    if (k < nbox) c = l4[(int64_t)kMaxList4 * b + k];
+    first[k] = (c >= 0) ? sse[2 * c] : 0;
+    pre[k + 1] = pre[k] + ((c >= 0) ? sse[2 * c + 1] - sse[2 * c] : 0);
+  }
+  const int total = pre[kMaxList4];
+  for (int f0 = 0; f0 < total; f0 += bat) {
+    const int cnt = min(bat, total - f0);
+    if (lane < cnt) {
+      const int f = f0 + lane;
+      int k = 0;
+      while (f >= pre[k + 1]) ++k;
+      const int j = first[k] + (f - pre[k]);
+      const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
+      const cplx zi = invc(z0);
+      const cplx u = r * zi;
+      cplx* row = W + lane * P;
+      cplx v = mkc(1.0);
+      row[0] = v;
+      for (int n = 1; n <= p; ++n) {
+        v = v * u;
+        row[n] = v;
       }
-#pragma unroll
-      for (int dd = 0; dd < DC; ++dd) {
-        if (d0 + dd >= nd) break;
-        loc[((int64_t)b * nd + d0 + dd) * P + nn] += acc[dd];
-      }
+      ZI[lane] = zi;
+      LG[lane] = log(absc(z0));
+      J[lane] = j;
     }
+    __syncthreads();
+    for (int idx = lane; idx < (p + 1) * nd; idx += kW) {
+      const int n = idx / nd, d = idx - n * nd;
+      const double cn = (n == 0) ? 0.0 : -1.0 / n;
+      cplx acc = mkc(0.0);
+      for (int jj = 0; jj < cnt; ++jj) {
+        const int64_t j = J[jj];
+        if (IFC) {
+          const cplx c = chs[j * nd + d];
+          acc += (n == 0) ? LG[jj] * c : cn * (W[jj * P + n] * c);
+        }
+        if (IFD) acc -= (ZI[jj] * W[jj * P + n]) * dps[j * nd + d];
+      }
+      loc[((int64_t)b * nd + d) * P + n] += acc;
+    }
+    __syncthreads();
   }
 }
-
 
 // ---------------------------------------------------------- L2L
 // l2dlocloc (laprouts2d.f:1020-1123): children at one level from their parent.
@@ -559,32 +687,62 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
 
     const int* sse = t.isrcse.p;
     const int* tse = t.itargse.p;
-    // ---- step 1: P2M
-    if (a.ifcharge && !a.ifdipole)
-      hipLaunchKernelGGL((k_p2m<1, 0>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, t.nchild.p,
-                         d_nterms.p, d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p);
-    else if (!a.ifcharge && a.ifdipole)
-      hipLaunchKernelGGL((k_p2m<0, 1>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, t.nchild.p,
-                         d_nterms.p, d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p);
-    else
-      hipLaunchKernelGGL((k_p2m<1, 1>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, t.nchild.p,
-                         d_nterms.p, d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p);
-    hip_throw(hipGetLastError(), "p2m");
-    tm.mark(0);
-    // ---- step 2: list 4
-    if (a.ifcharge && !a.ifdipole)
-      hipLaunchKernelGGL((k_p2l<1, 0>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p,
-                         d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p,
-                         dps.p, loc.p);
-    else if (!a.ifcharge && a.ifdipole)
-      hipLaunchKernelGGL((k_p2l<0, 1>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p,
-                         d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p,
-                         dps.p, loc.p);
-    else
-      hipLaunchKernelGGL((k_p2l<1, 1>), dim3(nb), dim3(kW), 0, st, nd, P, t.ilevel.p, d_nterms.p,
-                         d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p,
-                         dps.p, loc.p);
-    hip_throw(hipGetLastError(), "p2l");
+    // sources per LDS batch of P2M / P2L: at most 64, and the tables within 64 KiB
+    const size_t per_src = (size_t)(P + 1) * sizeof(cplx) + sizeof(double) + sizeof(int);
+    const int bat = (int)std::max<size_t>(1, std::min<size_t>(kW, 65536 / per_src));
+    const size_t lds_p2m = (size_t)bat * (P * sizeof(cplx) + sizeof(int));
+    const size_t lds_p2l = (size_t)bat * per_src;
+    // ---- step 1: P2M, step 2: list 4 (register forms up to 48 coefficients)
+    const int64_t nbd = (int64_t)nb * nd;
+#define FMM2D_P2M_REG(PT, C, D)                                                                \
+  launch1d_xcd(k_p2m_reg<PT, C, D>, nbd, st, nd, P, t.ilevel.p, t.nchild.p, d_nterms.p,       \
+               d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p)
+#define FMM2D_P2L_REG(PT, C, D)                                                                \
+  launch1d_xcd(k_p2l_reg<PT, C, D>, nbd, st, nd, P, t.ilevel.p, d_nterms.p, d_rsc.p,          \
+               t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p, dps.p, loc.p)
+#define FMM2D_BY_PT(M, C, D)          \
+  if (P <= 16) M(16, C, D);          \
+  else if (P <= 24) M(24, C, D);     \
+  else if (P <= 32) M(32, C, D);     \
+  else M(48, C, D)
+    if (P <= 48) {
+      if (a.ifcharge && !a.ifdipole) FMM2D_BY_PT(FMM2D_P2M_REG, 1, 0);
+      else if (!a.ifcharge && a.ifdipole) FMM2D_BY_PT(FMM2D_P2M_REG, 0, 1);
+      else FMM2D_BY_PT(FMM2D_P2M_REG, 1, 1);
+      tm.mark(0);
+      if (a.ifcharge && !a.ifdipole) FMM2D_BY_PT(FMM2D_P2L_REG, 1, 0);
+      else if (!a.ifcharge && a.ifdipole) FMM2D_BY_PT(FMM2D_P2L_REG, 0, 1);
+      else FMM2D_BY_PT(FMM2D_P2L_REG, 1, 1);
+    } else {
+      if (a.ifcharge && !a.ifdipole)
+        hipLaunchKernelGGL((k_p2m<1, 0>), dim3(nb), dim3(kW), lds_p2m, st, nd, P, t.ilevel.p, t.nchild.p,
+                           d_nterms.p, d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, bat, mp.p);
+      else if (!a.ifcharge && a.ifdipole)
+        hipLaunchKernelGGL((k_p2m<0, 1>), dim3(nb), dim3(kW), lds_p2m, st, nd, P, t.ilevel.p, t.nchild.p,
+                           d_nterms.p, d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, bat, mp.p);
+      else
+        hipLaunchKernelGGL((k_p2m<1, 1>), dim3(nb), dim3(kW), lds_p2m, st, nd, P, t.ilevel.p, t.nchild.p,
+                           d_nterms.p, d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, bat, mp.p);
+      hip_throw(hipGetLastError(), "p2m");
+      tm.mark(0);
+      // ---- step 2: list 4
+      if (a.ifcharge && !a.ifdipole)
+        hipLaunchKernelGGL((k_p2l<1, 0>), dim3(nb), dim3(kW), lds_p2l, st, nd, P, t.ilevel.p, d_nterms.p,
+                           d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p,
+                           dps.p, bat, loc.p);
+      else if (!a.ifcharge && a.ifdipole)
+        hipLaunchKernelGGL((k_p2l<0, 1>), dim3(nb), dim3(kW), lds_p2l, st, nd, P, t.ilevel.p, d_nterms.p,
+                           d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p,
+                           dps.p, bat, loc.p);
+      else
+        hipLaunchKernelGGL((k_p2l<1, 1>), dim3(nb), dim3(kW), lds_p2l, st, nd, P, t.ilevel.p, d_nterms.p,
+                           d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p,
+                           dps.p, bat, loc.p);
+      hip_throw(hipGetLastError(), "p2l");
+    }
+#undef FMM2D_BY_PT
+#undef FMM2D_P2L_REG
+#undef FMM2D_P2M_REG
     tm.mark(1);
     // ---- translation operators as MFMA fragments: 49 M2L offsets, 4 M2M, 4 L2L
     const int opM2M = kOff * kOff, opL2L = opM2M + 4;
@@ -668,7 +826,8 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     }
     tm.mark(10);
     unsigned long long hc[6] = {0, 0, 0, 0, 0, 0};
-    if (o) {
+    const bool counting = o && o->count_work;
+    if (counting) {
       DBuf<unsigned long long> dc(6, st);
       dc.zero();
       launch1d(k_counts, nb, st, t.ilevel.p, t.iparent.p, t.nchild.p, t.ichild.p, sse, tse, pg,
@@ -682,22 +841,24 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       for (int k = 0; k < 16; ++k) ow->timeinfo[k] = tinfo[k];
       ow->nlevels = nl;
       ow->nboxes = nb;
-      const double Pd = (double)P, ndd = (double)nd;
-      const int64_t neval = (pg > 0 ? ns : 0) + (pgt > 0 ? nt : 0);
-      const double gfac = (std::max(pg, pgt) >= 2) ? 2.0 : 1.0;
-      double w[16] = {0};
-      w[0] = 8.0 * Pd * ndd * (double)ns;
-      w[1] = 8.0 * Pd * ndd * (double)hc[5];
-      w[2] = 8.0 * Pd * Pd * ndd * (double)hc[2];
-      w[3] = 8.0 * Pd * Pd * ndd * (double)hc[1];
-      w[4] = 8.0 * Pd * Pd * ndd * (double)hc[3];
-      w[5] = 0.0;  // list 3 is fused with L2P: counted in [6]
-      w[6] = 8.0 * Pd * ndd * gfac * ((double)neval + (double)hc[4]);
-      w[7] = (12.0 + 12.0 * ndd) * (double)hc[0];
-      w[8] = 40.0 * (double)(ns + nt) + 80.0 * (double)nb;
-      for (int k = 0; k < 16; ++k) ow->work[k] = w[k];
-      ow->near_pairs = (int64_t)hc[0];
-      ow->m2l_pairs = (int64_t)hc[1];
+      if (counting) {
+        const double Pd = (double)P, ndd = (double)nd;
+        const int64_t neval = (pg > 0 ? ns : 0) + (pgt > 0 ? nt : 0);
+        const double gfac = (std::max(pg, pgt) >= 2) ? 2.0 : 1.0;
+        double w[16] = {0};
+        w[0] = 8.0 * Pd * ndd * (double)ns;
+        w[1] = 8.0 * Pd * ndd * (double)hc[5];
+        w[2] = 8.0 * Pd * Pd * ndd * (double)hc[2];
+        w[3] = 8.0 * Pd * Pd * ndd * (double)hc[1];
+        w[4] = 8.0 * Pd * Pd * ndd * (double)hc[3];
+        w[5] = 0.0;  // list 3 is fused with L2P: counted in [6]
+        w[6] = 8.0 * Pd * ndd * gfac * ((double)neval + (double)hc[4]);
+        w[7] = (12.0 + 12.0 * ndd) * (double)hc[0];
+        w[8] = 40.0 * (double)(ns + nt) + 80.0 * (double)nb;
+        for (int k = 0; k < 16; ++k) ow->work[k] = w[k];
+        ow->near_pairs = (int64_t)hc[0];
+        ow->m2l_pairs = (int64_t)hc[1];
+      }
     }
     return FMM2D_OK;
   } catch (const HipError& e) {

@@ -211,6 +211,16 @@ __global__ void k_total(int n, const int* flag, const int* offs, int* total) {
   *total = (n > 0) ? offs[n - 1] + flag[n - 1] : 0;
 }
 
+// cum[k] = exclusive-scan value at segment start cut[k]; cut[K] = n, the
+// scanned length, gives the grand total
+__global__ void k_seg_totals(int K, const int* cut, const int* flag, const int* offs, int* cum) {
+  const int n = cut[K];
+  for (int k = threadIdx.x; k <= K; k += blockDim.x) {
+    const int c = cut[k];
+    cum[k] = (c < n) ? offs[c] : (n > 0 ? offs[n - 1] + flag[n - 1] : 0);
+  }
+}
+
 // Children of the flagged boxes lo..lo+n-1 are numbered base + 4*offs + q, q =
 // 0..3 for (-,-), (+,-), (-,+), (+,+) around the parent centre
 // (tree_refine_boxes, tree_routs2d.f:25-52).  Their point ranges split the
@@ -450,28 +460,73 @@ struct Builder {
   Boxes B;
   int nboxes = 0;
   DBuf<int> flag, offs, total;
-  int* h_total = nullptr;
+  DBuf<char> scan_tmp;  // hipcub scan scratch, kept across scans
   Keys ks{}, kt{};
 
-  explicit Builder(hipStream_t s) : st(s) {
-    total.alloc(1, s);
-    hip_throw(hipHostMalloc((void**)&h_total, sizeof(int)), "hipHostMalloc");
-  }
-  ~Builder() {
-    if (h_total) (void)hipHostFree(h_total);
-  }
+  explicit Builder(hipStream_t s) : st(s) { total.alloc(64, s); }
 
+  void scan_only(int n) {
+    size_t bytes = 0;
+    hip_throw(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag.p, offs.p, n, st), "scan");
+    if (bytes > scan_tmp.n) scan_tmp.alloc(std::max(bytes, 2 * scan_tmp.n), st);
+    hip_throw(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, bytes, flag.p, offs.p, n, st), "scan");
+  }
   // exclusive scan of flag[0..n) into offs, returns the total
   int scan(int n) {
     if (n <= 0) return 0;
-    size_t bytes = 0;
-    hip_throw(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag.p, offs.p, n, st), "scan");
-    DBuf<char> tmp(bytes, st);
-    hip_throw(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, flag.p, offs.p, n, st), "scan");
+    scan_only(n);
     hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, st, n, flag.p, offs.p, total.p);
-    hip_throw(hipMemcpyAsync(h_total, total.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
+    int h = 0;
+    hip_throw(hipMemcpyAsync(&h, total.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
     hip_throw(hipStreamSynchronize(st), "sync");
-    return *h_total;
+    return h;
+  }
+  // Refine several box ranges with one scan: segment k covers boxes
+  // lo[k] .. lo[k]+n[k]-1 (flags from iflag > 0), children at level lev[k]
+  // with box size bsc[k].  Children are numbered in segment order, exactly as
+  // refining the segments one after the other.  first[k], last[k]: range of
+  // the children of segment k.
+  void refine_segments(const std::vector<int>& lo, const std::vector<int>& n,
+                       const std::vector<int>& lev, const std::vector<double>& bsc,
+                       std::vector<int>& first, std::vector<int>& last) {
+    const size_t K = lo.size();
+    std::vector<int> pos(K + 1, 0);
+    for (size_t k = 0; k < K; ++k) pos[k + 1] = pos[k] + n[k];
+    first.assign(K, nboxes);
+    last.assign(K, nboxes);
+    const int ntot = pos[K];
+    if (ntot == 0) return;
+    ensure_flag(ntot);
+    for (size_t k = 0; k < K; ++k)
+      launch1d(k_iflag_pos, n[k], st, lo[k], B.iflag.p, flag.p + pos[k]);
+    scan_only(ntot);
+    if (total.n < K + 1) total.alloc(K + 1, st);
+    std::vector<int> cut(pos);
+    DBuf<int> dcut = upload_ints(cut);
+    hipLaunchKernelGGL(k_seg_totals, dim3(1), dim3(64), 0, st, (int)K, dcut.p, flag.p, offs.p,
+                       total.p);
+    std::vector<int> cum(K + 1, 0);
+    hip_throw(hipMemcpyAsync(cum.data(), total.p, sizeof(int) * (K + 1), hipMemcpyDeviceToHost,
+                             st), "copy");
+    hip_throw(hipStreamSynchronize(st), "sync");
+    const int nref = cum[K];
+    if (nref == 0) return;
+    B.grow(nboxes + 4 * (size_t)nref, nboxes);
+    for (size_t k = 0; k < K; ++k) {
+      first[k] = nboxes + 4 * cum[k];
+      last[k] = nboxes + 4 * cum[k + 1];
+      if (last[k] > first[k])
+        launch1d(k_make_children, n[k], st, lo[k], flag.p + pos[k], offs.p + pos[k], nboxes,
+                 0.5 * bsc[k], lev[k], B.centers.p, B.ilevel.p, B.iparent.p, B.nchild.p,
+                 B.ichild.p, B.ses.p, B.set.p, B.iflag.p, 1, ks, kt);
+    }
+    nboxes += 4 * nref;
+  }
+  DBuf<int> upload_ints(const std::vector<int>& v) {
+    DBuf<int> d(v.size(), st);
+    hip_throw(hipMemcpyAsync(d.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, st),
+              "copy");
+    return d;  // the caller syncs before v dies
   }
   void ensure_flag(int n) {
     if ((size_t)n > flag.n) {
@@ -636,13 +691,21 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
                bld.B.coll.p, bld.B.nchild.p, bld.B.centers.p, iflag);
     }
     std::vector<int> ts(nlevels + 1, 0), te(nlevels + 1, 0);
-    for (int l = 1; l <= nlevels - 2; ++l) {
-      ts[l + 1] = bld.nboxes;
-      const int n = le[l] - ls[l];
-      bld.ensure_flag(n);
-      launch1d(k_iflag_pos, n, st, ls[l], bld.B.iflag.p, bld.flag.p);
-      bld.refine(ls[l], n, bs[l + 1], l + 1, 1);
-      te[l + 1] = bld.nboxes;
+    {
+      // the flags are fixed before this pass, so all levels refine in one scan
+      std::vector<int> lo, n, lev, first, last;
+      std::vector<double> bsc;
+      for (int l = 1; l <= nlevels - 2; ++l) {
+        lo.push_back(ls[l]);
+        n.push_back(le[l] - ls[l]);
+        lev.push_back(l + 1);
+        bsc.push_back(bs[l + 1]);
+      }
+      bld.refine_segments(lo, n, lev, bsc, first, last);
+      for (int l = 1; l <= nlevels - 2; ++l) {
+        ts[l + 1] = first[l - 1];
+        te[l + 1] = last[l - 1];
+      }
     }
     bld.reorg(ls, le, ts, te, nlevels);
     bld.computecoll(ls, le, bs, nlevels);
@@ -655,20 +718,14 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
                bld.B.ichild.p, bld.B.nchild.p, bld.B.centers.p, bld.B.iflag.p);
       launch1d(k_updateflags, te[l] - ts[l], st, ts[l], distest, bld.B.ncoll.p, bld.B.coll.p,
                bld.B.ichild.p, bld.B.nchild.p, bld.B.centers.p, bld.B.iflag.p);
-      ts[l + 1] = bld.nboxes;
       {
-        const int n = le[l] - ls[l];
-        bld.ensure_flag(n);
-        launch1d(k_iflag_pos, n, st, ls[l], bld.B.iflag.p, bld.flag.p);
-        bld.refine(ls[l], n, bs[l + 1], l + 1, 1);
+        // the original then the tail boxes of level l, in one scan
+        std::vector<int> first, last;
+        bld.refine_segments({ls[l], ts[l]}, {le[l] - ls[l], te[l] - ts[l]}, {l + 1, l + 1},
+                            {bs[l + 1], bs[l + 1]}, first, last);
+        ts[l + 1] = first[0];
+        te[l + 1] = last[1];
       }
-      {
-        const int n = te[l] - ts[l];
-        bld.ensure_flag(n);
-        launch1d(k_iflag_pos, n, st, ts[l], bld.B.iflag.p, bld.flag.p);
-        bld.refine(ts[l], n, bs[l + 1], l + 1, 1);
-      }
-      te[l + 1] = bld.nboxes;
       launch1d(k_coll, te[l + 1] - ts[l + 1], st, ts[l + 1], bs[l + 1], bld.B.iparent.p,
                bld.B.ichild.p, bld.B.centers.p, bld.B.ncoll.p, bld.B.coll.p);
     }
