@@ -33,6 +33,12 @@ SIGNATURES["fmm2d_tree_free"] = (None, [_vp])
 SIGNATURES["fmm2d_l2dterms"] = (_i, [_d, C.POINTER(_i)])
 SIGNATURES["fmm2d_h2dterms"] = (_i, [_d, _d, _d, _d, C.POINTER(_i)])
 SIGNATURES["fmm2d_ndiv"] = (_i, [_d, _i64, _i64, _i, C.POINTER(_i), C.POINTER(_i)])
+SIGNATURES["fmm2d_release_workspace"] = (_i, [])
+_f = C.c_float
+SIGNATURES["fmm2d_mlp_nparams"] = (_i64, [_i, _i])
+SIGNATURES["fmm2d_mlp_forward"] = (_i, [_i, _i, _vp, _i, _vp, _vp, _vp])
+SIGNATURES["fmm2d_mlp_loss_grad"] = (_i, [_i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp])
+SIGNATURES["fmm2d_mlp_adam"] = (_i, [_i64, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _i, _vp])
 # FMM drivers: (nd, eps, [zk_re, zk_im,] ns, sources, ifcharge, charge, ifdipole,
 # dipstr, [dipvec,] iper, ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg,
 # gradtarg, hesstarg, opts, stream)

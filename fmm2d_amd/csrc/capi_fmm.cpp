@@ -1,7 +1,9 @@
 // C-ABI entry points of the FMM drivers, the tree export and the parameter
 // rules.  Declarations and reference citations: include/fmm2d_amd.h.
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/fmm2d_amd.h"
@@ -10,6 +12,26 @@
 #include "tree.h"
 
 using namespace fmm2d;
+
+namespace fmm2d {
+
+void keep_pool_memory() {
+  static std::mutex mu;
+  static std::vector<char> done;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((size_t)dev < done.size() && done[dev]) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  if ((size_t)dev >= done.size()) done.resize(dev + 1, 0);
+  done[dev] = 1;
+}
+
+}  // namespace fmm2d
 
 namespace {
 
@@ -63,6 +85,7 @@ int fmm2d_cfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifch
   CauchyArgs a = cauchy_args(nd, eps, ns, sources, ifcharge, charge, ifdipole, dipstr, nullptr,
                              ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg,
                              hesstarg, opts);
+  keep_pool_memory();
   return laplace_fmm(0, a, reinterpret_cast<hipStream_t>(stream));
 }
 
@@ -76,6 +99,7 @@ int fmm2d_lfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifch
   CauchyArgs a = cauchy_args(nd, eps, ns, sources, ifcharge, charge, ifdipole, dipstr, dipvec,
                              ifpgh, pot, grad, hess, nt, targ, ifpghtarg, pottarg, gradtarg,
                              hesstarg, opts);
+  keep_pool_memory();
   return laplace_fmm(1, a, reinterpret_cast<hipStream_t>(stream));
 }
 
@@ -108,6 +132,7 @@ int fmm2d_hfmm2d(int nd, double eps, double zk_re, double zk_im, int64_t ns,
   a.gradtarg = reinterpret_cast<cplx*>(gradtarg);
   a.hesstarg = reinterpret_cast<cplx*>(hesstarg);
   a.opts = opts;
+  keep_pool_memory();
   return helmholtz_fmm(a, reinterpret_cast<hipStream_t>(stream));
 }
 
@@ -120,6 +145,7 @@ int fmm2d_tree_build(const double* sources, int64_t ns, const double* targ, int6
   TreeHandle* h = new TreeHandle;
   h->st = reinterpret_cast<hipStream_t>(stream);
   try {
+    keep_pool_memory();
     build_tree(sources, ns, targ, nt, ndiv, idivflag, nlmax, h->t, h->st);
     hip_throw(hipStreamSynchronize(h->st), "sync");
   } catch (const HipError& e) {
@@ -229,5 +255,7 @@ int fmm2d_ndiv(double eps, int64_t ns, int64_t nt, int helmholtz, int* ndiv, int
   *idivflag = 0;
   return FMM2D_OK;
 }
+
+int fmm2d_release_workspace(void) { return ws_release(); }
 
 }  // extern "C"

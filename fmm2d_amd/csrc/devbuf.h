@@ -1,6 +1,10 @@
-// Stream-ordered device buffers (hipMallocAsync / hipFreeAsync on the call's
-// stream; the default memory pool keeps freed blocks for reuse, so repeated
-// FMM calls do not go back to the driver).
+// Device workspace buffers.  Blocks come from a cache kept per (device,
+// stream): a freed block goes back to its stream's cache and is handed out
+// again to a later request on the same stream, so reuse is ordered after
+// every earlier use by the stream itself.  Returning large blocks to the
+// runtime costs up to milliseconds of host time each, which inside an FMM
+// call (dozens of temporaries between host synchronisations) would dominate
+// the small stages; fmm2d_release_workspace() hands the cache back.
 #pragma once
 #include "common.h"
 #include <utility>
@@ -15,6 +19,13 @@ struct HipError {
 inline void hip_throw(hipError_t e, const char* what) {
   if (e != hipSuccess) throw HipError{e, what};
 }
+
+// Workspace cache (ws.cpp).  ws_alloc returns at least `bytes` bytes usable
+// in stream order on `st`; ws_free returns the block to st's cache.
+void* ws_alloc(size_t bytes, hipStream_t st);
+void ws_free(void* p, hipStream_t st);
+// Free every cached block of the current device (after a device sync).
+int ws_release();
 
 template <typename T> struct DBuf {
   T* p = nullptr;
@@ -32,7 +43,7 @@ template <typename T> struct DBuf {
     reset();
     st = s;
     n = count;
-    if (count) hip_throw(hipMallocAsync((void**)&p, sizeof(T) * count, s), "hipMallocAsync");
+    if (count) p = static_cast<T*>(ws_alloc(sizeof(T) * count, s));
   }
   void zero() {
     if (n) hip_throw(hipMemsetAsync(p, 0, sizeof(T) * n, st), "hipMemsetAsync");
@@ -43,20 +54,26 @@ template <typename T> struct DBuf {
   // grow to at least count elements, keeping the first `keep` elements
   void grow(size_t count, size_t keep) {
     if (count <= n) return;
-    T* q = nullptr;
-    hip_throw(hipMallocAsync((void**)&q, sizeof(T) * count, st), "hipMallocAsync");
+    T* q = static_cast<T*>(ws_alloc(sizeof(T) * count, st));
     if (keep) hip_throw(hipMemcpyAsync(q, p, sizeof(T) * keep, hipMemcpyDeviceToDevice, st), "copy");
-    if (p) hip_throw(hipFreeAsync(p, st), "hipFreeAsync");
+    if (p) ws_free(p, st);
     p = q;
     n = count;
   }
   void reset() {
-    if (p) (void)hipFreeAsync(p, st);
+    if (p) ws_free(p, st);
     p = nullptr;
     n = 0;
   }
   T* get() const { return p; }
 };
+
+// Keep freed stream-ordered workspace in the current device's default memory
+// pool instead of returning it to the driver at every synchronisation (the
+// pool's default release threshold is 0), so that repeated calls and the
+// buffers a call allocates between its host synchronisations are served from
+// the pool.  Done once per device; the memory stays with the process.
+void keep_pool_memory();
 
 template <typename F, typename... A>
 inline void launch1d(F kernel, int64_t n, hipStream_t st, A... args) {

@@ -58,46 +58,64 @@ namespace {
 
 // lfmm2d split (laplace/lfmm2d.f:139-156): density d of the Laplace problem
 // becomes Cauchy densities 2d (real part) and 2d+1 (imaginary part); a dipole
-// (d, v) becomes -(v1 + i v2) times each part.
-__global__ void k_gather_split(int64_t n, int nd, const int* perm, const cplx* ch,
+// (d, v) becomes -(v1 + i v2) times each part.  With half = 1 every density
+// has zero imaginary part, the imaginary Cauchy densities vanish identically
+// and only the real parts are kept (Cauchy density d).
+__global__ void k_gather_split(int64_t n, int nd, int half, const int* perm, const cplx* ch,
                                const cplx* ds, const double* dv, cplx* och, cplx* ods) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t j = perm[i];
+  const int np = half ? 1 : 2;
   for (int d = 0; d < nd; ++d) {
+    const int64_t a = i * np * nd + np * d;
     if (ch) {
       const cplx c = ch[j * nd + d];
-      och[i * 2 * nd + 2 * d] = mkc(c.re);
-      och[i * 2 * nd + 2 * d + 1] = mkc(c.im);
+      och[a] = mkc(c.re);
+      if (!half) och[a + 1] = mkc(c.im);
     }
     if (ds) {
       const cplx s = ds[j * nd + d];
       const cplx v = cplx{-dv[nd * (2 * j) + d], -dv[nd * (2 * j + 1) + d]};
-      ods[i * 2 * nd + 2 * d] = s.re * v;
-      ods[i * 2 * nd + 2 * d + 1] = s.im * v;
+      ods[a] = s.re * v;
+      if (!half) ods[a + 1] = s.im * v;
     }
   }
 }
 
-// lfmm2d unpack (laplace/lfmm2d.f:169-223).  in: Cauchy outputs of 2nd
-// densities; pot(nd,n), grad(nd,2,n), hess(nd,3,n).
-__global__ void k_scatter_unpack(int64_t n, int nd, int pg, const int* perm, const cplx* p1,
-                                 const cplx* g1, const cplx* h1, cplx* pot, cplx* grad,
-                                 cplx* hess) {
+// flag = 1 if some value has a non-zero (or NaN) imaginary part
+__global__ void k_any_imag(int64_t n, const cplx* v, int* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double im = v[i].im;
+  if (!(im == 0.0)) *flag = 1;
+}
+
+// lfmm2d unpack (laplace/lfmm2d.f:169-223).  in: Cauchy outputs of the split
+// densities; pot(nd,n), grad(nd,2,n), hess(nd,3,n).  With half = 1 the
+// imaginary densities' outputs are the exact zeros they would sum to.
+__global__ void k_scatter_unpack(int64_t n, int nd, int half, int pg, const int* perm,
+                                 const cplx* p1, const cplx* g1, const cplx* h1, cplx* pot,
+                                 cplx* grad, cplx* hess) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t j = perm[i];
+  const int np = half ? 1 : 2;
+  const cplx z = mkc(0.0);
   for (int d = 0; d < nd; ++d) {
-    const int64_t a = i * 2 * nd + 2 * d;
-    pot[j * nd + d] = cplx{p1[a].re, p1[a + 1].re};
+    const int64_t a = i * np * nd + np * d;
+    const cplx pr = p1[a], pi = half ? z : p1[a + 1];
+    pot[j * nd + d] = cplx{pr.re, pi.re};
     if (pg >= 2) {
-      grad[nd * (2 * j) + d] = cplx{g1[a].re, g1[a + 1].re};
-      grad[nd * (2 * j + 1) + d] = cplx{-g1[a].im, -g1[a + 1].im};
+      const cplx gr = g1[a], gi = half ? z : g1[a + 1];
+      grad[nd * (2 * j) + d] = cplx{gr.re, gi.re};
+      grad[nd * (2 * j + 1) + d] = cplx{-gr.im, -gi.im};
     }
     if (pg >= 3) {
-      const cplx hxx = cplx{h1[a].re, h1[a + 1].re};
+      const cplx hr = h1[a], hi = half ? z : h1[a + 1];
+      const cplx hxx = cplx{hr.re, hi.re};
       hess[nd * (3 * j) + d] = hxx;
-      hess[nd * (3 * j + 1) + d] = cplx{-h1[a].im, -h1[a + 1].im};
+      hess[nd * (3 * j + 1) + d] = cplx{-hr.im, -hi.im};
       hess[nd * (3 * j + 2) + d] = -hxx;
     }
   }
@@ -470,9 +488,47 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
 }
 
 // ---------------------------------------------------------- near field
-// cfmm2dpart_direct (laprouts2d.f -> cfmm2d.f:1450-1616): list-1 sources of
-// the point's leaf through the c2d_direct* variant matching the flags.
+// Mixed-precision form of pair_cauchy (p2p_math.h): the displacement and
+// the exclusion test stay FP64 (so near-coincident pairs and the thresh rule
+// are exactly those of the FP64 path); log|z|, 1/z, -1/z^2, 2/z^3 and their
+// products with the strengths are FP32; the sums are accumulated in FP64.
 template <int IFC, int IFD, int PG, int NDT>
+__device__ __forceinline__ void pair_cauchy_f32(double dx, double dy, const SrcStr<NDT>& s,
+                                                double thresh, double thresh2,
+                                                Acc<CAUCHY, PG, NDT>& a) {
+  const double rr = dx * dx + dy * dy;
+  double absz = 0.0;
+  if (!IFC) absz = hypot(dx, dy);
+  if (skip_pair<CAUCHY, IFC, IFD, PG>(rr, thresh, thresh2, absz)) return;
+  const float fx = (float)dx, fy = (float)dy;
+  const float frr = fx * fx + fy * fy;
+  const float ri = 1.0f / frr;
+  const float z1r = fx * ri, z1i = -fy * ri;                          // 1/z
+  const float z2r = z1i * z1i - z1r * z1r, z2i = -2.0f * z1r * z1i;   // -1/z^2
+  const float z3r = -2.0f * (z2r * z1r - z2i * z1i);                 // 2/z^3
+  const float z3i = -2.0f * (z2r * z1i + z2i * z1r);
+  const float lg = IFC ? 0.5f * logf(frr) : 0.0f;
+#pragma unroll
+  for (int d = 0; d < NDT; ++d) {
+    if (IFC) {
+      const float cr = (float)s.charge[d].re, ci = (float)s.charge[d].im;
+      a.pot[d] += cplx{(double)(lg * cr), (double)(lg * ci)};
+      if (PG >= 2) a.grad[d][0] += cplx{(double)(z1r * cr - z1i * ci), (double)(z1r * ci + z1i * cr)};
+      if (PG >= 3) a.hess[d][0] += cplx{(double)(z2r * cr - z2i * ci), (double)(z2r * ci + z2i * cr)};
+    }
+    if (IFD) {
+      const float cr = (float)s.dipstr[d].re, ci = (float)s.dipstr[d].im;
+      a.pot[d] += cplx{(double)(z1r * cr - z1i * ci), (double)(z1r * ci + z1i * cr)};
+      if (PG >= 2) a.grad[d][0] += cplx{(double)(z2r * cr - z2i * ci), (double)(z2r * ci + z2i * cr)};
+      if (PG >= 3) a.hess[d][0] += cplx{(double)(z3r * cr - z3i * ci), (double)(z3r * ci + z3i * cr)};
+    }
+  }
+}
+
+// cfmm2dpart_direct (laprouts2d.f -> cfmm2d.f:1450-1616): list-1 sources of
+// the point's leaf through the c2d_direct* variant matching the flags (F32:
+// the mixed-precision pair above).
+template <int IFC, int IFD, int PG, int NDT, int F32>
 __global__ void k_near(int64_t n, int nd, int d0, const double* xy, const int* leaf,
                        const int* n1, const int* l1, const int* sse, const double* xs,
                        const cplx* chs, const cplx* dps, double thresh, cplx* pot, cplx* grad,
@@ -495,7 +551,11 @@ __global__ void k_near(int64_t n, int nd, int d0, const double* xy, const int* l
         if (IFC) s.charge[dd] = ok ? chs[(int64_t)j * nd + d0 + dd] : mkc(0.0);
         if (IFD) s.dipstr[dd] = ok ? dps[(int64_t)j * nd + d0 + dd] : mkc(0.0);
       }
-      pair_cauchy<IFC, IFD, PG, NDT>(x - xs[2 * j], y - xs[2 * j + 1], s, thresh, thresh2, a);
+      if (F32)
+        pair_cauchy_f32<IFC, IFD, PG, NDT>(x - xs[2 * j], y - xs[2 * j + 1], s, thresh, thresh2,
+                                           a);
+      else
+        pair_cauchy<IFC, IFD, PG, NDT>(x - xs[2 * j], y - xs[2 * j + 1], s, thresh, thresh2, a);
     }
   }
 #pragma unroll
@@ -562,36 +622,58 @@ void make_matrices(int P, std::vector<cplx>& A, std::vector<cplx>& B, std::vecto
     }
 }
 
-template <int PG, int NDT>
+template <int PG, int NDT, int F32>
 void launch_near_pg(int ifc, int ifd, int64_t n, hipStream_t st, int nd, int d0, const double* xy,
                     const int* leaf, const Lists& L, const int* sse, const double* xs,
                     const cplx* chs, const cplx* dps, double thresh, cplx* p, cplx* g, cplx* h) {
   if (ifc && !ifd)
-    launch1d_xcd(k_near<1, 0, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
-             thresh, p, g, h);
+    launch1d_xcd(k_near<1, 0, PG, NDT, F32>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs,
+                 chs, dps, thresh, p, g, h);
   else if (!ifc && ifd)
-    launch1d_xcd(k_near<0, 1, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
-             thresh, p, g, h);
+    launch1d_xcd(k_near<0, 1, PG, NDT, F32>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs,
+                 chs, dps, thresh, p, g, h);
   else
-    launch1d_xcd(k_near<1, 1, PG, NDT>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs, dps,
-             thresh, p, g, h);
+    launch1d_xcd(k_near<1, 1, PG, NDT, F32>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs,
+                 chs, dps, thresh, p, g, h);
 }
 
-void launch_near(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd, const double* xy,
-                 const int* leaf, const Lists& L, const int* sse, const double* xs,
-                 const cplx* chs, const cplx* dps, double thresh, cplx* p, cplx* g, cplx* h) {
+template <int F32>
+void launch_near_f(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd,
+                   const double* xy, const int* leaf, const Lists& L, const int* sse,
+                   const double* xs, const cplx* chs, const cplx* dps, double thresh, cplx* p,
+                   cplx* g, cplx* h) {
   const int NDT = (nd == 1) ? 1 : 2;
   for (int d0 = 0; d0 < nd; d0 += NDT) {
+#define FMM2D_NEAR(PG, NDT_)                                                                 \
+  launch_near_pg<PG, NDT_, F32>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, \
+                                p, g, h)
     if (NDT == 1) {
-      if (pg == 1) launch_near_pg<1, 1>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
-      else if (pg == 2) launch_near_pg<2, 1>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
-      else launch_near_pg<3, 1>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+      if (pg == 1) FMM2D_NEAR(1, 1);
+      else if (pg == 2) FMM2D_NEAR(2, 1);
+      else FMM2D_NEAR(3, 1);
     } else {
-      if (pg == 1) launch_near_pg<1, 2>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
-      else if (pg == 2) launch_near_pg<2, 2>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
-      else launch_near_pg<3, 2>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+      if (pg == 1) FMM2D_NEAR(1, 2);
+      else if (pg == 2) FMM2D_NEAR(2, 2);
+      else FMM2D_NEAR(3, 2);
     }
+#undef FMM2D_NEAR
   }
+}
+
+void launch_near(bool f32, int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd,
+                 const double* xy, const int* leaf, const Lists& L, const int* sse,
+                 const double* xs, const cplx* chs, const cplx* dps, double thresh, cplx* p,
+                 cplx* g, cplx* h) {
+  if (f32)
+    launch_near_f<1>(pg, ifc, ifd, n, st, nd, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+  else
+    launch_near_f<0>(pg, ifc, ifd, n, st, nd, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+}
+
+// mixed precision: FP32 copy of the multipole expansions for the FP32 M2L
+__global__ void k_to_f32(int64_t n, const double* in, float* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
 }
 
 void launch_eval(int pg, int64_t n, hipStream_t st, int nd, int P, const double* xy,
@@ -624,13 +706,28 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
   if (a.ns > INT32_MAX || a.nt > INT32_MAX) return FMM2D_ERR_ARG;
   if (!a.ifcharge && !a.ifdipole) return FMM2D_OK;
   const fmm2d_opts* o = a.opts;
-  if (o && o->precision != 0) return FMM2D_ERR_UNSUPPORTED;
+  if (o && (o->precision < 0 || o->precision > 1)) return FMM2D_ERR_ARG;
+  // precision 1: FP32 M2L on MFMA and FP32 near-field kernels with FP64 sums;
+  // tree, expansions, P2M/L2P, M2M/L2L stay FP64
+  const bool mixed = o && o->precision == 1;
   try {
     double tinfo[16] = {0};
     Timer tm(o != nullptr, st);
     const int64_t ns = a.ns;
     const int64_t nt = (a.ifpghtarg > 0) ? a.nt : 0;
-    const int nd = (kind == 1) ? 2 * a.nd : a.nd;  // Cauchy densities
+    // lfmm2d: 2 Cauchy densities per density, or 1 when all are real
+    int half = 0;
+    if (kind == 1) {
+      DBuf<int> flag(1, st);
+      flag.zero();
+      if (a.ifcharge) launch1d(k_any_imag, a.ns * a.nd, st, a.charge, flag.p);
+      if (a.ifdipole) launch1d(k_any_imag, a.ns * a.nd, st, a.dipstr, flag.p);
+      int h = 0;
+      hip_throw(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
+      hip_throw(hipStreamSynchronize(st), "sync");
+      half = (h == 0) ? 1 : 0;
+    }
+    const int nd = (kind == 1 && !half) ? 2 * a.nd : a.nd;  // Cauchy densities
     int ndiv = lndiv2d(a.eps, ns, nt);
     int idivflag = 0;
     if (o && o->ndiv > 0) {
@@ -667,7 +764,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       if (a.ifcharge) launch1d(k_gather_dens, ns, st, nd, t.isrc.p, a.charge, chs.p);
       if (a.ifdipole) launch1d(k_gather_dens, ns, st, nd, t.isrc.p, a.dipstr, dps.p);
     } else {
-      launch1d(k_gather_split, ns, st, a.nd, t.isrc.p, a.ifcharge ? a.charge : nullptr,
+      launch1d(k_gather_split, ns, st, a.nd, half, t.isrc.p, a.ifcharge ? a.charge : nullptr,
                a.ifdipole ? a.dipstr : nullptr, a.dipvec, chs.p, dps.p);
     }
     DBuf<int> leaf_s(ns, st), leaf_t(std::max<int64_t>(nt, 1), st);
@@ -774,11 +871,23 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
         std::vector<TrGroup> groups;
         for (int q = 0; q < 4; ++q)
           groups.push_back(TrGroup{m2l.cls_seg[q], m2l.cls_seg[q + 1], 0, q});
-        tm.mark(3);  // planning
-        launch_translate(0, 1, nd, TrBufs{mp.p, 0, P, loc.p, 0, P}, kSlots, groups,
-                         m2l.tlist.p, m2l.slots.p, d_soff.p, d_af.p, t.ilevel.p, d_band.p,
-                         d_logr.p, st);
-        tm.mark(13);  // the M2L kernel alone
+        if (mixed) {
+          // FP32 operators and an FP32 copy of the multipoles
+          DBuf<float> af32 = upload(swizzle_ops<float>(P, T, kOff * kOff), st);
+          DBuf<float> mp32(2 * nexp, st);
+          launch1d(k_to_f32, 2 * nexp, st, reinterpret_cast<const double*>(mp.p), mp32.p);
+          tm.mark(3);  // planning
+          launch_translate(0, 1, nd, TrBufs{mp32.p, 0, P, loc.p, 0, P}, kSlots, groups,
+                           m2l.tlist.p, m2l.slots.p, d_soff.p, af32.p, t.ilevel.p, d_band.p,
+                           d_logr.p, st, true);
+          tm.mark(13);  // the M2L kernel alone
+        } else {
+          tm.mark(3);  // planning
+          launch_translate(0, 1, nd, TrBufs{mp.p, 0, P, loc.p, 0, P}, kSlots, groups,
+                           m2l.tlist.p, m2l.slots.p, d_soff.p, d_af.p, t.ilevel.p, d_band.p,
+                           d_logr.p, st);
+          tm.mark(13);  // the M2L kernel alone
+        }
       }
     }
     tm.mark(3);
@@ -801,10 +910,10 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     if (ifnear) {
       const double thresh = t.boxsize[0] * std::ldexp(1.0, -51);
       if (pg > 0)
-        launch_near(pg, a.ifcharge, a.ifdipole, ns, st, nd, xs.p, leaf_s.p, L, sse, xs.p, chs.p,
+        launch_near(mixed, pg, a.ifcharge, a.ifdipole, ns, st, nd, xs.p, leaf_s.p, L, sse, xs.p, chs.p,
                     dps.p, thresh, ps.p, gs.p, hs.p);
       if (pgt > 0)
-        launch_near(pgt, a.ifcharge, a.ifdipole, nt, st, nd, xt.p, leaf_t.p, L, sse, xs.p,
+        launch_near(mixed, pgt, a.ifcharge, a.ifdipole, nt, st, nd, xt.p, leaf_t.p, L, sse, xs.p,
                     chs.p, dps.p, thresh, pt.p, gt.p, ht.p);
     }
     tm.mark(7);
@@ -818,10 +927,10 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       if (pgt >= 3) launch1d(k_scatter, nt, st, nd, t.itarg.p, ht.p, a.hesstarg);
     } else {
       if (pg >= 1)
-        launch1d(k_scatter_unpack, ns, st, a.nd, pg, t.isrc.p, ps.p, gs.p, hs.p, a.pot, a.grad,
+        launch1d(k_scatter_unpack, ns, st, a.nd, half, pg, t.isrc.p, ps.p, gs.p, hs.p, a.pot, a.grad,
                  a.hess);
       if (pgt >= 1)
-        launch1d(k_scatter_unpack, nt, st, a.nd, pgt, t.itarg.p, pt.p, gt.p, ht.p, a.pottarg,
+        launch1d(k_scatter_unpack, nt, st, a.nd, half, pgt, t.itarg.p, pt.p, gt.p, ht.p, a.pottarg,
                  a.gradtarg, a.hesstarg);
     }
     tm.mark(10);

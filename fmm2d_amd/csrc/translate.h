@@ -42,12 +42,38 @@ struct TrTable {
 // loop is unrolled to KSU steps (operators of at most 16 RT rows and 4 KSU
 // columns, the whole operator in one pass), with KSU == 0 it is a runtime
 // loop and the row tiles are covered in chunks of RT.
-template <int RT, int KSU, int MODE, int LOGT>
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 MFMA step in the element type T (double: v_mfma_f64_16x16x4f64,
+// float: v_mfma_f32_16x16x4f32), and the output row of accumulator element i
+// for lane group kr: the two instructions lay their 16x16 results out
+// differently (f64: row kr + 4 i, f32: row 4 kr + i).
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4v acc_t;
+  static __device__ __forceinline__ acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int kr, int i) { return kr + 4 * i; }
+};
+template <> struct Mfma<float> {
+  typedef f4v acc_t;
+  static __device__ __forceinline__ acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int kr, int i) { return 4 * kr + i; }
+};
+
+// T is the arithmetic type of the operator (Afrag) and the source
+// expansions (src, 2 Pin values of T per box and density); the results are
+// added to the FP64 destination expansions.
+template <typename T, int RT, int KSU, int MODE, int LOGT>
 __global__ __launch_bounds__(256) void k_translate(
     int nd, int Pin, int Pout, int nslots, int64_t nwaves, int per_xcd, TrTable tab,
-    const int* tlist,
-    const int* slots, const int* slot_op, const double* Afrag, int64_t afs, int sbox0, int dbox0,
-    const int* ilevel, const int2* band, const double* logr, const cplx* src, cplx* dst) {
+    const int* tlist, const int* slots, const int* slot_op, const T* Afrag, int64_t afs,
+    int sbox0, int dbox0, const int* ilevel, const int2* band, const double* logr, const T* src,
+    cplx* dst) {
+  typedef typename Mfma<T>::acc_t acc_t;
   // Workgroups are dispatched round-robin over the 8 XCDs; block b works on
   // logical block (b % 8) * per_xcd + b / 8 so that each XCD walks a
   // contiguous range of targets, whose neighbouring source boxes then stay
@@ -70,30 +96,29 @@ __global__ __launch_bounds__(256) void k_translate(
   const int ob = tab.opbase[g];
   for (int rc = 0; rc < RTT; rc += RT) {
     if (MODE == 0) {
-      d4v acc[RT];
+      acc_t acc[RT];
 #pragma unroll
-      for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
-      double m0 = 0.0;
+      for (int r = 0; r < RT; ++r) acc[r] = acc_t{0, 0, 0, 0};
+      T m0 = 0;
       for (int s = 0; s < nslots; ++s) {
         const int c = valid ? slots[tpos * nslots + s] : -1;
-        const double* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
-        const double* M = nullptr;
+        const T* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
+        const T* M = nullptr;
         int klo = 0, khi = 0;
         if (c >= 0) {
-          M = reinterpret_cast<const double*>(src + ((int64_t)(c - sbox0) * nd + d) * Pin);
+          M = src + ((int64_t)(c - sbox0) * nd + d) * 2 * Pin;
           const int2 bd = band ? band[ilevel[c]] : full_in;
           klo = bd.x;
           khi = bd.y;
         }
         auto step = [&](int ks) {
           const int k = 4 * ks + kr;
-          const double b = (k >= klo && k < khi) ? M[k] : 0.0;
+          const T b = (k >= klo && k < khi) ? M[k] : T(0);
           if (LOGT && rc == 0 && ks == 0 && lane < 32) m0 += b;
 #pragma unroll
           for (int r = 0; r < RT; ++r)
             if (RT == 1 || KSU > 0 || rc + r < RTT)
-              acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                  A[((int64_t)(rc + r) * KS + ks) * 64 + lane], b, acc[r], 0, 0, 0);
+              acc[r] = Mfma<T>::step(A[((int64_t)(rc + r) * KS + ks) * 64 + lane], b, acc[r]);
         };
         if (KSU > 0) {
 #pragma unroll
@@ -104,21 +129,21 @@ __global__ __launch_bounds__(256) void k_translate(
         }
       }
       if (valid) {
-        if (LOGT && rc == 0 && lane < 32) acc[0][0] += logr[ilevel[tb]] * m0;
         const int2 bd = band ? band[ilevel[tb]] : full_out;
         double* L = reinterpret_cast<double*>(dst + ((int64_t)(tb - dbox0) * nd + d) * Pout);
+        // level-dependent part of L_0: lanes 0..15 hold Re M_0 (row 0), lanes
+        // 16..31 Im M_0 (row 1) of their column
+        if (LOGT && rc == 0 && lane < 32) L[kr] += logr[ilevel[tb]] * (double)m0;
 #pragma unroll
         for (int r = 0; r < RT; ++r)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const int row = 16 * (rc + r) + kr + 4 * i;
-            if (rc + r < RTT && row >= bd.x && row < bd.y) L[row] += acc[r][i];
+            const int row = 16 * (rc + r) + Mfma<T>::row(kr, i);
+            if (rc + r < RTT && row >= bd.x && row < bd.y) L[row] += (double)acc[r][i];
           }
       }
     } else {
-      const double* M =
-          valid ? reinterpret_cast<const double*>(src + ((int64_t)(tb - sbox0) * nd + d) * Pin)
-                : nullptr;
+      const T* M = valid ? src + ((int64_t)(tb - sbox0) * nd + d) * 2 * Pin : nullptr;
       int klo = 0, khi = 0;
       if (valid) {
         const int2 bd = band ? band[ilevel[tb]] : full_in;
@@ -127,26 +152,25 @@ __global__ __launch_bounds__(256) void k_translate(
       }
       // the source coefficients are reused by every slot: keep them in
       // registers when the k loop is unrolled
-      double bv[KSU > 0 ? KSU : 1];
+      T bv[KSU > 0 ? KSU : 1];
       if (KSU > 0) {
 #pragma unroll
         for (int ks = 0; ks < (KSU > 0 ? KSU : 1); ++ks) {
           const int k = 4 * ks + kr;
-          bv[ks] = (ks < KS && k >= klo && k < khi) ? M[k] : 0.0;
+          bv[ks] = (ks < KS && k >= klo && k < khi) ? M[k] : T(0);
         }
       }
       for (int s = 0; s < nslots; ++s) {
         const int c = valid ? slots[tpos * nslots + s] : -1;
-        const double* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
-        d4v acc[RT];
+        const T* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
+        acc_t acc[RT];
 #pragma unroll
-        for (int r = 0; r < RT; ++r) acc[r] = d4v{0.0, 0.0, 0.0, 0.0};
-        auto step = [&](int ks, double b) {
+        for (int r = 0; r < RT; ++r) acc[r] = acc_t{0, 0, 0, 0};
+        auto step = [&](int ks, T b) {
 #pragma unroll
           for (int r = 0; r < RT; ++r)
             if (RT == 1 || KSU > 0 || rc + r < RTT)
-              acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                  A[((int64_t)(rc + r) * KS + ks) * 64 + lane], b, acc[r], 0, 0, 0);
+              acc[r] = Mfma<T>::step(A[((int64_t)(rc + r) * KS + ks) * 64 + lane], b, acc[r]);
         };
         if (KSU > 0) {
 #pragma unroll
@@ -155,7 +179,7 @@ __global__ __launch_bounds__(256) void k_translate(
         } else {
           for (int ks = 0; ks < KS; ++ks) {
             const int k = 4 * ks + kr;
-            step(ks, (k >= klo && k < khi) ? M[k] : 0.0);
+            step(ks, (k >= klo && k < khi) ? M[k] : T(0));
           }
         }
         if (c >= 0) {
@@ -165,8 +189,8 @@ __global__ __launch_bounds__(256) void k_translate(
           for (int r = 0; r < RT; ++r)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              const int row = 16 * (rc + r) + kr + 4 * i;
-              if (rc + r < RTT && row >= bd.x && row < bd.y) L[row] += acc[r][i];
+              const int row = 16 * (rc + r) + Mfma<T>::row(kr, i);
+              if (rc + r < RTT && row >= bd.x && row < bd.y) L[row] += (double)acc[r][i];
             }
         }
       }
@@ -176,9 +200,10 @@ __global__ __launch_bounds__(256) void k_translate(
 
 // Real interleaved form of square complex operators ops[o][j][n] (input j,
 // output n), swizzled as Afrag[o][row tile][k step][lane].
-inline std::vector<double> swizzle_ops(int P, const std::vector<cplx>& ops, int nops) {
+template <typename T = double>
+inline std::vector<T> swizzle_ops(int P, const std::vector<cplx>& ops, int nops) {
   const int K2 = 2 * P, RTT = (K2 + 15) / 16, KS = (K2 + 3) / 4;
-  std::vector<double> out((size_t)nops * RTT * KS * 64, 0.0);
+  std::vector<T> out((size_t)nops * RTT * KS * 64, T(0));
   for (int o = 0; o < nops; ++o)
     for (int r = 0; r < RTT; ++r)
       for (int ks = 0; ks < KS; ++ks)
@@ -190,7 +215,7 @@ inline std::vector<double> swizzle_ops(int P, const std::vector<cplx>& ops, int 
             const int a = row & 1, b = col & 1;
             v = (a == b) ? t.re : (a == 0 ? -t.im : t.im);
           }
-          out[(((size_t)o * RTT + r) * KS + ks) * 64 + l] = v;
+          out[(((size_t)o * RTT + r) * KS + ks) * 64 + l] = (T)v;
         }
   return out;
 }
@@ -208,21 +233,28 @@ struct TrGroup {
 
 // Launch translate passes over the given groups (at most kTrMaxGroups per
 // launch; launches are stream-ordered, so groups must not depend on each other).
+// Buffers of one translate launch.  src holds 2 Pin values per (box,
+// density) of the launch's element type (double: the cplx expansions).
 struct TrBufs {
-  const cplx* src;
+  const void* src;
   int sbox0, Pin;
   cplx* dst;
   int dbox0, Pout;
 };
 
+// Launch translate passes over the given groups (at most kTrMaxGroups per
+// launch; launches are stream-ordered, so groups must not depend on each other).
+// fp32: operators (Afrag) and sources are float, accumulation FP32 on
+// v_mfma_f32_16x16x4f32 (gather mode only); otherwise FP64.
 inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int nslots,
                              const std::vector<TrGroup>& groups, const int* tlist,
-                             const int* slots, const int* slot_op, const double* Afrag,
+                             const int* slots, const int* slot_op, const void* Afrag,
                              const int* ilevel, const int2* band, const double* logr,
-                             hipStream_t st) {
+                             hipStream_t st, bool fp32 = false) {
   const int Pin = io.Pin, Pout = io.Pout;
   const int64_t afs = afrag_stride(Pin, Pout);
   const int RTT = (2 * Pout + 15) / 16, KS = (2 * Pin + 3) / 4;
+  if (fp32 && mode != 0) throw HipError{hipErrorInvalidValue, "translate fp32 scatter"};
   size_t gi = 0;
   while (gi < groups.size()) {
     TrTable tab{};
@@ -242,25 +274,31 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     tab.tile0[tab.ng] = tiles;
     const int nblk = (tiles + 3) / 4, per_xcd = (nblk + 7) / 8;
     const dim3 grid((unsigned)(8 * per_xcd));
-#define FMM2D_TR(RT, KSU, MODE, LOGT)                                                           \
-  hipLaunchKernelGGL((k_translate<RT, KSU, MODE, LOGT>), grid, dim3(256), 0, st, nd, Pin, Pout, \
-                     nslots, (int64_t)tiles, per_xcd, tab, tlist, slots, slot_op, Afrag, afs,    \
-                     io.sbox0,                                                                   \
-                     io.dbox0, ilevel, band, logr, io.src, io.dst)
-#define FMM2D_TR_RT(MODE, LOGT)                 \
-  switch (KS <= 4 * RTT ? RTT : 0) {            \
-    case 1: FMM2D_TR(1, 4, MODE, LOGT); break;  \
-    case 2: FMM2D_TR(2, 8, MODE, LOGT); break;  \
-    case 3: FMM2D_TR(3, 12, MODE, LOGT); break; \
-    case 4: FMM2D_TR(4, 16, MODE, LOGT); break; \
-    default: FMM2D_TR(4, 0, MODE, LOGT); break; \
+#define FMM2D_TR(T, RT, KSU, MODE, LOGT)                                                        \
+  hipLaunchKernelGGL((k_translate<T, RT, KSU, MODE, LOGT>), grid, dim3(256), 0, st, nd, Pin,    \
+                     Pout, nslots, (int64_t)tiles, per_xcd, tab, tlist, slots, slot_op,         \
+                     static_cast<const T*>(Afrag), afs, io.sbox0, io.dbox0, ilevel, band, logr, \
+                     static_cast<const T*>(io.src), io.dst)
+#define FMM2D_TR_RT(T, MODE, LOGT)                 \
+  switch (KS <= 4 * RTT ? RTT : 0) {               \
+    case 1: FMM2D_TR(T, 1, 4, MODE, LOGT); break;  \
+    case 2: FMM2D_TR(T, 2, 8, MODE, LOGT); break;  \
+    case 3: FMM2D_TR(T, 3, 12, MODE, LOGT); break; \
+    case 4: FMM2D_TR(T, 4, 16, MODE, LOGT); break; \
+    default: FMM2D_TR(T, 4, 0, MODE, LOGT); break; \
   }
-    if (mode == 0 && logt) {
-      FMM2D_TR_RT(0, 1)
+    if (fp32) {
+      if (logt) {
+        FMM2D_TR_RT(float, 0, 1)
+      } else {
+        FMM2D_TR_RT(float, 0, 0)
+      }
+    } else if (mode == 0 && logt) {
+      FMM2D_TR_RT(double, 0, 1)
     } else if (mode == 0) {
-      FMM2D_TR_RT(0, 0)
+      FMM2D_TR_RT(double, 0, 0)
     } else {
-      FMM2D_TR_RT(1, 0)
+      FMM2D_TR_RT(double, 1, 0)
     }
 #undef FMM2D_TR_RT
 #undef FMM2D_TR

@@ -211,6 +211,23 @@ __global__ void k_total(int n, const int* flag, const int* offs, int* total) {
   *total = (n > 0) ? offs[n - 1] + flag[n - 1] : 0;
 }
 
+// box 1 (index 0): the root, holding every point
+__global__ void k_init_root(double cx, double cy, int ns, int nt, double* centers, int* ilevel,
+                            int* iparent, int* nchild, int* ichild, int* ses, int* set,
+                            int* iflag) {
+  centers[0] = cx;
+  centers[1] = cy;
+  ilevel[0] = 0;
+  iparent[0] = -1;
+  nchild[0] = 0;
+  for (int q = 0; q < 4; ++q) ichild[q] = -1;
+  ses[0] = 0;
+  ses[1] = ns;
+  set[0] = 0;
+  set[1] = nt;
+  iflag[0] = 0;
+}
+
 // cum[k] = exclusive-scan value at segment start cut[k]; cut[K] = n, the
 // scanned length, gives the grand total
 __global__ void k_seg_totals(int K, const int* cut, const int* flag, const int* offs, int* cum) {
@@ -626,20 +643,9 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   size_t cap0 = std::max<size_t>(1024, (size_t)(8 * (ns + nt) / std::max(ndiv, 1)));
   bld.B.alloc(cap0, st);
   bld.nboxes = 1;
-  {
-    const int none4[4] = {-1, -1, -1, -1};
-    const int zero = 0, mone = -1;
-    const int se[4] = {0, (int)ns, 0, (int)nt};
-    hip_throw(hipMemcpyAsync(bld.B.centers.p, root, sizeof(root), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipMemcpyAsync(bld.B.ilevel.p, &zero, sizeof(int), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipMemcpyAsync(bld.B.iparent.p, &mone, sizeof(int), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipMemcpyAsync(bld.B.nchild.p, &zero, sizeof(int), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipMemcpyAsync(bld.B.ichild.p, none4, sizeof(none4), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipMemcpyAsync(bld.B.ses.p, &se[0], 2 * sizeof(int), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipMemcpyAsync(bld.B.set.p, &se[2], 2 * sizeof(int), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipMemcpyAsync(bld.B.iflag.p, &zero, sizeof(int), hipMemcpyHostToDevice, st), "c");
-    hip_throw(hipStreamSynchronize(st), "sync");
-  }
+  hipLaunchKernelGGL(k_init_root, dim3(1), dim3(1), 0, st, root[0], root[1], (int)ns, (int)nt,
+                     bld.B.centers.p, bld.B.ilevel.p, bld.B.iparent.p, bld.B.nchild.p,
+                     bld.B.ichild.p, bld.B.ses.p, bld.B.set.p, bld.B.iflag.p);
 
   // ---- level-by-level refinement (pts_tree2d.f:551-631)
   std::vector<int> ls{0}, le{1};

@@ -1,0 +1,118 @@
+"""Learned parameter proposer for the FMM drivers.
+
+A perceptron 20 -> 64 -> 64 -> 17 (tanh, tanh, linear; FP32) run and trained
+by the library's own device kernels (fmm2d_mlp_forward / _loss_grad / _adam,
+csrc/mlp.hip).  It maps a problem to expansion orders per level and a leaf
+capacity; the drivers take them through fmm2d_opts.nterms / ndiv.  The
+reference's rules (laplace/l2dterms.f:38, helmholtz/h2dterms.f:35,
+laplace/lndiv2d.f) stay the fallback: a proposal whose error against the
+FP64 direct sum on sampled points exceeds eps is rejected.
+
+Inputs (20): log10 N / 7, log10(1 + |zk| boxsize0) / 3, -log10 eps / 15, a
+Helmholtz flag, and the fraction of points held by leaves at levels 0..15 of
+the tree built with the default leaf capacity.
+Outputs (17): r_l, l = 0..15, the fractional reduction of level l's default
+order (nterms_l = max(2, round(p_l (1 - r_l))), r_l clipped to [0, 0.7]), and
+log2 of the leaf capacity relative to the default one (clipped to [-1.5, 2]).
+"""
+import os
+
+import numpy as np
+
+from . import _native as nat
+
+NIN, NOUT, NLEV = 20, 17, 16
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "proposer.npz")
+
+
+def default_ndiv(eps, ns, nt):
+    """lndiv2d / hndiv2d (laplace/lndiv2d.f)."""
+    import ctypes as C
+    L = nat.lib()
+    nd, idf = C.c_int(0), C.c_int(0)
+    nat.check(L.fmm2d_ndiv(float(eps), int(ns), int(nt), 0, C.byref(nd), C.byref(idf)), "ndiv")
+    return nd.value
+
+
+def default_nterms(kind, eps, boxsize, zk=None):
+    """Per-level orders of the reference's rules for the given box sizes."""
+    import ctypes as C
+    L = nat.lib()
+    out = []
+    for bs in boxsize:
+        v = C.c_int(0)
+        if kind == "h":
+            z = complex(zk)
+            nat.check(L.fmm2d_h2dterms(float(bs), z.real, z.imag, float(eps), C.byref(v)),
+                      "h2dterms")
+        else:
+            nat.check(L.fmm2d_l2dterms(float(eps), C.byref(v)), "l2dterms")
+        out.append(v.value)
+    return out
+
+
+def tree_features(sources, targets, eps, zk, kind, ndiv):
+    """Feature vector (NIN floats) and the default tree's box sizes."""
+    from .tree import build_tree
+    src = np.asarray(sources, dtype=np.float64)
+    ns = src.shape[1]
+    nt = 0 if targets is None else np.asarray(targets).shape[1]
+    tr = build_tree(src, targets, ndiv=ndiv)
+    leaf = tr["nchild"] == 0
+    cnt = (tr["isrcse"][1] - tr["isrcse"][0] + 1) + (tr["itargse"][1] - tr["itargse"][0] + 1)
+    lev = np.minimum(tr["ilevel"], NLEV - 1)
+    hist = np.bincount(lev[leaf], weights=cnt[leaf], minlength=NLEV)[:NLEV]
+    hist = hist / max(1.0, hist.sum())
+    kz = 0.0 if kind != "h" else abs(complex(zk)) * float(tr["boxsize"][0])
+    f = np.concatenate([[np.log10(ns + nt) / 7.0, np.log10(1.0 + kz) / 3.0,
+                         -np.log10(eps) / 15.0, 1.0 if kind == "h" else 0.0], hist])
+    return f.astype(np.float32), tr["boxsize"]
+
+
+def decode(y, p_default, ndiv_default):
+    """Network output -> (nterms per level, ndiv)."""
+    r = np.clip(np.asarray(y[:NLEV], dtype=np.float64), 0.0, 0.7)
+    nl = len(p_default)
+    rr = np.array([r[min(l, NLEV - 1)] for l in range(nl)])
+    nterms = [int(max(2, round(p * (1.0 - x)))) for p, x in zip(p_default, rr)]
+    e = float(np.clip(y[NLEV], -1.5, 2.0))
+    ndiv = int(np.clip(round(ndiv_default * 2.0 ** e), 4, 400))
+    return nterms, ndiv
+
+
+class Proposer:
+    """Holds the network parameters on the device."""
+
+    def __init__(self, params=None, path=WEIGHTS):
+        import torch
+        if params is None:
+            with np.load(path) as z:
+                params = z["params"]
+        self.L = nat.lib()
+        assert len(params) == self.L.fmm2d_mlp_nparams(NIN, NOUT)
+        self.params = torch.from_numpy(np.asarray(params, dtype=np.float32)).to(
+            nat.torch_device())
+
+    def predict(self, feats):
+        """Network outputs for a batch of feature rows (numpy [batch][NIN])."""
+        import torch
+        x = torch.from_numpy(np.ascontiguousarray(feats, dtype=np.float32)).to(
+            self.params.device)
+        b = x.shape[0]
+        y = torch.zeros((b, NOUT), dtype=torch.float32, device=x.device)
+        nat.check(self.L.fmm2d_mlp_forward(NIN, NOUT, nat.ptr(self.params), b, nat.ptr(x),
+                                           nat.ptr(y), nat.stream_ptr()), "mlp_forward")
+        return y.cpu().numpy()
+
+    def propose(self, kind, sources, eps, targets=None, zk=None):
+        """dict(nterms=[...per level of the proposed tree...], ndiv=k)."""
+        ns = np.asarray(sources).shape[1]
+        nt = 0 if targets is None else np.asarray(targets).shape[1]
+        nd0 = default_ndiv(eps, ns, nt)
+        f, bs = tree_features(sources, targets, eps, zk, kind, nd0)
+        y = self.predict(f[None, :])[0]
+        # orders for the levels of a tree up to 4 levels deeper than the default one
+        bsx = list(bs) + [bs[-1] / 2 ** k for k in range(1, 5)]
+        p0 = default_nterms(kind, eps, bsx, zk)
+        nterms, ndiv = decode(y, p0, nd0)
+        return {"nterms": nterms, "ndiv": ndiv}

@@ -1,0 +1,166 @@
+"""Generate training tuples with the FMM itself and train the proposer on the GPU.
+
+  python -m fmm2d_amd.train_proposer --problems 160 --out fmm2d_amd/data/proposer.npz
+
+For each random problem (Laplace lfmm2d or Helmholtz hfmm2d; points uniform in
+a square, Gaussian clusters or on the curve r = 1 + 0.3 cos 5t; N log-uniform;
+zk log-uniform in [1, 300]; eps in {1e-4, 1e-6, 1e-9}) the FMM runs on a grid
+of parameters -- order reduction r in {0, .1, ..., .5} at every level and leaf
+capacity 2^e times the default, e in {-1, 0, 1} -- and each run's GPU time
+(the driver's own event timing, timeinfo[11]) and relative L2 error of
+pot+grad against the FP64 direct sum at 256 sampled sources are recorded.
+The target of a problem is the fastest setting whose error is at most eps/2.
+The network is then fitted by the library's device kernels (Adam on the
+weighted squared error).  Every tuple (features, parameters, error, time) is
+saved next to the weights.
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import _native as nat
+from . import fmm2d as F
+from . import proposer as PR
+
+R_GRID = [0.0, 0.1, 0.2, 0.3, 0.4, 0.5]
+E_GRID = [-1, 0, 1]
+NSAMPLE = 256
+
+
+def make_points(rng, n, dist):
+    if dist == "uniform":
+        return rng.random((2, n))
+    if dist == "cluster":
+        k = int(rng.integers(4, 33))
+        cen = rng.uniform(0.1, 0.9, (2, k))
+        sig = np.exp(rng.uniform(np.log(1e-3), np.log(3e-2), k))
+        lab = rng.integers(0, k, n)
+        p = cen[:, lab] + sig[lab] * rng.standard_normal((2, n))
+        m = n // 10
+        p[:, :m] = rng.random((2, m))
+        return p
+    t = rng.uniform(0, 2 * np.pi, n)
+    r = 1 + 0.3 * np.cos(5 * t)
+    return np.stack([r * np.cos(t), r * np.sin(t)]) + 1e-4 * rng.standard_normal((2, n))
+
+
+def run_problem(rng):
+    kind = "h" if rng.random() < 0.5 else "l"
+    dist = ["uniform", "cluster", "curve"][int(rng.integers(0, 3))]
+    n = int(np.exp(rng.uniform(np.log(1e4), np.log(2e5))))
+    eps = float([1e-4, 1e-6, 1e-9][int(rng.integers(0, 3))])
+    zk = complex(np.exp(rng.uniform(0, np.log(300.0)))) if kind == "h" else None
+    src = make_points(rng, n, dist)
+    ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    idx = np.sort(rng.choice(n, NSAMPLE, replace=False))
+    (p, g, _), _ = F._direct(kind, zk, src, src[:, idx], ch, None, None, 2, 1, 1e-16)
+    ex = np.concatenate([p[0], g[0].ravel()])
+    nd0 = PR.default_ndiv(eps, n, 0)
+    feats, bs = PR.tree_features(src, None, eps, zk, kind, nd0)
+    bsx = list(bs) + [bs[-1] / 2 ** k for k in range(1, 5)]
+    p0 = PR.default_nterms(kind, eps, bsx, zk)
+    rows = []
+    for e in E_GRID:
+        for r in R_GRID:
+            y = np.zeros(PR.NOUT)
+            y[:PR.NLEV] = r
+            y[PR.NLEV] = e
+            nterms, ndiv = PR.decode(y, p0, nd0)
+            opts = {"nterms": nterms, "ndiv": ndiv, "timeinfo": np.zeros(16)}
+            F._fmm(kind, eps, zk, src, ch, None, None, None, 2, 0, 1, opts=dict(opts))
+            best = None
+            for _ in range(2):
+                res, ier = F._fmm(kind, eps, zk, src, ch, None, None, None, 2, 0, 1, opts=opts)
+                t = float(opts["timeinfo"][11])
+                best = t if best is None else min(best, t)
+            got = np.concatenate([res["pot"][idx], res["grad"][:, idx].ravel()])
+            err = float(np.linalg.norm(got - ex) / np.linalg.norm(ex))
+            rows.append({"r": r, "e": e, "time": best, "err": err, "ier": int(ier)})
+    ok = [w for w in rows if w["err"] <= 0.5 * eps and w["ier"] == 0]
+    tgt = min(ok, key=lambda w: w["time"]) if ok else {"r": 0.0, "e": 0}
+    default = [w for w in rows if w["r"] == 0.0 and w["e"] == 0][0]
+    return {"kind": kind, "dist": dist, "n": n, "eps": eps,
+            "zk": None if zk is None else [zk.real, zk.imag], "features": feats.tolist(),
+            "grid": rows, "target": [tgt["r"], tgt["e"]], "default": default,
+            "chosen": tgt}
+
+
+def fit(feats, targets, steps=3000, lr=3e-3, seed=0):
+    """Fit the network on the device (Glorot-uniform start, Adam on the
+    squared error); returns (params, [(step, loss)])."""
+    import torch
+    L = nat.lib()
+    dev = nat.torch_device()
+    n = L.fmm2d_mlp_nparams(PR.NIN, PR.NOUT)
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fi, fo in ((PR.NIN, 64), (64, 64), (64, PR.NOUT)):
+        a = np.sqrt(6.0 / (fi + fo))
+        parts.append(rng.uniform(-a, a, fi * fo))
+        parts.append(np.zeros(fo))
+    p = torch.from_numpy(np.concatenate(parts).astype(np.float32)).to(dev)
+    m = torch.zeros(n, dtype=torch.float32, device=dev)
+    v = torch.zeros_like(m)
+    g = torch.zeros_like(m)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    x = torch.from_numpy(np.asarray(feats, dtype=np.float32)).to(dev)
+    t = torch.from_numpy(np.asarray(targets, dtype=np.float32)).to(dev)
+    b = x.shape[0]
+    hist = []
+    for step in range(1, steps + 1):
+        nat.check(L.fmm2d_mlp_loss_grad(PR.NIN, PR.NOUT, nat.ptr(p), b, nat.ptr(x), nat.ptr(t),
+                                        None, nat.ptr(g), nat.ptr(loss), nat.stream_ptr()),
+                  "mlp_loss_grad")
+        nat.check(L.fmm2d_mlp_adam(n, nat.ptr(p), nat.ptr(g), nat.ptr(m), nat.ptr(v), lr, 0.9,
+                                   0.999, 1e-8, step, nat.stream_ptr()), "mlp_adam")
+        if step % 100 == 0 or step == 1:
+            hist.append((step, float(loss.cpu())))
+    return p.cpu().numpy(), hist
+
+
+def targets_of(problems):
+    y = np.zeros((len(problems), PR.NOUT), dtype=np.float32)
+    for i, pb in enumerate(problems):
+        y[i, :PR.NLEV] = pb["target"][0]
+        y[i, PR.NLEV] = pb["target"][1]
+    return y
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--problems", type=int, default=160)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--out", default=PR.WEIGHTS)
+    ap.add_argument("--data", default=None, help="reuse tuples from this JSON file")
+    args = ap.parse_args()
+    if args.data:
+        with open(args.data) as f:
+            problems = json.load(f)
+    else:
+        rng = np.random.default_rng(args.seed)
+        problems = []
+        t0 = time.time()
+        for i in range(args.problems):
+            problems.append(run_problem(rng))
+            pb = problems[-1]
+            print(f"problem {i}: {pb['kind']} {pb['dist']} n={pb['n']} eps={pb['eps']:.0e} "
+                  f"target r={pb['target'][0]} e={pb['target'][1]} "
+                  f"t={pb['chosen'].get('time', 0):.4f}s default={pb['default']['time']:.4f}s "
+                  f"({time.time() - t0:.0f}s)", flush=True)
+    feats = np.array([pb["features"] for pb in problems], dtype=np.float32)
+    params, hist = fit(feats, targets_of(problems), steps=args.steps)
+    for s, l in hist[-3:]:
+        print(f"step {s}: loss {l:.5f}", flush=True)
+    np.savez(args.out, params=params, nin=PR.NIN, nout=PR.NOUT)
+    with open(args.out.replace(".npz", "_tuples.json"), "w") as f:
+        json.dump(problems, f)
+    with open(args.out.replace(".npz", "_loss.json"), "w") as f:
+        json.dump(hist, f)
+    print("saved", args.out, flush=True)
+
+
+if __name__ == "__main__":
+    main()

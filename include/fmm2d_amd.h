@@ -49,7 +49,7 @@ typedef struct {
   int32_t ifnear;      /* 1: add the list-1 near field (reference default), 0: omit it */
   int32_t precision;   /* 0: FP64 throughout; 1: mixed FP32/FP64 */
   int32_t n_nterms;    /* entries in nterms; 0: reference rule (l2dterms / h2dterms) */
-  int32_t pad0;
+  int32_t count_work;  /* 1: fill work[], near_pairs, m2l_pairs (one extra counting pass) */
   const int32_t* nterms; /* HOST array: expansion order for levels 0..n_nterms-1 */
   /* out: seconds per stage.  [0..7] are the reference's timeinfo(1..8): P2M,
    * list-4 P2L, M2M, M2L, L2L, list-3 M2P, L2P, near field (list-3 and L2P
@@ -60,7 +60,7 @@ typedef struct {
   double timeinfo[16];
   int32_t nlevels, nboxes; /* out: tree depth and box count */
   int32_t pad1, pad2;
-  /* out: algorithmic work per stage, same slots as timeinfo: flops for
+  /* out (count_work = 1): algorithmic work per stage, same slots as timeinfo: flops for
    * [0..7] (P2P 12+12*nd per Cauchy pair for pot+grad, M2L 8(p+1)^2 nd per
    * list-2 pair, 8(p+1) nd per point for P2M/L2P, 8(p+1)^2 nd per M2M/L2L
    * edge), bytes for [8] (40 per point + 80 per box); 0 when not counted. */
@@ -180,6 +180,37 @@ void fmm2d_tree_free(void* handle);
 int fmm2d_l2dterms(double eps, int* nterms);
 int fmm2d_h2dterms(double bsize, double zk_re, double zk_im, double eps, int* nterms);
 int fmm2d_ndiv(double eps, int64_t ns, int64_t nt, int helmholtz, int* ndiv, int* idivflag);
+
+/*
+ * Return the FMM drivers' cached device workspace on the current device to
+ * the runtime (synchronises the device first).  No reference counterpart:
+ * the reference allocates and frees its workspace inside every call
+ * (laplace/cfmm2d.f:251-262, helmholtz/hfmm2d.f:354-371); here workspace
+ * blocks are cached per (device, stream) across calls and kept until this is
+ * called or the process ends.
+ */
+int fmm2d_release_workspace(void);
+
+/*
+ * Parameter-proposer network (no reference counterpart: the reference picks
+ * nterms and the leaf size by the fixed rules of laplace/l2dterms.f:38,
+ * helmholtz/h2dterms.f:35 and laplace/lndiv2d.f; a proposal reaches the
+ * drivers through fmm2d_opts.nterms / ndiv).  A perceptron nin -> 64 -> 64 ->
+ * nout with tanh, tanh, linear layers in FP32.  params is a device array of
+ * fmm2d_mlp_nparams(nin, nout) floats laid out W1[64][nin], b1[64],
+ * W2[64][64], b2[64], W3[nout][64], b3[nout]; x[batch][nin], y[batch][nout].
+ */
+int64_t fmm2d_mlp_nparams(int nin, int nout);
+int fmm2d_mlp_forward(int nin, int nout, const float* params, int batch, const float* x,
+                      float* y, void* stream);
+/* loss = sum_{b,o} weight[b][o] (y[b][o] - target[b][o])^2 / batch (weight NULL = 1) and
+ * its gradient with respect to params (grad: nparams floats); loss: 1 device float. */
+int fmm2d_mlp_loss_grad(int nin, int nout, const float* params, int batch, const float* x,
+                        const float* target, const float* weight, float* grad, float* loss,
+                        void* stream);
+/* Adam step (Kingma & Ba 2015) on n parameters; step counts from 1. */
+int fmm2d_mlp_adam(int64_t n, float* params, const float* grad, float* m, float* v, float lr,
+                   float beta1, float beta2, float eps, int step, void* stream);
 
 #ifdef __cplusplus
 }

@@ -194,8 +194,37 @@ def hfmm_fixtures():
     np.savez_compressed(os.path.join(HERE, "hfmm_ref.npz"), **out)
 
 
+# (eps, nd, ifc, ifd, pg, pgt) for lfmm2d with real charges and dipole
+# strengths (zero imaginary parts)
+LFMM_REAL_CASES = [
+    (1e-6, 1, 1, 0, 2, 2),
+    (1e-12, 1, 1, 1, 3, 3),
+    (1e-9, 1, 0, 1, 1, 2),
+    (1e-9, 2, 1, 1, 2, 0),
+]
+
+
+def lfmm_real_fixtures():
+    """lfmm2d outputs of the reference for real densities (same points)."""
+    base = np.load(os.path.join(HERE, "fmm_ref.npz"))
+    src, trg = base["src"], base["trg"]
+    out = {"src": src, "trg": trg}
+    for i, (eps, nd, ifc, ifd, pg, pgt) in enumerate(LFMM_REAL_CASES):
+        ch = base[f"nd{nd}_charge"].real + 0j
+        ds = base[f"nd{nd}_dipstr"].real + 0j
+        dv = base[f"nd{nd}_dipvec"]
+        r = R.lfmm2d(nd=nd, eps=eps, sources=src, charges=ch if ifc else None,
+                     dipstr=ds if ifd else None, dipvec=dv if ifd else None,
+                     targets=trg if pgt else None, ifpgh=pg, ifpghtarg=pgt)
+        for k, v in r.items():
+            if k in ("pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"):
+                out[f"c{i}_{k}"] = v
+        out[f"c{i}_ier"] = np.array(r["ier"])
+    np.savez_compressed(os.path.join(HERE, "lfmm_real_ref.npz"), **out)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm", "hfmm"]
+    which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm", "hfmm", "lreal"]
     if "direct" in which:
         direct_fixtures()
     if "hankel" in which:
@@ -206,5 +235,7 @@ if __name__ == "__main__":
         fmm_fixtures()
     if "hfmm" in which:
         hfmm_fixtures()
+    if "lreal" in which:
+        lfmm_real_fixtures()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     txt = open(os.path.join(ROOT, "include", "fmm2d_amd.h")).read()
-    return sorted(set(re.findall(r"^(?:int|void)\s+(fmm2d_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|int64_t|void)\s+(fmm2d_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_symbols_exported():

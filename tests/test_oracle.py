@@ -152,3 +152,43 @@ def test_oracle_detects_planted_mistakes(golden_dir):
     flipped = O.direct("l", 1, src, trg, None, -ds, dv, 2, 1e-14)["gradtarg"]
     swapped = O.direct("l", 1, src[::-1], trg[::-1], None, ds, dv, 2, 1e-14)["gradtarg"]
     assert _rel(flipped, ref) > 1.0 and _rel(swapped, ref) > 1e-2
+
+
+# ---------------------------------------------------------------- proposer network
+def test_mlp_oracle_gradient_by_finite_differences():
+    """oracle/mlp_ref.py: the analytic gradient of the weighted squared-error
+    loss equals central differences of the loss (float64)."""
+    from oracle import mlp_ref as M
+    nin, nout, batch = 5, 3, 7
+    rng = np.random.default_rng(1)
+    p = M.init(nin, nout, seed=2) + 0.05 * rng.standard_normal(M.layout(nin, nout)[1])
+    x = rng.standard_normal((batch, nin))
+    t = rng.standard_normal((batch, nout))
+    w = rng.uniform(0.5, 2.0, (batch, nout))
+    loss, g = M.loss_grad(p, x, t, w, nin, nout)
+    idx = rng.choice(len(p), 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = h
+        fd = (M.loss_grad(p + e, x, t, w, nin, nout)[0] -
+              M.loss_grad(p - e, x, t, w, nin, nout)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_mlp_oracle_closed_forms():
+    """Zero hidden weights: the output is b3 + W3 tanh(b2) independent of x;
+    one Adam step from zero moments moves each parameter by lr sign(g)."""
+    from oracle import mlp_ref as M
+    nin, nout = 4, 2
+    p = np.zeros(M.layout(nin, nout)[1])
+    W1, b1, W2, b2, W3, b3 = M.unpack(p, nin, nout)
+    b2[:] = 0.3
+    W3[:] = 0.5
+    b3[:] = [1.0, -1.0]
+    y, _ = M.forward(p, np.random.default_rng(0).standard_normal((3, nin)), nin, nout)
+    expect = np.array([1.0, -1.0]) + 0.5 * 64 * np.tanh(0.3)
+    assert np.allclose(y, expect[None, :], rtol=1e-14)
+    g = np.array([2.0, -3.0, 0.5])
+    p2, m, v = M.adam(np.zeros(3), g, np.zeros(3), np.zeros(3), 0.1, 0.9, 0.999, 0.0, 1)
+    assert np.allclose(p2, -0.1 * np.sign(g), rtol=1e-12)
