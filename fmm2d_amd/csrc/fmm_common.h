@@ -78,25 +78,6 @@ __global__ void k_leaf_of(int64_t n, const int* nchild, const int* se, int* leaf
 // possible offsets form a fixed set of 27 (the 6x6 children of the parent's
 // 3x3 neighbourhood minus the 3x3 adjacent ones), so targets of the same q are
 // processed together by the translate kernel, one offset slot after another.
-constexpr int kSlots = 27;
-
-// slot of offset (dx,dy) for child position q, or -1 (adjacent / impossible)
-FMM2D_HD int m2l_slot(int q, int dx, int dy) {
-  const int bx = (q & 1) ? -3 : -2, by = (q & 2) ? -3 : -2;
-  const int ix = dx - bx, iy = dy - by;
-  if (ix < 0 || ix > 5 || iy < 0 || iy > 5) return -1;
-  if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) return -1;
-  int s = 0;
-  for (int a = 0; a < 6; ++a)
-    for (int b = 0; b < 6; ++b) {
-      const int ex = bx + a, ey = by + b;
-      if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1) continue;
-      if (a == ix && b == iy) return s;
-      ++s;
-    }
-  return -1;
-}
-
 // class key of every box: child position (0..3) of an M2L target, 4 otherwise
 __global__ void k_m2l_class(int64_t n, const int* ilevel, const int* iparent, const int* ichild,
                             const int* n2, const int* sse, const int* tse, int ifpgh,
@@ -112,26 +93,6 @@ __global__ void k_m2l_class(int64_t n, const int* ilevel, const int* iparent, co
   }
   key[i] = k;
   val[i] = b;
-}
-
-// slots[pos][s] = source box of slot s of target tlist[pos] (or -1)
-__global__ void k_m2l_slots(int64_t n, const int* tlist, const int* tkey, const double* centers,
-                            const double* bsz, const int* ilevel, const int* n2, const int* l2,
-                            int* slots) {
-  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= n) return;
-  const int b = tlist[pos], q = tkey[pos];
-  int* row = slots + pos * kSlots;
-  for (int s = 0; s < kSlots; ++s) row[s] = -1;
-  const double rinv = 1.0 / bsz[ilevel[b]];
-  const double cx = centers[2 * b], cy = centers[2 * b + 1];
-  for (int k = 0; k < n2[b]; ++k) {
-    const int c = l2[(int64_t)kMaxList2 * b + k];
-    const int ox = (int)rint((centers[2 * c] - cx) * rinv);
-    const int oy = (int)rint((centers[2 * c + 1] - cy) * rinv);
-    const int s = m2l_slot(q, ox, oy);
-    if (s >= 0) row[s] = c;
-  }
 }
 
 // Non-leaf boxes (in box order, hence level order) and their 4-child slots for
@@ -326,11 +287,11 @@ __global__ void k_class_level_bounds(const int* tlist, const int* cseg, const in
 }
 
 // M2L targets (level >= 2, non-empty list 2, points to evaluate) sorted by
-// child position q, box order within a class; slots[pos * 27 + s] = source
-// box of offset slot s (or -1).  cls_seg[q] .. cls_seg[q+1] is class q, and
+// child position q, box order within a class (their sources per offset slot
+// are Lists::s2).  cls_seg[q] .. cls_seg[q+1] is class q, and
 // lev_seg[q][l] .. lev_seg[q][l+1] its part at level l.
 struct M2LPlan {
-  DBuf<int> tlist, skey, slots;
+  DBuf<int> tlist, skey;
   int cls_seg[5] = {0, 0, 0, 0, 0};
   std::vector<int> lev_seg;  // 4 x (nl + 2)
   int ntg = 0;
@@ -364,13 +325,6 @@ void plan_m2l(const Tree& t, const Lists& L, int pg, int pgt, M2LPlan& mp, hipSt
                            hipMemcpyDeviceToHost, st), "copy");
   hip_throw(hipStreamSynchronize(st), "sync");
   mp.ntg = mp.cls_seg[4];
-  if (mp.ntg > 0) {
-    DBuf<double> d_bs = upload(t.boxsize, st);
-    mp.slots.alloc((size_t)mp.ntg * kSlots, st);
-    launch1d(k_m2l_slots, mp.ntg, st, mp.tlist.p, mp.skey.p, t.centers.p, d_bs.p,
-             t.ilevel.p, L.n2.p, L.l2.p, mp.slots.p);
-    hip_throw(hipStreamSynchronize(st), "sync");  // d_bs dies here
-  }
 }
 
 // slot -> offset index (dx+3)*7 + (dy+3) for each child position q

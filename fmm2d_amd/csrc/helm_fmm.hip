@@ -903,8 +903,8 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
           launch_translate(0, 0, nd,
                            TrBufs{mp.p + V.off, V.box0, 2 * V.p + 1, loc.p + V.off, V.box0,
                                   2 * V.p + 1},
-                           kSlots, groups, m2l.tlist.p, m2l.slots.p, d_soff.p, opptr(op_m2l[l]),
-                           t.ilevel.p, nullptr, nullptr, st);
+                           kSlots, groups, m2l.tlist.p, L.s2.p, d_soff.p, opptr(op_m2l[l]),
+                           t.ilevel.p, nullptr, nullptr, st, false, true);
         }
         tm.mark(13);  // the M2L kernels alone
       }

@@ -878,14 +878,14 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
           launch1d(k_to_f32, 2 * nexp, st, reinterpret_cast<const double*>(mp.p), mp32.p);
           tm.mark(3);  // planning
           launch_translate(0, 1, nd, TrBufs{mp32.p, 0, P, loc.p, 0, P}, kSlots, groups,
-                           m2l.tlist.p, m2l.slots.p, d_soff.p, af32.p, t.ilevel.p, d_band.p,
-                           d_logr.p, st, true);
+                           m2l.tlist.p, L.s2.p, d_soff.p, af32.p, t.ilevel.p, d_band.p,
+                           d_logr.p, st, true, true);
           tm.mark(13);  // the M2L kernel alone
         } else {
           tm.mark(3);  // planning
           launch_translate(0, 1, nd, TrBufs{mp.p, 0, P, loc.p, 0, P}, kSlots, groups,
-                           m2l.tlist.p, m2l.slots.p, d_soff.p, d_af.p, t.ilevel.p, d_band.p,
-                           d_logr.p, st);
+                           m2l.tlist.p, L.s2.p, d_soff.p, d_af.p, t.ilevel.p, d_band.p,
+                           d_logr.p, st, false, true);
           tm.mark(13);  // the M2L kernel alone
         }
       }

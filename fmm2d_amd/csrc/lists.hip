@@ -25,26 +25,34 @@ constexpr double kColl = (double)1.05f;
 __global__ void k_lists(int64_t n, const int* ilevel, const int* iparent, const int* nchild,
                         const int* ichild, const int* ncoll, const int* coll,
                         const double* centers, const double* bsl, int* n1, int* l1, int* n2,
-                        int* l2, int* n3, int* l3, int* n4, int* l4) {
+                        int* l2, int* n3, int* l3, int* n4, int* l4, int* s2) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = (int)i;
   const int lev = ilevel[b];
   const double bx = centers[2 * b], by = centers[2 * b + 1];
   int c1 = 0, c2 = 0, c3 = 0, c4 = 0;
+  int* srow = s2 + (int64_t)kSlots * b;
+  for (int k = 0; k < kSlots; ++k) srow[k] = -1;
   if (b == 0) {
     if (nchild[0] == 0) l1[c1++] = 0;
   } else {
     const int dad = iparent[b];
     const double lim = kColl * bsl[lev];
-    // list 2
+    const double binv = 1.0 / bsl[lev];
+    int qb = 0;  // child position of b
+    while (qb < 3 && ichild[4 * dad + qb] != b) ++qb;
+    // list 2, and each entry's M2L offset slot
     for (int k = 0; k < ncoll[dad]; ++k) {
       const int j = coll[9 * dad + k];
       for (int q = 0; q < nchild[j]; ++q) {
         const int kb = ichild[4 * j + q];
-        if (fabs(centers[2 * kb] - bx) >= lim || fabs(centers[2 * kb + 1] - by) >= lim) {
+        const double dx = centers[2 * kb] - bx, dy = centers[2 * kb + 1] - by;
+        if (fabs(dx) >= lim || fabs(dy) >= lim) {
           if (c2 < kMaxList2) l2[(int64_t)kMaxList2 * b + c2] = kb;
           ++c2;
+          const int sl = m2l_slot(qb, (int)rint(dx * binv), (int)rint(dy * binv));
+          if (sl >= 0) srow[sl] = kb;
         }
       }
     }
@@ -106,6 +114,7 @@ void build_lists(const Tree& t, Lists& L, hipStream_t st) {
   L.l2.alloc((size_t)kMaxList2 * nb, st);
   L.l3.alloc((size_t)kMaxList3 * nb, st);
   L.l4.alloc((size_t)kMaxList4 * nb, st);
+  L.s2.alloc((size_t)kSlots * nb, st);
   // boxsize per level, with one extra (finer) entry for the leaf test at the
   // deepest level, where computelists never reaches it (no non-leaf colleague)
   std::vector<double> bsl(t.boxsize);
@@ -115,7 +124,8 @@ void build_lists(const Tree& t, Lists& L, hipStream_t st) {
                            st), "copy");
   hip_throw(hipStreamSynchronize(st), "sync");  // bsl dies on return
   launch1d(k_lists, nb, st, t.ilevel.p, t.iparent.p, t.nchild.p, t.ichild.p, t.ncoll.p, t.coll.p,
-           t.centers.p, dbs.p, L.n1.p, L.l1.p, L.n2.p, L.l2.p, L.n3.p, L.l3.p, L.n4.p, L.l4.p);
+           t.centers.p, dbs.p, L.n1.p, L.l1.p, L.n2.p, L.l2.p, L.n3.p, L.l3.p, L.n4.p, L.l4.p,
+           L.s2.p);
 }  // dbs is released stream-ordered, after k_lists
 
 }  // namespace fmm2d

@@ -7,6 +7,8 @@
 // of v_mfma_f64_16x16x4f64 (lane l holds row l%16, column l/16 of a 16x4
 // block; the result tile holds row l/16 + 4 i, column l%16 in element i).
 //
+// Slot s of target t reads source slots[row * nslots + s] (-1: empty), row =
+// t's position in tlist, or its box id (slots_by_box).
 // Work is split into groups: group g owns targets tlist[seg0[g]..seg1[g])
 // and waves [tile0[g], tile0[g+1]); within a group every target has the same
 // slot -> operator map (opbase[g] + slot_op[cls[g]*nslots + s]), so the A
@@ -70,7 +72,8 @@ template <> struct Mfma<float> {
 template <typename T, int RT, int KSU, int MODE, int LOGT>
 __global__ __launch_bounds__(256) void k_translate(
     int nd, int Pin, int Pout, int nslots, int64_t nwaves, int per_xcd, TrTable tab,
-    const int* tlist, const int* slots, const int* slot_op, const T* Afrag, int64_t afs,
+    const int* tlist, const int* slots, int slots_by_box, const int* slot_op, const T* Afrag,
+    int64_t afs,
     int sbox0, int dbox0, const int* ilevel, const int2* band, const double* logr, const T* src,
     cplx* dst) {
   typedef typename Mfma<T>::acc_t acc_t;
@@ -90,6 +93,8 @@ __global__ __launch_bounds__(256) void k_translate(
   const int d = (int)(col % nd);
   const bool valid = tpos < tab.seg1[g];
   const int tb = valid ? tlist[tpos] : -1;
+  // slot row of this column's target: by position in tlist, or by box id
+  const int64_t srow = slots_by_box ? (int64_t)tb : tpos;
   const int kr = lane >> 4;
   const int RTT = (2 * Pout + 15) / 16, KS = (2 * Pin + 3) / 4;
   const int2 full_in = int2{0, 2 * Pin}, full_out = int2{0, 2 * Pout};
@@ -101,7 +106,7 @@ __global__ __launch_bounds__(256) void k_translate(
       for (int r = 0; r < RT; ++r) acc[r] = acc_t{0, 0, 0, 0};
       T m0 = 0;
       for (int s = 0; s < nslots; ++s) {
-        const int c = valid ? slots[tpos * nslots + s] : -1;
+        const int c = valid ? slots[srow * nslots + s] : -1;
         const T* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
         const T* M = nullptr;
         int klo = 0, khi = 0;
@@ -161,7 +166,7 @@ __global__ __launch_bounds__(256) void k_translate(
         }
       }
       for (int s = 0; s < nslots; ++s) {
-        const int c = valid ? slots[tpos * nslots + s] : -1;
+        const int c = valid ? slots[srow * nslots + s] : -1;
         const T* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
         acc_t acc[RT];
 #pragma unroll
@@ -250,7 +255,7 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
                              const std::vector<TrGroup>& groups, const int* tlist,
                              const int* slots, const int* slot_op, const void* Afrag,
                              const int* ilevel, const int2* band, const double* logr,
-                             hipStream_t st, bool fp32 = false) {
+                             hipStream_t st, bool fp32 = false, bool slots_by_box = false) {
   const int Pin = io.Pin, Pout = io.Pout;
   const int64_t afs = afrag_stride(Pin, Pout);
   const int RTT = (2 * Pout + 15) / 16, KS = (2 * Pin + 3) / 4;
@@ -276,7 +281,8 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     const dim3 grid((unsigned)(8 * per_xcd));
 #define FMM2D_TR(T, RT, KSU, MODE, LOGT)                                                        \
   hipLaunchKernelGGL((k_translate<T, RT, KSU, MODE, LOGT>), grid, dim3(256), 0, st, nd, Pin,    \
-                     Pout, nslots, (int64_t)tiles, per_xcd, tab, tlist, slots, slot_op,         \
+                     Pout, nslots, (int64_t)tiles, per_xcd, tab, tlist, slots,                  \
+                     (int)slots_by_box, slot_op,                                                \
                      static_cast<const T*>(Afrag), afs, io.sbox0, io.dbox0, ilevel, band, logr, \
                      static_cast<const T*>(io.src), io.dst)
 #define FMM2D_TR_RT(T, MODE, LOGT)                 \

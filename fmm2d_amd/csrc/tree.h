@@ -26,8 +26,32 @@ struct Tree {
   DBuf<int> isrc, itarg;      // sorted position -> original point index
 };
 
+// M2L offset slots.  A box in child position q of its parent (0..3 for
+// (-,-), (+,-), (-,+), (+,+)) can only have list-2 sources at the offsets
+// (dx, dy) (in box sizes) of the 6x6 children of its parent's 3x3
+// neighbourhood, minus the 3x3 adjacent ones: 27 slots per q.
+constexpr int kSlots = 27;
+
+// slot of offset (dx,dy) for child position q, or -1 (adjacent / impossible)
+FMM2D_HD int m2l_slot(int q, int dx, int dy) {
+  const int bx = (q & 1) ? -3 : -2, by = (q & 2) ? -3 : -2;
+  const int ix = dx - bx, iy = dy - by;
+  if (ix < 0 || ix > 5 || iy < 0 || iy > 5) return -1;
+  if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) return -1;
+  int s = 0;
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) {
+      const int ex = bx + a, ey = by + b;
+      if (ex >= -1 && ex <= 1 && ey >= -1 && ey <= 1) continue;
+      if (a == ix && b == iy) return s;
+      ++s;
+    }
+  return -1;
+}
+
 struct Lists {
   DBuf<int> n1, l1, n2, l2, n3, l3, n4, l4;  // counts and [box][cap] tables
+  DBuf<int> s2;  // [box][27]: list-2 source of each M2L offset slot, or -1
 };
 
 // Build the tree of the reference's pts_tree_build + pts_tree_sort
