@@ -22,6 +22,7 @@ import numpy as np
 from . import _native as nat
 
 NIN, NOUT, NLEV = 20, 17, 16
+MARGIN = 0.05  # taken off the predicted reductions (the training grid step is 0.1)
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "proposer.npz")
 
 
@@ -69,9 +70,10 @@ def tree_features(sources, targets, eps, zk, kind, ndiv):
     return f.astype(np.float32), tr["boxsize"]
 
 
-def decode(y, p_default, ndiv_default):
-    """Network output -> (nterms per level, ndiv)."""
-    r = np.clip(np.asarray(y[:NLEV], dtype=np.float64), 0.0, 0.7)
+def decode(y, p_default, ndiv_default, margin=0.0):
+    """Network output -> (nterms per level, ndiv); margin is taken off the
+    predicted order reductions (a proposal's safety allowance)."""
+    r = np.clip(np.asarray(y[:NLEV], dtype=np.float64) - margin, 0.0, 0.7)
     nl = len(p_default)
     rr = np.array([r[min(l, NLEV - 1)] for l in range(nl)])
     nterms = [int(max(2, round(p * (1.0 - x)))) for p, x in zip(p_default, rr)]
@@ -114,5 +116,43 @@ class Proposer:
         # orders for the levels of a tree up to 4 levels deeper than the default one
         bsx = list(bs) + [bs[-1] / 2 ** k for k in range(1, 5)]
         p0 = default_nterms(kind, eps, bsx, zk)
-        nterms, ndiv = decode(y, p0, nd0)
+        nterms, ndiv = decode(y, p0, nd0, margin=MARGIN)
         return {"nterms": nterms, "ndiv": ndiv}
+
+
+def fmm_validated(kind, eps, sources, charges=None, dipstr=None, dipvec=None, zk=None,
+                  pg=2, proposer=None, nsample=256, seed=0, opts=None):
+    """Run the FMM at sources with the network's proposal, check its relative
+    L2 error (pot and grad) against the FP64 direct sum at nsample sampled
+    sources, and fall back to the reference rules if it exceeds eps.
+
+    Returns (outputs dict, info) with info = {"used": "proposal" | "default",
+    "err": error of the returned outputs, "proposal": {...}, "err_proposal"}.
+    """
+    from . import fmm2d as F
+    src = np.asarray(sources, dtype=np.float64)
+    n = src.shape[1]
+    prop = (proposer or Proposer()).propose(kind, src, eps, zk=zk)
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(n, min(nsample, n), replace=False))
+    (p, g, _), _ = F._direct(kind, zk, src, src[:, idx], charges, dipstr, dipvec, 2, 1, 1e-16)
+
+    def err_of(res):
+        gp, gg = res["pot"][idx], res["grad"][..., idx].ravel()
+        if kind == "c":  # Re(pot) only, as comperr's Cauchy mode
+            gp, ep = gp.real, p[0].real
+        else:
+            ep = p[0]
+        got, want = np.concatenate([gp, gg]), np.concatenate([ep, g[0].ravel()])
+        return float(np.linalg.norm(got - want) / np.linalg.norm(want))
+
+    o = dict(opts or {})
+    o.update(nterms=prop["nterms"], ndiv=prop["ndiv"])
+    res, _ = F._fmm(kind, eps, zk, src, charges, dipstr, dipvec, None, pg, 0, 1, opts=o)
+    e = err_of(res)
+    info = {"used": "proposal", "err": e, "err_proposal": e, "proposal": prop}
+    if e > eps:
+        res, _ = F._fmm(kind, eps, zk, src, charges, dipstr, dipvec, None, pg, 0, 1,
+                        opts=dict(opts or {}))
+        info.update(used="default", err=err_of(res))
+    return res, info

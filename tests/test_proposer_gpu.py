@@ -1,0 +1,48 @@
+"""The trained proposer (fmm2d_amd/data/proposer.npz) on problems it was not
+trained on: its proposals meet eps against the FP64 direct sum and, summed
+over the problems, run faster than the reference's nterms / ndiv rules."""
+import numpy as np
+import pytest
+
+from fmm2d_amd import fmm2d as F
+from fmm2d_amd import proposer as PR
+from fmm2d_amd.train_proposer import make_points
+
+pytestmark = pytest.mark.gpu
+
+# (kind, dist, n, eps, zk): held out (seed differs from the training run's)
+HELD_OUT = [("l", "cluster", 120000, 1e-6, None), ("h", "uniform", 60000, 1e-6, 40.0),
+            ("l", "curve", 80000, 1e-9, None), ("h", "cluster", 50000, 1e-4, 120.0)]
+
+
+def _time(kind, eps, zk, src, ch, opts):
+    best = None
+    F._fmm(kind, eps, zk, src, ch, None, None, None, 2, 0, 1, opts=dict(opts))
+    for _ in range(3):
+        o = dict(opts)
+        o["timeinfo"] = np.zeros(16)
+        F._fmm(kind, eps, zk, src, ch, None, None, None, 2, 0, 1, opts=o)
+        t = float(o["timeinfo"][11])
+        best = t if best is None else min(best, t)
+    return best
+
+
+def test_proposals_meet_eps_and_beat_default_rules():
+    rng = np.random.default_rng(1234)
+    prop = PR.Proposer()
+    t_prop = t_def = 0.0
+    for kind, dist, n, eps, zk in HELD_OUT:
+        src = make_points(rng, n, dist)
+        ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        zkc = None if zk is None else complex(zk)
+        res, info = PR.fmm_validated(kind, eps, src, ch, zk=zkc, proposer=prop)
+        assert info["used"] == "proposal", info
+        assert info["err"] <= eps, info
+        p = info["proposal"]
+        tp = _time(kind, eps, zkc, src, ch, {"nterms": p["nterms"], "ndiv": p["ndiv"]})
+        td = _time(kind, eps, zkc, src, ch, {})
+        print(f"{kind} {dist} n={n} eps={eps:.0e}: proposal {tp*1e3:.2f} ms "
+              f"(err {info['err']:.2e}), default {td*1e3:.2f} ms")
+        t_prop += tp
+        t_def += td
+    assert t_prop < t_def, (t_prop, t_def)
