@@ -98,6 +98,136 @@ print(json.dumps({{"seconds": dt}}))
     return dt, None
 
 
+def c2_inputs(n, seed=2):
+    """Helmholtz workload: n sources and n targets uniform in [0,1]^2, charges and
+    dipole strengths CN(0,1), dipole directions at uniform random angles."""
+    rng = np.random.default_rng(seed)
+    src = rng.random((n, 2))
+    trg = rng.random((n, 2))
+    ch = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)
+    ds = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)
+    th = rng.uniform(0, 2 * np.pi, n)
+    dv = np.stack([np.cos(th), np.sin(th)], axis=1)  # (n,2) == Fortran (1,2,n)
+    return src, trg, ch, ds, dv
+
+
+def cpu_baseline_c2(n_sample, threads):
+    """Reference hfmm2d on the first n_sample sources and targets of c2_inputs."""
+    code = f"""
+import sys, time, json
+sys.path.insert(0, {ROOT!r})
+import numpy as np
+from bench import c2_inputs
+from oracle import ref_binding as R
+src, trg, ch, ds, dv = c2_inputs({n_sample})
+kw = dict(nd=1, eps=1e-9, sources=np.ascontiguousarray(src.T), charges=ch[None], dipstr=ds[None],
+          dipvec=np.ascontiguousarray(dv.T)[None], targets=np.ascontiguousarray(trg.T),
+          ifpgh=0, ifpghtarg=2)
+t = time.perf_counter()
+R.hfmm2d(zk=100.0 + 0j, **kw)
+print(json.dumps({{"seconds": time.perf_counter() - t}}))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                       timeout=900)
+    if r.returncode != 0:
+        return None, r.stderr[-400:]
+    return json.loads(r.stdout.strip().splitlines()[-1])["seconds"], None
+
+
+def main_hfmm2d(args):
+    """Secondary workload (not the headline metric): hfmm2d, 1e6 sources + 1e6
+    targets, zk = 100, eps = 1e-9, charges + dipoles, pot+grad at the targets,
+    FP64.  Rate = targets per second."""
+    import torch
+    from fmm2d_amd import _native as nat
+    L = nat.lib()
+    dev = nat.torch_device()
+    n = args.n if args.n != 10_000_000 else 1_000_000
+    eps, zk = 1e-9, 100.0
+    print(f"bench: hfmm2d workload, {n} sources + {n} targets", file=sys.stderr, flush=True)
+    s_, t_, c_, d_, v_ = c2_inputs(n)
+    src, trg = torch.from_numpy(s_).to(dev), torch.from_numpy(t_).to(dev)
+    chg, dst = torch.from_numpy(c_).to(dev), torch.from_numpy(d_).to(dev)
+    dvec = torch.from_numpy(v_).to(dev)
+    pt = torch.zeros(n, dtype=torch.complex128, device=dev)
+    gt = torch.zeros((n, 2), dtype=torch.complex128, device=dev)
+    stream = nat.stream_ptr()
+
+    def step(o):
+        ier = L.fmm2d_hfmm2d(1, eps, zk, 0.0, n, nat.ptr(src), 1, nat.ptr(chg), 1, nat.ptr(dst),
+                             nat.ptr(dvec), 0, 0, None, None, None, n, nat.ptr(trg), 2,
+                             nat.ptr(pt), nat.ptr(gt), None, nat.byref_opts(o), stream)
+        if ier != 0:
+            raise RuntimeError(f"fmm2d_hfmm2d returned {ier}")
+
+    opts = nat.opts_struct({})
+    for _ in range(args.warmup):
+        step(opts)
+    torch.cuda.synchronize()
+    stage = np.zeros(16)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(opts)
+        stage += np.array(opts.timeinfo[:])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    stage /= args.steps
+    copts = nat.opts_struct({"count_work": 1})
+    step(copts)
+    torch.cuda.synchronize()
+    work = np.array(copts.work[:])
+    # error at sampled targets against the FP64 direct sum
+    rng = np.random.default_rng(54321)
+    idx = torch.from_numpy(np.sort(rng.choice(n, min(args.nerr, n), replace=False))).to(dev)
+    m = len(idx)
+    tsel = trg[idx].contiguous()
+    pex = torch.zeros(m, dtype=torch.complex128, device=dev)
+    gex = torch.zeros((m, 2), dtype=torch.complex128, device=dev)
+    ier = L.fmm2d_h2d_direct(1, zk, 0.0, nat.ptr(src), n, 1, nat.ptr(chg), 1, nat.ptr(dst),
+                             nat.ptr(dvec), nat.ptr(tsel), m, 2, nat.ptr(pex), nat.ptr(gex),
+                             None, 1e-16, stream)
+    assert ier == 0
+    num = ((pt[idx] - pex).abs() ** 2).sum() + ((gt[idx] - gex).abs() ** 2).sum()
+    den = (pex.abs() ** 2).sum() + (gex.abs() ** 2).sum()
+    relerr = float(torch.sqrt(num / den))
+    print(f"bench[hfmm2d]: {dt*1e3:.1f} ms/step, levels {copts.nlevels}, boxes {copts.nboxes}, "
+          f"rel L2 err {relerr:.3e}", file=sys.stderr, flush=True)
+    for k, name in STAGES.items():
+        print(f"bench[hfmm2d]:   {name:12s} {stage[k]*1e3:9.3f} ms", file=sys.stderr)
+    t_m2l = stage[M2L_SLOT]
+    achieved = work[3] / t_m2l / 1e12 if t_m2l > 0 else 0.0
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        ns_cpu = 100_000
+        secs, errmsg = cpu_baseline_c2(ns_cpu, threads)
+        if secs is not None:
+            cpu = {"value": round(ns_cpu / secs, 1), "unit": "targets/s", "cores": threads,
+                   "kind": "reference",
+                   "sample": f"reference hfmm2d (oracle/_ref) on the first {ns_cpu} sources "
+                             f"and targets of the same generator; {secs:.2f} s"}
+        else:
+            print("bench: cpu baseline failed: " + str(errmsg), file=sys.stderr)
+    line = {
+        "metric": "hfmm2d targets/sec, 1e6 sources + 1e6 targets, zk=100, eps 1e-9 (secondary)",
+        "value": round(n / dt, 1), "unit": "targets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "hfmm2d uniform 1e6 sources + 1e6 targets, zk=100, eps=1e-9, "
+                               "charges+dipoles, pot+grad at targets",
+                   "n": n, "nlevels": int(copts.nlevels), "nboxes": int(copts.nboxes)},
+        "rel_l2_err": relerr,
+        "stage_ms": {name: round(stage[k] * 1e3, 4) for k, name in STAGES.items()},
+        "roofline": {"bound": "mfma", "kernel": "k_translate (M2L, FP64 MFMA)",
+                     "achieved": round(achieved, 3), "peak": PEAK["fp64_tflops"],
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK["fp64_tflops"], 4),
+                     "traffic": None, "launch_ms": round(t_m2l * 1e3, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,12 +245,16 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run one elementwise copy kernel over MIB MiB (reads and "
                          "writes MIB MiB), the reference kernel for scaling HBM counters")
+    ap.add_argument("--workload", default="lfmm2d", choices=["lfmm2d", "hfmm2d"],
+                    help="lfmm2d: the headline workload; hfmm2d: the secondary Helmholtz one")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
               "decomposition is described in DESIGN.md and not measured.", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "hfmm2d":
+        return main_hfmm2d(args)
     import ctypes as C
     import torch
     from fmm2d_amd import _native as nat

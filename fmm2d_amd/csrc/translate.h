@@ -293,6 +293,20 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     case 4: FMM2D_TR(T, 4, 16, MODE, LOGT); break; \
     default: FMM2D_TR(T, 4, 0, MODE, LOGT); break; \
   }
+// FP64 also unrolls operators of 5..8 row tiles (expansions of up to 64
+// complex coefficients: Helmholtz orders up to 31)
+#define FMM2D_TR_RT64(MODE, LOGT)                            \
+  switch (KS <= 4 * RTT ? RTT : 0) {                         \
+    case 1: FMM2D_TR(double, 1, 4, MODE, LOGT); break;       \
+    case 2: FMM2D_TR(double, 2, 8, MODE, LOGT); break;       \
+    case 3: FMM2D_TR(double, 3, 12, MODE, LOGT); break;      \
+    case 4: FMM2D_TR(double, 4, 16, MODE, LOGT); break;      \
+    case 5: FMM2D_TR(double, 5, 20, MODE, LOGT); break;      \
+    case 6: FMM2D_TR(double, 6, 24, MODE, LOGT); break;      \
+    case 7: FMM2D_TR(double, 7, 28, MODE, LOGT); break;      \
+    case 8: FMM2D_TR(double, 8, 32, MODE, LOGT); break;      \
+    default: FMM2D_TR(double, 4, 0, MODE, LOGT); break;      \
+  }
     if (fp32) {
       if (logt) {
         FMM2D_TR_RT(float, 0, 1)
@@ -302,10 +316,11 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     } else if (mode == 0 && logt) {
       FMM2D_TR_RT(double, 0, 1)
     } else if (mode == 0) {
-      FMM2D_TR_RT(double, 0, 0)
+      FMM2D_TR_RT64(0, 0)
     } else {
-      FMM2D_TR_RT(double, 1, 0)
+      FMM2D_TR_RT64(1, 0)
     }
+#undef FMM2D_TR_RT64
 #undef FMM2D_TR_RT
 #undef FMM2D_TR
     hip_throw(hipGetLastError(), "translate");
