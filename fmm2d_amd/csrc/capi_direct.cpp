@@ -71,4 +71,12 @@ int fmm2d_h2d_direct(int nd, double zk_re, double zk_im, const double* sources, 
                     targ, nt, ifpgh, pot, grad, hess, thresh, stream);
 }
 
+int fmm2d_r2d_direct(int nd, const double* sources, int64_t ns, int ifcharge, const double* charge,
+                     int ifdipole, const double* dipstr, const double* dipvec, const double* targ,
+                     int64_t nt, int ifpgh, double* pot, double* grad, double* hess,
+                     double thresh, void* stream) {
+  return direct_eval_real(nd, sources, ns, ifcharge, charge, ifdipole, dipstr, dipvec, targ, nt,
+                          ifpgh, thresh, pot, grad, hess, reinterpret_cast<hipStream_t>(stream));
+}
+
 }  // extern "C"

@@ -103,6 +103,24 @@ int fmm2d_lfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifch
   return laplace_fmm(1, a, reinterpret_cast<hipStream_t>(stream));
 }
 
+int fmm2d_rfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
+                 const double* charge, int ifdipole, const double* dipstr, const double* dipvec,
+                 int iper, int ifpgh, double* pot, double* grad, double* hess, int64_t nt,
+                 const double* targ, int ifpghtarg, double* pottarg, double* gradtarg,
+                 double* hesstarg, fmm2d_opts* opts, void* stream) {
+  if (iper != 0) return FMM2D_ERR_UNSUPPORTED;
+  CauchyArgs a = cauchy_args(nd, eps, ns, sources, ifcharge,
+                             reinterpret_cast<const fmm2d_cplx*>(charge), ifdipole,
+                             reinterpret_cast<const fmm2d_cplx*>(dipstr), dipvec, ifpgh,
+                             reinterpret_cast<fmm2d_cplx*>(pot), reinterpret_cast<fmm2d_cplx*>(grad),
+                             reinterpret_cast<fmm2d_cplx*>(hess), nt, targ, ifpghtarg,
+                             reinterpret_cast<fmm2d_cplx*>(pottarg),
+                             reinterpret_cast<fmm2d_cplx*>(gradtarg),
+                             reinterpret_cast<fmm2d_cplx*>(hesstarg), opts);
+  keep_pool_memory();
+  return laplace_fmm(2, a, reinterpret_cast<hipStream_t>(stream));
+}
+
 int fmm2d_hfmm2d(int nd, double eps, double zk_re, double zk_im, int64_t ns,
                  const double* sources, int ifcharge, const fmm2d_cplx* charge, int ifdipole,
                  const fmm2d_cplx* dipstr, const double* dipvec, int iper, int ifpgh,

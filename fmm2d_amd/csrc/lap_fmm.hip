@@ -83,6 +83,46 @@ __global__ void k_gather_split(int64_t n, int nd, int half, const int* perm, con
   }
 }
 
+// rfmm2d split (laplace/rfmm2d.f:65-84): real charge c -> Cauchy charge c,
+// real dipole (d, v) -> Cauchy dipole -(v1 + i v2) d.
+__global__ void k_gather_real(int64_t n, int nd, const int* perm, const double* ch,
+                              const double* ds, const double* dv, cplx* och, cplx* ods) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = perm[i];
+  for (int d = 0; d < nd; ++d) {
+    if (ch) och[i * nd + d] = mkc(ch[j * nd + d]);
+    if (ds) {
+      const double s = ds[j * nd + d];
+      ods[i * nd + d] = cplx{-dv[nd * (2 * j) + d] * s, -dv[nd * (2 * j + 1) + d] * s};
+    }
+  }
+}
+
+// rfmm2d unpack (laplace/rfmm2d.f:86-139): pot = Re, grad = (Re, -Im),
+// hess = (Re, -Im, -Re) of the Cauchy outputs, into real arrays.
+__global__ void k_scatter_real(int64_t n, int nd, int pg, const int* perm, const cplx* p1,
+                               const cplx* g1, const cplx* h1, double* pot, double* grad,
+                               double* hess) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t j = perm[i];
+  for (int d = 0; d < nd; ++d) {
+    const int64_t a = i * nd + d;
+    pot[j * nd + d] = p1[a].re;
+    if (pg >= 2) {
+      grad[nd * (2 * j) + d] = g1[a].re;
+      grad[nd * (2 * j + 1) + d] = -g1[a].im;
+    }
+    if (pg >= 3) {
+      const double hxx = h1[a].re;
+      hess[nd * (3 * j) + d] = hxx;
+      hess[nd * (3 * j + 1) + d] = -h1[a].im;
+      hess[nd * (3 * j + 2) + d] = -hxx;
+    }
+  }
+}
+
 // flag = 1 if some value has a non-zero (or NaN) imaginary part
 __global__ void k_any_imag(int64_t n, const cplx* v, int* flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -697,7 +737,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       a.ifpghtarg > 3)
     return FMM2D_ERR_ARG;
   if (!a.src || (a.nt > 0 && !a.targ) || (a.ifcharge && !a.charge) ||
-      (a.ifdipole && !a.dipstr) || (kind == 1 && a.ifdipole && !a.dipvec))
+      (a.ifdipole && !a.dipstr) || (kind >= 1 && a.ifdipole && !a.dipvec))
     return FMM2D_ERR_ARG;
   if ((a.ifpgh >= 1 && !a.pot) || (a.ifpgh >= 2 && !a.grad) || (a.ifpgh >= 3 && !a.hess) ||
       (a.ifpghtarg >= 1 && !a.pottarg) || (a.ifpghtarg >= 2 && !a.gradtarg) ||
@@ -763,6 +803,11 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     if (kind == 0) {
       if (a.ifcharge) launch1d(k_gather_dens, ns, st, nd, t.isrc.p, a.charge, chs.p);
       if (a.ifdipole) launch1d(k_gather_dens, ns, st, nd, t.isrc.p, a.dipstr, dps.p);
+    } else if (kind == 2) {
+      launch1d(k_gather_real, ns, st, nd, t.isrc.p,
+               a.ifcharge ? reinterpret_cast<const double*>(a.charge) : nullptr,
+               a.ifdipole ? reinterpret_cast<const double*>(a.dipstr) : nullptr, a.dipvec,
+               chs.p, dps.p);
     } else {
       launch1d(k_gather_split, ns, st, a.nd, half, t.isrc.p, a.ifcharge ? a.charge : nullptr,
                a.ifdipole ? a.dipstr : nullptr, a.dipvec, chs.p, dps.p);
@@ -925,6 +970,15 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       if (pgt >= 1) launch1d(k_scatter, nt, st, nd, t.itarg.p, pt.p, a.pottarg);
       if (pgt >= 2) launch1d(k_scatter, nt, st, nd, t.itarg.p, gt.p, a.gradtarg);
       if (pgt >= 3) launch1d(k_scatter, nt, st, nd, t.itarg.p, ht.p, a.hesstarg);
+    } else if (kind == 2) {
+      if (pg >= 1)
+        launch1d(k_scatter_real, ns, st, nd, pg, t.isrc.p, ps.p, gs.p, hs.p,
+                 reinterpret_cast<double*>(a.pot), reinterpret_cast<double*>(a.grad),
+                 reinterpret_cast<double*>(a.hess));
+      if (pgt >= 1)
+        launch1d(k_scatter_real, nt, st, nd, pgt, t.itarg.p, pt.p, gt.p, ht.p,
+                 reinterpret_cast<double*>(a.pottarg), reinterpret_cast<double*>(a.gradtarg),
+                 reinterpret_cast<double*>(a.hesstarg));
     } else {
       if (pg >= 1)
         launch1d(k_scatter_unpack, ns, st, a.nd, half, pg, t.isrc.p, ps.p, gs.p, hs.p, a.pot, a.grad,

@@ -14,13 +14,15 @@ struct CauchyArgs {
   int ifcharge, ifdipole;
   const cplx* charge;   // (nd,ns)
   const cplx* dipstr;   // (nd,ns)
-  const double* dipvec; // (nd,2,ns): lfmm2d only
+  const double* dipvec; // (nd,2,ns): lfmm2d, rfmm2d
   int ifpgh, ifpghtarg;
   cplx *pot, *grad, *hess, *pottarg, *gradtarg, *hesstarg;
   const fmm2d_opts* opts;  // may be null
 };
 
-// kind 0: cfmm2d (laplace/cfmm2d.f:23); kind 1: lfmm2d (laplace/lfmm2d.f:29)
+// kind 0: cfmm2d (laplace/cfmm2d.f:23); kind 1: lfmm2d (laplace/lfmm2d.f:29);
+// kind 2: rfmm2d (laplace/rfmm2d.f:27) -- charge, dipstr and the six outputs
+// then point to real (double) arrays of the same shapes.
 int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st);
 
 // l2dterms (laplace/l2dterms.f:38-82) and lndiv2d (laplace/lndiv2d.f)

@@ -23,4 +23,13 @@ struct DirectArgs {
 
 int direct_eval(const DirectArgs& a, hipStream_t st);
 
+// r2d_direct{c,d,cd}{p,g,h} (laplace/rlapkernels2d.f): the Laplace kernels on
+// real densities with real outputs.  The formulas are those of l2d_direct
+// (laplace/lapkernels2d.f) with real arguments, so the real parts of the
+// complex evaluation on the densities are exactly the real kernels' values.
+int direct_eval_real(int nd, const double* src, int64_t ns, int ifcharge, const double* charge,
+                     int ifdipole, const double* dipstr, const double* dipvec, const double* targ,
+                     int64_t nt, int pg, double thresh, double* pot, double* grad, double* hess,
+                     hipStream_t st);
+
 }  // namespace fmm2d

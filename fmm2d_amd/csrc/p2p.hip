@@ -8,6 +8,7 @@
 // too few target blocks to fill the 256 CUs the source range is split across
 // blocks; each split writes a private copy of the outputs and a second kernel
 // adds the copies into the caller's buffers in split order (deterministic).
+#include "devbuf.h"
 #include "p2p.h"
 #include "p2p_math.h"
 
@@ -189,6 +190,65 @@ int direct_eval(const DirectArgs& a, hipStream_t st) {
     if (nh) hipLaunchKernelGGL(add_splits_kernel, dim3(ceil_div(nh, 256)), dim3(256), 0, st, a.hess, h, nh, (int)nsplit);
     FMM2D_HIP_TRY(hipGetLastError());
     FMM2D_HIP_TRY(hipFreeAsync(work, st));
+  }
+  return FMM2D_OK;
+}
+
+namespace {
+__global__ void k_real_to_cplx(int64_t n, const double* in, cplx* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = mkc(in[i]);
+}
+__global__ void k_add_real(int64_t n, const cplx* in, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += in[i].re;
+}
+}  // namespace
+
+int direct_eval_real(int nd, const double* src, int64_t ns, int ifcharge, const double* charge,
+                     int ifdipole, const double* dipstr, const double* dipvec, const double* targ,
+                     int64_t nt, int pg, double thresh, double* pot, double* grad, double* hess,
+                     hipStream_t st) {
+  if (nd < 1 || ns < 0 || nt < 0 || pg < 1 || pg > 3 || (!ifcharge && !ifdipole))
+    return FMM2D_ERR_ARG;
+  if ((ifcharge && !charge) || (ifdipole && (!dipstr || !dipvec)) || !pot ||
+      (pg >= 2 && !grad) || (pg >= 3 && !hess) || (ns > 0 && !src) || (nt > 0 && !targ))
+    return FMM2D_ERR_ARG;
+  if (nt == 0 || ns == 0) return FMM2D_OK;
+  try {
+    const int64_t nsd = ns * nd, ntd = nt * nd;
+    DBuf<cplx> ch(ifcharge ? nsd : 0, st), ds(ifdipole ? nsd : 0, st);
+    DBuf<cplx> p(ntd, st), g(pg >= 2 ? 2 * ntd : 0, st), h(pg >= 3 ? 3 * ntd : 0, st);
+    if (ifcharge) launch1d(k_real_to_cplx, nsd, st, charge, ch.p);
+    if (ifdipole) launch1d(k_real_to_cplx, nsd, st, dipstr, ds.p);
+    p.zero();
+    g.zero();
+    h.zero();
+    DirectArgs a;
+    a.family = LAPLACE;
+    a.nd = nd;
+    a.zk = mkc(0.0);
+    a.src = src;
+    a.ns = ns;
+    a.ifcharge = ifcharge;
+    a.ifdipole = ifdipole;
+    a.charge = ch.p;
+    a.dipstr = ds.p;
+    a.dipvec = dipvec;
+    a.targ = targ;
+    a.nt = nt;
+    a.pg = pg;
+    a.thresh = thresh;
+    a.pot = p.p;
+    a.grad = g.p;
+    a.hess = h.p;
+    const int ier = direct_eval(a, st);
+    if (ier != FMM2D_OK) return ier;
+    launch1d(k_add_real, ntd, st, p.p, pot);
+    if (pg >= 2) launch1d(k_add_real, 2 * ntd, st, g.p, grad);
+    if (pg >= 3) launch1d(k_add_real, 3 * ntd, st, h.p, hess);
+  } catch (const HipError& e) {
+    return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
   }
   return FMM2D_OK;
 }

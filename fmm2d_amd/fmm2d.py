@@ -81,7 +81,7 @@ def _check_dipoles(dipstr, dipvec, nd, ns, need_vec=True):
 class _DevInputs:
     """Device copies of the inputs in the C-ABI's (Fortran) layout."""
 
-    def __init__(self, sources, ns, nd, charges, dipstr, dipvec, targets, need_vec):
+    def __init__(self, sources, ns, nd, charges, dipstr, dipvec, targets, need_vec, real=False):
         import torch
         self.ns = ns
         self.nd = nd
@@ -90,10 +90,11 @@ class _DevInputs:
         self.charge = None
         self.dipstr = None
         self.dipvec = None
+        dt = np.float64 if real else np.complex128
         if charges is not None:
-            self.charge = nat.to_dev_fortran(np.reshape(charges, (nd, ns)), np.complex128)
+            self.charge = nat.to_dev_fortran(np.reshape(charges, (nd, ns)), dt)
         if dipstr is not None:
-            self.dipstr = nat.to_dev_fortran(np.reshape(dipstr, (nd, ns)), np.complex128)
+            self.dipstr = nat.to_dev_fortran(np.reshape(dipstr, (nd, ns)), dt)
             if need_vec:
                 self.dipvec = nat.to_dev_fortran(np.reshape(dipvec, (nd, 2, ns)), np.float64)
         self.targ = None
@@ -111,14 +112,15 @@ def _fmm(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
          opts=None):
     """One call of the native driver; returns (dict of f2py-shaped outputs, ier).
 
-    kind: "c" (cfmm2d), "l" (lfmm2d) or "h" (hfmm2d).  pg/pgt are the
+    kind: "c" (cfmm2d), "l" (lfmm2d), "h" (hfmm2d) or "r" (rfmm2d, real
+    densities and outputs).  pg/pgt are the
     evaluation flags at sources/targets (0 = none); targets may be None.
     """
     import torch
     ns = _nsrc(sources)
     need_vec = kind != "c"
     dev = _DevInputs(sources, ns, nd, charges, dipstr, dipvec,
-                     targets if pgt > 0 else None, need_vec)
+                     targets if pgt > 0 else None, need_vec, real=(kind == "r"))
     nt = dev.nt
     ng = 1 if kind == "c" else 2
     nh = 1 if kind == "c" else 3
@@ -130,7 +132,8 @@ def _fmm(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
     for flag, n, names in ((pg, ns, ("pot", "grad", "hess")),
                            (pgt, nt, ("pottarg", "gradtarg", "hesstarg"))):
         for lvl, (name, shp) in enumerate(zip(names, shapes(n))):
-            bufs[name] = nat.dev_zeros_fortran(shp, torch.complex128) if flag > lvl else None
+            odt = torch.float64 if kind == "r" else torch.complex128
+            bufs[name] = nat.dev_zeros_fortran(shp, odt) if flag > lvl else None
     L = nat.lib()
     ifc = 1 if dev.charge is not None else 0
     ifd = 1 if dev.dipstr is not None else 0
@@ -145,7 +148,8 @@ def _fmm(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
     args += [0, pg, nat.ptr(bufs["pot"]), nat.ptr(bufs["grad"]), nat.ptr(bufs["hess"]),
              nt, nat.ptr(dev.targ), pgt, nat.ptr(bufs["pottarg"]), nat.ptr(bufs["gradtarg"]),
              nat.ptr(bufs["hesstarg"]), nat.byref_opts(o), nat.stream_ptr()]
-    fn = {"c": L.fmm2d_cfmm2d, "l": L.fmm2d_lfmm2d, "h": L.fmm2d_hfmm2d}[kind]
+    fn = {"c": L.fmm2d_cfmm2d, "l": L.fmm2d_lfmm2d, "h": L.fmm2d_hfmm2d,
+          "r": L.fmm2d_rfmm2d}[kind]
     ier = fn(*args)
     nat.check(ier, kind + "fmm2d")
     res = {}
@@ -223,6 +227,14 @@ def lfmm2d(*, eps, sources, charges=None, dipstr=None, dipvec=None,
                          nd, maxpg=3)
 
 
+def rfmm2d(*, eps, sources, charges=None, dipstr=None, dipvec=None,
+           targets=None, pg=0, pgt=0, nd=1):
+    """Laplace FMM with real densities and real outputs (python/fmm2dpy/fmm2d.py:201-456;
+    laplace/rfmm2d.f:27).  pg, pgt in {1, 2, 3}."""
+    return _fmm_dispatch("r", eps, None, sources, charges, dipstr, dipvec, targets, pg, pgt,
+                         nd, maxpg=3)
+
+
 def cfmm2d(*, eps, sources, charges=None, dipstr=None,
            targets=None, pg=0, pgt=0, nd=1):
     """Cauchy-kernel FMM (python/fmm2dpy/fmm2d.py:715-964; laplace/cfmm2d.f:23).
@@ -244,15 +256,17 @@ def _direct(family, zk, sources, targets, charges, dipstr, dipvec, pgt, nd, thre
     import torch
     ns = _nsrc(sources)
     need_vec = family != "c"
-    dev = _DevInputs(sources, ns, nd, charges, dipstr, dipvec, targets, need_vec)
+    dev = _DevInputs(sources, ns, nd, charges, dipstr, dipvec, targets, need_vec,
+                     real=(family == "r"))
     nt = dev.nt
     ng = 1 if family == "c" else 2
     nh = 1 if family == "c" else 3
-    pot = nat.dev_zeros_fortran((nd, nt), torch.complex128)
+    odt = torch.float64 if family == "r" else torch.complex128
+    pot = nat.dev_zeros_fortran((nd, nt), odt)
     gshape = (nd, nt) if ng == 1 else (nd, ng, nt)
     hshape = (nd, nt) if nh == 1 else (nd, nh, nt)
-    grad = nat.dev_zeros_fortran(gshape, torch.complex128) if pgt >= 2 else None
-    hess = nat.dev_zeros_fortran(hshape, torch.complex128) if pgt >= 3 else None
+    grad = nat.dev_zeros_fortran(gshape, odt) if pgt >= 2 else None
+    hess = nat.dev_zeros_fortran(hshape, odt) if pgt >= 3 else None
     L = nat.lib()
     ifc = 1 if dev.charge is not None else 0
     ifd = 1 if dev.dipstr is not None else 0
@@ -262,6 +276,9 @@ def _direct(family, zk, sources, targets, charges, dipstr, dipvec, pgt, nd, thre
         ier = L.fmm2d_c2d_direct(nd, *common, nat.ptr(dev.targ), nt, pgt, *outs)
     elif family == "l":
         ier = L.fmm2d_l2d_direct(nd, *common, nat.ptr(dev.dipvec), nat.ptr(dev.targ), nt, pgt,
+                                 *outs)
+    elif family == "r":
+        ier = L.fmm2d_r2d_direct(nd, *common, nat.ptr(dev.dipvec), nat.ptr(dev.targ), nt, pgt,
                                  *outs)
     else:
         zk = complex(zk)
@@ -317,6 +334,30 @@ def l2ddir(*, sources, targets, charges=None, dipstr=None, dipvec=None,
     if pgt not in (1, 2, 3) or not (ifcharge or ifdipole):
         return _reshape_ddir(out, nd, 0, targets, ndim_g=2)
     (p, g, h), nt = _direct("l", None, sources, targets, charges if ifcharge else None,
+                            dipstr if ifdipole else None, dipvec, pgt, nd, thresh)
+    out.pottarg = p
+    if pgt >= 2:
+        out.gradtarg = g
+    if pgt >= 3:
+        out.hesstarg = h
+    return _reshape_ddir(out, nd, pgt, targets, ndim_g=2)
+
+
+def r2ddir(*, sources, targets, charges=None, dipstr=None, dipvec=None,
+           pgt=0, nd=1, thresh=1e-16):
+    """Real Laplace direct evaluation (python/fmm2dpy/fmm2d.py:1370-1490;
+    laplace/rlapkernels2d.f): real densities and outputs, pgt in {1, 2, 3}."""
+    out = Output()
+    ns = _nsrc(sources)
+    if pgt == 0:
+        print("Nothing to compute, set either pg or pgt to non-zero")
+        return out
+    ifcharge = _check_charges(charges, nd, ns)
+    ifdipole = _check_dipoles(dipstr, dipvec, nd, ns)
+    assert targets.shape[0] == 2, "The first dimension of targets must be 2"
+    if pgt not in (1, 2, 3) or not (ifcharge or ifdipole):
+        return _reshape_ddir(out, nd, 0, targets, ndim_g=2)
+    (p, g, h), nt = _direct("r", None, sources, targets, charges if ifcharge else None,
                             dipstr if ifdipole else None, dipvec, pgt, nd, thresh)
     out.pottarg = p
     if pgt >= 2:
@@ -413,15 +454,13 @@ def comperr(*, ntest, out, outex, pg=0, pgt=0, nd=1, cauchy=0, bh=0):
 def _out_of_scope(name):
     def f(*args, **kwargs):
         raise NotImplementedError(
-            f"{name} is not provided by fmm2d_amd (only the Laplace, Cauchy and "
-            "Helmholtz FMMs and direct evaluators are)")
+            f"{name} is not provided by fmm2d_amd (only the Laplace (complex and real), "
+            "Cauchy and Helmholtz FMMs and direct evaluators are)")
     f.__name__ = name
     return f
 
 
-rfmm2d = _out_of_scope("rfmm2d")
 bhfmm2d = _out_of_scope("bhfmm2d")
 stfmm2d = _out_of_scope("stfmm2d")
-r2ddir = _out_of_scope("r2ddir")
 bh2ddir = _out_of_scope("bh2ddir")
 st2ddir = _out_of_scope("st2ddir")

@@ -93,6 +93,17 @@ int fmm2d_l2d_direct(int nd, const double* sources, int64_t ns, int ifcharge,
                      fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, double thresh,
                      void* stream);
 
+/*
+ * Direct evaluation of the real Laplace kernels, INCREMENTS pot/grad/hess:
+ *   replaces laplace/rlapkernels2d.f r2d_direct{c,d,cd}{p,g,h} (f2py: r2ddir,
+ *   python/fmm2dpy/fmm2d.py:1370), real densities and outputs, the same
+ *   per-variant exclusion rules as l2d_direct*.
+ */
+int fmm2d_r2d_direct(int nd, const double* sources, int64_t ns, int ifcharge, const double* charge,
+                     int ifdipole, const double* dipstr, const double* dipvec, const double* targ,
+                     int64_t nt, int ifpgh, double* pot, double* grad, double* hess,
+                     double thresh, void* stream);
+
 /*   replaces helmholtz/helmkernels2d.f h2d_direct{c,d,cd}{p,g,h}
  *   (lines 3, 71, 155, 250, 334, 435, 560, 651, 765); thresh applies to
  *   |zk|*|t-s| as in the reference. */
@@ -134,6 +145,20 @@ int fmm2d_lfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifch
                  fmm2d_cplx* hess, int64_t nt, const double* targ, int ifpghtarg,
                  fmm2d_cplx* pottarg, fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg,
                  fmm2d_opts* opts, void* stream);
+
+/*
+ * rfmm2d (laplace/rfmm2d.f:27; f2py wrappers laplace/rfmm2dwrap.f): the Laplace
+ * FMM on REAL charges (nd,ns), dipole strengths (nd,ns) and dipole vectors
+ * (nd,2,ns) with REAL outputs pot (nd,n), grad (nd,2,n), hess (nd,3,n).  As the
+ * reference, it runs the Cauchy FMM on the charges and on -(v1 + i v2) times
+ * the dipole strengths and returns pot = Re, grad = (Re, -Im), hess = (Re,
+ * -Im, -Re) (rfmm2d.f:65-139).
+ */
+int fmm2d_rfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
+                 const double* charge, int ifdipole, const double* dipstr, const double* dipvec,
+                 int iper, int ifpgh, double* pot, double* grad, double* hess, int64_t nt,
+                 const double* targ, int ifpghtarg, double* pottarg, double* gradtarg,
+                 double* hesstarg, fmm2d_opts* opts, void* stream);
 
 /*
  * Helmholtz FMM; OVERWRITES outputs (ifpgh, ifpghtarg <= 2).

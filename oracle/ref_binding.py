@@ -68,20 +68,20 @@ def _out(shape, dtype=np.complex128):
     return a, a.ctypes.data_as(C.c_void_p)
 
 
-def _src_arrays(nd, ns, charges, dipstr, dipvec, need_dipvec):
+def _src_arrays(nd, ns, charges, dipstr, dipvec, need_dipvec, dtype=np.complex128):
     keep = []
     if charges is None:
-        ch, pch = _out((nd, 1))
+        ch, pch = _out((nd, 1), dtype)
         ifc = 0
     else:
-        ch, pch = _farr(np.reshape(charges, (nd, ns), order="C"), np.complex128)
+        ch, pch = _farr(np.reshape(charges, (nd, ns), order="C"), dtype)
         ifc = 1
     if dipstr is None:
-        ds, pds = _out((nd, 1))
+        ds, pds = _out((nd, 1), dtype)
         dv, pdv = _out((nd, 2, 1), np.float64)
         ifd = 0
     else:
-        ds, pds = _farr(np.reshape(dipstr, (nd, ns), order="C"), np.complex128)
+        ds, pds = _farr(np.reshape(dipstr, (nd, ns), order="C"), dtype)
         if need_dipvec:
             dv, pdv = _farr(np.reshape(dipvec, (nd, 2, ns), order="C"), np.float64)
         else:
@@ -112,17 +112,18 @@ def _fmm(kind, nd, eps, sources, charges=None, dipstr=None, dipvec=None,
         nt = targets.shape[1]
         trg, ptrg = _farr(targets, np.float64)
     need_dv = kind != "c"
-    ifc, pch, ifd, pds, pdv, keep = _src_arrays(nd, ns, charges, dipstr, dipvec, need_dv)
+    dt = np.float64 if kind == "r" else np.complex128
+    ifc, pch, ifd, pds, pdv, keep = _src_arrays(nd, ns, charges, dipstr, dipvec, need_dv, dt)
     gdim = (nd,) if kind == "c" else (nd, 2)
     hdim = (nd,) if kind == "c" else (nd, 3)
     n1 = max(ns, 1)
     t1 = max(nt, 1)
-    pot, ppot = _out((nd, n1))
-    grad, pgrad = _out(gdim + (n1,))
-    hess, phess = _out(hdim + (n1,))
-    pt, ppt = _out((nd, t1))
-    gt, pgt = _out(gdim + (t1,))
-    ht, pht = _out(hdim + (t1,))
+    pot, ppot = _out((nd, n1), dt)
+    grad, pgrad = _out(gdim + (n1,), dt)
+    hess, phess = _out(hdim + (n1,), dt)
+    pt, ppt = _out((nd, t1), dt)
+    gt, pgt = _out(gdim + (t1,), dt)
+    ht, pht = _out(hdim + (t1,), dt)
     ier = C.c_int(0)
     timeinfo = np.zeros(8)
     args = []
@@ -136,7 +137,7 @@ def _fmm(kind, nd, eps, sources, charges=None, dipstr=None, dipvec=None,
     if kind != "c":
         args.append(pdv)
     args += [_i(0), _i(ifpgh), ppot, pgrad, phess, _i(nt), ptrg, _i(ifpghtarg), ppt, pgt, pht]
-    name = {"c": "cfmm2d", "l": "lfmm2d", "h": "hfmm2d"}[kind]
+    name = {"c": "cfmm2d", "l": "lfmm2d", "h": "hfmm2d", "r": "rfmm2d"}[kind]
     if ndiv is not None:
         args += [_i(ndiv), _i(idivflag), _i(ifnear), timeinfo.ctypes.data_as(C.c_void_p)]
         name += "_ndiv"
@@ -170,9 +171,14 @@ def hfmm2d(**kw):
     return _fmm("h", **kw)
 
 
+def rfmm2d(**kw):
+    return _fmm("r", **kw)
+
+
 def direct(kind, nd, sources, targets, charges=None, dipstr=None, dipvec=None,
            pgt=1, thresh=1e-16, zk=None):
-    """Reference direct kernels c2d_/l2d_/h2d_direct{c,d,cd}{p,g,h}; outputs start at 0."""
+    """Reference direct kernels c2d_/l2d_/h2d_/r2d_direct{c,d,cd}{p,g,h}; outputs start at 0
+    (kind "r": real densities and outputs)."""
     L = lib()
     sources = np.asarray(sources, dtype=np.float64)
     targets = np.asarray(targets, dtype=np.float64)
@@ -181,13 +187,14 @@ def direct(kind, nd, sources, targets, charges=None, dipstr=None, dipvec=None,
     src, psrc = _farr(sources, np.float64)
     trg, ptrg = _farr(targets, np.float64)
     need_dv = kind != "c"
-    ifc, pch, ifd, pds, pdv, keep = _src_arrays(nd, ns, charges, dipstr, dipvec, need_dv)
+    dt = np.float64 if kind == "r" else np.complex128
+    ifc, pch, ifd, pds, pdv, keep = _src_arrays(nd, ns, charges, dipstr, dipvec, need_dv, dt)
     tag = ("c" if ifc else "") + ("d" if ifd else "") + "pgh"[pgt - 1]
     gdim = (nd,) if kind == "c" else (nd, 2)
     hdim = (nd,) if kind == "c" else (nd, 3)
-    pot, ppot = _out((nd, nt))
-    grad, pgrad = _out(gdim + (nt,))
-    hess, phess = _out(hdim + (nt,))
+    pot, ppot = _out((nd, nt), dt)
+    grad, pgrad = _out(gdim + (nt,), dt)
+    hess, phess = _out(hdim + (nt,), dt)
     args = [_i(nd)]
     if kind == "h":
         zka, pzk = _z(zk)

@@ -223,8 +223,54 @@ def lfmm_real_fixtures():
     np.savez_compressed(os.path.join(HERE, "lfmm_real_ref.npz"), **out)
 
 
+# (eps, nd, ifc, ifd, pg, pgt) for rfmm2d (real densities and outputs)
+RFMM_CASES = [
+    (1e-6, 1, 1, 0, 2, 2),
+    (1e-12, 1, 1, 1, 3, 3),
+    (1e-9, 1, 0, 1, 1, 2),
+    (1e-9, 2, 1, 1, 3, 0),
+]
+
+
+def rfmm_fixtures():
+    """rfmm2d and r2d_direct outputs of the reference (points of fmm_ref.npz)."""
+    base = np.load(os.path.join(HERE, "fmm_ref.npz"))
+    src, trg = base["src"], base["trg"]
+    out = {"src": src, "trg": trg}
+    for i, (eps, nd, ifc, ifd, pg, pgt) in enumerate(RFMM_CASES):
+        ch = base[f"nd{nd}_charge"].real.copy()
+        ds = base[f"nd{nd}_dipstr"].real.copy()
+        dv = base[f"nd{nd}_dipvec"]
+        r = R.rfmm2d(nd=nd, eps=eps, sources=src, charges=ch if ifc else None,
+                     dipstr=ds if ifd else None, dipvec=dv if ifd else None,
+                     targets=trg if pgt else None, ifpgh=pg, ifpghtarg=pgt)
+        for k, v in r.items():
+            if k in ("pot", "grad", "hess", "pottarg", "gradtarg", "hesstarg"):
+                out[f"c{i}_{k}"] = v
+        out[f"c{i}_ier"] = np.array(r["ier"])
+    # r2d_direct: all 9 variants, nd = 1 and 3, on small point sets
+    rng = np.random.default_rng(17)
+    for nd in (1, 3):
+        s_ = rng.random((2, 37))
+        t_ = rng.random((2, 23))
+        t_[:, 0] = s_[:, 2]
+        ch = rng.standard_normal((nd, 37))
+        ds = rng.standard_normal((nd, 37))
+        dv = rng.standard_normal((nd, 2, 37))
+        out[f"dir{nd}_src"], out[f"dir{nd}_trg"] = s_, t_
+        out[f"dir{nd}_charge"], out[f"dir{nd}_dipstr"], out[f"dir{nd}_dipvec"] = ch, ds, dv
+        for ifc, ifd in ((1, 0), (0, 1), (1, 1)):
+            for pgt in (1, 2, 3):
+                r = R.direct("r", nd, s_, t_, charges=ch if ifc else None,
+                             dipstr=ds if ifd else None, dipvec=dv if ifd else None, pgt=pgt,
+                             thresh=1e-14)
+                for k, v in r.items():
+                    out[f"dir{nd}_{ifc}{ifd}{pgt}_{k}"] = v
+    np.savez_compressed(os.path.join(HERE, "rfmm_ref.npz"), **out)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm", "hfmm", "lreal"]
+    which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm", "hfmm", "lreal", "rfmm"]
     if "direct" in which:
         direct_fixtures()
     if "hankel" in which:
@@ -237,5 +283,7 @@ if __name__ == "__main__":
         hfmm_fixtures()
     if "lreal" in which:
         lfmm_real_fixtures()
+    if "rfmm" in which:
+        rfmm_fixtures()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))

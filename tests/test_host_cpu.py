@@ -69,8 +69,7 @@ def test_shape_assertions():
 
 
 def test_out_of_scope_families():
-    for fn in (fmm2d_amd.rfmm2d, fmm2d_amd.bhfmm2d, fmm2d_amd.stfmm2d, fmm2d_amd.r2ddir,
-               fmm2d_amd.bh2ddir, fmm2d_amd.st2ddir):
+    for fn in (fmm2d_amd.bhfmm2d, fmm2d_amd.stfmm2d, fmm2d_amd.bh2ddir, fmm2d_amd.st2ddir):
         with pytest.raises(NotImplementedError):
             fn(eps=1e-6, sources=np.zeros((2, 3)))
 
