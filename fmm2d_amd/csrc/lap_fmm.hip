@@ -528,51 +528,214 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
 }
 
 // ---------------------------------------------------------- near field
-// Mixed-precision form of pair_cauchy (p2p_math.h): the displacement and
-// the exclusion test stay FP64 (so near-coincident pairs and the thresh rule
-// are exactly those of the FP64 path); log|z|, 1/z, -1/z^2, 2/z^3 and their
-// products with the strengths are FP32; the sums are accumulated in FP64.
-template <int IFC, int IFD, int PG, int NDT>
-__device__ __forceinline__ void pair_cauchy_f32(double dx, double dy, const SrcStr<NDT>& s,
-                                                double thresh, double thresh2,
-                                                Acc<CAUCHY, PG, NDT>& a) {
-  const double rr = dx * dx + dy * dy;
-  double absz = 0.0;
-  if (!IFC) absz = hypot(dx, dy);
-  if (skip_pair<CAUCHY, IFC, IFD, PG>(rr, thresh, thresh2, absz)) return;
-  const float fx = (float)dx, fy = (float)dy;
-  const float frr = fx * fx + fy * fy;
-  const float ri = 1.0f / frr;
-  const float z1r = fx * ri, z1i = -fy * ri;                          // 1/z
-  const float z2r = z1i * z1i - z1r * z1r, z2i = -2.0f * z1r * z1i;   // -1/z^2
-  const float z3r = -2.0f * (z2r * z1r - z2i * z1i);                 // 2/z^3
-  const float z3i = -2.0f * (z2r * z1i + z2i * z1r);
-  const float lg = IFC ? 0.5f * logf(frr) : 0.0f;
+// cfmm2dpart_direct (laplace/cfmm2d.f:1450-1616): the list-1 sources of the
+// point's leaf through the c2d_direct* variant of the flags, one thread per
+// evaluation point.  The sources of a list-1 box are taken U = 4 at a time:
+// their loads are issued together and the exclusion rule masks a pair's
+// strengths instead of branching; the last (s1 - s0) mod 4 go one by one.
+//
+// RE = 1: the charges are real and only Re(pot) is consumed (lfmm2d and
+// rfmm2d split to Cauchy densities with zero imaginary parts,
+// laplace/lfmm2d.f:139-156, laplace/rfmm2d.f:135-148), so the charge terms
+// take the real charge alone.
+//
+// F32 = 1 (mixed precision): the displacement is formed in FP64 and rounded
+// to FP32; log|z|, 1/z, -1/z^2, 2/z^3 and their products with FP32 copies of
+// the strengths are FP32 (hardware reciprocal and log2), summed in FP32 over
+// one list-1 box and added to FP64 accumulators after it.  The FP32 form
+// takes the pairs with lo < |z|^2 < 1e20 (FP32), lo = max(1e-20, 4 thresh^2)
+// rounded up: no such pair can be excluded, and the powers of 1/z stay in
+// the normal FP32 range.  Every other pair (coincident points among them)
+// goes through the exact FP64 exclusion rule and the FP64 form.
+
+// c2d_direct* for one pair, FP64 (RE: real charges, Re(pot) only)
+template <int IFC, int IFD, int PG, int NDT, int RE>
+__device__ __forceinline__ void pair_c64(double dx, double dy, double rr, const SrcStr<NDT>& s,
+                                         Acc<CAUCHY, PG, NDT>& a) {
+  const double ri = 1.0 / rr;
+  const cplx zinv = cplx{dx * ri, -dy * ri};  // 1/z
+  const cplx z2 = -(zinv * zinv);             // -1/z^2
+  const cplx z3 = (-2.0) * (z2 * zinv);       // 2/z^3
+  const double lg = IFC ? 0.5 * log(rr) : 0.0;
 #pragma unroll
   for (int d = 0; d < NDT; ++d) {
     if (IFC) {
-      const float cr = (float)s.charge[d].re, ci = (float)s.charge[d].im;
-      a.pot[d] += cplx{(double)(lg * cr), (double)(lg * ci)};
-      if (PG >= 2) a.grad[d][0] += cplx{(double)(z1r * cr - z1i * ci), (double)(z1r * ci + z1i * cr)};
-      if (PG >= 3) a.hess[d][0] += cplx{(double)(z2r * cr - z2i * ci), (double)(z2r * ci + z2i * cr)};
+      if (RE) {
+        const double c = s.charge[d].re;
+        a.pot[d].re += lg * c;
+        if (PG >= 2) a.grad[d][0] += c * zinv;
+        if (PG >= 3) a.hess[d][0] += c * z2;
+      } else {
+        a.pot[d] += lg * s.charge[d];
+        if (PG >= 2) a.grad[d][0] += zinv * s.charge[d];
+        if (PG >= 3) a.hess[d][0] += z2 * s.charge[d];
+      }
     }
     if (IFD) {
-      const float cr = (float)s.dipstr[d].re, ci = (float)s.dipstr[d].im;
-      a.pot[d] += cplx{(double)(z1r * cr - z1i * ci), (double)(z1r * ci + z1i * cr)};
-      if (PG >= 2) a.grad[d][0] += cplx{(double)(z2r * cr - z2i * ci), (double)(z2r * ci + z2i * cr)};
-      if (PG >= 3) a.hess[d][0] += cplx{(double)(z3r * cr - z3i * ci), (double)(z3r * ci + z3i * cr)};
+      if (RE) {
+        a.pot[d].re += zinv.re * s.dipstr[d].re - zinv.im * s.dipstr[d].im;
+      } else {
+        a.pot[d] += zinv * s.dipstr[d];
+      }
+      if (PG >= 2) a.grad[d][0] += z2 * s.dipstr[d];
+      if (PG >= 3) a.hess[d][0] += z3 * s.dipstr[d];
     }
   }
 }
 
-// cfmm2dpart_direct (laprouts2d.f -> cfmm2d.f:1450-1616): list-1 sources of
-// the point's leaf through the c2d_direct* variant matching the flags (F32:
-// the mixed-precision pair above).
-template <int IFC, int IFD, int PG, int NDT, int F32>
-__global__ void k_near(int64_t n, int nd, int d0, const double* xy, const int* leaf,
-                       const int* n1, const int* l1, const int* sse, const double* xs,
-                       const cplx* chs, const cplx* dps, double thresh, cplx* pot, cplx* grad,
-                       cplx* hess) {
+// FP32 sums of one list-1 box
+// (lr, li: charge times log2|z|^2, scaled by ln(2)/2 when folded)
+template <int PG, int NDT> struct AccF {
+  float pr[NDT], pi[NDT], lr[NDT], li[NDT], gr[NDT], gi[NDT], hr[NDT], hi[NDT];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int d = 0; d < NDT; ++d)
+      pr[d] = pi[d] = lr[d] = li[d] = gr[d] = gi[d] = hr[d] = hi[d] = 0.0f;
+  }
+  __device__ __forceinline__ void fold(Acc<CAUCHY, PG, NDT>& a) const {
+    constexpr double kHalfLn2 = 0.34657359027997264;
+#pragma unroll
+    for (int d = 0; d < NDT; ++d) {
+      a.pot[d] += cplx{(double)pr[d] + kHalfLn2 * (double)lr[d],
+                       (double)pi[d] + kHalfLn2 * (double)li[d]};
+      if (PG >= 2) a.grad[d][0] += cplx{(double)gr[d], (double)gi[d]};
+      if (PG >= 3) a.hess[d][0] += cplx{(double)hr[d], (double)hi[d]};
+    }
+  }
+};
+
+// FP32 strengths of one source: charge (cr, ci; ci unused with RE) and
+// dipole strength (dr, di)
+template <int NDT> struct StrF {
+  float cr[NDT], ci[NDT], dr[NDT], di[NDT];
+};
+
+// FP32 strength layout: charges ns x nd floats (RE) or (re, im) pairs,
+// dipole strengths (re, im) pairs
+template <int IFC, int IFD, int NDT, int RE>
+__device__ __forceinline__ void load_strf(int64_t j, int nd, int d0, const float* ch32,
+                                          const float* dp32, StrF<NDT>& s) {
+#pragma unroll
+  for (int dd = 0; dd < NDT; ++dd) {
+    const bool ok = dd == 0 || d0 + dd < nd;  // d0 < nd always
+    const int64_t e = j * nd + d0 + dd;
+    s.cr[dd] = s.ci[dd] = s.dr[dd] = s.di[dd] = 0.0f;
+    if (IFC && ok) {
+      if (RE) {
+        s.cr[dd] = ch32[e];
+      } else {
+        s.cr[dd] = ch32[2 * e];
+        s.ci[dd] = ch32[2 * e + 1];
+      }
+    }
+    if (IFD && ok) {
+      s.dr[dd] = dp32[2 * e];
+      s.di[dd] = dp32[2 * e + 1];
+    }
+  }
+}
+
+// the FP32 form; w = 0 drops the pair (its frr is replaced by 1)
+template <int IFC, int IFD, int PG, int NDT, int RE>
+__device__ __forceinline__ void pair_f32(float fx, float fy, float frr, bool w,
+                                         const StrF<NDT>& s, AccF<PG, NDT>& f) {
+  frr = w ? frr : 1.0f;
+  const float ri = __builtin_amdgcn_rcpf(frr);
+  const float z1r = fx * ri, z1i = -fy * ri;                          // 1/z
+  const float z2r = z1i * z1i - z1r * z1r, z2i = -2.0f * z1r * z1i;   // -1/z^2
+  const float z3r = -2.0f * (z2r * z1r - z2i * z1i);                 // 2/z^3
+  const float z3i = -2.0f * (z2r * z1i + z2i * z1r);
+  const float l2 = IFC ? __builtin_amdgcn_logf(frr) : 0.0f;           // log2 |z|^2
+#pragma unroll
+  for (int d = 0; d < NDT; ++d) {
+    if (IFC) {
+      const float cr = w ? s.cr[d] : 0.0f;
+      f.lr[d] += l2 * cr;
+      if (RE) {
+        if (PG >= 2) { f.gr[d] += z1r * cr; f.gi[d] += z1i * cr; }
+        if (PG >= 3) { f.hr[d] += z2r * cr; f.hi[d] += z2i * cr; }
+      } else {
+        const float ci = w ? s.ci[d] : 0.0f;
+        f.li[d] += l2 * ci;
+        if (PG >= 2) { f.gr[d] += z1r * cr - z1i * ci; f.gi[d] += z1r * ci + z1i * cr; }
+        if (PG >= 3) { f.hr[d] += z2r * cr - z2i * ci; f.hi[d] += z2r * ci + z2i * cr; }
+      }
+    }
+    if (IFD) {
+      const float cr = w ? s.dr[d] : 0.0f, ci = w ? s.di[d] : 0.0f;
+      f.pr[d] += z1r * cr - z1i * ci;
+      if (!RE) f.pi[d] += z1r * ci + z1i * cr;
+      if (PG >= 2) { f.gr[d] += z2r * cr - z2i * ci; f.gi[d] += z2r * ci + z2i * cr; }
+      if (PG >= 3) { f.hr[d] += z3r * cr - z3i * ci; f.hi[d] += z3r * ci + z3i * cr; }
+    }
+  }
+}
+
+constexpr int kNearU = 4;
+
+// strengths of source j (zero beyond nd)
+template <int IFC, int IFD, int NDT>
+__device__ __forceinline__ void load_str(int64_t j, int nd, int d0, const cplx* chs,
+                                         const cplx* dps, SrcStr<NDT>& s) {
+#pragma unroll
+  for (int dd = 0; dd < NDT; ++dd) {
+    const bool ok = dd == 0 || d0 + dd < nd;  // d0 < nd always
+    if (IFC) s.charge[dd] = ok ? chs[j * nd + d0 + dd] : mkc(0.0);
+    if (IFD) s.dipstr[dd] = ok ? dps[j * nd + d0 + dd] : mkc(0.0);
+  }
+}
+
+template <int NDT> __device__ __forceinline__ void zero_str(SrcStr<NDT>& s) {
+#pragma unroll
+  for (int dd = 0; dd < NDT; ++dd) s.charge[dd] = s.dipstr[dd] = mkc(0.0);
+}
+
+// one pair through the exact exclusion rule and the FP64 form
+template <int IFC, int IFD, int PG, int NDT, int RE>
+__device__ __forceinline__ void near_pair64(double dx, double dy, SrcStr<NDT>& s, double thresh,
+                                            double thresh2, Acc<CAUCHY, PG, NDT>& a) {
+  double rr = dx * dx + dy * dy;
+  // exclusion rule of c2d_direct* (|z| computed only where it can decide)
+  double absz = 2.0 * thresh;
+  if (!IFC && rr <= 4.0 * thresh2) absz = hypot(dx, dy);
+  if (skip_pair<CAUCHY, IFC, IFD, PG>(rr, thresh, thresh2, absz)) {
+    zero_str(s);
+    rr = 1.0;
+    dx = 1.0;
+    dy = 0.0;
+  }
+  pair_c64<IFC, IFD, PG, NDT, RE>(dx, dy, rr, s, a);
+}
+
+// one pair of the mixed-precision path
+template <int IFC, int IFD, int PG, int NDT, int RE>
+__device__ __forceinline__ void near_pair32(double dx, double dy, const StrF<NDT>& sf, int64_t j,
+                                            int nd, int d0, const cplx* chs, const cplx* dps,
+                                            double thresh, double thresh2, float lo,
+                                            Acc<CAUCHY, PG, NDT>& a, AccF<PG, NDT>& f) {
+  const float fx = (float)dx, fy = (float)dy;
+  const float frr = fx * fx + fy * fy;
+  const bool fast = frr > lo && frr < 1e20f;
+  pair_f32<IFC, IFD, PG, NDT, RE>(fx, fy, frr, fast, sf, f);
+  if (!fast) {  // coincident points (excluded) or |z| out of the FP32 range
+    const double rr = dx * dx + dy * dy;
+    double absz = 2.0 * thresh;
+    if (!IFC && rr <= 4.0 * thresh2) absz = hypot(dx, dy);
+    if (!skip_pair<CAUCHY, IFC, IFD, PG>(rr, thresh, thresh2, absz)) {
+      SrcStr<NDT> s;
+      load_str<IFC, IFD, NDT>(j, nd, d0, chs, dps, s);
+      pair_c64<IFC, IFD, PG, NDT, RE>(dx, dy, rr, s, a);
+    }
+  }
+}
+
+template <int IFC, int IFD, int PG, int NDT, int F32, int RE>
+__global__ __launch_bounds__(256) void k_near(int64_t n, int nd, int d0, const double* xy,
+                                              const int* leaf, const int* n1, const int* l1,
+                                              const int* sse, const double* xs, const cplx* chs,
+                                              const cplx* dps, const float* ch32,
+                                              const float* dp32, double thresh, float lo,
+                                              cplx* pot, cplx* grad, cplx* hess) {
   const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = leaf[i];
@@ -581,29 +744,64 @@ __global__ void k_near(int64_t n, int nd, int d0, const double* xy, const int* l
   Acc<CAUCHY, PG, NDT> a;
   a.zero();
   const int c1 = n1[b];
-  for (int k = 0; k < c1; ++k) {
-    const int c = l1[(int64_t)kMaxList1 * b + k];
-    for (int j = sse[2 * c]; j < sse[2 * c + 1]; ++j) {
-      SrcStr<NDT> s;
+  for (int kk = 0; kk < c1; ++kk) {
+    const int c = l1[(int64_t)kMaxList1 * b + kk];
+    const int s0 = sse[2 * c], s1 = sse[2 * c + 1];
+    int j = s0;
+    if (F32) {
+      AccF<PG, NDT> f;
+      f.zero();
+      for (; j + kNearU <= s1; j += kNearU) {
+        double sx[kNearU], sy[kNearU];
+        StrF<NDT> sf[kNearU];
 #pragma unroll
-      for (int dd = 0; dd < NDT; ++dd) {
-        const bool ok = d0 + dd < nd;
-        if (IFC) s.charge[dd] = ok ? chs[(int64_t)j * nd + d0 + dd] : mkc(0.0);
-        if (IFD) s.dipstr[dd] = ok ? dps[(int64_t)j * nd + d0 + dd] : mkc(0.0);
+        for (int u = 0; u < kNearU; ++u) {
+          sx[u] = xs[2 * (j + u)];
+          sy[u] = xs[2 * (j + u) + 1];
+          load_strf<IFC, IFD, NDT, RE>(j + u, nd, d0, ch32, dp32, sf[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kNearU; ++u)
+          near_pair32<IFC, IFD, PG, NDT, RE>(x - sx[u], y - sy[u], sf[u], j + u, nd, d0, chs,
+                                             dps, thresh, thresh2, lo, a, f);
       }
-      if (F32)
-        pair_cauchy_f32<IFC, IFD, PG, NDT>(x - xs[2 * j], y - xs[2 * j + 1], s, thresh, thresh2,
+      for (; j < s1; ++j) {
+        StrF<NDT> sf;
+        load_strf<IFC, IFD, NDT, RE>(j, nd, d0, ch32, dp32, sf);
+        near_pair32<IFC, IFD, PG, NDT, RE>(x - xs[2 * j], y - xs[2 * j + 1], sf, j, nd, d0, chs,
+                                           dps, thresh, thresh2, lo, a, f);
+      }
+      f.fold(a);
+    } else {
+      for (; j + kNearU <= s1; j += kNearU) {
+        double sx[kNearU], sy[kNearU];
+        SrcStr<NDT> s[kNearU];
+#pragma unroll
+        for (int u = 0; u < kNearU; ++u) {
+          sx[u] = xs[2 * (j + u)];
+          sy[u] = xs[2 * (j + u) + 1];
+          load_str<IFC, IFD, NDT>(j + u, nd, d0, chs, dps, s[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kNearU; ++u)
+          near_pair64<IFC, IFD, PG, NDT, RE>(x - sx[u], y - sy[u], s[u], thresh, thresh2, a);
+      }
+      for (; j < s1; ++j) {
+        SrcStr<NDT> s;
+        load_str<IFC, IFD, NDT>(j, nd, d0, chs, dps, s);
+        near_pair64<IFC, IFD, PG, NDT, RE>(x - xs[2 * j], y - xs[2 * j + 1], s, thresh, thresh2,
                                            a);
-      else
-        pair_cauchy<IFC, IFD, PG, NDT>(x - xs[2 * j], y - xs[2 * j + 1], s, thresh, thresh2, a);
+      }
     }
   }
 #pragma unroll
   for (int dd = 0; dd < NDT; ++dd) {
     if (d0 + dd >= nd) break;
-    pot[i * nd + d0 + dd] += a.pot[dd];
-    if (PG >= 2) grad[i * nd + d0 + dd] += a.grad[dd][0];
-    if (PG >= 3) hess[i * nd + d0 + dd] += a.hess[dd][0];
+    const int64_t o = i * nd + d0 + dd;
+    if (RE) pot[o].re += a.pot[dd].re;
+    else pot[o] += a.pot[dd];
+    if (PG >= 2) grad[o] += a.grad[dd][0];
+    if (PG >= 3) hess[o] += a.hess[dd][0];
   }
 }
 
@@ -662,58 +860,72 @@ void make_matrices(int P, std::vector<cplx>& A, std::vector<cplx>& B, std::vecto
     }
 }
 
-template <int PG, int NDT, int F32>
-void launch_near_pg(int ifc, int ifd, int64_t n, hipStream_t st, int nd, int d0, const double* xy,
-                    const int* leaf, const Lists& L, const int* sse, const double* xs,
-                    const cplx* chs, const cplx* dps, double thresh, cplx* p, cplx* g, cplx* h) {
-  if (ifc && !ifd)
-    launch1d_xcd(k_near<1, 0, PG, NDT, F32>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs,
-                 chs, dps, thresh, p, g, h);
-  else if (!ifc && ifd)
-    launch1d_xcd(k_near<0, 1, PG, NDT, F32>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs,
-                 chs, dps, thresh, p, g, h);
-  else
-    launch1d_xcd(k_near<1, 1, PG, NDT, F32>, n, st, nd, d0, xy, leaf, L.n1.p, L.l1.p, sse, xs,
-                 chs, dps, thresh, p, g, h);
+struct NearArgs {
+  int nd;
+  const double* xy;  // sorted evaluation points
+  const int* leaf;   // leaf box of each
+  const Lists* L;
+  const int* sse;
+  const double* xs;
+  const cplx* chs;
+  const cplx* dps;
+  const float* ch32;  // FP32 strengths (mixed precision), see load_strf
+  const float* dp32;
+  double thresh;
+  cplx *p, *g, *h;
+};
+
+template <int PG, int NDT, int F32, int RE>
+void launch_near_pg(int ifc, int ifd, int64_t n, hipStream_t st, int d0, const NearArgs& q) {
+  // FP32 pairs need |z|^2 > max(1e-20, 4 thresh^2): no exclusion, normal range
+  const float lo = (float)(std::max(1e-20, 4.0 * q.thresh * q.thresh) * 1.001);
+#define FMM2D_NEAR_L(IFC, IFD)                                                                   \
+  launch1d_xcd(k_near<IFC, IFD, PG, NDT, F32, RE>, n, st, q.nd, d0, q.xy, q.leaf, q.L->n1.p,    \
+               q.L->l1.p, q.sse, q.xs, q.chs, q.dps, q.ch32, q.dp32, q.thresh, lo, q.p, q.g, q.h)
+  if (ifc && !ifd) FMM2D_NEAR_L(1, 0);
+  else if (!ifc && ifd) FMM2D_NEAR_L(0, 1);
+  else FMM2D_NEAR_L(1, 1);
+#undef FMM2D_NEAR_L
 }
 
-template <int F32>
-void launch_near_f(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd,
-                   const double* xy, const int* leaf, const Lists& L, const int* sse,
-                   const double* xs, const cplx* chs, const cplx* dps, double thresh, cplx* p,
-                   cplx* g, cplx* h) {
-  const int NDT = (nd == 1) ? 1 : 2;
-  for (int d0 = 0; d0 < nd; d0 += NDT) {
-#define FMM2D_NEAR(PG, NDT_)                                                                 \
-  launch_near_pg<PG, NDT_, F32>(ifc, ifd, n, st, nd, d0, xy, leaf, L, sse, xs, chs, dps, thresh, \
-                                p, g, h)
+template <int F32, int RE>
+void launch_near_f(int pg, int ifc, int ifd, int64_t n, hipStream_t st, const NearArgs& q) {
+  const int NDT = (q.nd == 1) ? 1 : 2;
+  for (int d0 = 0; d0 < q.nd; d0 += NDT) {
     if (NDT == 1) {
-      if (pg == 1) FMM2D_NEAR(1, 1);
-      else if (pg == 2) FMM2D_NEAR(2, 1);
-      else FMM2D_NEAR(3, 1);
+      if (pg == 1) launch_near_pg<1, 1, F32, RE>(ifc, ifd, n, st, d0, q);
+      else if (pg == 2) launch_near_pg<2, 1, F32, RE>(ifc, ifd, n, st, d0, q);
+      else launch_near_pg<3, 1, F32, RE>(ifc, ifd, n, st, d0, q);
     } else {
-      if (pg == 1) FMM2D_NEAR(1, 2);
-      else if (pg == 2) FMM2D_NEAR(2, 2);
-      else FMM2D_NEAR(3, 2);
+      if (pg == 1) launch_near_pg<1, 2, F32, RE>(ifc, ifd, n, st, d0, q);
+      else if (pg == 2) launch_near_pg<2, 2, F32, RE>(ifc, ifd, n, st, d0, q);
+      else launch_near_pg<3, 2, F32, RE>(ifc, ifd, n, st, d0, q);
     }
-#undef FMM2D_NEAR
   }
 }
 
-void launch_near(bool f32, int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd,
-                 const double* xy, const int* leaf, const Lists& L, const int* sse,
-                 const double* xs, const cplx* chs, const cplx* dps, double thresh, cplx* p,
-                 cplx* g, cplx* h) {
-  if (f32)
-    launch_near_f<1>(pg, ifc, ifd, n, st, nd, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
-  else
-    launch_near_f<0>(pg, ifc, ifd, n, st, nd, xy, leaf, L, sse, xs, chs, dps, thresh, p, g, h);
+// re: charges real and only Re(pot) consumed (lfmm2d, rfmm2d)
+void launch_near(bool f32, bool re, int pg, int ifc, int ifd, int64_t n, hipStream_t st,
+                 const NearArgs& q) {
+  if (f32) {
+    if (re) launch_near_f<1, 1>(pg, ifc, ifd, n, st, q);
+    else launch_near_f<1, 0>(pg, ifc, ifd, n, st, q);
+  } else {
+    if (re) launch_near_f<0, 1>(pg, ifc, ifd, n, st, q);
+    else launch_near_f<0, 0>(pg, ifc, ifd, n, st, q);
+  }
 }
 
 // mixed precision: FP32 copy of the multipole expansions for the FP32 M2L
 __global__ void k_to_f32(int64_t n, const double* in, float* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (float)in[i];
+}
+
+// FP32 real parts (the near field's real charges)
+__global__ void k_re_to_f32(int64_t n, const cplx* in, float* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i].re;
 }
 
 void launch_eval(int pg, int64_t n, hipStream_t st, int nd, int P, const double* xy,
@@ -954,12 +1166,25 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     const int ifnear = o ? o->ifnear : 1;
     if (ifnear) {
       const double thresh = t.boxsize[0] * std::ldexp(1.0, -51);
+      const bool re = kind != 0;  // real Cauchy charges, Re(pot) consumed
+      DBuf<float> ch32, dp32;
+      if (mixed && a.ifcharge) {
+        ch32 = DBuf<float>((re ? 1 : 2) * ns * nd, st);
+        if (re) launch1d(k_re_to_f32, ns * nd, st, chs.p, ch32.p);
+        else launch1d(k_to_f32, 2 * ns * nd, st, reinterpret_cast<const double*>(chs.p), ch32.p);
+      }
+      if (mixed && a.ifdipole) {
+        dp32 = DBuf<float>(2 * ns * nd, st);
+        launch1d(k_to_f32, 2 * ns * nd, st, reinterpret_cast<const double*>(dps.p), dp32.p);
+      }
       if (pg > 0)
-        launch_near(mixed, pg, a.ifcharge, a.ifdipole, ns, st, nd, xs.p, leaf_s.p, L, sse, xs.p, chs.p,
-                    dps.p, thresh, ps.p, gs.p, hs.p);
+        launch_near(mixed, re, pg, a.ifcharge, a.ifdipole, ns, st,
+                    NearArgs{nd, xs.p, leaf_s.p, &L, sse, xs.p, chs.p, dps.p, ch32.p, dp32.p,
+                             thresh, ps.p, gs.p, hs.p});
       if (pgt > 0)
-        launch_near(mixed, pgt, a.ifcharge, a.ifdipole, nt, st, nd, xt.p, leaf_t.p, L, sse, xs.p,
-                    chs.p, dps.p, thresh, pt.p, gt.p, ht.p);
+        launch_near(mixed, re, pgt, a.ifcharge, a.ifdipole, nt, st,
+                    NearArgs{nd, xt.p, leaf_t.p, &L, sse, xs.p, chs.p, dps.p, ch32.p, dp32.p,
+                             thresh, pt.p, gt.p, ht.p});
     }
     tm.mark(7);
     // ---- unsort
