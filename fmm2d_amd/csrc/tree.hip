@@ -14,8 +14,8 @@
 // GPU formulation: one thread per point for classification into children
 // (y - cy < 0 then x - cx < 0, as sort_pts_to_children 659-754), one thread
 // per box for every box-level step, hipCUB scans where the reference uses
-// cumsum, and the final point order as a stable radix sort of the points by
-// the depth-first start offset of their leaf -- which is exactly the order the
+// cumsum, and the final point order as the points of each leaf (a contiguous
+// range of the path order) in input order -- which is exactly the order the
 // reference's recursive stable partition produces.  Box-level steps run level
 // by level where the reference's loops carry a dependency between levels.
 #include <hipcub/hipcub.hpp>
@@ -124,9 +124,10 @@ __global__ void k_bbox_final(const double* part, int nb, double* out) {
 // y - cy < 0 / x - cx < 0.  The path does not depend on which boxes get
 // refined, so it is computed once per point; digits are child indices 0..3 in
 // the reference's order (1=(-,-), 2=(+,-), 3=(-,+), 4=(+,+)), most significant
-// digit = level 1.  Levels 1..31 go to key1, levels 32..62 to key2 (only
-// computed when the tree is that deep).
-constexpr int kD1 = 31;
+// digit = level 1.  Levels 1..20 go to key1 (a 40-bit radix sort), levels
+// 21..51 to key2 (only computed when the tree is that deep).
+constexpr int kD1 = 20;
+constexpr int kD2Max = 31;
 
 __global__ void k_path_keys(int64_t n, const double* __restrict__ xy, double cx0, double cy0,
                             const double* __restrict__ half, int d1, int d2,
@@ -421,6 +422,44 @@ __global__ void k_leaf_label(int64_t n, const int* nchild, const int* se, const 
   for (int i = s; i < se[2 * b + 1]; ++i) label[perm[i]] = s;
 }
 
+// The same order without the sort: inside each leaf (a contiguous range of
+// the path order) the input indices in increasing order, placed by rank.
+// One wavefront per box; leaves of up to 64 points rank through lane
+// shuffles, leaves of up to kLeafSortMax points through LDS; a larger leaf
+// sets *big and the caller falls back to the labelled sort.
+constexpr int kLeafSortMax = 2048;
+
+__global__ __launch_bounds__(64) void k_leaf_sort(int64_t nthreads, const int* nchild,
+                                                  const int* se, const int* perm, int* out,
+                                                  int* big) {
+  __shared__ int sh[kLeafSortMax];
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gt >= nthreads) return;
+  const int b = (int)(gt >> 6), lane = threadIdx.x;
+  if (nchild[b] != 0) return;
+  const int s = se[2 * b], m = se[2 * b + 1] - s;
+  if (m <= 0) return;
+  if (m > kLeafSortMax) {
+    if (lane == 0) *big = 1;
+    return;
+  }
+  if (m <= 64) {
+    const int v = lane < m ? perm[s + lane] : 0x7fffffff;
+    int rank = 0;
+    for (int j = 0; j < m; ++j) rank += (__shfl(v, j) < v) ? 1 : 0;
+    if (lane < m) out[s + rank] = v;
+    return;
+  }
+  for (int i = lane; i < m; i += 64) sh[i] = perm[s + i];
+  __syncthreads();
+  for (int i = lane; i < m; i += 64) {
+    const int v = sh[i];
+    int rank = 0;
+    for (int j = 0; j < m; ++j) rank += (sh[j] < v) ? 1 : 0;
+    out[s + rank] = v;
+  }
+}
+
 __global__ void k_iota(int64_t n, int* v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] = (int)i;
@@ -659,7 +698,7 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
       // a box at level d1 may need splitting: extend the paths (rare: only
       // for points closer than 2^-31 of the domain)
       if (bld.scan(n) > 0) {
-        const int d2 = std::min(62, nlmax) - d1;
+        const int d2 = std::min(kD2Max, std::min(62, nlmax) - d1);
         pk[0].build(src, ns, root[0], root[1], dhalf.p, d1, d2, st);
         pk[1].build(trg, nt, root[0], root[1], dhalf.p, d1, d2, st);
         bld.ks = pk[0].view();
@@ -683,6 +722,8 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   bld.computecoll(ls, le, bs, nlevels);
 
   // ---- level restriction (pts_tree_fix_lr, pts_tree2d.f:758-1061)
+  std::vector<int> ts1(nlevels + 1, 0), te1(nlevels + 1, 0);
+  int added1 = 0;
   if (nlevels >= 2) {
     int* iflag = bld.B.iflag.p;
     hip_throw(hipMemsetAsync(iflag, 0, sizeof(int) * bld.nboxes, st), "memset");
@@ -696,7 +737,6 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
       launch1d(k_flag2, le[l] - ls[l], st, ls[l], distest, bld.B.iparent.p, bld.B.ncoll.p,
                bld.B.coll.p, bld.B.nchild.p, bld.B.centers.p, iflag);
     }
-    std::vector<int> ts(nlevels + 1, 0), te(nlevels + 1, 0);
     {
       // the flags are fixed before this pass, so all levels refine in one scan
       std::vector<int> lo, n, lev, first, last;
@@ -709,15 +749,21 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
       }
       bld.refine_segments(lo, n, lev, bsc, first, last);
       for (int l = 1; l <= nlevels - 2; ++l) {
-        ts[l + 1] = first[l - 1];
-        te[l + 1] = last[l - 1];
+        ts1[l + 1] = first[l - 1];
+        te1[l + 1] = last[l - 1];
+        added1 += last[l - 1] - first[l - 1];
       }
     }
-    bld.reorg(ls, le, ts, te, nlevels);
+  }
+  // With no box added by the flagged refinement no box carries flag 3, the
+  // updateflags sweep below refines nothing and both reorganisations are the
+  // identity: the tree is already level-restricted.
+  if (nlevels >= 2 && added1 > 0) {
+    bld.reorg(ls, le, ts1, te1, nlevels);
     bld.computecoll(ls, le, bs, nlevels);
     launch1d(k_keep3, bld.nboxes, st, bld.B.iflag.p);
-    std::fill(ts.begin(), ts.end(), 0);
-    std::fill(te.begin(), te.end(), 0);
+    std::vector<int> ts(nlevels + 1, 0), te(nlevels + 1, 0);
+    int added2 = 0;
     for (int l = 2; l <= nlevels - 2; ++l) {
       const double distest = 1.05 * (bs[l] + bs[l + 1]) / 2.0;
       launch1d(k_updateflags, le[l] - ls[l], st, ls[l], distest, bld.B.ncoll.p, bld.B.coll.p,
@@ -731,12 +777,17 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
                             {bs[l + 1], bs[l + 1]}, first, last);
         ts[l + 1] = first[0];
         te[l + 1] = last[1];
+        added2 += te[l + 1] - ts[l + 1];
       }
       launch1d(k_coll, te[l + 1] - ts[l + 1], st, ts[l + 1], bs[l + 1], bld.B.iparent.p,
                bld.B.ichild.p, bld.B.centers.p, bld.B.ncoll.p, bld.B.coll.p);
     }
-    bld.reorg(ls, le, ts, te, nlevels);
-    bld.computecoll(ls, le, bs, nlevels);
+    // (with nothing added the second reorganisation is the identity and the
+    // colleagues are those computed above)
+    if (added2 > 0) {
+      bld.reorg(ls, le, ts, te, nlevels);
+      bld.computecoll(ls, le, bs, nlevels);
+    }
   }
 
   // ---- final point order (pts_tree_sort, pts_tree2d.f:1222-1246)
@@ -751,8 +802,21 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   DBuf<int>* perm[2] = {&t.isrc, &t.itarg};
   const int* se[2] = {bld.B.ses.p, bld.B.set.p};
   const int64_t npts[2] = {ns, nt};
+  DBuf<int> big(2, st);
+  big.zero();
   for (int k = 0; k < 2; ++k) {
     if (npts[k] == 0) continue;
+    const int64_t nthr = (int64_t)nb * 64;
+    hipLaunchKernelGGL(k_leaf_sort, dim3((unsigned)ceil_div(nthr, 64)), dim3(64), 0, st, nthr,
+                       bld.B.nchild.p, se[k], pk[k].perm.p, perm[k]->p, big.p + k);
+    hip_throw(hipGetLastError(), "kernel launch");
+  }
+  int hbig[2] = {0, 0};
+  hip_throw(hipMemcpyAsync(hbig, big.p, sizeof(hbig), hipMemcpyDeviceToHost, st), "copy");
+  hip_throw(hipStreamSynchronize(st), "sync");
+  for (int k = 0; k < 2; ++k) {
+    if (npts[k] == 0 || !hbig[k]) continue;
+    // a leaf too large for k_leaf_sort: stable sort of all points by leaf
     DBuf<int> label(npts[k], st), iota(npts[k], st), lsorted(npts[k], st);
     launch1d(k_leaf_label, nb, st, bld.B.nchild.p, se[k], pk[k].perm.p, label.p);
     int bits = 1;
