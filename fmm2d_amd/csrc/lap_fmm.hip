@@ -24,6 +24,8 @@
 #include <complex>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "devbuf.h"
 #include "fmm_common.h"
 #include "lap_ops.h"
@@ -217,15 +219,34 @@ __global__ __launch_bounds__(kW) void k_p2m(int nd, int P, const int* ilevel, co
   }
 }
 
-// Register forms of P2M and list-4 P2L for expansions of at most PT
-// coefficients: one thread per (box, density) runs over the box's sources
-// with the running products w^n, accumulating all coefficients in registers
-// (same sums as k_p2m / k_p2l above, which serve longer expansions).
-template <int PT, int IFC, int IFD>
-__global__ void k_p2m_reg(int64_t n, int nd, int P, const int* ilevel, const int* nchild,
-                          const int* nterms, const double* rsc, const double* centers,
-                          const int* sse, const double* xs, const cplx* chs, const cplx* dps,
-                          cplx* mp) {
+// Boxes with list-4 work (levels >= 2, a non-empty list 4, points to
+// evaluate): flag, then (after an exclusive scan of the flags) the list in
+// box order.
+__global__ void k_p2l_flag(int64_t nb, const int* ilevel, const int* n4, const int* sse,
+                           const int* tse, int ifpgh, int ifpght, int* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int b = (int)i;
+  flag[b] = (ilevel[b] >= 2 && n4[b] > 0 && npts_eval(b, sse, tse, ifpgh, ifpght) > 0) ? 1 : 0;
+}
+
+__global__ void k_p2l_list(int64_t nb, const int* flag, const int* pos, int* list, int* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  if (flag[i]) list[pos[i]] = (int)i;
+  if (i == nb - 1) *cnt = pos[i] + flag[i];
+}
+
+// P2M (l2dformmp{c,d,cd}, laprouts2d.f:84-282) for expansions of at most PT
+// coefficients: one thread per (box, density), for the leaves at levels >= 2,
+// runs over the leaf's sources with the running products w^n, w = (z - c)/r, accumulating
+// every coefficient in registers.  RE: real charges.
+template <int PT, int IFC, int IFD, int RE>
+__global__ __launch_bounds__(256) void k_p2m_reg(int64_t n, int nd, int P, const int* ilevel,
+                                                 const int* nchild, const int* nterms,
+                                                 const double* rsc, const double* centers,
+                                                 const int* sse, const double* xs,
+                                                 const cplx* chs, const cplx* dps, cplx* mp) {
   const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = (int)(i / nd), d = (int)(i - (int64_t)b * nd);
@@ -250,7 +271,14 @@ __global__ void k_p2m_reg(int64_t n, int nd, int P, const int* ilevel, const int
       if (k <= p) {
         if (IFD) acc[k] += v * dp;
         v = v * w;
-        if (IFC) acc[k] += (-1.0 / k) * (v * c);
+        if (IFC) {
+          if (RE) {
+            const double ck = (-1.0 / k) * c.re;
+            acc[k] += cplx{v.re * ck, v.im * ck};
+          } else {
+            acc[k] += (-1.0 / k) * (v * c);
+          }
+        }
       }
     }
   }
@@ -260,48 +288,123 @@ __global__ void k_p2m_reg(int64_t n, int nd, int P, const int* ilevel, const int
     if (k <= p) out[k] += acc[k];
 }
 
-template <int PT, int IFC, int IFD>
-__global__ void k_p2l_reg(int64_t n, int nd, int P, const int* ilevel, const int* nterms,
-                          const double* rsc, const double* centers, const int* n4, const int* l4,
-                          const int* sse, const int* tse, int ifpgh, int ifpght, const double* xs,
-                          const cplx* chs, const cplx* dps, cplx* loc) {
-  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = (int)(i / nd), d = (int)(i - (int64_t)b * nd);
-  const int lev = ilevel[b];
-  const int nbox = n4[b];
-  if (lev < 2 || nbox == 0 || npts_eval(b, sse, tse, ifpgh, ifpght) == 0) return;
-  const int p = nterms[lev];
-  const double r = rsc[lev];
-  const double cx = centers[2 * b], cy = centers[2 * b + 1];
-  cplx acc[PT];
-#pragma unroll
-  for (int k = 0; k < PT; ++k) acc[k] = mkc(0.0);
-  for (int kb = 0; kb < nbox; ++kb) {
-    const int c4 = l4[(int64_t)kMaxList4 * b + kb];
-    for (int j = sse[2 * c4]; j < sse[2 * c4 + 1]; ++j) {
-      const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
-      const cplx zi = invc(z0);
-      const cplx u = r * zi;
-      const cplx c = IFC ? chs[(int64_t)j * nd + d] : mkc(0.0);
-      const cplx dz = IFD ? -(zi * dps[(int64_t)j * nd + d]) : mkc(0.0);
-      if (IFC) acc[0] += log(absc(z0)) * c;
-      if (IFD) acc[0] += dz;
-      cplx v = mkc(1.0);  // u^k
-#pragma unroll
-      for (int k = 1; k < PT; ++k) {
-        if (k <= p) {
-          v = v * u;
-          if (IFC) acc[k] += (-1.0 / k) * (v * c);
-          if (IFD) acc[k] += v * dz;
-        }
+// Points to expansion: P2M from a leaf's own sources (l2dformmp{c,d,cd},
+// laprouts2d.f:84-282) or list-4 P2L from the sources of a box's list-4 boxes
+// (l2dformta{c,d,cd}, laprouts2d.f:501-690).  Each wavefront takes boxes of
+// `boxes` in a grid-stride loop (4 wavefronts per block, each with its own
+// LDS rows).  A box's sources are one flat sequence taken in batches of up to
+// `bat`: lane j writes row j, source j's terms S_j[n], n = 0..p, and lane
+// (g, n) of the G = floor(64/(p+1)) lane groups adds rows g, g+G, ... of
+// column n.  The 1/n of every term is applied once to the column sum:
+//   P2M, w = (z - c)/r:     T_0 = q,                 T_n = -(1/n) (w^n q - n w^(n-1) d/r)
+//   P2L, u = r/(z - c):     T_0 = log|z - c| q + e,  T_n = -(1/n) (u^n q - n u^n e),
+//                           e = -d/(z - c)
+// (q: charge, d: Cauchy dipole strength).  RE: real charges.
+constexpr int kP2xWaves = 4;
+
+template <int MODE, int IFC, int IFD, int RE>  // MODE 0: P2M, 1: P2L
+__global__ __launch_bounds__(kP2xWaves * kW) void k_p2x(
+    const int* boxes, const int* nboxes, int nd, int P, const int* ilevel, const int* nterms,
+    const double* rsc, const double* centers, const int* n4, const int* l4, const int* sse,
+    const double* xs, const cplx* chs, const cplx* dps, int bat, cplx* out) {
+  extern __shared__ cplx shall[];  // [kP2xWaves][bat][P]
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  cplx* sh = shall + (size_t)wv * bat * P;
+  const int nbx = *nboxes;
+  constexpr int KR = MODE == 0 ? 1 : kMaxList4;
+  for (int ib = blockIdx.x * kP2xWaves + wv; ib < nbx; ib += gridDim.x * kP2xWaves) {
+    const int b = boxes[ib];
+    const int lev = ilevel[b];
+    // flat numbering of the sources: range k holds [pre[k], pre[k+1])
+    int pre[KR + 1], first[KR];
+    pre[0] = 0;
+    if (MODE == 0) {
+      first[0] = sse[2 * b];
+      pre[1] = sse[2 * b + 1] - sse[2 * b];
+    } else {
+      const int nbox = n4[b];
+      for (int k = 0; k < KR; ++k) {
+        const int c = (k < nbox) ? l4[(int64_t)kMaxList4 * b + k] : -1;
+        first[k] = (c >= 0) ? sse[2 * c] : 0;
+        pre[k + 1] = pre[k] + ((c >= 0) ? sse[2 * c + 1] - sse[2 * c] : 0);
       }
     }
+    const int total = pre[KR];
+    const int p = nterms[lev], np1 = p + 1;
+    const int G = min(kW / np1, bat);  // group sums use rows 0..G-1
+    const int g = lane / np1, n = lane - g * np1;
+    const bool act = g < G;
+    const double r = rsc[lev], rinv = 1.0 / r;
+    const double cx = centers[2 * b], cy = centers[2 * b + 1];
+    for (int d = 0; d < nd; ++d) {
+      cplx acc = mkc(0.0);
+      for (int f0 = 0; f0 < total; f0 += bat) {
+        const int cnt = min(bat, total - f0);
+        if (lane < cnt) {
+          const int f = f0 + lane;
+          int k = 0;
+          while (k + 1 < KR && f >= pre[k + 1]) ++k;
+          const int64_t j = first[k] + (f - pre[k]);
+          cplx q = mkc(0.0), e = mkc(0.0);
+          if (IFC) {
+            q = chs[j * nd + d];
+            if (RE) q.im = 0.0;
+          }
+          cplx* row = sh + lane * P;
+          if (MODE == 0) {
+            const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
+            if (IFD) e = rinv * dps[j * nd + d];
+            row[0] = q;
+            cplx v = mkc(1.0);  // w^(m-1)
+            for (int m = 1; m <= p; ++m) {
+              cplx s = mkc(0.0);
+              if (IFD) s = (-(double)m) * (v * e);
+              v = v * w;
+              if (IFC) s += RE ? cplx{v.re * q.re, v.im * q.re} : v * q;
+              row[m] = s;
+            }
+          } else {
+            const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
+            const cplx zi = invc(z0);
+            const cplx u = r * zi;
+            if (IFD) e = -(zi * dps[j * nd + d]);
+            cplx t0 = e;
+            if (IFC) t0 += log(absc(z0)) * q;
+            row[0] = t0;
+            cplx v = mkc(1.0);  // u^m
+            for (int m = 1; m <= p; ++m) {
+              v = v * u;
+              cplx s = mkc(0.0);
+              if (IFC) s = RE ? cplx{v.re * q.re, v.im * q.re} : v * q;
+              if (IFD) s -= (double)m * (v * e);
+              row[m] = s;
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (act)
+          for (int jj = g; jj < cnt; jj += G) acc += sh[jj * P + n];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      if (act) sh[g * P + n] = acc;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (lane < np1) {
+        cplx s = mkc(0.0);
+        for (int gg = 0; gg < G; ++gg) s += sh[gg * P + lane];
+        if (lane > 0) s = (-1.0 / lane) * s;
+        out[((int64_t)b * nd + d) * P + lane] += s;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
-  cplx* out = loc + ((int64_t)b * nd + d) * P;
-#pragma unroll
-  for (int k = 0; k < PT; ++k)
-    if (k <= p) out[k] += acc[k];
 }
 
 // ---------------------------------------------------------- M2M
@@ -467,6 +570,8 @@ __global__ __launch_bounds__(kW) void k_l2l(int lo, int nd, int P, int pp, int p
 // ---------------------------------------------------------- list 3 + L2P
 // l2dtaeval{p,g,h} (laprouts2d.f:696-897) from the point's leaf and
 // l2dmpeval{p,g,h} (laprouts2d.f:289-495) of the leaf's list-3 boxes.
+constexpr int kEvU = 4;
+
 template <int PG>
 __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* leaf,
                        const int* ilevel, const int* nterms, const double* rsc,
@@ -485,16 +590,25 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
     cplx ap = mkc(0.0), ag = mkc(0.0), ah = mkc(0.0);
     // local expansion: sum L_n w^n, d/dz = sum n L_n w^(n-1) / r, ...
     {
+      // coefficients taken kEvU at a time (loads issued together; indices
+      // past p read zeros or unused slots of the P-row and are masked)
       const cplx* L = loc + ((int64_t)b * nd + d) * P;
       cplx wm2 = mkc(0.0), wm1 = mkc(0.0), wn = mkc(1.0);
-      for (int k = 0; k <= p; ++k) {
-        const cplx Lk = L[k];
-        ap += Lk * wn;
-        if (PG >= 2 && k >= 1) ag += (double)k * (Lk * wm1);
-        if (PG >= 3 && k >= 2) ah += (double)(k * (k - 1)) * (Lk * wm2);
-        wm2 = wm1;
-        wm1 = wn;
-        wn = wn * w;
+      for (int k0 = 0; k0 <= p; k0 += kEvU) {
+        cplx Lq[kEvU];
+#pragma unroll
+        for (int u = 0; u < kEvU; ++u) Lq[u] = L[min(k0 + u, P - 1)];
+#pragma unroll
+        for (int u = 0; u < kEvU; ++u) {
+          const int k = k0 + u;
+          const cplx Lk = (k <= p) ? Lq[u] : mkc(0.0);
+          ap += Lk * wn;
+          if (PG >= 2 && k >= 1) ag += (double)k * (Lk * wm1);
+          if (PG >= 3 && k >= 2) ah += (double)(k * (k - 1)) * (Lk * wm2);
+          wm2 = wm1;
+          wm1 = wn;
+          wn = wn * w;
+        }
       }
       if (PG >= 2) ag = ag * rinv;
       if (PG >= 3) ah = ah * (rinv * rinv);
@@ -512,13 +626,20 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
       if (PG >= 2) ag += (u * rci) * M[0];
       if (PG >= 3) ah -= (u * u) * (rci * rci) * M[0];
       cplx un = u;  // u^n
-      for (int k = 1; k <= pc; ++k) {
-        const cplx Mk = M[k];
-        ap += Mk * un;
-        const cplx un1 = un * u;
-        if (PG >= 2) ag -= ((double)k * rci) * (Mk * un1);
-        if (PG >= 3) ah += ((double)(k * (k + 1)) * rci * rci) * (Mk * (un1 * u));
-        un = un1;
+      for (int k0 = 1; k0 <= pc; k0 += kEvU) {
+        cplx Mq[kEvU];
+#pragma unroll
+        for (int v = 0; v < kEvU; ++v) Mq[v] = M[min(k0 + v, P - 1)];
+#pragma unroll
+        for (int v = 0; v < kEvU; ++v) {
+          const int k = k0 + v;
+          const cplx Mk = (k <= pc) ? Mq[v] : mkc(0.0);
+          ap += Mk * un;
+          const cplx un1 = un * u;
+          if (PG >= 2) ag -= ((double)k * rci) * (Mk * un1);
+          if (PG >= 3) ah += ((double)(k * (k + 1)) * rci * rci) * (Mk * (un1 * u));
+          un = un1;
+        }
       }
     }
     pot[i * nd + d] += ap;
@@ -1046,27 +1167,67 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     const int bat = (int)std::max<size_t>(1, std::min<size_t>(kW, 65536 / per_src));
     const size_t lds_p2m = (size_t)bat * (P * sizeof(cplx) + sizeof(int));
     const size_t lds_p2l = (size_t)bat * per_src;
-    // ---- step 1: P2M, step 2: list 4 (register forms up to 48 coefficients)
-    const int64_t nbd = (int64_t)nb * nd;
-#define FMM2D_P2M_REG(PT, C, D)                                                                \
-  launch1d_xcd(k_p2m_reg<PT, C, D>, nbd, st, nd, P, t.ilevel.p, t.nchild.p, d_nterms.p,       \
+    // ---- step 1: P2M, step 2: list 4 (wavefronts over compact box lists up
+    // to 64 coefficients; the LDS forms above beyond)
+    const bool re_ch = kind != 0;  // real Cauchy charges
+    DBuf<int> p2x_list(nb, st), p2x_cnt(1, st);
+    // rows of P coefficients per wavefront: 32, or 16 beyond P = 32 (LDS
+    // 4 x bat x P x 16 bytes <= 64 KiB)
+    const int bat_x = P <= 32 ? 32 : 16;
+    const size_t lds_p2x = (size_t)kP2xWaves * bat_x * P * sizeof(cplx);
+    int p2x_grid = 0;
+    if (P <= kW) {
+      DBuf<int> flag(nb, st), pos(nb, st);
+      launch1d(k_p2l_flag, nb, st, t.ilevel.p, L.n4.p, sse, tse, pg, pgt, flag.p);
+      size_t sbytes = 0;
+      hip_throw(hipcub::DeviceScan::ExclusiveSum(nullptr, sbytes, flag.p, pos.p, nb, st), "scan");
+      DBuf<char> stmp(sbytes, st);
+      hip_throw(hipcub::DeviceScan::ExclusiveSum(stmp.p, sbytes, flag.p, pos.p, nb, st), "scan");
+      launch1d(k_p2l_list, nb, st, flag.p, pos.p, p2x_list.p, p2x_cnt.p);
+      int dev = 0, ncu = 0, per_cu = 0;
+      hip_throw(hipGetDevice(&dev), "device");
+      hip_throw(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "cu");
+      hip_throw(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2x<1, 1, 1, 1>,
+                                                             kP2xWaves * kW, lds_p2x),
+                "occupancy");
+      p2x_grid = std::max(1, ncu * std::max(1, per_cu));
+    }
+#define FMM2D_P2X(MODE, C, D, RE)                                                              \
+  hipLaunchKernelGGL((k_p2x<MODE, C, D, RE>), dim3(p2x_grid), dim3(kP2xWaves * kW), lds_p2x,  \
+                     st, p2x_list.p, p2x_cnt.p, nd, P,                                         \
+                     t.ilevel.p, d_nterms.p, d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, xs.p, \
+                     chs.p, dps.p, bat_x, MODE == 0 ? mp.p : loc.p)
+#define FMM2D_P2X_RE(MODE, C, D) \
+  if (re_ch) FMM2D_P2X(MODE, C, D, 1); \
+  else FMM2D_P2X(MODE, C, D, 0)
+#define FMM2D_P2X_CD(MODE)                                               \
+  if (a.ifcharge && !a.ifdipole) { FMM2D_P2X_RE(MODE, 1, 0); }           \
+  else if (!a.ifcharge && a.ifdipole) { FMM2D_P2X(MODE, 0, 1, 0); }      \
+  else { FMM2D_P2X_RE(MODE, 1, 1); }                                     \
+  hip_throw(hipGetLastError(), "p2x")
+#define FMM2D_P2M_REG(PT, C, D, RE)                                                           \
+  launch1d_xcd(k_p2m_reg<PT, C, D, RE>, nbd, st, nd, P, t.ilevel.p, t.nchild.p, d_nterms.p,   \
                d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p)
-#define FMM2D_P2L_REG(PT, C, D)                                                                \
-  launch1d_xcd(k_p2l_reg<PT, C, D>, nbd, st, nd, P, t.ilevel.p, d_nterms.p, d_rsc.p,          \
-               t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p, dps.p, loc.p)
-#define FMM2D_BY_PT(M, C, D)          \
-  if (P <= 16) M(16, C, D);          \
-  else if (P <= 24) M(24, C, D);     \
-  else if (P <= 32) M(32, C, D);     \
-  else M(48, C, D)
-    if (P <= 48) {
-      if (a.ifcharge && !a.ifdipole) FMM2D_BY_PT(FMM2D_P2M_REG, 1, 0);
-      else if (!a.ifcharge && a.ifdipole) FMM2D_BY_PT(FMM2D_P2M_REG, 0, 1);
-      else FMM2D_BY_PT(FMM2D_P2M_REG, 1, 1);
+#define FMM2D_P2M_PT(C, D, RE)                  \
+  if (P <= 16) FMM2D_P2M_REG(16, C, D, RE);     \
+  else if (P <= 20) FMM2D_P2M_REG(20, C, D, RE); \
+  else if (P <= 24) FMM2D_P2M_REG(24, C, D, RE); \
+  else if (P <= 32) FMM2D_P2M_REG(32, C, D, RE); \
+  else if (P <= 48) FMM2D_P2M_REG(48, C, D, RE); \
+  else FMM2D_P2M_REG(64, C, D, RE)
+    if (P <= kW) {
+      const int64_t nbd = (int64_t)nb * nd;
+      if (a.ifcharge && !a.ifdipole) {
+        if (re_ch) FMM2D_P2M_PT(1, 0, 1);
+        else FMM2D_P2M_PT(1, 0, 0);
+      } else if (!a.ifcharge && a.ifdipole) {
+        FMM2D_P2M_PT(0, 1, 0);
+      } else {
+        if (re_ch) FMM2D_P2M_PT(1, 1, 1);
+        else FMM2D_P2M_PT(1, 1, 0);
+      }
       tm.mark(0);
-      if (a.ifcharge && !a.ifdipole) FMM2D_BY_PT(FMM2D_P2L_REG, 1, 0);
-      else if (!a.ifcharge && a.ifdipole) FMM2D_BY_PT(FMM2D_P2L_REG, 0, 1);
-      else FMM2D_BY_PT(FMM2D_P2L_REG, 1, 1);
+      FMM2D_P2X_CD(1);
     } else {
       if (a.ifcharge && !a.ifdipole)
         hipLaunchKernelGGL((k_p2m<1, 0>), dim3(nb), dim3(kW), lds_p2m, st, nd, P, t.ilevel.p, t.nchild.p,
@@ -1094,9 +1255,11 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
                            dps.p, bat, loc.p);
       hip_throw(hipGetLastError(), "p2l");
     }
-#undef FMM2D_BY_PT
-#undef FMM2D_P2L_REG
+#undef FMM2D_P2M_PT
 #undef FMM2D_P2M_REG
+#undef FMM2D_P2X_CD
+#undef FMM2D_P2X_RE
+#undef FMM2D_P2X
     tm.mark(1);
     // ---- translation operators as MFMA fragments: 49 M2L offsets, 4 M2M, 4 L2L
     const int opM2M = kOff * kOff, opL2L = opM2M + 4;
