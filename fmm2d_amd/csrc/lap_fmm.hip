@@ -22,6 +22,8 @@
 // atomics in the FMM itself (pull form, as the reference's loops).
 #include <algorithm>
 #include <complex>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -219,257 +221,6 @@ __global__ __launch_bounds__(kW) void k_p2m(int nd, int P, const int* ilevel, co
   }
 }
 
-// Boxes with list-4 work (levels >= 2, a non-empty list 4, points to
-// evaluate): flag, then (after an exclusive scan of the flags) the list in
-// box order.
-__global__ void k_p2l_flag(int64_t nb, const int* ilevel, const int* n4, const int* sse,
-                           const int* tse, int ifpgh, int ifpght, int* flag) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  const int b = (int)i;
-  flag[b] = (ilevel[b] >= 2 && n4[b] > 0 && npts_eval(b, sse, tse, ifpgh, ifpght) > 0) ? 1 : 0;
-}
-
-__global__ void k_p2l_list(int64_t nb, const int* flag, const int* pos, int* list, int* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  if (flag[i]) list[pos[i]] = (int)i;
-  if (i == nb - 1) *cnt = pos[i] + flag[i];
-}
-
-// P2M (l2dformmp{c,d,cd}, laprouts2d.f:84-282) for expansions of at most PT
-// coefficients: one thread per (box, density), for the leaves at levels >= 2,
-// runs over the leaf's sources with the running products w^n, w = (z - c)/r, accumulating
-// every coefficient in registers.  RE: real charges.
-template <int PT, int IFC, int IFD, int RE>
-__global__ __launch_bounds__(256) void k_p2m_reg(int64_t n, int nd, int P, const int* ilevel,
-                                                 const int* nchild, const int* nterms,
-                                                 const double* rsc, const double* centers,
-                                                 const int* sse, const double* xs,
-                                                 const cplx* chs, const cplx* dps, cplx* mp) {
-  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = (int)(i / nd), d = (int)(i - (int64_t)b * nd);
-  const int lev = ilevel[b];
-  if (lev < 2 || nchild[b] != 0) return;
-  const int s0 = sse[2 * b], s1 = sse[2 * b + 1];
-  if (s1 <= s0) return;
-  const int p = nterms[lev];
-  const double rinv = 1.0 / rsc[lev];
-  const double cx = centers[2 * b], cy = centers[2 * b + 1];
-  cplx acc[PT];
-#pragma unroll
-  for (int k = 0; k < PT; ++k) acc[k] = mkc(0.0);
-  for (int j = s0; j < s1; ++j) {
-    const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
-    const cplx c = IFC ? chs[(int64_t)j * nd + d] : mkc(0.0);
-    const cplx dp = IFD ? rinv * dps[(int64_t)j * nd + d] : mkc(0.0);
-    if (IFC) acc[0] += c;
-    cplx v = mkc(1.0);  // w^(k-1)
-#pragma unroll
-    for (int k = 1; k < PT; ++k) {
-      if (k <= p) {
-        if (IFD) acc[k] += v * dp;
-        v = v * w;
-        if (IFC) {
-          if (RE) {
-            const double ck = (-1.0 / k) * c.re;
-            acc[k] += cplx{v.re * ck, v.im * ck};
-          } else {
-            acc[k] += (-1.0 / k) * (v * c);
-          }
-        }
-      }
-    }
-  }
-  cplx* out = mp + ((int64_t)b * nd + d) * P;
-#pragma unroll
-  for (int k = 0; k < PT; ++k)
-    if (k <= p) out[k] += acc[k];
-}
-
-// Points to expansion: P2M from a leaf's own sources (l2dformmp{c,d,cd},
-// laprouts2d.f:84-282) or list-4 P2L from the sources of a box's list-4 boxes
-// (l2dformta{c,d,cd}, laprouts2d.f:501-690).  Each wavefront takes boxes of
-// `boxes` in a grid-stride loop (4 wavefronts per block, each with its own
-// LDS rows).  A box's sources are one flat sequence taken in batches of up to
-// `bat`: lane j writes row j, source j's terms S_j[n], n = 0..p, and lane
-// (g, n) of the G = floor(64/(p+1)) lane groups adds rows g, g+G, ... of
-// column n.  The 1/n of every term is applied once to the column sum:
-//   P2M, w = (z - c)/r:     T_0 = q,                 T_n = -(1/n) (w^n q - n w^(n-1) d/r)
-//   P2L, u = r/(z - c):     T_0 = log|z - c| q + e,  T_n = -(1/n) (u^n q - n u^n e),
-//                           e = -d/(z - c)
-// (q: charge, d: Cauchy dipole strength).  RE: real charges.
-constexpr int kP2xWaves = 4;
-
-template <int MODE, int IFC, int IFD, int RE>  // MODE 0: P2M, 1: P2L
-__global__ __launch_bounds__(kP2xWaves * kW) void k_p2x(
-    const int* boxes, const int* nboxes, int nd, int P, const int* ilevel, const int* nterms,
-    const double* rsc, const double* centers, const int* n4, const int* l4, const int* sse,
-    const double* xs, const cplx* chs, const cplx* dps, int bat, cplx* out) {
-  extern __shared__ cplx shall[];  // [kP2xWaves][bat][P]
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  cplx* sh = shall + (size_t)wv * bat * P;
-  const int nbx = *nboxes;
-  constexpr int KR = MODE == 0 ? 1 : kMaxList4;
-  for (int ib = blockIdx.x * kP2xWaves + wv; ib < nbx; ib += gridDim.x * kP2xWaves) {
-    const int b = boxes[ib];
-    const int lev = ilevel[b];
-    // flat numbering of the sources: range k holds [pre[k], pre[k+1])
-    int pre[KR + 1], first[KR];
-    pre[0] = 0;
-    if (MODE == 0) {
-      first[0] = sse[2 * b];
-      pre[1] = sse[2 * b + 1] - sse[2 * b];
-    } else {
-      const int nbox = n4[b];
-      for (int k = 0; k < KR; ++k) {
-        const int c = (k < nbox) ? l4[(int64_t)kMaxList4 * b + k] : -1;
-        first[k] = (c >= 0) ? sse[2 * c] : 0;
-        pre[k + 1] = pre[k] + ((c >= 0) ? sse[2 * c + 1] - sse[2 * c] : 0);
-      }
-    }
-    const int total = pre[KR];
-    const int p = nterms[lev], np1 = p + 1;
-    const int G = min(kW / np1, bat);  // group sums use rows 0..G-1
-    const int g = lane / np1, n = lane - g * np1;
-    const bool act = g < G;
-    const double r = rsc[lev], rinv = 1.0 / r;
-    const double cx = centers[2 * b], cy = centers[2 * b + 1];
-    for (int d = 0; d < nd; ++d) {
-      cplx acc = mkc(0.0);
-      for (int f0 = 0; f0 < total; f0 += bat) {
-        const int cnt = min(bat, total - f0);
-        if (lane < cnt) {
-          const int f = f0 + lane;
-          int k = 0;
-          while (k + 1 < KR && f >= pre[k + 1]) ++k;
-          const int64_t j = first[k] + (f - pre[k]);
-          cplx q = mkc(0.0), e = mkc(0.0);
-          if (IFC) {
-            q = chs[j * nd + d];
-            if (RE) q.im = 0.0;
-          }
-          cplx* row = sh + lane * P;
-          if (MODE == 0) {
-            const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
-            if (IFD) e = rinv * dps[j * nd + d];
-            row[0] = q;
-            cplx v = mkc(1.0);  // w^(m-1)
-            for (int m = 1; m <= p; ++m) {
-              cplx s = mkc(0.0);
-              if (IFD) s = (-(double)m) * (v * e);
-              v = v * w;
-              if (IFC) s += RE ? cplx{v.re * q.re, v.im * q.re} : v * q;
-              row[m] = s;
-            }
-          } else {
-            const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
-            const cplx zi = invc(z0);
-            const cplx u = r * zi;
-            if (IFD) e = -(zi * dps[j * nd + d]);
-            cplx t0 = e;
-            if (IFC) t0 += log(absc(z0)) * q;
-            row[0] = t0;
-            cplx v = mkc(1.0);  // u^m
-            for (int m = 1; m <= p; ++m) {
-              v = v * u;
-              cplx s = mkc(0.0);
-              if (IFC) s = RE ? cplx{v.re * q.re, v.im * q.re} : v * q;
-              if (IFD) s -= (double)m * (v * e);
-              row[m] = s;
-            }
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (act)
-          for (int jj = g; jj < cnt; jj += G) acc += sh[jj * P + n];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      }
-      if (act) sh[g * P + n] = acc;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (lane < np1) {
-        cplx s = mkc(0.0);
-        for (int gg = 0; gg < G; ++gg) s += sh[gg * P + lane];
-        if (lane > 0) s = (-1.0 / lane) * s;
-        out[((int64_t)b * nd + d) * P + lane] += s;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-  }
-}
-
-// ---------------------------------------------------------- M2M
-// l2dmpmp (laprouts2d.f:905-1015): parents at one level from their children.
-// A[q][j][n]: child quadrant q, child coefficient j, parent coefficient n.
-__global__ __launch_bounds__(kW) void k_m2m(int lo, int nd, int P, int pc, int pp,
-                                            const int* nchild, const int* ichild, const int* sse,
-                                            const cplx* A, cplx* mp) {
-  const int b = lo + blockIdx.x;
-  if (nchild[b] == 0) return;
-  for (int nn = threadIdx.x; nn <= pp; nn += kW) {
-    for (int d = 0; d < nd; ++d) {
-      cplx acc = mkc(0.0);
-      for (int q = 0; q < 4; ++q) {
-        const int c = ichild[4 * b + q];
-        if (sse[2 * c + 1] <= sse[2 * c]) continue;
-        const cplx* m = mp + ((int64_t)c * nd + d) * P;
-        const int jm = min(nn, pc);
-        for (int j = 0; j <= jm; ++j) acc += A[((int64_t)q * P + j) * P + nn] * m[j];
-      }
-      mp[((int64_t)b * nd + d) * P + nn] += acc;
-    }
-  }
-}
-
-// ---------------------------------------------------------- M2L
-// l2dmploc (laprouts2d.f:1130-1244) over list 2, all levels at once.
-// T[o][j][n] for offset o = (dx+3)*7 + (dy+3) of the source box.
-__global__ __launch_bounds__(kW) void k_m2l(int nd, int P, const int* ilevel, const int* nterms,
-                                            const double* rsc, const double* logr,
-                                            const double* centers, const int* n2, const int* l2,
-                                            const int* sse, const int* tse, int ifpgh,
-                                            int ifpght, const cplx* T, const cplx* mp,
-                                            cplx* loc) {
-  const int b = blockIdx.x;
-  const int lev = ilevel[b];
-  const int cnt = n2[b];
-  if (lev < 2 || cnt == 0 || npts_eval(b, sse, tse, ifpgh, ifpght) == 0) return;
-  const int p = nterms[lev];
-  const double rinv = 1.0 / rsc[lev], lr = logr[lev];
-  const double cx = centers[2 * b], cy = centers[2 * b + 1];
-  __shared__ int so[kMaxList2], sc[kMaxList2];
-  if ((int)threadIdx.x < cnt) {
-    const int c = l2[(int64_t)kMaxList2 * b + threadIdx.x];
-    const int ox = (int)rint((centers[2 * c] - cx) * rinv);
-    const int oy = (int)rint((centers[2 * c + 1] - cy) * rinv);
-    so[threadIdx.x] = (ox + 3) * kOff + (oy + 3);
-    sc[threadIdx.x] = c;
-  }
-  __syncthreads();
-  for (int nn = threadIdx.x; nn <= p; nn += kW) {
-    for (int d = 0; d < nd; ++d) {
-      cplx acc = mkc(0.0);
-      for (int k = 0; k < cnt; ++k) {
-        const cplx* m = mp + ((int64_t)sc[k] * nd + d) * P;
-        const cplx* t = T + (int64_t)so[k] * P * P + nn;
-        for (int j = 0; j <= p; ++j) acc += t[(int64_t)j * P] * m[j];
-        if (nn == 0) acc += lr * m[0];
-      }
-      loc[((int64_t)b * nd + d) * P + nn] += acc;
-    }
-  }
-}
-
-// ---------------------------------------------------------- list 4 (P2L)
 // l2dformta{c,d,cd} (laprouts2d.f:501-690): sources of the coarser leaves in
 // list 4 of a box.  One wavefront per box; the list-4 sources are taken as one
 // flat sequence in batches of up to 64: lane j stores u_j^n (u = r/z_j, z_j = source
@@ -545,24 +296,169 @@ __global__ __launch_bounds__(kW) void k_p2l(int nd, int P, const int* ilevel, co
   }
 }
 
-// ---------------------------------------------------------- L2L
-// l2dlocloc (laprouts2d.f:1020-1123): children at one level from their parent.
-// B[q][j][n]: child quadrant q, parent coefficient j, child coefficient n.
-__global__ __launch_bounds__(kW) void k_l2l(int lo, int nd, int P, int pp, int pc,
-                                            const int* iparent, const int* ichild,
-                                            const int* sse, const int* tse, int ifpgh,
-                                            int ifpght, const cplx* B, cplx* loc) {
-  const int b = lo + blockIdx.x;
-  const int dad = iparent[b];
-  if (npts_eval(dad, sse, tse, ifpgh, ifpght) == 0) return;
-  int q = 0;
-  while (q < 3 && ichild[4 * dad + q] != b) ++q;
-  for (int nn = threadIdx.x; nn <= pc; nn += kW) {
+// Boxes with list-4 work (levels >= 2, a non-empty list 4, points to
+// evaluate): flag, then (after an exclusive scan of the flags) the list in
+// box order.
+__global__ void k_p2l_flag(int64_t nb, const int* ilevel, const int* n4, const int* sse,
+                           const int* tse, int ifpgh, int ifpght, int* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int b = (int)i;
+  flag[b] = (ilevel[b] >= 2 && n4[b] > 0 && npts_eval(b, sse, tse, ifpgh, ifpght) > 0) ? 1 : 0;
+}
+
+__global__ void k_p2l_list(int64_t nb, const int* flag, const int* pos, int* list, int* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  if (flag[i]) list[pos[i]] = (int)i;
+  if (i == nb - 1) *cnt = pos[i] + flag[i];
+}
+
+// P2M (l2dformmp{c,d,cd}, laprouts2d.f:84-282) for expansions of at most PT
+// coefficients: one thread per (box, density), for the leaves at levels >= 2,
+// runs over the leaf's sources with the running products w^n, w = (z - c)/r, accumulating
+// every coefficient in registers.  RE: real charges.
+template <int PT, int IFC, int IFD, int RE>
+__global__ __launch_bounds__(256) void k_p2m_reg(int64_t n, int nd, int P, const int* ilevel,
+                                                 const int* nchild, const int* nterms,
+                                                 const double* rsc, const double* centers,
+                                                 const int* sse, const double* xs,
+                                                 const cplx* chs, const cplx* dps, cplx* mp) {
+  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = (int)(i / nd), d = (int)(i - (int64_t)b * nd);
+  const int lev = ilevel[b];
+  if (lev < 2 || nchild[b] != 0) return;
+  const int s0 = sse[2 * b], s1 = sse[2 * b + 1];
+  if (s1 <= s0) return;
+  const int p = nterms[lev];
+  const double rinv = 1.0 / rsc[lev];
+  const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  cplx acc[PT];
+#pragma unroll
+  for (int k = 0; k < PT; ++k) acc[k] = mkc(0.0);
+  for (int j = s0; j < s1; ++j) {
+    const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
+    const cplx c = IFC ? chs[(int64_t)j * nd + d] : mkc(0.0);
+    const cplx dp = IFD ? rinv * dps[(int64_t)j * nd + d] : mkc(0.0);
+    if (IFC) acc[0] += c;
+    cplx v = mkc(1.0);  // w^(k-1)
+#pragma unroll
+    for (int k = 1; k < PT; ++k) {
+      if (k <= p) {
+        if (IFD) acc[k] += v * dp;
+        v = v * w;
+        if (IFC) {
+          if (RE) {
+            const double ck = (-1.0 / k) * c.re;
+            acc[k] += cplx{v.re * ck, v.im * ck};
+          } else {
+            acc[k] += (-1.0 / k) * (v * c);
+          }
+        }
+      }
+    }
+  }
+  cplx* out = mp + ((int64_t)b * nd + d) * P;
+#pragma unroll
+  for (int k = 0; k < PT; ++k)
+    if (k <= p) out[k] += acc[k];
+}
+
+// Orders one wavefront's LDS writes before its later LDS reads (the
+// wavefront's LDS accesses complete in order; this keeps the compiler from
+// moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// List-4 P2L (l2dformta{c,d,cd}, laprouts2d.f:501-690): the sources of a
+// box's list-4 boxes into its local expansion.  Each wavefront takes boxes of
+// `boxes` in a grid-stride loop (4 wavefronts per block, each with its own
+// LDS rows).  A box's sources are one flat sequence taken in batches of up to
+// `bat`: lane j writes row j, source j's terms S_j[n], n = 0..p, and lane
+// (g, n) of the G = floor(64/(p+1)) lane groups adds rows g, g+G, ... of
+// column n.  With u = r/(z - c) and e = -d/(z - c) (q: charge, d: Cauchy
+// dipole strength)
+//   T_0 = log|z - c| q + e,   T_n = -(1/n) (u^n q - n u^n e),
+// the 1/n applied once to the column sum.  RE: real charges.
+constexpr int kP2lWaves = 4;
+
+template <int IFC, int IFD, int RE>
+__global__ __launch_bounds__(kP2lWaves * kW) void k_p2l_w(
+    const int* boxes, const int* nboxes, int nd, int P, const int* ilevel, const int* nterms,
+    const double* rsc, const double* centers, const int* n4, const int* l4, const int* sse,
+    const double* xs, const cplx* chs, const cplx* dps, int bat, cplx* loc) {
+  extern __shared__ cplx shall[];  // [kP2lWaves][bat][P]
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  cplx* sh = shall + (size_t)wv * bat * P;
+  const int nbx = *nboxes;
+  for (int ib = blockIdx.x * kP2lWaves + wv; ib < nbx; ib += gridDim.x * kP2lWaves) {
+    const int b = boxes[ib];
+    const int lev = ilevel[b];
+    // flat numbering of the sources: list-4 box k holds [pre[k], pre[k+1])
+    int pre[kMaxList4 + 1], first[kMaxList4];
+    pre[0] = 0;
+    const int nbox = n4[b];
+    for (int k = 0; k < kMaxList4; ++k) {
+      const int c = (k < nbox) ? l4[(int64_t)kMaxList4 * b + k] : -1;
+      first[k] = (c >= 0) ? sse[2 * c] : 0;
+      pre[k + 1] = pre[k] + ((c >= 0) ? sse[2 * c + 1] - sse[2 * c] : 0);
+    }
+    const int total = pre[kMaxList4];
+    const int p = nterms[lev], np1 = p + 1;
+    const int G = min(kW / np1, bat);  // group sums use rows 0..G-1
+    const int g = lane / np1, n = lane - g * np1;
+    const bool act = g < G;
+    const double r = rsc[lev];
+    const double cx = centers[2 * b], cy = centers[2 * b + 1];
     for (int d = 0; d < nd; ++d) {
-      const cplx* L = loc + ((int64_t)dad * nd + d) * P;
       cplx acc = mkc(0.0);
-      for (int j = nn; j <= pp; ++j) acc += B[((int64_t)q * P + j) * P + nn] * L[j];
-      loc[((int64_t)b * nd + d) * P + nn] += acc;
+      for (int f0 = 0; f0 < total; f0 += bat) {
+        const int cnt = min(bat, total - f0);
+        if (lane < cnt) {
+          const int f = f0 + lane;
+          int k = 0;
+          while (k + 1 < kMaxList4 && f >= pre[k + 1]) ++k;
+          const int64_t j = first[k] + (f - pre[k]);
+          cplx q = mkc(0.0), e = mkc(0.0);
+          if (IFC) {
+            q = chs[j * nd + d];
+            if (RE) q.im = 0.0;
+          }
+          cplx* row = sh + lane * P;
+          const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
+          const cplx zi = invc(z0);
+          const cplx u = r * zi;
+          if (IFD) e = -(zi * dps[j * nd + d]);
+          cplx t0 = e;
+          if (IFC) t0 += log(absc(z0)) * q;
+          row[0] = t0;
+          cplx v = mkc(1.0);  // u^m
+          for (int m = 1; m <= p; ++m) {
+            v = v * u;
+            cplx s = mkc(0.0);
+            if (IFC) s = RE ? cplx{v.re * q.re, v.im * q.re} : v * q;
+            if (IFD) s -= (double)m * (v * e);
+            row[m] = s;
+          }
+        }
+        wave_lds_sync();
+        if (act)
+          for (int jj = g; jj < cnt; jj += G) acc += sh[jj * P + n];
+        wave_lds_sync();
+      }
+      if (act) sh[g * P + n] = acc;
+      wave_lds_sync();
+      if (lane < np1) {
+        cplx s = mkc(0.0);
+        for (int gg = 0; gg < G; ++gg) s += sh[gg * P + lane];
+        if (lane > 0) s = (-1.0 / lane) * s;
+        loc[((int64_t)b * nd + d) * P + lane] += s;
+      }
+      wave_lds_sync();
     }
   }
 }
@@ -981,6 +877,41 @@ void make_matrices(int P, std::vector<cplx>& A, std::vector<cplx>& B, std::vecto
     }
 }
 
+// Translation operators of one expansion length P as MFMA fragments: the 49
+// M2L offsets, 4 M2M and 4 L2L quadrants in FP64, and the M2L offsets in FP32
+// for mixed precision.  They depend on P alone (the Laplace operators are
+// level independent), so each (device, P) builds them once; they stay for
+// the life of the process.
+struct LapOps {
+  double* af = nullptr;
+  float* af32 = nullptr;
+};
+
+const LapOps& lap_ops(int P) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, LapOps> cache;
+  int dev = 0;
+  hip_throw(hipGetDevice(&dev), "device");
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find({dev, P});
+  if (it != cache.end()) return it->second;
+  std::vector<cplx> A, B, T;
+  make_matrices(P, A, B, T);
+  std::vector<cplx> ops(T);
+  ops.insert(ops.end(), A.begin(), A.end());
+  ops.insert(ops.end(), B.begin(), B.end());
+  const std::vector<double> f64 = swizzle_ops(P, ops, kOff * kOff + 8);
+  const std::vector<float> f32 = swizzle_ops<float>(P, T, kOff * kOff);
+  LapOps o;
+  hip_throw(hipMalloc(&o.af, f64.size() * sizeof(double)), "alloc");
+  hip_throw(hipMalloc(&o.af32, f32.size() * sizeof(float)), "alloc");
+  hip_throw(hipMemcpy(o.af, f64.data(), f64.size() * sizeof(double), hipMemcpyHostToDevice),
+            "copy");
+  hip_throw(hipMemcpy(o.af32, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice),
+            "copy");
+  return cache.emplace(std::make_pair(dev, P), o).first->second;
+}
+
 struct NearArgs {
   int nd;
   const double* xy;  // sorted evaluation points
@@ -1122,11 +1053,9 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     const int P = pmax + 1;
     std::vector<double> rsc(t.boxsize), logr(nl + 1);
     for (int l = 0; l <= nl; ++l) logr[l] = std::log(rsc[l]);
-    std::vector<cplx> A, B, T;
-    make_matrices(P, A, B, T);
+    const LapOps& ops = lap_ops(P);
     DBuf<int> d_nterms = upload(nterms, st);
     DBuf<double> d_rsc = upload(rsc, st), d_logr = upload(logr, st);
-    DBuf<cplx> d_A = upload(A, st), d_B = upload(B, st), d_T = upload(T, st);
 
     // ---- sort and gather
     DBuf<double> xs(2 * ns, st), xt(2 * std::max<int64_t>(nt, 1), st);
@@ -1174,7 +1103,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     // rows of P coefficients per wavefront: 32, or 16 beyond P = 32 (LDS
     // 4 x bat x P x 16 bytes <= 64 KiB)
     const int bat_x = P <= 32 ? 32 : 16;
-    const size_t lds_p2x = (size_t)kP2xWaves * bat_x * P * sizeof(cplx);
+    const size_t lds_p2x = (size_t)kP2lWaves * bat_x * P * sizeof(cplx);
     int p2x_grid = 0;
     if (P <= kW) {
       DBuf<int> flag(nb, st), pos(nb, st);
@@ -1187,24 +1116,15 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       int dev = 0, ncu = 0, per_cu = 0;
       hip_throw(hipGetDevice(&dev), "device");
       hip_throw(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "cu");
-      hip_throw(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2x<1, 1, 1, 1>,
-                                                             kP2xWaves * kW, lds_p2x),
+      hip_throw(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2l_w<1, 1, 1>,
+                                                             kP2lWaves * kW, lds_p2x),
                 "occupancy");
       p2x_grid = std::max(1, ncu * std::max(1, per_cu));
     }
-#define FMM2D_P2X(MODE, C, D, RE)                                                              \
-  hipLaunchKernelGGL((k_p2x<MODE, C, D, RE>), dim3(p2x_grid), dim3(kP2xWaves * kW), lds_p2x,  \
-                     st, p2x_list.p, p2x_cnt.p, nd, P,                                         \
-                     t.ilevel.p, d_nterms.p, d_rsc.p, t.centers.p, L.n4.p, L.l4.p, sse, xs.p, \
-                     chs.p, dps.p, bat_x, MODE == 0 ? mp.p : loc.p)
-#define FMM2D_P2X_RE(MODE, C, D) \
-  if (re_ch) FMM2D_P2X(MODE, C, D, 1); \
-  else FMM2D_P2X(MODE, C, D, 0)
-#define FMM2D_P2X_CD(MODE)                                               \
-  if (a.ifcharge && !a.ifdipole) { FMM2D_P2X_RE(MODE, 1, 0); }           \
-  else if (!a.ifcharge && a.ifdipole) { FMM2D_P2X(MODE, 0, 1, 0); }      \
-  else { FMM2D_P2X_RE(MODE, 1, 1); }                                     \
-  hip_throw(hipGetLastError(), "p2x")
+#define FMM2D_P2L_W(C, D, RE)                                                                 \
+  hipLaunchKernelGGL((k_p2l_w<C, D, RE>), dim3(p2x_grid), dim3(kP2lWaves * kW), lds_p2x, st,  \
+                     p2x_list.p, p2x_cnt.p, nd, P, t.ilevel.p, d_nterms.p, d_rsc.p,            \
+                     t.centers.p, L.n4.p, L.l4.p, sse, xs.p, chs.p, dps.p, bat_x, loc.p)
 #define FMM2D_P2M_REG(PT, C, D, RE)                                                           \
   launch1d_xcd(k_p2m_reg<PT, C, D, RE>, nbd, st, nd, P, t.ilevel.p, t.nchild.p, d_nterms.p,   \
                d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p)
@@ -1227,7 +1147,16 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
         else FMM2D_P2M_PT(1, 1, 0);
       }
       tm.mark(0);
-      FMM2D_P2X_CD(1);
+      if (a.ifcharge && !a.ifdipole) {
+        if (re_ch) FMM2D_P2L_W(1, 0, 1);
+        else FMM2D_P2L_W(1, 0, 0);
+      } else if (!a.ifcharge && a.ifdipole) {
+        FMM2D_P2L_W(0, 1, 0);
+      } else {
+        if (re_ch) FMM2D_P2L_W(1, 1, 1);
+        else FMM2D_P2L_W(1, 1, 0);
+      }
+      hip_throw(hipGetLastError(), "p2l");
     } else {
       if (a.ifcharge && !a.ifdipole)
         hipLaunchKernelGGL((k_p2m<1, 0>), dim3(nb), dim3(kW), lds_p2m, st, nd, P, t.ilevel.p, t.nchild.p,
@@ -1257,19 +1186,11 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     }
 #undef FMM2D_P2M_PT
 #undef FMM2D_P2M_REG
-#undef FMM2D_P2X_CD
-#undef FMM2D_P2X_RE
-#undef FMM2D_P2X
+#undef FMM2D_P2L_W
     tm.mark(1);
     // ---- translation operators as MFMA fragments: 49 M2L offsets, 4 M2M, 4 L2L
     const int opM2M = kOff * kOff, opL2L = opM2M + 4;
-    DBuf<double> d_af;
-    {
-      std::vector<cplx> ops(T);
-      ops.insert(ops.end(), A.begin(), A.end());
-      ops.insert(ops.end(), B.begin(), B.end());
-      d_af = upload(swizzle_ops(P, ops, opL2L + 4), st);
-    }
+    const double* d_af = ops.af;
     std::vector<int2> band(nl + 1);
     for (int l = 0; l <= nl; ++l) band[l] = int2{0, 2 * (nterms[l] + 1)};
     DBuf<int2> d_band = upload(band, st);
@@ -1280,7 +1201,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     for (int l = nl - 1; l >= 1; --l)
       launch_translate(0, 0, nd, TrBufs{mp.p, 0, P, mp.p, 0, P}, 4,
                        {TrGroup{cp.lev_pos[l], cp.lev_pos[l + 1], opM2M, 0}}, cp.list.p, cp.up.p,
-                       d_op4.p, d_af.p, t.ilevel.p, d_band.p, d_logr.p, st);
+                       d_op4.p, d_af, t.ilevel.p, d_band.p, d_logr.p, st);
     tm.mark(2);
     // ---- step 4: M2L, targets grouped by child position (box order within a group)
     {
@@ -1293,18 +1214,17 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
           groups.push_back(TrGroup{m2l.cls_seg[q], m2l.cls_seg[q + 1], 0, q});
         if (mixed) {
           // FP32 operators and an FP32 copy of the multipoles
-          DBuf<float> af32 = upload(swizzle_ops<float>(P, T, kOff * kOff), st);
           DBuf<float> mp32(2 * nexp, st);
           launch1d(k_to_f32, 2 * nexp, st, reinterpret_cast<const double*>(mp.p), mp32.p);
           tm.mark(3);  // planning
           launch_translate(0, 1, nd, TrBufs{mp32.p, 0, P, loc.p, 0, P}, kSlots, groups,
-                           m2l.tlist.p, L.s2.p, d_soff.p, af32.p, t.ilevel.p, d_band.p,
+                           m2l.tlist.p, L.s2.p, d_soff.p, ops.af32, t.ilevel.p, d_band.p,
                            d_logr.p, st, true, true);
           tm.mark(13);  // the M2L kernel alone
         } else {
           tm.mark(3);  // planning
           launch_translate(0, 1, nd, TrBufs{mp.p, 0, P, loc.p, 0, P}, kSlots, groups,
-                           m2l.tlist.p, L.s2.p, d_soff.p, d_af.p, t.ilevel.p, d_band.p,
+                           m2l.tlist.p, L.s2.p, d_soff.p, d_af, t.ilevel.p, d_band.p,
                            d_logr.p, st, false, true);
           tm.mark(13);  // the M2L kernel alone
         }
@@ -1315,7 +1235,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     for (int l = 1; l <= nl - 1; ++l)
       launch_translate(1, 0, nd, TrBufs{loc.p, 0, P, loc.p, 0, P}, 4,
                        {TrGroup{cp.lev_pos[l], cp.lev_pos[l + 1], opL2L, 0}}, cp.list.p,
-                       cp.down.p, d_op4.p, d_af.p, t.ilevel.p, d_band.p, d_logr.p, st);
+                       cp.down.p, d_op4.p, d_af, t.ilevel.p, d_band.p, d_logr.p, st);
     tm.mark(4);
     // ---- steps 6+7: list 3 and L2P
     if (pg > 0)
