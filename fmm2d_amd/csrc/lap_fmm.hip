@@ -549,7 +549,7 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
 // point's leaf through the c2d_direct* variant of the flags, one thread per
 // evaluation point.  The sources of a list-1 box are taken U = 4 at a time:
 // their loads are issued together and the exclusion rule masks a pair's
-// strengths instead of branching; the last (s1 - s0) mod 4 go one by one.
+// strengths instead of branching; the last (s1 - s0) mod U go one by one.
 //
 // RE = 1: the charges are real and only Re(pot) is consumed (lfmm2d and
 // rfmm2d split to Cauchy densities with zero imaginary parts,
@@ -724,6 +724,27 @@ __device__ __forceinline__ void near_pair64(double dx, double dy, SrcStr<NDT>& s
   pair_c64<IFC, IFD, PG, NDT, RE>(dx, dy, rr, s, a);
 }
 
+// The FP64 exclusion rule and form for one pair, out of line: it serves only
+// coincident or out-of-range pairs, and keeping its registers out of the
+// FP32 loop keeps that loop's occupancy.
+template <int IFC, int IFD, int PG, int NDT, int RE>
+__device__ __noinline__ Acc<CAUCHY, PG, NDT> slow_pair(double dx, double dy, int64_t j, int nd,
+                                                        int d0, const cplx* chs,
+                                                        const cplx* dps, double thresh,
+                                                        double thresh2) {
+  Acc<CAUCHY, PG, NDT> r;
+  r.zero();
+  const double rr = dx * dx + dy * dy;
+  double absz = 2.0 * thresh;
+  if (!IFC && rr <= 4.0 * thresh2) absz = hypot(dx, dy);
+  if (!skip_pair<CAUCHY, IFC, IFD, PG>(rr, thresh, thresh2, absz)) {
+    SrcStr<NDT> s;
+    load_str<IFC, IFD, NDT>(j, nd, d0, chs, dps, s);
+    pair_c64<IFC, IFD, PG, NDT, RE>(dx, dy, rr, s, r);
+  }
+  return r;
+}
+
 // one pair of the mixed-precision path
 template <int IFC, int IFD, int PG, int NDT, int RE>
 __device__ __forceinline__ void near_pair32(double dx, double dy, const StrF<NDT>& sf, int64_t j,
@@ -735,13 +756,13 @@ __device__ __forceinline__ void near_pair32(double dx, double dy, const StrF<NDT
   const bool fast = frr > lo && frr < 1e20f;
   pair_f32<IFC, IFD, PG, NDT, RE>(fx, fy, frr, fast, sf, f);
   if (!fast) {  // coincident points (excluded) or |z| out of the FP32 range
-    const double rr = dx * dx + dy * dy;
-    double absz = 2.0 * thresh;
-    if (!IFC && rr <= 4.0 * thresh2) absz = hypot(dx, dy);
-    if (!skip_pair<CAUCHY, IFC, IFD, PG>(rr, thresh, thresh2, absz)) {
-      SrcStr<NDT> s;
-      load_str<IFC, IFD, NDT>(j, nd, d0, chs, dps, s);
-      pair_c64<IFC, IFD, PG, NDT, RE>(dx, dy, rr, s, a);
+    const Acc<CAUCHY, PG, NDT> r =
+        slow_pair<IFC, IFD, PG, NDT, RE>(dx, dy, j, nd, d0, chs, dps, thresh, thresh2);
+#pragma unroll
+    for (int d = 0; d < NDT; ++d) {
+      a.pot[d] += r.pot[d];
+      if (PG >= 2) a.grad[d][0] += r.grad[d][0];
+      if (PG >= 3) a.hess[d][0] += r.hess[d][0];
     }
   }
 }
@@ -765,16 +786,17 @@ __global__ __launch_bounds__(256) void k_near(int64_t n, int nd, int d0, const d
     const int c = l1[(int64_t)kMaxList1 * b + kk];
     const int s0 = sse[2 * c], s1 = sse[2 * c + 1];
     int j = s0;
+    const double* px = xs + 2 * (int64_t)s0;  // source j at px (immediate offsets)
     if (F32) {
       AccF<PG, NDT> f;
       f.zero();
-      for (; j + kNearU <= s1; j += kNearU) {
+      for (; j + kNearU <= s1; j += kNearU, px += 2 * kNearU) {
         double sx[kNearU], sy[kNearU];
         StrF<NDT> sf[kNearU];
 #pragma unroll
         for (int u = 0; u < kNearU; ++u) {
-          sx[u] = xs[2 * (j + u)];
-          sy[u] = xs[2 * (j + u) + 1];
+          sx[u] = px[2 * u];
+          sy[u] = px[2 * u + 1];
           load_strf<IFC, IFD, NDT, RE>(j + u, nd, d0, ch32, dp32, sf[u]);
         }
 #pragma unroll
@@ -782,32 +804,31 @@ __global__ __launch_bounds__(256) void k_near(int64_t n, int nd, int d0, const d
           near_pair32<IFC, IFD, PG, NDT, RE>(x - sx[u], y - sy[u], sf[u], j + u, nd, d0, chs,
                                              dps, thresh, thresh2, lo, a, f);
       }
-      for (; j < s1; ++j) {
+      for (; j < s1; ++j, px += 2) {
         StrF<NDT> sf;
         load_strf<IFC, IFD, NDT, RE>(j, nd, d0, ch32, dp32, sf);
-        near_pair32<IFC, IFD, PG, NDT, RE>(x - xs[2 * j], y - xs[2 * j + 1], sf, j, nd, d0, chs,
-                                           dps, thresh, thresh2, lo, a, f);
+        near_pair32<IFC, IFD, PG, NDT, RE>(x - px[0], y - px[1], sf, j, nd, d0, chs, dps, thresh,
+                                           thresh2, lo, a, f);
       }
       f.fold(a);
     } else {
-      for (; j + kNearU <= s1; j += kNearU) {
+      for (; j + kNearU <= s1; j += kNearU, px += 2 * kNearU) {
         double sx[kNearU], sy[kNearU];
         SrcStr<NDT> s[kNearU];
 #pragma unroll
         for (int u = 0; u < kNearU; ++u) {
-          sx[u] = xs[2 * (j + u)];
-          sy[u] = xs[2 * (j + u) + 1];
+          sx[u] = px[2 * u];
+          sy[u] = px[2 * u + 1];
           load_str<IFC, IFD, NDT>(j + u, nd, d0, chs, dps, s[u]);
         }
 #pragma unroll
         for (int u = 0; u < kNearU; ++u)
           near_pair64<IFC, IFD, PG, NDT, RE>(x - sx[u], y - sy[u], s[u], thresh, thresh2, a);
       }
-      for (; j < s1; ++j) {
+      for (; j < s1; ++j, px += 2) {
         SrcStr<NDT> s;
         load_str<IFC, IFD, NDT>(j, nd, d0, chs, dps, s);
-        near_pair64<IFC, IFD, PG, NDT, RE>(x - xs[2 * j], y - xs[2 * j + 1], s, thresh, thresh2,
-                                           a);
+        near_pair64<IFC, IFD, PG, NDT, RE>(x - px[0], y - px[1], s, thresh, thresh2, a);
       }
     }
   }
