@@ -15,9 +15,10 @@ against the FP64 direct sum at 1e4 sampled sources is within eps; every
 pipeline's rate and error are in "pipelines".
 
 Also reported: per-stage times from HIP events recorded by the library on the
-call's stream, the M2L kernel's roofline fraction against MI355X's published
-MFMA peak, its HBM traffic from rocprofv3 counters (profiles/), and the
-reference library timed on the host cores on a bounded sample.
+call's stream, the roofline fractions of the near-field and M2L kernels
+against MI355X's published peaks ("roofline" is the larger of the two), their
+HBM traffic from rocprofv3 counters (profiles/), and the reference library
+timed on the host cores on a bounded sample.
 
   python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--n 10000000]
                   [--precision mixed|fp64] [--no-learned] [--no-cpu]
@@ -38,8 +39,12 @@ EPS = 1e-6
 PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
 # timeinfo slots of fmm2d_opts (include/fmm2d_amd.h)
 STAGES = {0: "p2m", 1: "list4", 2: "m2m", 3: "m2l plan", 13: "m2l kernel", 4: "l2l",
-          6: "l2p+list3", 7: "near", 8: "tree+lists", 9: "sort+gather", 10: "scatter"}
-M2L_SLOT = 13
+          6: "l2p+list3", 15: "near prep", 7: "near", 8: "tree+lists", 9: "sort+gather",
+          10: "scatter"}
+M2L_SLOT, NEAR_SLOT = 13, 7
+# keys of TRAFFIC_FILE for (M2L, near field) of each pipeline
+TRAFFIC_KEYS = {"fp64": ("m2l_fp64", "near_fp64"), "mixed": ("m2l_mixed", "near_mixed"),
+                "mixed+learned": ("m2l_learned", "near_learned")}
 # HBM bytes per M2L launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
 # scaled by the counters' reading for a copy kernel of known size
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "m2l_lfmm2d_n1e7_traffic.txt")
@@ -355,19 +360,30 @@ def main():
     R = runs[head]
     dt, stage, work, relerr = R["dt"], R["stage"], R["work"], R["relerr"]
 
-    # ---- roofline of the M2L translate kernel: algorithmic flops of the
-    # list-2 translations (8 (p+1)^2 nd per pair) over the kernel's own time,
-    # from HIP events recorded around its launch on the call's stream
-    t_m2l = stage[M2L_SLOT]
-    achieved = work[3] / t_m2l / 1e12 if t_m2l > 0 else 0.0
+    # ---- rooflines of the two largest kernels, from algorithmic flops over
+    # each kernel's own time (HIP events recorded around its launches on the
+    # call's stream): the near field (list-1 pairs, 12 + 12 nd flops each;
+    # vector-ALU bound: FP32 pairs in mixed precision, FP64 otherwise) and the
+    # M2L translations (8 (p+1)^2 nd flops per pair, MFMA).  "roofline" is the
+    # one that takes longer.
     peak = PEAK["fp64_tflops"] if head == "fp64" else PEAK["fp32_tflops"]
-    traffic_key = "fp64" if head == "fp64" else "mixed"
-    roofline = {"bound": "mfma",
-                "kernel": "k_translate (M2L, %s MFMA)" % ("FP64" if head == "fp64" else "FP32"),
-                "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4),
-                "traffic": read_traffic(traffic_key) if head != "mixed+learned" else None,
-                "launch_ms": round(t_m2l * 1e3, 4), "flops_per_launch": work[3]}
+    prec = "FP64" if head == "fp64" else "FP32"
+    t_m2l, t_near = stage[M2L_SLOT], stage[NEAR_SLOT]
+    a_m2l = work[3] / t_m2l / 1e12 if t_m2l > 0 else 0.0
+    a_near = work[7] / t_near / 1e12 if t_near > 0 else 0.0
+    rl = {
+        "m2l": {"bound": "mfma", "kernel": "k_translate (M2L, %s MFMA)" % prec,
+                "achieved": round(a_m2l, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(a_m2l / peak, 4),
+                "traffic": read_traffic(TRAFFIC_KEYS[head][0]),
+                "launch_ms": round(t_m2l * 1e3, 4), "flops_per_launch": work[3]},
+        "near": {"bound": "valu", "kernel": "k_near (list-1 pairs, %s vector ALU)" % prec,
+                 "achieved": round(a_near, 3), "peak": peak, "unit": "TFLOP/s",
+                 "frac": round(a_near / peak, 4),
+                 "traffic": read_traffic(TRAFFIC_KEYS[head][1]),
+                 "launch_ms": round(t_near * 1e3, 4), "flops_per_launch": work[7]},
+    }
+    roofline = rl["near"] if t_near >= t_m2l else rl["m2l"]
 
     cpu = None
     if not args.no_cpu:
@@ -404,6 +420,7 @@ def main():
         "rel_l2_err": relerr,
         "stage_ms": {name: round(stage[k] * 1e3, 4) for k, name in STAGES.items()},
         "roofline": roofline,
+        "rooflines": rl,
         "cpu_baseline": cpu,
     }
     f = runs["fp64"]

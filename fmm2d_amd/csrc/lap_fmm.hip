@@ -65,11 +65,17 @@ namespace {
 // (d, v) becomes -(v1 + i v2) times each part.  With half = 1 every density
 // has zero imaginary part, the imaginary Cauchy densities vanish identically
 // and only the real parts are kept (Cauchy density d).
+// (with xy given, the point coordinates are gathered into oxy in the same pass)
 __global__ void k_gather_split(int64_t n, int nd, int half, const int* perm, const cplx* ch,
-                               const cplx* ds, const double* dv, cplx* och, cplx* ods) {
+                               const cplx* ds, const double* dv, cplx* och, cplx* ods,
+                               const double* xy, double* oxy) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t j = perm[i];
+  if (xy) {
+    oxy[2 * i] = xy[2 * j];
+    oxy[2 * i + 1] = xy[2 * j + 1];
+  }
   const int np = half ? 1 : 2;
   for (int d = 0; d < nd; ++d) {
     const int64_t a = i * np * nd + np * d;
@@ -337,23 +343,32 @@ __global__ __launch_bounds__(256) void k_p2m_reg(int64_t n, int nd, int P, const
   cplx acc[PT];
 #pragma unroll
   for (int k = 0; k < PT; ++k) acc[k] = mkc(0.0);
-  for (int j = s0; j < s1; ++j) {
-    const cplx w = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
-    const cplx c = IFC ? chs[(int64_t)j * nd + d] : mkc(0.0);
-    const cplx dp = IFD ? rinv * dps[(int64_t)j * nd + d] : mkc(0.0);
-    if (IFC) acc[0] += c;
-    cplx v = mkc(1.0);  // w^(k-1)
+  // sources two at a time: two independent power chains in flight
+  for (int j = s0; j < s1; j += 2) {
+    const bool two = j + 1 < s1;
+    const int j1 = two ? j + 1 : j;
+    const cplx w0 = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
+    const cplx w1 = cplx{(xs[2 * j1] - cx) * rinv, (xs[2 * j1 + 1] - cy) * rinv};
+    cplx c0 = IFC ? chs[(int64_t)j * nd + d] : mkc(0.0);
+    cplx c1 = IFC ? chs[(int64_t)j1 * nd + d] : mkc(0.0);
+    cplx e0 = IFD ? rinv * dps[(int64_t)j * nd + d] : mkc(0.0);
+    cplx e1 = IFD ? rinv * dps[(int64_t)j1 * nd + d] : mkc(0.0);
+    if (!two) c1 = e1 = mkc(0.0);
+    if (IFC) acc[0] += c0 + c1;
+    cplx v0 = mkc(1.0), v1 = mkc(1.0);  // w^(k-1)
 #pragma unroll
     for (int k = 1; k < PT; ++k) {
       if (k <= p) {
-        if (IFD) acc[k] += v * dp;
-        v = v * w;
+        if (IFD) acc[k] += v0 * e0 + v1 * e1;
+        v0 = v0 * w0;
+        v1 = v1 * w1;
         if (IFC) {
           if (RE) {
-            const double ck = (-1.0 / k) * c.re;
-            acc[k] += cplx{v.re * ck, v.im * ck};
+            const double ck = -1.0 / k;
+            const double a0 = ck * c0.re, a1 = ck * c1.re;
+            acc[k] += cplx{v0.re * a0 + v1.re * a1, v0.im * a0 + v1.im * a1};
           } else {
-            acc[k] += (-1.0 / k) * (v * c);
+            acc[k] += (-1.0 / k) * (v0 * c0 + v1 * c1);
           }
         }
       }
@@ -468,6 +483,29 @@ __global__ __launch_bounds__(kP2lWaves * kW) void k_p2l_w(
 // l2dmpeval{p,g,h} (laprouts2d.f:289-495) of the leaf's list-3 boxes.
 constexpr int kEvU = 4;
 
+// f = sum_{k=0}^{kmax} a_k t^k with f1 = f'(t) and f2 = f''(t)/2, by Horner's
+// rule with synthetic division, coefficients a[k] read kEvU at a time from the
+// highest down (loads issued together); a_0 is taken as zero when zero0.
+template <int PG>
+__device__ __forceinline__ void horner(const cplx* a, int kmax, bool zero0, cplx t, cplx& f,
+                                       cplx& f1, cplx& f2) {
+  f = f1 = f2 = mkc(0.0);
+  for (int k0 = kmax; k0 >= 0; k0 -= kEvU) {
+    cplx q[kEvU];
+#pragma unroll
+    for (int u = 0; u < kEvU; ++u) q[u] = a[max(k0 - u, 0)];
+#pragma unroll
+    for (int u = 0; u < kEvU; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        if (PG >= 3) f2 = f2 * t + f1;
+        if (PG >= 2) f1 = f1 * t + f;
+        f = f * t + ((zero0 && k == 0) ? mkc(0.0) : q[u]);
+      }
+    }
+  }
+}
+
 template <int PG>
 __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* leaf,
                        const int* ilevel, const int* nterms, const double* rsc,
@@ -483,33 +521,16 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
   const cplx w = cplx{(x - centers[2 * b]) * rinv, (y - centers[2 * b + 1]) * rinv};
   const int c3 = n3[b];
   for (int d = 0; d < nd; ++d) {
-    cplx ap = mkc(0.0), ag = mkc(0.0), ah = mkc(0.0);
-    // local expansion: sum L_n w^n, d/dz = sum n L_n w^(n-1) / r, ...
+    cplx ap, ag = mkc(0.0), ah = mkc(0.0);
+    // local expansion f(w) = sum L_n w^n: pot f, d/dz = f'(w)/r, d2/dz2 = f''(w)/r^2
     {
-      // coefficients taken kEvU at a time (loads issued together; indices
-      // past p read zeros or unused slots of the P-row and are masked)
-      const cplx* L = loc + ((int64_t)b * nd + d) * P;
-      cplx wm2 = mkc(0.0), wm1 = mkc(0.0), wn = mkc(1.0);
-      for (int k0 = 0; k0 <= p; k0 += kEvU) {
-        cplx Lq[kEvU];
-#pragma unroll
-        for (int u = 0; u < kEvU; ++u) Lq[u] = L[min(k0 + u, P - 1)];
-#pragma unroll
-        for (int u = 0; u < kEvU; ++u) {
-          const int k = k0 + u;
-          const cplx Lk = (k <= p) ? Lq[u] : mkc(0.0);
-          ap += Lk * wn;
-          if (PG >= 2 && k >= 1) ag += (double)k * (Lk * wm1);
-          if (PG >= 3 && k >= 2) ah += (double)(k * (k - 1)) * (Lk * wm2);
-          wm2 = wm1;
-          wm1 = wn;
-          wn = wn * w;
-        }
-      }
-      if (PG >= 2) ag = ag * rinv;
-      if (PG >= 3) ah = ah * (rinv * rinv);
+      cplx f1, f2;
+      horner<PG>(loc + ((int64_t)b * nd + d) * P, p, false, w, ap, f1, f2);
+      if (PG >= 2) ag = f1 * rinv;
+      if (PG >= 3) ah = (2.0 * rinv * rinv) * f2;
     }
-    // multipoles of the list-3 boxes
+    // multipoles of the list-3 boxes: M_0 log|z| + g(u), g(u) = sum_{n>=1} M_n u^n,
+    // u = r/z; d/dz = M_0/z - g'(u) u^2/r, d2/dz2 = -M_0/z^2 + (g''(u) u^4 + 2 g'(u) u^3)/r^2
     for (int k3 = 0; k3 < c3; ++k3) {
       const int c = l3[(int64_t)kMaxList3 * b + k3];
       const int lc = ilevel[c];
@@ -518,23 +539,16 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
       const cplx z = cplx{x - centers[2 * c], y - centers[2 * c + 1]};
       const cplx u = rc * invc(z);  // r/z
       const cplx* M = mp + ((int64_t)c * nd + d) * P;
-      ap += log(absc(z)) * M[0];
-      if (PG >= 2) ag += (u * rci) * M[0];
-      if (PG >= 3) ah -= (u * u) * (rci * rci) * M[0];
-      cplx un = u;  // u^n
-      for (int k0 = 1; k0 <= pc; k0 += kEvU) {
-        cplx Mq[kEvU];
-#pragma unroll
-        for (int v = 0; v < kEvU; ++v) Mq[v] = M[min(k0 + v, P - 1)];
-#pragma unroll
-        for (int v = 0; v < kEvU; ++v) {
-          const int k = k0 + v;
-          const cplx Mk = (k <= pc) ? Mq[v] : mkc(0.0);
-          ap += Mk * un;
-          const cplx un1 = un * u;
-          if (PG >= 2) ag -= ((double)k * rci) * (Mk * un1);
-          if (PG >= 3) ah += ((double)(k * (k + 1)) * rci * rci) * (Mk * (un1 * u));
-          un = un1;
+      const cplx m0 = M[0];
+      cplx g, g1, g2;
+      horner<PG>(M, pc, true, u, g, g1, g2);
+      ap += log(absc(z)) * m0 + g;
+      if (PG >= 2) {
+        const cplx u2 = u * u;
+        ag += (u * rci) * m0 - (u2 * rci) * g1;
+        if (PG >= 3) {
+          const cplx u3 = u2 * u;
+          ah += (rci * rci) * (-(u2 * m0) + 2.0 * ((u3 * u) * g2 + u3 * g1));
         }
       }
     }
@@ -1081,7 +1095,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     // ---- sort and gather
     DBuf<double> xs(2 * ns, st), xt(2 * std::max<int64_t>(nt, 1), st);
     DBuf<cplx> chs(a.ifcharge ? ns * nd : 1, st), dps(a.ifdipole ? ns * nd : 1, st);
-    launch1d(k_gather_xy, ns, st, t.isrc.p, a.src, xs.p);
+    if (kind != 1) launch1d(k_gather_xy, ns, st, t.isrc.p, a.src, xs.p);
     if (nt > 0) launch1d(k_gather_xy, nt, st, t.itarg.p, a.targ, xt.p);
     if (kind == 0) {
       if (a.ifcharge) launch1d(k_gather_dens, ns, st, nd, t.isrc.p, a.charge, chs.p);
@@ -1093,7 +1107,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
                chs.p, dps.p);
     } else {
       launch1d(k_gather_split, ns, st, a.nd, half, t.isrc.p, a.ifcharge ? a.charge : nullptr,
-               a.ifdipole ? a.dipstr : nullptr, a.dipvec, chs.p, dps.p);
+               a.ifdipole ? a.dipstr : nullptr, a.dipvec, chs.p, dps.p, a.src, xs.p);
     }
     DBuf<int> leaf_s(ns, st), leaf_t(std::max<int64_t>(nt, 1), st);
     launch1d(k_leaf_of, nb, st, t.nchild.p, t.isrcse.p, leaf_s.p);
@@ -1281,6 +1295,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
         dp32 = DBuf<float>(2 * ns * nd, st);
         launch1d(k_to_f32, 2 * ns * nd, st, reinterpret_cast<const double*>(dps.p), dp32.p);
       }
+      tm.mark(15);  // FP32 strength copies; [7] then holds the near-field kernels alone
       if (pg > 0)
         launch_near(mixed, re, pg, a.ifcharge, a.ifdipole, ns, st,
                     NearArgs{nd, xs.p, leaf_s.p, &L, sse, xs.p, chs.p, dps.p, ch32.p, dp32.p,

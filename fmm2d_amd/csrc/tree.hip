@@ -30,6 +30,9 @@ namespace {
 // The reference writes `1.05*boxsize(ilev)` with a default-real literal in
 // computecoll / computelists: 1.05 rounded to single precision, then promoted.
 constexpr double kColl = (double)1.05f;
+constexpr int kMaxLevels = 51;  // nlmax of the reference (pts_tree2d.f)
+constexpr int kBatch = 8;             // refinement levels per host synchronisation
+constexpr int64_t kBatchBoxes = 1 << 24;  // box capacity a batch may reserve
 
 struct Boxes {
   DBuf<double> centers;
@@ -212,6 +215,36 @@ __global__ void k_total(int n, const int* flag, const int* offs, int* total) {
   *total = (n > 0) ? offs[n - 1] + flag[n - 1] : 0;
 }
 
+// Refinement of several levels between host synchronisations: dlev[2 l],
+// dlev[2 l + 1] hold the first box and the box count of level l, written by
+// k_level_next on the device; the launches cover an upper bound `ub` of the
+// level's size and read the count from dlev.
+__global__ void k_refine_flags_lv(int64_t ub, const int* dlev, int l, const int* ses,
+                                  const int* set, int idivflag, int ndiv, int* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ub) return;
+  const int lo = dlev[2 * l], n = dlev[2 * l + 1];
+  int f = 0;
+  if (i < n) {
+    const int b = lo + (int)i;
+    const int cs = ses[2 * b + 1] - ses[2 * b], ct = set[2 * b + 1] - set[2 * b];
+    int nn = cs;
+    if (idivflag == 1) nn = ct;
+    if (idivflag == 2) nn = max(cs, ct);
+    f = nn > ndiv ? 1 : 0;
+  }
+  flag[i] = f;
+}
+
+// level l+1 = the children of level l's flagged boxes, numbered from *dnb
+__global__ void k_level_next(int64_t ub, const int* flag, const int* offs, int* dlev, int l,
+                             int* dnb) {
+  const int nref = offs[ub - 1] + flag[ub - 1];
+  dlev[2 * (l + 1)] = *dnb;
+  dlev[2 * (l + 1) + 1] = 4 * nref;
+  *dnb += 4 * nref;
+}
+
 // box 1 (index 0): the root, holding every point
 __global__ void k_init_root(double cx, double cy, int ns, int nt, double* centers, int* ilevel,
                             int* iparent, int* nchild, int* ichild, int* ses, int* set,
@@ -244,14 +277,11 @@ __global__ void k_seg_totals(int K, const int* cut, const int* flag, const int* 
 // (tree_refine_boxes, tree_routs2d.f:25-52).  Their point ranges split the
 // parent's by the path digit at level lev.  mode 1 (level-restriction
 // refinement): child iflag = 3 when the parent's flag is 1, else 0.
-__global__ void k_make_children(int64_t n, int lo, const int* flag, const int* offs, int base,
-                                double half, int lev, double* centers, int* ilevel, int* iparent,
-                                int* nchild, int* ichild, int* ses, int* set, int* iflag, int mode,
-                                Keys ks, Keys kt) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !flag[i]) return;
-  const int b = lo + (int)i;
-  const int c0 = base + 4 * offs[i];
+// the four children of box b, numbered c0 .. c0+3, at level lev
+__device__ void make_children_of(int b, int c0, double half, int lev, double* centers,
+                                 int* ilevel, int* iparent, int* nchild, int* ichild, int* ses,
+                                 int* set, int* iflag, int mode, const Keys& ks,
+                                 const Keys& kt) {
   const double cx = centers[2 * b], cy = centers[2 * b + 1];
   nchild[b] = 4;
   const int pf = (mode == 1) ? iflag[b] : 0;
@@ -279,6 +309,26 @@ __global__ void k_make_children(int64_t n, int lo, const int* flag, const int* o
     set[2 * c + 1] = bt_[q + 1];
     iflag[c] = (mode == 1 && pf == 1) ? 3 : 0;
   }
+}
+
+__global__ void k_make_children(int64_t n, int lo, const int* flag, const int* offs, int base,
+                                double half, int lev, double* centers, int* ilevel, int* iparent,
+                                int* nchild, int* ichild, int* ses, int* set, int* iflag, int mode,
+                                Keys ks, Keys kt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  make_children_of(lo + (int)i, base + 4 * offs[i], half, lev, centers, ilevel, iparent, nchild,
+                   ichild, ses, set, iflag, mode, ks, kt);
+}
+
+__global__ void k_make_children_lv(int64_t ub, const int* dlev, int l, const int* flag,
+                                   const int* offs, double half, double* centers, int* ilevel,
+                                   int* iparent, int* nchild, int* ichild, int* ses, int* set,
+                                   int* iflag, Keys ks, Keys kt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ub || i >= dlev[2 * l + 1] || !flag[i]) return;
+  make_children_of(dlev[2 * l] + (int)i, dlev[2 * (l + 1)] + 4 * offs[i], half, l + 1, centers,
+                   ilevel, iparent, nchild, ichild, ses, set, iflag, 0, ks, kt);
 }
 
 // ------------------------------------------------------------ colleagues
@@ -441,11 +491,20 @@ __global__ void k_root_coll(int* ncoll, int* coll) {
 // ------------------------------------------------------------ level restriction
 // flag pass (pts_tree2d.f:797-839): leaves among the grandparent's colleagues
 // that touch the parent of a box at level lev.
-__global__ void k_flag1(int64_t n, int lo, double distest, const int* iparent, const int* ncoll,
-                        const int* coll, const int* nchild, const double* centers, int* iflag) {
+// It only turns flags 0 into 1, so the levels (the reference sweeps them from
+// the finest up) run in one launch; distest = 1.05 (bs_{l-1} + bs_{l-2}) / 2 of
+// the box's level l comes from `dist`.
+struct LevelReals {
+  double v[kMaxLevels + 1];
+};
+
+__global__ void k_flag1(int64_t n, int lo, LevelReals dist, const int* ilevel, const int* iparent,
+                        const int* ncoll, const int* coll, const int* nchild,
+                        const double* centers, int* iflag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = lo + (int)i;
+  const double distest = dist.v[ilevel[b]];
   const int dad = iparent[b];
   const int gd = iparent[dad];
   const double dx0 = centers[2 * dad], dy0 = centers[2 * dad + 1];
@@ -509,10 +568,25 @@ __global__ void k_keep3(int64_t n, int* iflag) {
 }
 
 // ------------------------------------------------------------ reorganisation
-__global__ void k_map_range(int64_t n, int from, int to, int* map) {
+// The new number of every box: the boxes are covered by segments (a level's
+// original boxes, a level's tail boxes), sorted by first box; segment k moves
+// [from[k], from[k+1]) to to[k], to[k] + 1, ...
+constexpr int kMaxSeg = 2 * (kMaxLevels + 1);
+struct SegTable {
+  int n;
+  int from[kMaxSeg], to[kMaxSeg];
+};
+
+__global__ void k_map_segments(int64_t nb, SegTable T, int* map) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  map[from + i] = to + (int)i;
+  if (i >= nb) return;
+  int lo = 0, hi = T.n - 1;  // last segment with from <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T.from[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  map[i] = T.to[lo] + (int)(i - T.from[lo]);
 }
 
 __global__ void k_reorg_apply(int64_t n, const int* map, const double* c0, const int* lev0,
@@ -792,18 +866,28 @@ struct Builder {
   void reorg(std::vector<int>& ls, std::vector<int>& le, const std::vector<int>& ts,
              const std::vector<int>& te, int nlevels) {
     DBuf<int> map(nboxes, st);
+    std::vector<std::pair<int, int>> seg;  // (from, to) of non-empty segments
     for (int l = 0; l <= std::min(1, nlevels); ++l)
-      launch1d(k_map_range, le[l] - ls[l], st, ls[l], ls[l], map.p);
+      if (le[l] > ls[l]) seg.push_back({ls[l], ls[l]});
     int cur = (nlevels >= 2) ? ls[2] : 0;
     for (int l = 2; l <= nlevels; ++l) {
       const int s = cur;
-      launch1d(k_map_range, le[l] - ls[l], st, ls[l], cur, map.p);
+      if (le[l] > ls[l]) seg.push_back({ls[l], cur});
       cur += le[l] - ls[l];
-      launch1d(k_map_range, te[l] - ts[l], st, ts[l], cur, map.p);
+      if (te[l] > ts[l]) seg.push_back({ts[l], cur});
       cur += te[l] - ts[l];
       ls[l] = s;
       le[l] = cur;
     }
+    std::sort(seg.begin(), seg.end());
+    SegTable T{};
+    T.n = (int)seg.size();
+    if (T.n > kMaxSeg) throw HipError{hipErrorInvalidValue, "tree reorganisation segments"};
+    for (int k = 0; k < T.n; ++k) {
+      T.from[k] = seg[k].first;
+      T.to[k] = seg[k].second;
+    }
+    launch1d(k_map_segments, nboxes, st, T, map.p);
     Boxes N;
     N.alloc(B.cap, st);
     launch1d(k_reorg_apply, nboxes, st, map.p, B.centers.p, B.ilevel.p, B.iparent.p, B.nchild.p,
@@ -824,6 +908,7 @@ struct Builder {
 
 void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, int ndiv,
                 int idivflag, int nlmax, Tree& t, hipStream_t st) {
+  if (nlmax > kMaxLevels) throw HipError{hipErrorInvalidValue, "nlmax beyond 51"};
   Builder bld(st);
   t.ns = ns;
   t.nt = nt;
@@ -867,35 +952,106 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
                      bld.B.centers.p, bld.B.ilevel.p, bld.B.iparent.p, bld.B.nchild.p,
                      bld.B.ichild.p, bld.B.ses.p, bld.B.set.p, bld.B.iflag.p);
 
-  // ---- level-by-level refinement (pts_tree2d.f:551-631)
+  // ---- level-by-level refinement (pts_tree2d.f:551-631).  While the
+  // bound on a level's size is moderate, kBatch levels run between host
+  // synchronisations (level ranges kept on the device); a refined box holds
+  // more than ndiv points, so a level has at most 4 (ns + nt) / (ndiv + 1)
+  // boxes.  Otherwise, and for the key-extension check at level d1, one level
+  // at a time with its exact count.
   std::vector<int> ls{0}, le{1};
   std::vector<double> bs{bsall[0]};
   int nlevels = nlmax;
-  for (int l = 0; l < nlmax; ++l) {
+  const int64_t lev_cap = 4 * ((ns + nt) / (std::max(ndiv, 0) + 1) + 1);
+  DBuf<int> dlev(2 * (nlmax + 2), st), dnb(1, st);
+  int l = 0;
+  bool done = false;
+  while (!done && l < nlmax) {
     const int lo = ls[l], n = le[l] - ls[l];
-    bld.ensure_flag(n);
-    launch1d(k_refine_flags, n, st, lo, bld.B.ses.p, bld.B.set.p, idivflag, ndiv, bld.flag.p);
-    if (l + 1 > d1 && pk[0].d2 == 0) {
-      // a box at level d1 may need splitting: extend the paths (rare: only
-      // for points closer than 2^-31 of the domain)
-      if (bld.scan(n) > 0) {
-        const int d2 = std::min(kD2Max, std::min(62, nlmax) - d1);
-        pk[0].build(src, ns, root[0], root[1], dhalf.p, d1, d2, st);
-        pk[1].build(trg, nt, root[0], root[1], dhalf.p, d1, d2, st);
-        bld.ks = pk[0].view();
-        bld.kt = pk[1].view();
+    // the batch: levels l .. l+K-1 refined, sizes bounded by ub[k]
+    int K = std::min(kBatch, nlmax - l);
+    if (pk[0].d2 == 0 && l < d1) K = std::min(K, d1 - l);
+    // as many levels as fit the capacity a batch may reserve
+    const int64_t limit = std::max<int64_t>(kBatchBoxes, 2 * (int64_t)bld.nboxes);
+    std::vector<int64_t> ub{n};
+    int64_t reserve = 0;
+    for (int k = 1; k <= K; ++k) {
+      const int64_t u = std::min<int64_t>(4 * ub[k - 1], lev_cap);
+      if (reserve + u > limit) {
+        K = k - 1;
+        break;
+      }
+      ub.push_back(u);
+      reserve += u;
+    }
+    const bool batch = (pk[0].d2 != 0 || l < d1) && K > 1;
+    if (!batch) {
+      // one level, exact (pts_tree2d.f:551-631 as written)
+      bld.ensure_flag(n);
+      launch1d(k_refine_flags, n, st, lo, bld.B.ses.p, bld.B.set.p, idivflag, ndiv, bld.flag.p);
+      if (l + 1 > d1 && pk[0].d2 == 0) {
+        // a box at level d1 may need splitting: extend the paths (rare: only
+        // for points closer than 2^-21 of the domain)
+        if (bld.scan(n) > 0) {
+          const int d2 = std::min(kD2Max, std::min(62, nlmax) - d1);
+          pk[0].build(src, ns, root[0], root[1], dhalf.p, d1, d2, st);
+          pk[1].build(trg, nt, root[0], root[1], dhalf.p, d1, d2, st);
+          bld.ks = pk[0].view();
+          bld.kt = pk[1].view();
+        }
+      }
+      const int base = bld.nboxes;
+      bs.push_back(bsall[l + 1]);
+      const int nref = bld.refine(lo, n, bs[l + 1], l + 1, 0);
+      if (nref == 0) {
+        bs.pop_back();
+        nlevels = l;
+        break;
+      }
+      ls.push_back(base);
+      le.push_back(bld.nboxes);
+      ++l;
+      continue;
+    }
+    bld.B.grow(bld.nboxes + (size_t)reserve, bld.nboxes);
+    int64_t ubmax = 1;
+    for (int k = 0; k < K; ++k) ubmax = std::max(ubmax, ub[k]);
+    bld.ensure_flag((int)ubmax);
+    {
+      const int h[3] = {lo, n, bld.nboxes};
+      hip_throw(hipMemcpyAsync(dlev.p + 2 * l, h, 2 * sizeof(int), hipMemcpyHostToDevice, st),
+                "copy");
+      hip_throw(hipMemcpyAsync(dnb.p, h + 2, sizeof(int), hipMemcpyHostToDevice, st), "copy");
+      for (int k = 0; k < K; ++k) {
+        const int L = l + k;
+        launch1d(k_refine_flags_lv, ub[k], st, dlev.p, L, bld.B.ses.p, bld.B.set.p, idivflag,
+                 ndiv, bld.flag.p);
+        bld.scan_only((int)ub[k]);
+        hipLaunchKernelGGL(k_level_next, dim3(1), dim3(1), 0, st, ub[k], bld.flag.p, bld.offs.p,
+                           dlev.p, L, dnb.p);
+        launch1d(k_make_children_lv, ub[k], st, dlev.p, L, bld.flag.p, bld.offs.p,
+                 0.5 * bsall[L + 1], bld.B.centers.p, bld.B.ilevel.p, bld.B.iparent.p,
+                 bld.B.nchild.p, bld.B.ichild.p, bld.B.ses.p, bld.B.set.p, bld.B.iflag.p, bld.ks,
+                 bld.kt);
+      }
+      std::vector<int> hl(2 * (K + 1));
+      int hnb = 0;
+      hip_throw(hipMemcpyAsync(hl.data(), dlev.p + 2 * l, sizeof(int) * 2 * (K + 1),
+                               hipMemcpyDeviceToHost, st), "copy");
+      hip_throw(hipMemcpyAsync(&hnb, dnb.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
+      hip_throw(hipStreamSynchronize(st), "sync");  // h, hl, hnb live until here
+      bld.nboxes = hnb;
+      for (int k = 1; k <= K; ++k) {
+        if (hl[2 * k + 1] == 0) {
+          nlevels = l + k - 1;
+          done = true;
+          break;
+        }
+        bs.push_back(bsall[l + k]);
+        ls.push_back(hl[2 * k]);
+        le.push_back(hl[2 * k] + hl[2 * k + 1]);
       }
     }
-    const int base = bld.nboxes;
-    bs.push_back(bsall[l + 1]);
-    const int nref = bld.refine(lo, n, bs[l + 1], l + 1, 0);
-    if (nref == 0) {
-      bs.pop_back();
-      nlevels = l;
-      break;
-    }
-    ls.push_back(base);
-    le.push_back(bld.nboxes);
+    l += K;
   }
   ls.resize(nlevels + 1);
   le.resize(nlevels + 1);
@@ -908,10 +1064,11 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   if (nlevels >= 2) {
     int* iflag = bld.B.iflag.p;
     hip_throw(hipMemsetAsync(iflag, 0, sizeof(int) * bld.nboxes, st), "memset");
-    for (int l = nlevels; l >= 2; --l) {
-      const double distest = 1.05 * (bs[l - 1] + bs[l - 2]) / 2.0;
-      launch1d(k_flag1, le[l] - ls[l], st, ls[l], distest, bld.B.iparent.p, bld.B.ncoll.p,
-               bld.B.coll.p, bld.B.nchild.p, bld.B.centers.p, iflag);
+    {
+      LevelReals dist{};
+      for (int l = 2; l <= nlevels; ++l) dist.v[l] = 1.05 * (bs[l - 1] + bs[l - 2]) / 2.0;
+      launch1d(k_flag1, le[nlevels] - ls[2], st, ls[2], dist, bld.B.ilevel.p, bld.B.iparent.p,
+               bld.B.ncoll.p, bld.B.coll.p, bld.B.nchild.p, bld.B.centers.p, iflag);
     }
     for (int l = nlevels; l >= 1; --l) {
       const double distest = 1.05 * (bs[l] + bs[l - 1]) / 2.0;

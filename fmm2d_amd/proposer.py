@@ -13,7 +13,7 @@ Helmholtz flag, and the fraction of points held by leaves at levels 0..15 of
 the tree built with the default leaf capacity.
 Outputs (17): r_l, l = 0..15, the fractional reduction of level l's default
 order (nterms_l = max(2, round(p_l (1 - r_l))), r_l clipped to [0, 0.7]), and
-log2 of the leaf capacity relative to the default one (clipped to [-1.5, 2]).
+log2 of the leaf capacity relative to the default one (clipped to [-1.5, 3]).
 """
 import os
 
@@ -77,7 +77,7 @@ def decode(y, p_default, ndiv_default, margin=0.0):
     nl = len(p_default)
     rr = np.array([r[min(l, NLEV - 1)] for l in range(nl)])
     nterms = [int(max(2, round(p * (1.0 - x)))) for p, x in zip(p_default, rr)]
-    e = float(np.clip(y[NLEV], -1.5, 2.0))
+    e = float(np.clip(y[NLEV], -1.5, 3.0))
     ndiv = int(np.clip(round(ndiv_default * 2.0 ** e), 4, 400))
     return nterms, ndiv
 

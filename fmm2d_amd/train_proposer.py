@@ -3,10 +3,11 @@
   python -m fmm2d_amd.train_proposer --problems 160 --out fmm2d_amd/data/proposer.npz
 
 For each random problem (Laplace lfmm2d or Helmholtz hfmm2d; points uniform in
-a square, Gaussian clusters or on the curve r = 1 + 0.3 cos 5t; N log-uniform;
-zk log-uniform in [1, 300]; eps in {1e-4, 1e-6, 1e-9}) the FMM runs on a grid
-of parameters -- order reduction r in {0, .1, ..., .5} at every level and leaf
-capacity 2^e times the default, e in {-1, 0, 1} -- and each run's GPU time
+a square, 4-64 Gaussian clusters or on the curve r = 1 + 0.3 cos 5t; N log-uniform
+in [1e4, 1e7]; zk log-uniform in [1, 300]; eps in {1e-4, 1e-6, 1e-9}) the FMM
+runs on a grid of parameters -- order reduction r in {0, .1, ..., .5} at every
+level and leaf capacity 2^e times the default, e in {-1, 0, 1, 2, 3} -- and each
+run's GPU time
 (the driver's own event timing, timeinfo[11]) and relative L2 error of
 pot+grad against the FP64 direct sum at 256 sampled sources are recorded.
 The target of a problem is the fastest setting whose error is at most eps/2.
@@ -25,7 +26,7 @@ from . import fmm2d as F
 from . import proposer as PR
 
 R_GRID = [0.0, 0.1, 0.2, 0.3, 0.4, 0.5]
-E_GRID = [-1, 0, 1]
+E_GRID = [-1, 0, 1, 2, 3]
 NSAMPLE = 256
 REPEATS = 1  # timed calls per setting (after one warm-up call); the minimum is kept
 
@@ -34,7 +35,7 @@ def make_points(rng, n, dist):
     if dist == "uniform":
         return rng.random((2, n))
     if dist == "cluster":
-        k = int(rng.integers(4, 33))
+        k = int(rng.integers(4, 65))
         cen = rng.uniform(0.1, 0.9, (2, k))
         sig = np.exp(rng.uniform(np.log(1e-3), np.log(3e-2), k))
         lab = rng.integers(0, k, n)
@@ -50,7 +51,7 @@ def make_points(rng, n, dist):
 def run_problem(rng):
     kind = "h" if rng.random() < 0.5 else "l"
     dist = ["uniform", "cluster", "curve"][int(rng.integers(0, 3))]
-    n = int(np.exp(rng.uniform(np.log(1e4), np.log(2e5))))
+    n = int(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
     eps = float([1e-4, 1e-6, 1e-9][int(rng.integers(0, 3))])
     zk = complex(np.exp(rng.uniform(0, np.log(300.0)))) if kind == "h" else None
     src = make_points(rng, n, dist)

@@ -56,7 +56,9 @@ typedef struct {
    * run fused and are reported together in [6]); [8] tree + lists, [9] sort
    * and gather, [10] scatter, [11] whole call, [12] translation operators
    * (Helmholtz), [13] the M2L translate kernels alone ([3] then holds the
-   * M2L planning: target classes and source slots). */
+   * M2L planning: target classes and source slots), [15] the FP32 copies of
+   * the strengths for the mixed-precision near field (Laplace; [7] then holds
+   * the near-field kernels alone). */
   double timeinfo[16];
   int32_t nlevels, nboxes; /* out: tree depth and box count */
   int32_t pad1, pad2;
