@@ -357,7 +357,7 @@ __global__ void k_coll(int64_t n, int lo, double bs, const int* iparent, const i
   for (int k = cnt; k < kMaxColl; ++k) coll[9 * b + k] = -1;
 }
 
-// computecoll for every box at once (trees of at most kHashLevels levels).
+// computecoll for every box at once (trees of at most kWalkLevels levels).
 // The colleagues of a box are the boxes of its level whose integer
 // coordinates (ix, iy) differ by at most one in each direction -- exactly the
 // boxes the reference finds among the children of the parent's colleagues
@@ -365,8 +365,7 @@ __global__ void k_coll(int64_t n, int lo, double bs, const int* iparent, const i
 // parent's colleagues and then of the child slots (-,-), (+,-), (-,+), (+,+),
 // which, by induction from the root, is increasing path order: the
 // interleaved code of (iy, ix), iy bit above ix bit, coarsest level first.
-constexpr int kHashLevels = 29;
-constexpr unsigned long long kHashEmpty = ~0ull;
+constexpr int kWalkLevels = 29;
 
 // (ix, iy) of every box, walking up to the root: the child centre is the
 // parent centre +- half the child size, so the comparisons are exact.
@@ -391,10 +390,6 @@ __global__ void k_box_ij(int64_t nb, const int* iparent, const int* ilevel,
   ij[2 * i + 1] = iy;
 }
 
-__device__ __forceinline__ unsigned long long box_key(int lev, unsigned ix, unsigned iy) {
-  return ((unsigned long long)lev << 58) | ((unsigned long long)ix << 29) | iy;
-}
-
 // spread the 29 bits of v to the even bit positions
 __device__ __forceinline__ unsigned long long spread_bits(unsigned v) {
   unsigned long long x = v;
@@ -411,64 +406,43 @@ __device__ __forceinline__ unsigned long long path_code(unsigned ix, unsigned iy
   return spread_bits(ix) | (spread_bits(iy) << 1);
 }
 
-// table slot: multiplicative hash of the key (open addressing, linear probing)
-__device__ __forceinline__ unsigned long long hash_slot(int lev, unsigned ix, unsigned iy,
-                                                        int logh) {
-  return (box_key(lev, ix, iy) * 0x9E3779B97F4A7C15ull) >> (64 - logh);
-}
-
-__global__ void k_hash_fill(int64_t n, unsigned long long* keys) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) keys[i] = kHashEmpty;
-}
-
-__global__ void k_hash_insert(int64_t nb, const int* ilevel, const unsigned* ij, int logh,
-                              unsigned long long* keys, int* vals) {
+// Colleagues of every box in one pass.  The neighbour (jx, jy) of box b
+// (level lev, coordinates (ix, iy)) shares b's ancestors down to level
+// m = lev - max(bitlength(ix ^ jx), bitlength(iy ^ jy)); it is found by going
+// up lev - m parents and down lev - m child slots, slot 2 qy + qx taken from
+// the bits of (jx, jy) (the reference's child order (-,-), (+,-), (-,+),
+// (+,+)); a missing child ends the search.
+__global__ void k_coll_walk(int64_t nb, const int* ilevel, const int* iparent, const int* ichild,
+                            const unsigned* ij, int* ncoll, int* coll) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
-  const int lev = ilevel[i];
-  const unsigned ix = ij[2 * i], iy = ij[2 * i + 1];
-  const unsigned long long key = box_key(lev, ix, iy);
-  const unsigned long long mask = (1ull << logh) - 1;
-  for (unsigned long long h = hash_slot(lev, ix, iy, logh);; h = (h + 1) & mask) {
-    const unsigned long long prev = atomicCAS(&keys[h], kHashEmpty, key);
-    if (prev == kHashEmpty || prev == key) {
-      vals[h] = (int)i;
-      return;
-    }
-  }
-}
-
-__device__ __forceinline__ int hash_find(int lev, unsigned ix, unsigned iy, int logh,
-                                         const unsigned long long* keys, const int* vals) {
-  const unsigned long long key = box_key(lev, ix, iy);
-  const unsigned long long mask = (1ull << logh) - 1;
-  for (unsigned long long h = hash_slot(lev, ix, iy, logh);; h = (h + 1) & mask) {
-    const unsigned long long k = keys[h];
-    if (k == key) return vals[h];
-    if (k == kHashEmpty) return -1;
-  }
-}
-
-__global__ void k_coll_hash(int64_t nb, const int* ilevel, const unsigned* ij, int logh,
-                            const unsigned long long* keys, const int* vals, int* ncoll,
-                            int* coll) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb) return;
-  const int lev = ilevel[i];
+  const int b = (int)i;
+  const int lev = ilevel[b];
   const long long side = 1ll << lev;
-  const long long ix = ij[2 * i], iy = ij[2 * i + 1];
+  const unsigned ix = ij[2 * i], iy = ij[2 * i + 1];
   int id[kMaxColl];
   unsigned long long code[kMaxColl];
   int cnt = 0;
   for (int dy = -1; dy <= 1; ++dy)
     for (int dx = -1; dx <= 1; ++dx) {
-      const long long jx = ix + dx, jy = iy + dy;
-      if (jx < 0 || jy < 0 || jx >= side || jy >= side) continue;
-      const int j = hash_find(lev, (unsigned)jx, (unsigned)jy, logh, keys, vals);
+      const long long jxl = (long long)ix + dx, jyl = (long long)iy + dy;
+      if (jxl < 0 || jyl < 0 || jxl >= side || jyl >= side) continue;
+      const unsigned jx = (unsigned)jxl, jy = (unsigned)jyl;
+      int j = b;
+      if (dx != 0 || dy != 0) {
+        const unsigned diff = (ix ^ jx) | (iy ^ jy);
+        const int up = 32 - __clz(diff);  // levels to the common ancestor
+        int a = b;
+        for (int t = 0; t < up; ++t) a = iparent[a];
+        for (int t = up - 1; t >= 0 && a >= 0; --t) {
+          const int q = (int)((((jy >> t) & 1u) << 1) | ((jx >> t) & 1u));
+          a = ichild[4 * a + q];
+        }
+        j = a;
+      }
       if (j < 0) continue;
       // insertion by path code
-      const unsigned long long cj = path_code((unsigned)jx, (unsigned)jy);
+      const unsigned long long cj = path_code(jx, jy);
       int k = cnt++;
       while (k > 0 && code[k - 1] > cj) {
         code[k] = code[k - 1];
@@ -754,7 +728,7 @@ struct Builder {
   Keys ks{}, kt{};
   // the centre comparisons of k_box_ij are exact when the domain is not far
   // from the origin relative to its size
-  bool hash_ok = false;
+  bool walk_ok = false;
 
   explicit Builder(hipStream_t s) : st(s) { total.alloc(64, s); }
 
@@ -840,20 +814,12 @@ struct Builder {
   }
   void computecoll(const std::vector<int>& ls, const std::vector<int>& le,
                    const std::vector<double>& bs, int nlevels) {
-    // all levels at once through a hash of the boxes' (level, ix, iy) where
-    // that is cheaper than one launch per level: measured on MI355X, about
-    // 0.38 us per 1000 boxes against about 18 us per level
-    if (nlevels <= kHashLevels && hash_ok && (double)nboxes * 0.38e-3 < 18.0 * nlevels) {
-      int logh = 1;
-      while ((int64_t(1) << logh) < 2 * (int64_t)nboxes) ++logh;
-      const int64_t hn = int64_t(1) << logh;
+    // all levels at once by walking the tree from every box (integer
+    // coordinates exact: at most kWalkLevels levels, domain near the origin)
+    if (nlevels <= kWalkLevels && walk_ok) {
       DBuf<unsigned> ij(2 * (int64_t)nboxes, st);
-      DBuf<unsigned long long> keys(hn, st);
-      DBuf<int> vals(hn, st);
       launch1d(k_box_ij, nboxes, st, B.iparent.p, B.ilevel.p, B.centers.p, ij.p);
-      launch1d(k_hash_fill, hn, st, keys.p);
-      launch1d(k_hash_insert, nboxes, st, B.ilevel.p, ij.p, logh, keys.p, vals.p);
-      launch1d(k_coll_hash, nboxes, st, B.ilevel.p, ij.p, logh, keys.p, vals.p, B.ncoll.p,
+      launch1d(k_coll_walk, nboxes, st, B.ilevel.p, B.iparent.p, B.ichild.p, ij.p, B.ncoll.p,
                B.coll.p);
       return;
     }
@@ -929,7 +895,7 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   if (sizey > bsall[0]) bsall[0] = sizey;
   for (int l = 1; l < (int)bsall.size(); ++l) bsall[l] = bsall[l - 1] / 2;
   const double root[2] = {(bb[0] + bb[1]) / 2, (bb[2] + bb[3]) / 2};
-  bld.hash_ok = std::max(std::fabs(root[0]), std::fabs(root[1])) <= std::ldexp(bsall[0], 20);
+  bld.walk_ok = std::max(std::fabs(root[0]), std::fabs(root[1])) <= std::ldexp(bsall[0], 20);
 
   // ---- point paths, sorted (sources and targets separately)
   std::vector<double> half(bsall.size());
