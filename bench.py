@@ -47,7 +47,7 @@ TRAFFIC_KEYS = {"fp64": ("m2l_fp64", "near_fp64"), "mixed": ("m2l_mixed", "near_
                 "mixed+learned": ("m2l_learned", "near_learned")}
 # HBM bytes per M2L launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
 # scaled by the counters' reading for a copy kernel of known size
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "m2l_lfmm2d_n1e7_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lfmm2d_n1e7_traffic.txt")
 
 
 def read_traffic(precision):
