@@ -10,7 +10,7 @@
 // Slot s of target t reads source slots[row * nslots + s] (-1: empty), row =
 // t's position in tlist, or its box id (slots_by_box).
 // Work is split into groups: group g owns targets tlist[seg0[g]..seg1[g])
-// and waves [tile0[g], tile0[g+1]); within a group every target has the same
+// and tiles [tile0[g], tile0[g+1]) of 16 columns; within a group every target has the same
 // slot -> operator map (opbase[g] + slot_op[cls[g]*nslots + s]), so the A
 // operand is uniform across a wavefront, whose 16 columns are (target,
 // density) pairs.  MODE 0 gathers: dst[t] += sum_s T_s src[slot(t,s)]; MODE 1
@@ -31,7 +31,7 @@ namespace fmm2d {
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 // Group table of one launch, passed by value: group g owns targets
-// tlist[seg0[g]..seg1[g]) and waves [tile0[g], tile0[g+1]).
+// tlist[seg0[g]..seg1[g]) and tiles [tile0[g], tile0[g+1]).
 constexpr int kTrMaxGroups = 8;
 struct TrTable {
   int ng;
@@ -82,9 +82,16 @@ __global__ __launch_bounds__(256) void k_translate(
   // contiguous range of targets, whose neighbouring source boxes then stay
   // in that XCD's L2.
   const int64_t lblk = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-  const int64_t wave = lblk * 4 + (threadIdx.x >> 6);
-  if (wave >= nwaves) return;
+  const int64_t wv = lblk * 4 + (threadIdx.x >> 6);
+  if (wv >= nwaves) return;
   const int lane = threadIdx.x & 63;
+  // With the k loop not unrolled (long operators), the chunks of RT row tiles
+  // go to separate wavefronts: wavefront wv works on tile wv / nrc, chunk
+  // wv % nrc (the rows are disjoint outputs).
+  const int RTT0 = (2 * Pout + 15) / 16;
+  const int nrc = KSU > 0 ? 1 : (RTT0 + RT - 1) / RT;
+  const int64_t wave = wv / nrc;
+  const int rc0 = (int)(wv - wave * nrc) * RT;
   int g = 0;
   while (g + 1 < tab.ng && wave >= tab.tile0[g + 1]) ++g;
   const int q = tab.cls[g];
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(256) void k_translate(
   const int RTT = (2 * Pout + 15) / 16, KS = (2 * Pin + 3) / 4;
   const int2 full_in = int2{0, 2 * Pin}, full_out = int2{0, 2 * Pout};
   const int ob = tab.opbase[g];
-  for (int rc = 0; rc < RTT; rc += RT) {
+  for (int rc = rc0; rc < (KSU > 0 ? RTT : rc0 + 1); rc += RT) {
     if (MODE == 0) {
       acc_t acc[RT];
 #pragma unroll
@@ -277,14 +284,16 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     }
     if (tab.ng == 0) continue;
     tab.tile0[tab.ng] = tiles;
-    const int nblk = (tiles + 3) / 4, per_xcd = (nblk + 7) / 8;
-    const dim3 grid((unsigned)(8 * per_xcd));
 #define FMM2D_TR(T, RT, KSU, MODE, LOGT)                                                        \
-  hipLaunchKernelGGL((k_translate<T, RT, KSU, MODE, LOGT>), grid, dim3(256), 0, st, nd, Pin,    \
-                     Pout, nslots, (int64_t)tiles, per_xcd, tab, tlist, slots,                  \
-                     (int)slots_by_box, slot_op,                                                \
-                     static_cast<const T*>(Afrag), afs, io.sbox0, io.dbox0, ilevel, band, logr, \
-                     static_cast<const T*>(io.src), io.dst)
+  {                                                                                             \
+    const int64_t nw = (int64_t)tiles * (KSU > 0 ? 1 : (RTT + RT - 1) / RT);                    \
+    const int64_t nblk = (nw + 3) / 4;                                                          \
+    const int per_xcd = (int)((nblk + 7) / 8);                                                  \
+    hipLaunchKernelGGL((k_translate<T, RT, KSU, MODE, LOGT>), dim3((unsigned)(8 * per_xcd)),    \
+                       dim3(256), 0, st, nd, Pin, Pout, nslots, nw, per_xcd, tab, tlist, slots, \
+                       (int)slots_by_box, slot_op, static_cast<const T*>(Afrag), afs, io.sbox0, \
+                       io.dbox0, ilevel, band, logr, static_cast<const T*>(io.src), io.dst);    \
+  }
 #define FMM2D_TR_RT(T, MODE, LOGT)                 \
   switch (KS <= 4 * RTT ? RTT : 0) {               \
     case 1: FMM2D_TR(T, 1, 4, MODE, LOGT); break;  \
