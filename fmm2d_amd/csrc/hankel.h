@@ -29,15 +29,20 @@ namespace fmm2d {
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kEuler = 0.57721566490153286061;
 
-// nterm ascending-series terms; the k-th term is bounded by (|z|^2/4)^k/(k!)^2.
+constexpr int kSeriesMax = 26;
+
+// nterm (<= kSeriesMax) ascending-series terms; the k-th term is bounded by
+// (|z|^2/4)^k/(k!)^2.  The loop is unrolled so that the factors 1/k^2,
+// 1/(k(k+1)), 1/k, 1/(k+1) are compile-time constants.
 FMM2D_HD void hank01_series(cplx z, cplx* h0, cplx* h1, int nterm) {
   cplx q = -0.25 * (z * z);
   cplx t0 = mkc(1.0), t1 = mkc(1.0);
   cplx J0 = t0, S1sum = t1, S0 = mkc(0.0), S1 = mkc(1.0);  // k = 0 terms
   double Hk = 0.0;
-#pragma unroll 1
-  for (int k = 1; k <= nterm; ++k) {
-    double dk = (double)k;
+#pragma unroll
+  for (int k = 1; k <= kSeriesMax; ++k) {
+    if (k > nterm) break;
+    const double dk = (double)k;
     t0 = t0 * q * (1.0 / (dk * dk));
     t1 = t1 * q * (1.0 / (dk * (dk + 1.0)));
     Hk += 1.0 / dk;
@@ -54,6 +59,32 @@ FMM2D_HD void hank01_series(cplx z, cplx* h0, cplx* h1, int nterm) {
   cplx Y1 = (2.0 / kPi) * (L * J1) - (2.0 / kPi) * invc(z) - (1.0 / kPi) * (zh * S1);
   *h0 = cplx{J0.re - Y0.im, J0.im + Y0.re};
   *h1 = cplx{J1.re - Y1.im, J1.im + Y1.re};
+}
+
+// The same series for a real argument x > 0 (real wavenumbers): J and Y are
+// real, H = J + i Y.
+FMM2D_HD void hank01_series_real(double x, cplx* h0, cplx* h1, int nterm) {
+  const double q = -0.25 * (x * x);
+  double t0 = 1.0, t1 = 1.0, J0 = 1.0, S1sum = 1.0, S0 = 0.0, S1 = 1.0, Hk = 0.0;
+#pragma unroll
+  for (int k = 1; k <= kSeriesMax; ++k) {
+    if (k > nterm) break;
+    const double dk = (double)k;
+    t0 = t0 * q * (1.0 / (dk * dk));
+    t1 = t1 * q * (1.0 / (dk * (dk + 1.0)));
+    Hk += 1.0 / dk;
+    J0 += t0;
+    S1sum += t1;
+    S0 += Hk * t0;
+    S1 += (2.0 * Hk + 1.0 / (dk + 1.0)) * t1;
+  }
+  const double xh = 0.5 * x;
+  const double J1 = xh * S1sum;
+  const double L = log(xh) + kEuler;
+  const double Y0 = (2.0 / kPi) * (L * J0 - S0);
+  const double Y1 = (2.0 / kPi) * (L * J1) - (2.0 / kPi) / x - (1.0 / kPi) * (xh * S1);
+  *h0 = cplx{J0, Y0};
+  *h1 = cplx{J1, Y1};
 }
 
 // sqrt(2/(pi z)) * exp(i (z - pi/4))
@@ -119,7 +150,11 @@ FMM2D_HD void hank01(cplx z, cplx* h0, cplx* h1) {
   // positive imaginary axis it hands over to the quadrature already at 1.75.
   if (r < 1.75 || (r < 2.5 && fabs(z.re) >= z.im)) {
     // term bound below 1e-17 at k = 8 (r<0.5), 10 (r<1.2), 14 (r<2.5)
-    hank01_series(z, h0, h1, r < 0.5 ? 8 : (r < 1.2 ? 10 : 14));
+    const int nterm = r < 0.5 ? 8 : (r < 1.2 ? 10 : 14);
+    if (z.im == 0.0 && z.re > 0.0)
+      hank01_series_real(z.re, h0, h1, nterm);
+    else
+      hank01_series(z, h0, h1, nterm);
   } else if (z.im >= 0.0 || atan2(z.im, z.re) >= -0.3) {
     if (r < 17.0)
       hank01_laguerre(z, h0, h1);
