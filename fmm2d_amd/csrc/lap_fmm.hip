@@ -444,12 +444,14 @@ __global__ __launch_bounds__(kP2lWaves * kW) void k_p2l_w(
             if (RE) q.im = 0.0;
           }
           cplx* row = sh + lane * P;
-          const cplx z0 = cplx{xs[2 * j] - cx, xs[2 * j + 1] - cy};
-          const cplx zi = invc(z0);
+          // z0 = source - centre of a well-separated box (no overflow guards)
+          const double zx = xs[2 * j] - cx, zy = xs[2 * j + 1] - cy;
+          const double rr = zx * zx + zy * zy, ri = 1.0 / rr;
+          const cplx zi = cplx{zx * ri, -zy * ri};
           const cplx u = r * zi;
           if (IFD) e = -(zi * dps[j * nd + d]);
           cplx t0 = e;
-          if (IFC) t0 += log(absc(z0)) * q;
+          if (IFC) t0 += (0.5 * log(rr)) * q;
           row[0] = t0;
           cplx v = mkc(1.0);  // u^m
           for (int m = 1; m <= p; ++m) {
@@ -536,13 +538,16 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
       const int lc = ilevel[c];
       const int pc = nterms[lc];
       const double rc = rsc[lc], rci = 1.0 / rc;
-      const cplx z = cplx{x - centers[2 * c], y - centers[2 * c + 1]};
-      const cplx u = rc * invc(z);  // r/z
+      // z = target - centre of a well-separated box: 1/z = conj(z) / |z|^2
+      // and log|z| = log(|z|^2) / 2 need no overflow guards
+      const double zx = x - centers[2 * c], zy = y - centers[2 * c + 1];
+      const double rr = zx * zx + zy * zy, rcr = rc / rr;
+      const cplx u = cplx{zx * rcr, -zy * rcr};  // r/z
       const cplx* M = mp + ((int64_t)c * nd + d) * P;
       const cplx m0 = M[0];
       cplx g, g1, g2;
       horner<PG>(M, pc, true, u, g, g1, g2);
-      ap += log(absc(z)) * m0 + g;
+      ap += (0.5 * log(rr)) * m0 + g;
       if (PG >= 2) {
         const cplx u2 = u * u;
         ag += (u * rci) * m0 - (u2 * rci) * g1;
