@@ -72,6 +72,76 @@ __global__ void k_leaf_of(int64_t n, const int* nchild, const int* se, int* leaf
 }
 
 
+// ---------------------------------------------------------- near-field segments
+// The sources a leaf interacts with directly (the boxes of its list 1,
+// common/tree_routs2d.f:486-710) as a few contiguous ranges of the sorted
+// sources: empty boxes dropped, ranges sorted by start and merged where one
+// ends at the next one's start (sibling leaves are adjacent in the sorted
+// order).  Row b of kSegRow int2: [0] = {sources in total, ranges}, then
+// the ranges [s0, s1), then an end mark {0, 0} (every box has a row; that of
+// a box with an empty list 1 holds no range).
+constexpr int kSegRow = 16;
+static_assert(kSegRow >= kMaxList1 + 2, "segment row holds the list-1 ranges and the end mark");
+
+__global__ __launch_bounds__(256) void k_near_segs(int64_t nb, const int* n1, const int* l1,
+                                                   const int* sse, int2* seg) {
+  __shared__ int2 buf[256][kMaxList1];
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int c1 = n1[b];
+  int2* r = buf[threadIdx.x];
+  int m = 0;
+  for (int k = 0; k < c1; ++k) {  // insertion sort by start
+    const int c = l1[(int64_t)kMaxList1 * b + k];
+    const int s0 = sse[2 * c], s1 = sse[2 * c + 1];
+    if (s1 <= s0) continue;
+    int q = m++;
+    for (; q > 0 && r[q - 1].x > s0; --q) r[q] = r[q - 1];
+    r[q] = make_int2(s0, s1);
+  }
+  int2* out = seg + (int64_t)kSegRow * b;
+  int ns = 0, tot = 0;
+  for (int k = 0; k < m;) {
+    int2 s = r[k++];
+    for (; k < m && r[k].x == s.y; ++k) s.y = r[k].y;
+    out[1 + ns++] = s;
+    tot += s.y - s.x;
+  }
+  out[1 + ns] = make_int2(0, 0);
+  out[0] = make_int2(tot, ns);
+}
+
+// true when pred holds on every active lane of the wavefront
+__device__ __forceinline__ bool wave_all(bool pred) { return __ballot(pred) == __ballot(1); }
+
+// Walks the ranges of one segment row: current range [j, e) and the next
+// one prefetched, advanced when the current one runs out.
+struct SegCursor {
+  const int2* row;
+  int j, e, k, last;
+  int2 nxt;
+  __device__ __forceinline__ int init(const int2* r) {
+    row = r;
+    const int2 hd = r[0];
+    last = hd.y + 1;  // slot of the end mark
+    const int2 cur = r[1];
+    nxt = r[min(2, last)];
+    k = 3;
+    j = cur.x;
+    e = cur.y;
+    return hd.x;
+  }
+  __device__ __forceinline__ void step() {
+    if (j == e) {
+      j = nxt.x;
+      e = nxt.y;
+      nxt = row[min(k, last)];
+      ++k;
+    }
+  }
+};
+
+
 // ---------------------------------------------------------- list-2 plan
 // The M2L of one target box is sum over its list-2 sources of T_o M_c, with o
 // the source's offset.  For a target in child position q of its parent the

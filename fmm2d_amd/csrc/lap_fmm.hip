@@ -578,7 +578,7 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
 // F32 = 1 (mixed precision): the displacement is formed in FP64 and rounded
 // to FP32; log|z|, 1/z, -1/z^2, 2/z^3 and their products with FP32 copies of
 // the strengths are FP32 (hardware reciprocal and log2), summed in FP32 over
-// one list-1 box and added to FP64 accumulators after it.  The FP32 form
+// at most kFoldT sources and added to FP64 accumulators after them.  The FP32 form
 // takes the pairs with lo < |z|^2 < 1e20 (FP32), lo = max(1e-20, 4 thresh^2)
 // rounded up: no such pair can be excluded, and the powers of 1/z stay in
 // the normal FP32 range.  Every other pair (coincident points among them)
@@ -654,7 +654,7 @@ __device__ __forceinline__ void load_strf(int64_t j, int nd, int d0, const float
 #pragma unroll
   for (int dd = 0; dd < NDT; ++dd) {
     const bool ok = dd == 0 || d0 + dd < nd;  // d0 < nd always
-    const int64_t e = j * nd + d0 + dd;
+    const int64_t e = NDT == 1 ? j : j * nd + d0 + dd;  // NDT = 1 only with nd = 1
     s.cr[dd] = s.ci[dd] = s.dr[dd] = s.di[dd] = 0.0f;
     if (IFC && ok) {
       if (RE) {
@@ -716,8 +716,9 @@ __device__ __forceinline__ void load_str(int64_t j, int nd, int d0, const cplx* 
 #pragma unroll
   for (int dd = 0; dd < NDT; ++dd) {
     const bool ok = dd == 0 || d0 + dd < nd;  // d0 < nd always
-    if (IFC) s.charge[dd] = ok ? chs[j * nd + d0 + dd] : mkc(0.0);
-    if (IFD) s.dipstr[dd] = ok ? dps[j * nd + d0 + dd] : mkc(0.0);
+    const int64_t e = NDT == 1 ? j : j * nd + d0 + dd;  // NDT = 1 only with nd = 1
+    if (IFC) s.charge[dd] = ok ? chs[e] : mkc(0.0);
+    if (IFD) s.dipstr[dd] = ok ? dps[e] : mkc(0.0);
   }
 }
 
@@ -743,73 +744,90 @@ __device__ __forceinline__ void near_pair64(double dx, double dy, SrcStr<NDT>& s
   pair_c64<IFC, IFD, PG, NDT, RE>(dx, dy, rr, s, a);
 }
 
-// The FP64 exclusion rule and form for one pair, out of line: it serves only
-// coincident or out-of-range pairs, and keeping its registers out of the
-// FP32 loop keeps that loop's occupancy.
+// The FP64 exclusion rule and form for one pair, added to the outputs of
+// point i (k_near_slow: coincident or out-of-range pairs of the FP32 path).
 template <int IFC, int IFD, int PG, int NDT, int RE>
-__device__ __noinline__ Acc<CAUCHY, PG, NDT> slow_pair(double dx, double dy, int64_t j, int nd,
-                                                        int d0, const cplx* chs,
-                                                        const cplx* dps, double thresh,
-                                                        double thresh2) {
-  Acc<CAUCHY, PG, NDT> r;
-  r.zero();
+__device__ __forceinline__ void slow_pair(double dx, double dy, int64_t j, int64_t i, int nd,
+                                       int d0, const cplx* chs, const cplx* dps, double thresh,
+                                       double thresh2, cplx* pot, cplx* grad, cplx* hess) {
   const double rr = dx * dx + dy * dy;
   double absz = 2.0 * thresh;
   if (!IFC && rr <= 4.0 * thresh2) absz = hypot(dx, dy);
-  if (!skip_pair<CAUCHY, IFC, IFD, PG>(rr, thresh, thresh2, absz)) {
-    SrcStr<NDT> s;
-    load_str<IFC, IFD, NDT>(j, nd, d0, chs, dps, s);
-    pair_c64<IFC, IFD, PG, NDT, RE>(dx, dy, rr, s, r);
+  if (skip_pair<CAUCHY, IFC, IFD, PG>(rr, thresh, thresh2, absz)) return;
+  SrcStr<NDT> s;
+  load_str<IFC, IFD, NDT>(j, nd, d0, chs, dps, s);
+  Acc<CAUCHY, PG, NDT> r;
+  r.zero();
+  pair_c64<IFC, IFD, PG, NDT, RE>(dx, dy, rr, s, r);
+#pragma unroll
+  for (int dd = 0; dd < NDT; ++dd) {
+    if (d0 + dd >= nd) break;
+    const int64_t o = i * nd + d0 + dd;
+    if (RE) pot[o].re += r.pot[dd].re;
+    else pot[o] += r.pot[dd];
+    if (PG >= 2) grad[o] += r.grad[dd][0];
+    if (PG >= 3) hess[o] += r.hess[dd][0];
   }
-  return r;
 }
 
-// one pair of the mixed-precision path
-template <int IFC, int IFD, int PG, int NDT, int RE>
-__device__ __forceinline__ void near_pair32(double dx, double dy, const StrF<NDT>& sf, int64_t j,
-                                            int nd, int d0, const cplx* chs, const cplx* dps,
-                                            double thresh, double thresh2, float lo,
-                                            Acc<CAUCHY, PG, NDT>& a, AccF<PG, NDT>& f) {
-  const float fx = (float)dx, fy = (float)dy;
-  const float frr = fx * fx + fy * fy;
-  const bool fast = frr > lo && frr < 1e20f;
-  pair_f32<IFC, IFD, PG, NDT, RE>(fx, fy, frr, fast, sf, f);
-  if (!fast) {  // coincident points (excluded) or |z| out of the FP32 range
-    const Acc<CAUCHY, PG, NDT> r =
-        slow_pair<IFC, IFD, PG, NDT, RE>(dx, dy, j, nd, d0, chs, dps, thresh, thresh2);
-#pragma unroll
-    for (int d = 0; d < NDT; ++d) {
-      a.pot[d] += r.pot[d];
-      if (PG >= 2) a.grad[d][0] += r.grad[d][0];
-      if (PG >= 3) a.hess[d][0] += r.hess[d][0];
-    }
-  }
+// |z|^2 of the FP32 form, with the contraction pinned so that k_near and
+// k_near_slow classify every pair alike
+__device__ __forceinline__ float near_frr(float fx, float fy) {
+  return __builtin_fmaf(fx, fx, fy * fy);
 }
+
+// one pair of the mixed-precision path; ok = 0: the point's own pair,
+// dropped.  Returns true for a pair the FP32 form does not take (coincident
+// points or |z| out of the FP32 range), left to k_near_slow.
+template <int IFC, int IFD, int PG, int NDT, int RE>
+__device__ __forceinline__ bool near_pair32(double dx, double dy, bool ok, const StrF<NDT>& sf,
+                                            float lo, AccF<PG, NDT>& f) {
+  const float fx = (float)dx, fy = (float)dy;
+  const float frr = near_frr(fx, fy);
+  const bool in = frr > lo && frr < 1e20f;
+  pair_f32<IFC, IFD, PG, NDT, RE>(fx, fy, frr, in && ok, sf, f);
+  return !in && ok;
+}
+
+// Thread per evaluation point.  The sources of the point's leaf come from its
+// segment row (k_near_segs) through one loop over all of them: kNearU
+// consecutive sources per step while every active lane has that many left in
+// its current range, one source per step otherwise, so lanes of different
+// leaves in one wavefront stay busy until their own totals run out.  The step
+// counter t advances alike on every active lane; FP32 sums are folded into
+// the FP64 accumulators every kFoldT sources.  self: the points are the
+// sources themselves (point i is source i), whose own pair the FP32 path drops
+// by index.  A point with a pair the FP32 form does not take is appended to
+// slow (at most once, so n entries suffice) for k_near_slow.
+constexpr int kFoldT = 32;
 
 template <int IFC, int IFD, int PG, int NDT, int F32, int RE>
 __global__ __launch_bounds__(256) void k_near(int64_t n, int nd, int d0, const double* xy,
-                                              const int* leaf, const int* n1, const int* l1,
-                                              const int* sse, const double* xs, const cplx* chs,
+                                              const int* leaf, const int2* seg, int self,
+                                              const double* xs, const cplx* chs,
                                               const cplx* dps, const float* ch32,
                                               const float* dp32, double thresh, float lo,
-                                              cplx* pot, cplx* grad, cplx* hess) {
+                                              int* slow, int* nslow, cplx* pot, cplx* grad,
+                                              cplx* hess) {
   const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = leaf[i];
   const double x = xy[2 * i], y = xy[2 * i + 1];
   const double thresh2 = thresh * thresh;
+  const int iself = self ? (int)i : -1;
+  bool any_slow = false;
   Acc<CAUCHY, PG, NDT> a;
   a.zero();
-  const int c1 = n1[b];
-  for (int kk = 0; kk < c1; ++kk) {
-    const int c = l1[(int64_t)kMaxList1 * b + kk];
-    const int s0 = sse[2 * c], s1 = sse[2 * c + 1];
-    int j = s0;
-    const double* px = xs + 2 * (int64_t)s0;  // source j at px (immediate offsets)
-    if (F32) {
-      AccF<PG, NDT> f;
-      f.zero();
-      for (; j + kNearU <= s1; j += kNearU, px += 2 * kNearU) {
+  AccF<PG, NDT> f;
+  f.zero();
+  SegCursor sc;
+  const int T = sc.init(seg + (int64_t)kSegRow * b);
+  int tf = kFoldT;
+  for (int t = 0; t < T;) {
+    const int j = sc.j;
+    const double* px = xs + 2 * (int64_t)j;
+    if (wave_all(sc.e - j >= kNearU)) {
+      if (F32) {
         double sx[kNearU], sy[kNearU];
         StrF<NDT> sf[kNearU];
 #pragma unroll
@@ -818,20 +836,12 @@ __global__ __launch_bounds__(256) void k_near(int64_t n, int nd, int d0, const d
           sy[u] = px[2 * u + 1];
           load_strf<IFC, IFD, NDT, RE>(j + u, nd, d0, ch32, dp32, sf[u]);
         }
+        const int ds = iself - j;
 #pragma unroll
         for (int u = 0; u < kNearU; ++u)
-          near_pair32<IFC, IFD, PG, NDT, RE>(x - sx[u], y - sy[u], sf[u], j + u, nd, d0, chs,
-                                             dps, thresh, thresh2, lo, a, f);
-      }
-      for (; j < s1; ++j, px += 2) {
-        StrF<NDT> sf;
-        load_strf<IFC, IFD, NDT, RE>(j, nd, d0, ch32, dp32, sf);
-        near_pair32<IFC, IFD, PG, NDT, RE>(x - px[0], y - px[1], sf, j, nd, d0, chs, dps, thresh,
-                                           thresh2, lo, a, f);
-      }
-      f.fold(a);
-    } else {
-      for (; j + kNearU <= s1; j += kNearU, px += 2 * kNearU) {
+          any_slow |= near_pair32<IFC, IFD, PG, NDT, RE>(x - sx[u], y - sy[u], ds != u, sf[u],
+                                                         lo, f);
+      } else {
         double sx[kNearU], sy[kNearU];
         SrcStr<NDT> s[kNearU];
 #pragma unroll
@@ -844,12 +854,32 @@ __global__ __launch_bounds__(256) void k_near(int64_t n, int nd, int d0, const d
         for (int u = 0; u < kNearU; ++u)
           near_pair64<IFC, IFD, PG, NDT, RE>(x - sx[u], y - sy[u], s[u], thresh, thresh2, a);
       }
-      for (; j < s1; ++j, px += 2) {
+      sc.j = j + kNearU;
+      t += kNearU;
+    } else {
+      if (F32) {
+        StrF<NDT> sf;
+        load_strf<IFC, IFD, NDT, RE>(j, nd, d0, ch32, dp32, sf);
+        any_slow |= near_pair32<IFC, IFD, PG, NDT, RE>(x - px[0], y - px[1], j != iself, sf, lo,
+                                                       f);
+      } else {
         SrcStr<NDT> s;
         load_str<IFC, IFD, NDT>(j, nd, d0, chs, dps, s);
         near_pair64<IFC, IFD, PG, NDT, RE>(x - px[0], y - px[1], s, thresh, thresh2, a);
       }
+      sc.j = j + 1;
+      t += 1;
     }
+    sc.step();
+    if (F32 && t >= tf) {
+      f.fold(a);
+      f.zero();
+      tf = t + kFoldT;
+    }
+  }
+  if (F32) {
+    f.fold(a);
+    if (any_slow) slow[atomicAdd(nslow, 1)] = (int)i;
   }
 #pragma unroll
   for (int dd = 0; dd < NDT; ++dd) {
@@ -859,6 +889,36 @@ __global__ __launch_bounds__(256) void k_near(int64_t n, int nd, int d0, const d
     else pot[o] += a.pot[dd];
     if (PG >= 2) grad[o] += a.grad[dd][0];
     if (PG >= 3) hess[o] += a.hess[dd][0];
+  }
+}
+
+// The pairs of the points listed by k_near that its FP32 form left out,
+// through the exact FP64 exclusion rule and the FP64 form, added to those
+// points' outputs (grid-stride over the list).
+template <int IFC, int IFD, int PG, int NDT, int RE>
+__global__ __launch_bounds__(256) void k_near_slow(int nd, int d0, const double* xy,
+                                                   const int* leaf, const int2* seg, int self,
+                                                   const double* xs, const cplx* chs,
+                                                   const cplx* dps, double thresh, float lo,
+                                                   const int* slow, const int* nslow, cplx* pot,
+                                                   cplx* grad, cplx* hess) {
+  const int cnt = *nslow;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+    const int i = slow[q];
+    const int iself = self ? i : -1;
+    const double x = xy[2 * (int64_t)i], y = xy[2 * (int64_t)i + 1];
+    SegCursor sc;
+    const int T = sc.init(seg + (int64_t)kSegRow * leaf[i]);
+    for (int t = 0; t < T; ++t) {
+      const int j = sc.j;
+      const double dx = x - xs[2 * (int64_t)j], dy = y - xs[2 * (int64_t)j + 1];
+      const float frr = near_frr((float)dx, (float)dy);
+      if (!(frr > lo && frr < 1e20f) && j != iself)
+        slow_pair<IFC, IFD, PG, NDT, RE>(dx, dy, j, i, nd, d0, chs, dps, thresh, thresh * thresh,
+                                         pot, grad, hess);
+      sc.j = j + 1;
+      sc.step();
+    }
   }
 }
 
@@ -956,8 +1016,10 @@ struct NearArgs {
   int nd;
   const double* xy;  // sorted evaluation points
   const int* leaf;   // leaf box of each
-  const Lists* L;
-  const int* sse;
+  const int2* seg;   // near-field segment rows (k_near_segs)
+  int self;          // the points are the sorted sources
+  int* slow;         // FP32 path: points with pairs left to k_near_slow (n entries)
+  int* nslow;        // and their count
   const double* xs;
   const cplx* chs;
   const cplx* dps;
@@ -967,13 +1029,25 @@ struct NearArgs {
   cplx *p, *g, *h;
 };
 
+constexpr int kNearSlowBlocks = 256;
+
 template <int PG, int NDT, int F32, int RE>
 void launch_near_pg(int ifc, int ifd, int64_t n, hipStream_t st, int d0, const NearArgs& q) {
   // FP32 pairs need |z|^2 > max(1e-20, 4 thresh^2): no exclusion, normal range
   const float lo = (float)(std::max(1e-20, 4.0 * q.thresh * q.thresh) * 1.001);
 #define FMM2D_NEAR_L(IFC, IFD)                                                                   \
-  launch1d_xcd(k_near<IFC, IFD, PG, NDT, F32, RE>, n, st, q.nd, d0, q.xy, q.leaf, q.L->n1.p,    \
-               q.L->l1.p, q.sse, q.xs, q.chs, q.dps, q.ch32, q.dp32, q.thresh, lo, q.p, q.g, q.h)
+  do {                                                                                           \
+    if (F32) hip_throw(hipMemsetAsync(q.nslow, 0, sizeof(int), st), "hipMemsetAsync");          \
+    launch1d_xcd(k_near<IFC, IFD, PG, NDT, F32, RE>, n, st, q.nd, d0, q.xy, q.leaf, q.seg,     \
+                 q.self, q.xs, q.chs, q.dps, q.ch32, q.dp32, q.thresh, lo, q.slow, q.nslow,   \
+                 q.p, q.g, q.h);                                                               \
+    if (F32) {                                                                                   \
+      hipLaunchKernelGGL((k_near_slow<IFC, IFD, PG, NDT, RE>), dim3(kNearSlowBlocks), dim3(256), \
+                         0, st, q.nd, d0, q.xy, q.leaf, q.seg, q.self, q.xs, q.chs, q.dps,       \
+                         q.thresh, lo, q.slow, q.nslow, q.p, q.g, q.h);                        \
+      hip_throw(hipGetLastError(), "kernel launch");                                             \
+    }                                                                                            \
+  } while (0)
   if (ifc && !ifd) FMM2D_NEAR_L(1, 0);
   else if (!ifc && ifd) FMM2D_NEAR_L(0, 1);
   else FMM2D_NEAR_L(1, 1);
@@ -1300,15 +1374,22 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
         dp32 = DBuf<float>(2 * ns * nd, st);
         launch1d(k_to_f32, 2 * ns * nd, st, reinterpret_cast<const double*>(dps.p), dp32.p);
       }
-      tm.mark(15);  // FP32 strength copies; [7] then holds the near-field kernels alone
+      DBuf<int2> seg((size_t)kSegRow * t.nboxes, st);
+      launch1d(k_near_segs, t.nboxes, st, L.n1.p, L.l1.p, sse, seg.p);
+      DBuf<int> slow, nslow;
+      if (mixed) {
+        slow = DBuf<int>((size_t)std::max(ns, nt), st);
+        nslow = DBuf<int>(1, st);
+      }
+      tm.mark(15);  // FP32 strength copies, segment rows; [7] the near-field kernels alone
       if (pg > 0)
         launch_near(mixed, re, pg, a.ifcharge, a.ifdipole, ns, st,
-                    NearArgs{nd, xs.p, leaf_s.p, &L, sse, xs.p, chs.p, dps.p, ch32.p, dp32.p,
-                             thresh, ps.p, gs.p, hs.p});
+                    NearArgs{nd, xs.p, leaf_s.p, seg.p, 1, slow.p, nslow.p, xs.p, chs.p, dps.p,
+                             ch32.p, dp32.p, thresh, ps.p, gs.p, hs.p});
       if (pgt > 0)
         launch_near(mixed, re, pgt, a.ifcharge, a.ifdipole, nt, st,
-                    NearArgs{nd, xt.p, leaf_t.p, &L, sse, xs.p, chs.p, dps.p, ch32.p, dp32.p,
-                             thresh, pt.p, gt.p, ht.p});
+                    NearArgs{nd, xt.p, leaf_t.p, seg.p, 0, slow.p, nslow.p, xs.p, chs.p, dps.p,
+                             ch32.p, dp32.p, thresh, pt.p, gt.p, ht.p});
     }
     tm.mark(7);
     // ---- unsort
