@@ -457,36 +457,40 @@ __global__ void k_heval(int64_t n, int nd, cplx zk, const HLev* lev, const doubl
 }
 
 // ---------------------------------------------------------- near field
-// hfmm2dpart_direct (hfmm2d.f:1560-1600 -> helmkernels2d.f) over list 1.
+// hfmm2dpart_direct (hfmm2d.f:1560-1600 -> helmkernels2d.f) over list 1,
+// one loop per point over the merged source ranges of its leaf's segment
+// row (k_near_segs), so lanes of different leaves in one wavefront stay busy
+// until their own totals run out.
 template <int IFC, int IFD, int PG, int NDT>
 __global__ void k_hnear(int64_t n, int nd, int d0, cplx zk, const double* xy, const int* leaf,
-                        const int* n1, const int* l1, const int* sse, const double* xs,
-                        const cplx* chs, const cplx* dps, const double* dvs, double thresh,
-                        cplx* pot, cplx* grad, cplx* hess) {
+                        const int2* seg, const double* xs, const cplx* chs, const cplx* dps,
+                        const double* dvs, double thresh, cplx* pot, cplx* grad, cplx* hess) {
   const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = leaf[i];
   const double x = xy[2 * i], y = xy[2 * i + 1];
   Acc<HELM, PG, NDT> a;
   a.zero();
-  const int c1 = n1[b];
-  for (int k = 0; k < c1; ++k) {
-    const int c = l1[(int64_t)kMaxList1 * b + k];
-    for (int j = sse[2 * c]; j < sse[2 * c + 1]; ++j) {
-      SrcStr<NDT> s;
+  SegCursor sc;
+  const int T = sc.init(seg + (int64_t)kSegRow * b);
+  for (int t = 0; t < T; ++t) {
+    const int j = sc.j;
+    SrcStr<NDT> s;
 #pragma unroll
-      for (int dd = 0; dd < NDT; ++dd) {
-        const bool ok = d0 + dd < nd;
-        const int dc = ok ? d0 + dd : d0;
-        if (IFC) s.charge[dd] = ok ? chs[(int64_t)j * nd + dc] : mkc(0.0);
-        if (IFD) {
-          s.dipstr[dd] = ok ? dps[(int64_t)j * nd + dc] : mkc(0.0);
-          s.dv[dd][0] = dvs[(2 * (int64_t)j) * nd + dc];
-          s.dv[dd][1] = dvs[(2 * (int64_t)j + 1) * nd + dc];
-        }
+    for (int dd = 0; dd < NDT; ++dd) {
+      const bool ok = d0 + dd < nd;
+      const int dc = ok ? d0 + dd : d0;
+      if (IFC) s.charge[dd] = ok ? chs[(int64_t)j * nd + dc] : mkc(0.0);
+      if (IFD) {
+        s.dipstr[dd] = ok ? dps[(int64_t)j * nd + dc] : mkc(0.0);
+        s.dv[dd][0] = dvs[(2 * (int64_t)j) * nd + dc];
+        s.dv[dd][1] = dvs[(2 * (int64_t)j + 1) * nd + dc];
       }
-      pair_helm<IFC, IFD, PG, NDT>(x - xs[2 * j], y - xs[2 * j + 1], zk, s, thresh, a);
     }
+    pair_helm<IFC, IFD, PG, NDT>(x - xs[2 * (int64_t)j], y - xs[2 * (int64_t)j + 1], zk, s,
+                                 thresh, a);
+    sc.j = j + 1;
+    sc.step();
   }
 #pragma unroll
   for (int dd = 0; dd < NDT; ++dd) {
@@ -617,47 +621,47 @@ void launch_heval(int64_t n, hipStream_t st, int nd, cplx zk, const HLev* lev, c
 
 template <int IFC, int IFD, int NDT>
 void launch_hnear_f(int pg, int64_t n, hipStream_t st, int nd, int d0, cplx zk, const double* xy,
-                    const int* leaf, const Lists& L, const int* sse, const double* xs,
+                    const int* leaf, const int2* seg, const double* xs,
                     const cplx* chs, const cplx* dps, const double* dvs, double thresh, cplx* p,
                     cplx* g, cplx* h) {
   if (pg == 1)
-    launch1d_xcd(k_hnear<IFC, IFD, 1, NDT>, n, st, nd, d0, zk, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs,
-             dps, dvs, thresh, p, g, h);
+    launch1d_xcd(k_hnear<IFC, IFD, 1, NDT>, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps,
+             dvs, thresh, p, g, h);
   else if (pg == 2)
-    launch1d_xcd(k_hnear<IFC, IFD, 2, NDT>, n, st, nd, d0, zk, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs,
-             dps, dvs, thresh, p, g, h);
+    launch1d_xcd(k_hnear<IFC, IFD, 2, NDT>, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps,
+             dvs, thresh, p, g, h);
   else
-    launch1d_xcd(k_hnear<IFC, IFD, 3, NDT>, n, st, nd, d0, zk, xy, leaf, L.n1.p, L.l1.p, sse, xs, chs,
-             dps, dvs, thresh, p, g, h);
+    launch1d_xcd(k_hnear<IFC, IFD, 3, NDT>, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps,
+             dvs, thresh, p, g, h);
 }
 
 template <int NDT>
 void launch_hnear_n(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd, int d0, cplx zk,
-                    const double* xy, const int* leaf, const Lists& L, const int* sse,
+                    const double* xy, const int* leaf, const int2* seg,
                     const double* xs, const cplx* chs, const cplx* dps, const double* dvs,
                     double thresh, cplx* p, cplx* g, cplx* h) {
   if (ifc && !ifd)
-    launch_hnear_f<1, 0, NDT>(pg, n, st, nd, d0, zk, xy, leaf, L, sse, xs, chs, dps, dvs, thresh,
+    launch_hnear_f<1, 0, NDT>(pg, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs, thresh,
                               p, g, h);
   else if (!ifc && ifd)
-    launch_hnear_f<0, 1, NDT>(pg, n, st, nd, d0, zk, xy, leaf, L, sse, xs, chs, dps, dvs, thresh,
+    launch_hnear_f<0, 1, NDT>(pg, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs, thresh,
                               p, g, h);
   else
-    launch_hnear_f<1, 1, NDT>(pg, n, st, nd, d0, zk, xy, leaf, L, sse, xs, chs, dps, dvs, thresh,
+    launch_hnear_f<1, 1, NDT>(pg, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs, thresh,
                               p, g, h);
 }
 
 void launch_hnear(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd, cplx zk,
-                  const double* xy, const int* leaf, const Lists& L, const int* sse,
+                  const double* xy, const int* leaf, const int2* seg,
                   const double* xs, const cplx* chs, const cplx* dps, const double* dvs,
                   double thresh, cplx* p, cplx* g, cplx* h) {
   const int NDT = (nd == 1) ? 1 : 2;
   for (int d0 = 0; d0 < nd; d0 += NDT) {
     if (NDT == 1)
-      launch_hnear_n<1>(pg, ifc, ifd, n, st, nd, d0, zk, xy, leaf, L, sse, xs, chs, dps, dvs,
+      launch_hnear_n<1>(pg, ifc, ifd, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs,
                         thresh, p, g, h);
     else
-      launch_hnear_n<2>(pg, ifc, ifd, n, st, nd, d0, zk, xy, leaf, L, sse, xs, chs, dps, dvs,
+      launch_hnear_n<2>(pg, ifc, ifd, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs,
                         thresh, p, g, h);
   }
 }
@@ -936,11 +940,13 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     const int ifnear = o ? o->ifnear : 1;
     if (ifnear) {
       const double thresh = std::hypot(zk.re, zk.im) * t.boxsize[0] * std::ldexp(1.0, -51);
+      DBuf<int2> seg((size_t)kSegRow * t.nboxes, st);
+      launch1d(k_near_segs, t.nboxes, st, L.n1.p, L.l1.p, sse, seg.p);
       if (pg > 0)
-        launch_hnear(pg, a.ifcharge, a.ifdipole, ns, st, nd, zk, xs.p, leaf_s.p, L, sse, xs.p,
+        launch_hnear(pg, a.ifcharge, a.ifdipole, ns, st, nd, zk, xs.p, leaf_s.p, seg.p, xs.p,
                      chs.p, dps.p, dvs.p, thresh, ps.p, gs.p, hs.p);
       if (pgt > 0)
-        launch_hnear(pgt, a.ifcharge, a.ifdipole, nt, st, nd, zk, xt.p, leaf_t.p, L, sse, xs.p,
+        launch_hnear(pgt, a.ifcharge, a.ifdipole, nt, st, nd, zk, xt.p, leaf_t.p, seg.p, xs.p,
                      chs.p, dps.p, dvs.p, thresh, pt.p, gt.p, ht.p);
     }
     tm.mark(7);
