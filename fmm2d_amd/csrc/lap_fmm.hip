@@ -508,6 +508,55 @@ __device__ __forceinline__ void horner(const cplx* a, int kmax, bool zero0, cplx
   }
 }
 
+// multipole of box c at (x, y), added to ap, ag, ah: M_0 log|z| + g(u),
+// g(u) = sum_{n>=1} M_n u^n, u = r/z; d/dz = M_0/z - g'(u) u^2/r,
+// d2/dz2 = -M_0/z^2 + (g''(u) u^4 + 2 g'(u) u^3)/r^2
+template <int PG>
+__device__ __forceinline__ void mp_eval(double x, double y, int c, int nd, int d, int P,
+                                        const int* ilevel, const int* nterms,
+                                        const double* rsc, const double* centers,
+                                        const cplx* mp, cplx& ap, cplx& ag, cplx& ah) {
+  const int lc = ilevel[c];
+  const int pc = nterms[lc];
+  const double rc = rsc[lc], rci = 1.0 / rc;
+  // z = target - centre of a well-separated box: 1/z = conj(z) / |z|^2
+  // and log|z| = log(|z|^2) / 2 need no overflow guards
+  const double zx = x - centers[2 * c], zy = y - centers[2 * c + 1];
+  const double rr = zx * zx + zy * zy, rcr = rc / rr;
+  const cplx u = cplx{zx * rcr, -zy * rcr};  // r/z
+  const cplx* M = mp + ((int64_t)c * nd + d) * P;
+  const cplx m0 = M[0];
+  cplx g, g1, g2;
+  horner<PG>(M, pc, true, u, g, g1, g2);
+  ap += (0.5 * log(rr)) * m0 + g;
+  if (PG >= 2) {
+    const cplx u2 = u * u;
+    ag += (u * rci) * m0 - (u2 * rci) * g1;
+    if (PG >= 3) {
+      const cplx u3 = u2 * u;
+      ah += (rci * rci) * (-(u2 * m0) + 2.0 * ((u3 * u) * g2 + u3 * g1));
+    }
+  }
+}
+
+// local expansion of leaf b at (x, y): f(w) = sum L_n w^n, w = (z - c)/r;
+// pot f, d/dz = f'(w)/r, d2/dz2 = f''(w)/r^2
+template <int PG>
+__device__ __forceinline__ void loc_eval(double x, double y, int b, int nd, int d, int P,
+                                         const int* ilevel, const int* nterms,
+                                         const double* rsc, const double* centers,
+                                         const cplx* loc, cplx& ap, cplx& ag, cplx& ah) {
+  const int lev = ilevel[b];
+  const double r = rsc[lev], rinv = 1.0 / r;
+  const cplx w = cplx{(x - centers[2 * b]) * rinv, (y - centers[2 * b + 1]) * rinv};
+  cplx f1, f2;
+  horner<PG>(loc + ((int64_t)b * nd + d) * P, nterms[lev], false, w, ap, f1, f2);
+  ag = ah = mkc(0.0);
+  if (PG >= 2) ag = f1 * rinv;
+  if (PG >= 3) ah = (2.0 * rinv * rinv) * f2;
+}
+
+// Thread per point (any nd): local expansion, then the point's list-3 boxes.
 template <int PG>
 __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* leaf,
                        const int* ilevel, const int* nterms, const double* rsc,
@@ -517,58 +566,93 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
   if (i >= n) return;
   const int b = leaf[i];
   const double x = xy[2 * i], y = xy[2 * i + 1];
-  const int lev = ilevel[b];
-  const int p = nterms[lev];
-  const double r = rsc[lev], rinv = 1.0 / r;
-  const cplx w = cplx{(x - centers[2 * b]) * rinv, (y - centers[2 * b + 1]) * rinv};
   const int c3 = n3[b];
   for (int d = 0; d < nd; ++d) {
-    cplx ap, ag = mkc(0.0), ah = mkc(0.0);
-    // local expansion f(w) = sum L_n w^n: pot f, d/dz = f'(w)/r, d2/dz2 = f''(w)/r^2
-    {
-      cplx f1, f2;
-      horner<PG>(loc + ((int64_t)b * nd + d) * P, p, false, w, ap, f1, f2);
-      if (PG >= 2) ag = f1 * rinv;
-      if (PG >= 3) ah = (2.0 * rinv * rinv) * f2;
-    }
-    // multipoles of the list-3 boxes: M_0 log|z| + g(u), g(u) = sum_{n>=1} M_n u^n,
-    // u = r/z; d/dz = M_0/z - g'(u) u^2/r, d2/dz2 = -M_0/z^2 + (g''(u) u^4 + 2 g'(u) u^3)/r^2
-    for (int k3 = 0; k3 < c3; ++k3) {
-      const int c = l3[(int64_t)kMaxList3 * b + k3];
-      const int lc = ilevel[c];
-      const int pc = nterms[lc];
-      const double rc = rsc[lc], rci = 1.0 / rc;
-      // z = target - centre of a well-separated box: 1/z = conj(z) / |z|^2
-      // and log|z| = log(|z|^2) / 2 need no overflow guards
-      const double zx = x - centers[2 * c], zy = y - centers[2 * c + 1];
-      const double rr = zx * zx + zy * zy, rcr = rc / rr;
-      const cplx u = cplx{zx * rcr, -zy * rcr};  // r/z
-      const cplx* M = mp + ((int64_t)c * nd + d) * P;
-      const cplx m0 = M[0];
-      cplx g, g1, g2;
-      horner<PG>(M, pc, true, u, g, g1, g2);
-      ap += (0.5 * log(rr)) * m0 + g;
-      if (PG >= 2) {
-        const cplx u2 = u * u;
-        ag += (u * rci) * m0 - (u2 * rci) * g1;
-        if (PG >= 3) {
-          const cplx u3 = u2 * u;
-          ah += (rci * rci) * (-(u2 * m0) + 2.0 * ((u3 * u) * g2 + u3 * g1));
-        }
-      }
-    }
+    cplx ap, ag, ah;
+    loc_eval<PG>(x, y, b, nd, d, P, ilevel, nterms, rsc, centers, loc, ap, ag, ah);
+    for (int k3 = 0; k3 < c3; ++k3)
+      mp_eval<PG>(x, y, l3[(int64_t)kMaxList3 * b + k3], nd, d, P, ilevel, nterms, rsc, centers,
+                  mp, ap, ag, ah);
     pot[i * nd + d] += ap;
     if (PG >= 2) grad[i * nd + d] += ag;
     if (PG >= 3) hess[i * nd + d] += ah;
   }
 }
 
+// nd = 1: a block of kEvB consecutive points shares out their (point, list-3
+// box) pairs evenly -- pair k to thread k mod kEvB, its point found by binary
+// search in the block's scan of list-3 counts -- and sums each point's
+// multipole terms in LDS (FP64 atomic adds); leaves with different list-3
+// lengths in one wavefront then leave no lane idle.  Each thread then adds
+// its own point's local expansion.
+constexpr int kEvB = 256;
+
+template <int PG>
+__global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double* xy,
+                                                 const int* leaf, const int* ilevel,
+                                                 const int* nterms, const double* rsc,
+                                                 const double* centers, const int* n3,
+                                                 const int* l3, const cplx* mp,
+                                                 const cplx* loc, cplx* pot, cplx* grad,
+                                                 cplx* hess) {
+  using Scan = hipcub::BlockScan<int, kEvB>;
+  __shared__ typename Scan::TempStorage scan_tmp;
+  __shared__ int off[kEvB];
+  __shared__ int sb[kEvB];
+  __shared__ double sx[kEvB], sy[kEvB];
+  __shared__ double acc[2 * PG][kEvB];  // re, im of pot, grad, hess sums
+  const int tid = threadIdx.x;
+  const int64_t i = xcd_block() * kEvB + tid;
+  const bool live = i < n;
+  const int b = live ? leaf[i] : 0;
+  const double x = live ? xy[2 * i] : 0.0, y = live ? xy[2 * i + 1] : 0.0;
+  const int cnt = live ? n3[b] : 0;
+  sb[tid] = b;
+  sx[tid] = x;
+  sy[tid] = y;
+#pragma unroll
+  for (int a = 0; a < 2 * PG; ++a) acc[a][tid] = 0.0;
+  int o, m;
+  Scan(scan_tmp).ExclusiveSum(cnt, o, m);
+  off[tid] = o;
+  __syncthreads();
+  for (int k = tid; k < m; k += kEvB) {
+    int lo = 0, hi = kEvB - 1;  // last slot t with off[t] <= k
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    const int bt = sb[lo];
+    cplx ap = mkc(0.0), ag = mkc(0.0), ah = mkc(0.0);
+    mp_eval<PG>(sx[lo], sy[lo], l3[(int64_t)kMaxList3 * bt + (k - off[lo])], 1, 0, P, ilevel,
+                nterms, rsc, centers, mp, ap, ag, ah);
+    atomicAdd(&acc[0][lo], ap.re);
+    atomicAdd(&acc[1][lo], ap.im);
+    if (PG >= 2) {
+      atomicAdd(&acc[2][lo], ag.re);
+      atomicAdd(&acc[3][lo], ag.im);
+    }
+    if (PG >= 3) {
+      atomicAdd(&acc[4][lo], ah.re);
+      atomicAdd(&acc[5][lo], ah.im);
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  cplx ap, ag, ah;
+  loc_eval<PG>(x, y, b, 1, 0, P, ilevel, nterms, rsc, centers, loc, ap, ag, ah);
+  pot[i] += ap + cplx{acc[0][tid], acc[1][tid]};
+  if (PG >= 2) grad[i] += ag + cplx{acc[2][tid], acc[3][tid]};
+  if (PG >= 3) hess[i] += ah + cplx{acc[4][tid], acc[5][tid]};
+}
+
 // ---------------------------------------------------------- near field
 // cfmm2dpart_direct (laplace/cfmm2d.f:1450-1616): the list-1 sources of the
 // point's leaf through the c2d_direct* variant of the flags, one thread per
-// evaluation point.  The sources of a list-1 box are taken U = 4 at a time:
-// their loads are issued together and the exclusion rule masks a pair's
-// strengths instead of branching; the last (s1 - s0) mod U go one by one.
+// evaluation point (loop structure at k_near).  Where U = 4 sources are taken
+// together their loads are issued together, and the exclusion rule masks a
+// pair's strengths instead of branching.
 //
 // RE = 1: the charges are real and only Re(pot) is consumed (lfmm2d and
 // rfmm2d split to Cauchy densities with zero imaginary parts,
@@ -1097,15 +1181,20 @@ __global__ void k_re_to_f32(int64_t n, const cplx* in, float* out) {
 void launch_eval(int pg, int64_t n, hipStream_t st, int nd, int P, const double* xy,
                  const int* leaf, const Tree& t, const int* nterms, const double* rsc,
                  const Lists& L, const cplx* mp, const cplx* loc, cplx* p, cplx* g, cplx* h) {
-  if (pg == 1)
-    launch1d_xcd(k_eval<1>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
-             L.l3.p, mp, loc, p, g, h);
-  else if (pg == 2)
-    launch1d_xcd(k_eval<2>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
-             L.l3.p, mp, loc, p, g, h);
-  else
-    launch1d_xcd(k_eval<3>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p, L.n3.p,
-             L.l3.p, mp, loc, p, g, h);
+#define FMM2D_EVAL(PG)                                                                         \
+  do {                                                                                         \
+    if (nd == 1)                                                                               \
+      launch1d_xcd(k_eval_b<PG>, n, st, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p,     \
+                   L.n3.p, L.l3.p, mp, loc, p, g, h);                                          \
+    else                                                                                       \
+      launch1d_xcd(k_eval<PG>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p,   \
+                   L.n3.p, L.l3.p, mp, loc, p, g, h);                                          \
+  } while (0)
+  static_assert(kEvB == 256, "launch1d_xcd launches blocks of 256 threads");
+  if (pg == 1) FMM2D_EVAL(1);
+  else if (pg == 2) FMM2D_EVAL(2);
+  else FMM2D_EVAL(3);
+#undef FMM2D_EVAL
 }
 
 }  // namespace
