@@ -456,6 +456,93 @@ __global__ void k_heval(int64_t n, int nd, cplx zk, const HLev* lev, const doubl
   }
 }
 
+// nd = 1: a block of kHEvB consecutive points shares out their (point,
+// list-3 box) pairs evenly -- pair k to thread k mod kHEvB, its point found by
+// binary search in the block's scan of list-3 counts -- and sums each point's
+// multipole terms in LDS (FP64 atomic adds), so leaves with different list-3
+// lengths in one wavefront leave no lane idle; each thread then adds its own
+// point's local expansion.
+constexpr int kHEvB = 256;
+
+template <int PG>
+__global__ __launch_bounds__(kHEvB) void k_heval_b(int64_t n, cplx zk, const HLev* lev,
+                                                   const double* xy, const int* leaf,
+                                                   const int* ilevel, const double* centers,
+                                                   const int* n3, const int* l3, const cplx* mp,
+                                                   const cplx* loc, cplx* pot, cplx* grad,
+                                                   cplx* hess) {
+  constexpr int NC = NComp<PG>::v;
+  using Scan = hipcub::BlockScan<int, kHEvB>;
+  __shared__ typename Scan::TempStorage scan_tmp;
+  __shared__ int off[kHEvB];
+  __shared__ int sb[kHEvB];
+  __shared__ double sx[kHEvB], sy[kHEvB];
+  __shared__ double acc[2 * NC][kHEvB];  // re, im of each component's sum
+  const int tid = threadIdx.x;
+  const int64_t i = xcd_block() * kHEvB + tid;
+  const bool live = i < n;
+  const int b = live ? leaf[i] : 0;
+  const double x = live ? xy[2 * i] : 0.0, y = live ? xy[2 * i + 1] : 0.0;
+  const int cnt = live ? n3[b] : 0;
+  sb[tid] = b;
+  sx[tid] = x;
+  sy[tid] = y;
+#pragma unroll
+  for (int a = 0; a < 2 * NC; ++a) acc[a][tid] = 0.0;
+  int o, m;
+  Scan(scan_tmp).ExclusiveSum(cnt, o, m);
+  off[tid] = o;
+  __syncthreads();
+  for (int k = tid; k < m; k += kHEvB) {
+    int lo = 0, hi = kHEvB - 1;  // last slot t with off[t] <= k
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    const int c = l3[(int64_t)kMaxList3 * sb[lo] + (k - off[lo])];
+    const HLev Lc = lev[ilevel[c]];
+    if (!Lc.on) continue;
+    const double dx = sx[lo] - centers[2 * c], dy = sy[lo] - centers[2 * c + 1];
+    const double r = hypot(dx, dy);
+    const cplx w = cplx{dx / r, dy / r};
+    const Deriv D{hexp(mp, Lc, c, 1, 0), Lc.p, 0.5 * zk, Lc.s, 1.0 / Lc.s, -1};
+    cplx res[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) res[a] = mkc(0.0);
+    eval_mpole<PG>(D, zk * r, Lc.s, w, res);
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+      atomicAdd(&acc[2 * a][lo], res[a].re);
+      atomicAdd(&acc[2 * a + 1][lo], res[a].im);
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  cplx res[NC];
+#pragma unroll
+  for (int a = 0; a < NC; ++a) res[a] = cplx{acc[2 * a][tid], acc[2 * a + 1][tid]};
+  const HLev Lb = lev[ilevel[b]];
+  if (Lb.on) {
+    const double dx = x - centers[2 * b], dy = y - centers[2 * b + 1];
+    const double r = hypot(dx, dy);
+    const cplx w = (r > 0.0) ? cplx{dx / r, dy / r} : mkc(1.0);
+    const Deriv D{hexp(loc, Lb, b, 1, 0), Lb.p, 0.5 * zk, Lb.s, 1.0 / Lb.s, 1};
+    eval_local<PG>(D, zk * r, Lb.s, w, res);
+  }
+  const cplx ik4 = cplx{0.0, 0.25};
+  pot[i] += ik4 * res[0];
+  if (PG >= 2) {
+    grad[2 * i] += ik4 * res[1];
+    grad[2 * i + 1] += ik4 * res[2];
+  }
+  if (PG >= 3) {
+    hess[3 * i] += ik4 * res[3];
+    hess[3 * i + 1] += ik4 * res[4];
+    hess[3 * i + 2] += ik4 * res[5];
+  }
+}
+
 // ---------------------------------------------------------- near field
 // hfmm2dpart_direct (hfmm2d.f:1560-1600 -> helmkernels2d.f) over list 1,
 // one loop per point over the merged source ranges of its leaf's segment
@@ -615,8 +702,13 @@ template <int PG>
 void launch_heval(int64_t n, hipStream_t st, int nd, cplx zk, const HLev* lev, const double* xy,
                   const int* leaf, const Tree& t, const Lists& L, const cplx* mp, const cplx* loc,
                   cplx* p, cplx* g, cplx* h) {
-  launch1d_xcd(k_heval<PG>, n, st, nd, zk, lev, xy, leaf, t.ilevel.p, t.centers.p, L.n3.p, L.l3.p, mp,
-           loc, p, g, h);
+  static_assert(kHEvB == 256, "launch1d_xcd launches blocks of 256 threads");
+  if (nd == 1)
+    launch1d_xcd(k_heval_b<PG>, n, st, zk, lev, xy, leaf, t.ilevel.p, t.centers.p, L.n3.p,
+                 L.l3.p, mp, loc, p, g, h);
+  else
+    launch1d_xcd(k_heval<PG>, n, st, nd, zk, lev, xy, leaf, t.ilevel.p, t.centers.p, L.n3.p,
+                 L.l3.p, mp, loc, p, g, h);
 }
 
 template <int IFC, int IFD, int NDT>
