@@ -188,16 +188,18 @@ __global__ void k_child_slots(int64_t n, const int* list, const int* ichild, con
   }
 }
 
+// out[l] = first position of list (sorted) at or beyond lstart[l], one thread
+// per l (launch >= nl + 2 threads)
 __global__ void k_level_bounds(const int* list, int n, const int* lstart, int nl, int* out) {
-  for (int l = 0; l <= nl + 1; ++l) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-      const int mid = (lo + hi) / 2;
-      if (list[mid] < lstart[l]) lo = mid + 1;
-      else hi = mid;
-    }
-    out[l] = lo;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l > nl + 1) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (list[mid] < lstart[l]) lo = mid + 1;
+    else hi = mid;
   }
+  out[l] = lo;
 }
 
 
@@ -248,16 +250,18 @@ __global__ void k_counts(int64_t n, const int* ilevel, const int* iparent, const
 }
 
 
+// seg[q] = first position of keys (sorted) at or beyond q, one thread per q
+// (launch >= 5 threads)
 __global__ void k_class_bounds(const int* keys, int n, int* seg) {
-  for (int q = 0; q <= 4; ++q) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-      const int mid = (lo + hi) / 2;
-      if (keys[mid] < q) lo = mid + 1;
-      else hi = mid;
-    }
-    seg[q] = lo;
+  const int q = threadIdx.x;
+  if (q > 4) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (keys[mid] < q) lo = mid + 1;
+    else hi = mid;
   }
+  seg[q] = lo;
 }
 
 template <typename T> DBuf<T> upload(const std::vector<T>& v, hipStream_t st) {
@@ -329,7 +333,8 @@ void plan_children(const Tree& t, int pg, int pgt, ChildPlan& cp, hipStream_t st
   std::vector<int> lsx(t.lstart);
   lsx.push_back(nb);
   DBuf<int> d_lsx = upload(lsx, st);
-  hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(1), 0, st, cp.list.p, nnl, d_lsx.p, nl,
+  if (nl + 2 > 64) throw HipError{hipErrorInvalidValue, "tree levels beyond 62"};
+  hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(64), 0, st, cp.list.p, nnl, d_lsx.p, nl,
                      dlb.p);
   hip_throw(hipMemcpyAsync(cp.lev_pos.data(), dlb.p, sizeof(int) * (nl + 2),
                            hipMemcpyDeviceToHost, st), "copy");
@@ -341,19 +346,19 @@ void plan_children(const Tree& t, int pg, int pgt, ChildPlan& cp, hipStream_t st
 }
 
 // seg[q * (nl + 2) + l]: first position in tlist of class q at level >= l
+// one thread per (class q, level l) (launch >= 4 (nl + 2) threads)
 __global__ void k_class_level_bounds(const int* tlist, const int* cseg, const int* lstart,
                                      int nl, int* seg) {
-  const int q = threadIdx.x;
-  if (q >= 4) return;
-  for (int l = 0; l <= nl + 1; ++l) {
-    int lo = cseg[q], hi = cseg[q + 1];
-    while (lo < hi) {
-      const int mid = (lo + hi) / 2;
-      if (tlist[mid] < lstart[l]) lo = mid + 1;
-      else hi = mid;
-    }
-    seg[q * (nl + 2) + l] = lo;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * (nl + 2)) return;
+  const int q = t / (nl + 2), l = t % (nl + 2);
+  int lo = cseg[q], hi = cseg[q + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (tlist[mid] < lstart[l]) lo = mid + 1;
+    else hi = mid;
   }
+  seg[t] = lo;
 }
 
 // M2L targets (level >= 2, non-empty list 2, points to evaluate) sorted by
@@ -381,13 +386,14 @@ void plan_m2l(const Tree& t, const Lists& L, int pg, int pgt, M2LPlan& mp, hipSt
   DBuf<char> tmp(bytes, st);
   hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, mp.skey.p, val.p,
                                                mp.tlist.p, nb, 0, 3, st), "sort");
-  hipLaunchKernelGGL(k_class_bounds, dim3(1), dim3(1), 0, st, mp.skey.p, nb, seg.p);
+  hipLaunchKernelGGL(k_class_bounds, dim3(1), dim3(64), 0, st, mp.skey.p, nb, seg.p);
   std::vector<int> lsx(t.lstart);
   lsx.push_back(nb);
   DBuf<int> d_lsx = upload(lsx, st);
   DBuf<int> dls(4 * (nl + 2), st);
-  hipLaunchKernelGGL(k_class_level_bounds, dim3(1), dim3(4), 0, st, mp.tlist.p, seg.p, d_lsx.p,
-                     nl, dls.p);
+  if (4 * (nl + 2) > 256) throw HipError{hipErrorInvalidValue, "tree levels beyond 62"};
+  hipLaunchKernelGGL(k_class_level_bounds, dim3(1), dim3(256), 0, st, mp.tlist.p, seg.p,
+                     d_lsx.p, nl, dls.p);
   mp.lev_seg.assign(4 * (size_t)(nl + 2), 0);
   hip_throw(hipMemcpyAsync(mp.cls_seg, seg.p, sizeof(mp.cls_seg), hipMemcpyDeviceToHost, st),
             "copy");

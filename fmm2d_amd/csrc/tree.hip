@@ -180,15 +180,30 @@ __device__ __forceinline__ int digit_at(const Keys& K, int64_t i, int lev) {
   return (int)((K.k2[i] >> (2 * (K.d2 - (lev - K.d1)))) & 3ull);
 }
 
-// first i in [lo, hi) whose digit at level lev is >= q (digits are sorted
-// there because all keys in a box's range share the path above lev)
-__device__ int64_t digit_bound(const Keys& K, int64_t lo, int64_t hi, int lev, int q) {
-  while (lo < hi) {
-    const int64_t mid = lo + ((hi - lo) >> 1);
-    if (digit_at(K, mid, lev) < q) lo = mid + 1;
-    else hi = mid;
+// b[q] = first i in [lo0, hi0) whose digit at level lev is >= q + 1, q = 0..2
+// (digits are sorted there because all keys in a box's range share the path
+// above lev).  The three binary searches run interleaved over the whole
+// range with their loads in flight together: at the first levels a box holds
+// millions of points, and chained searches were ~60 dependent loads long.
+__device__ void digit_bounds3(const Keys& K, int64_t lo0, int64_t hi0, int lev, int64_t* b) {
+  int64_t lo[3] = {lo0, lo0, lo0}, hi[3] = {hi0, hi0, hi0};
+  while (lo[0] < hi[0] || lo[1] < hi[1] || lo[2] < hi[2]) {
+    int64_t mid[3];
+    int dg[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) mid[q] = min(lo[q] + ((hi[q] - lo[q]) >> 1), hi0 - 1);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) dg[q] = digit_at(K, mid[q], lev);  // lo0 <= mid < hi0
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (lo[q] < hi[q]) {
+        if (dg[q] < q + 1) lo[q] = mid[q] + 1;
+        else hi[q] = mid[q];
+      }
+    }
   }
-  return lo;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) b[q] = lo[q];
 }
 
 // ------------------------------------------------------------ refinement
@@ -205,12 +220,6 @@ __global__ void k_refine_flags(int64_t n, int lo, const int* ses, const int* set
 }
 
 // flag[i] = iflag[lo+i] > 0 (tree_refine_boxes_flag, tree_routs2d.f:334-388)
-__global__ void k_iflag_pos(int64_t n, int lo, const int* iflag, int* flag) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  flag[i] = iflag[lo + (int)i] > 0 ? 1 : 0;
-}
-
 __global__ void k_total(int n, const int* flag, const int* offs, int* total) {
   *total = (n > 0) ? offs[n - 1] + flag[n - 1] : 0;
 }
@@ -264,10 +273,41 @@ __global__ void k_init_root(double cx, double cy, int ns, int nt, double* center
 
 // cum[k] = exclusive-scan value at segment start cut[k]; cut[K] = n, the
 // scanned length, gives the grand total
-__global__ void k_seg_totals(int K, const int* cut, const int* flag, const int* offs, int* cum) {
-  const int n = cut[K];
-  for (int k = threadIdx.x; k <= K; k += blockDim.x) {
-    const int c = cut[k];
+// Box ranges refined together with one scan (Builder::refine_segments),
+// passed by value: segment j covers flag positions [pos[j], pos[j+1]) and
+// boxes lo[j] ..; its children are at level lev[j] with half box size half[j].
+constexpr int kRefSeg = 64;
+struct RefSegs {
+  int k;
+  int pos[kRefSeg + 1];
+  int lo[kRefSeg];
+  int lev[kRefSeg];
+  double half[kRefSeg];
+};
+
+// the segment of flag position i: the last j with pos[j] <= i (skips empty ones)
+__device__ __forceinline__ int seg_of(const RefSegs& R, int64_t i) {
+  int a = 0, b = R.k - 1;
+  while (a < b) {
+    const int m = (a + b + 1) >> 1;
+    if (R.pos[m] <= i) a = m;
+    else b = m - 1;
+  }
+  return a;
+}
+
+__global__ void k_iflag_segs(int64_t n, RefSegs R, const int* iflag, int* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = seg_of(R, i);
+  flag[i] = iflag[R.lo[j] + (int)(i - R.pos[j])] > 0 ? 1 : 0;
+}
+
+// cum[k] = flagged boxes before segment k (k = 0..K)
+__global__ void k_seg_totals(RefSegs R, const int* flag, const int* offs, int* cum) {
+  const int n = R.pos[R.k];
+  for (int k = threadIdx.x; k <= R.k; k += blockDim.x) {
+    const int c = R.pos[k];
     cum[k] = (c < n) ? offs[c] : (n > 0 ? offs[n - 1] + flag[n - 1] : 0);
   }
 }
@@ -290,10 +330,11 @@ __device__ void make_children_of(int b, int c0, double half, int lev, double* ce
   bs_[4] = ses[2 * b + 1];
   bt_[0] = set[2 * b];
   bt_[4] = set[2 * b + 1];
-  for (int q = 1; q < 4; ++q) {
-    bs_[q] = (int)digit_bound(ks, bs_[q - 1], bs_[4], lev, q);
-    bt_[q] = (int)digit_bound(kt, bt_[q - 1], bt_[4], lev, q);
-  }
+  int64_t bq[3];
+  digit_bounds3(ks, bs_[0], bs_[4], lev, bq);
+  for (int q = 1; q < 4; ++q) bs_[q] = (int)bq[q - 1];
+  digit_bounds3(kt, bt_[0], bt_[4], lev, bq);
+  for (int q = 1; q < 4; ++q) bt_[q] = (int)bq[q - 1];
   for (int q = 0; q < 4; ++q) {
     const int c = c0 + q;
     centers[2 * c] = cx + ((q & 1) ? half : -half);
@@ -319,6 +360,17 @@ __global__ void k_make_children(int64_t n, int lo, const int* flag, const int* o
   if (i >= n || !flag[i]) return;
   make_children_of(lo + (int)i, base + 4 * offs[i], half, lev, centers, ilevel, iparent, nchild,
                    ichild, ses, set, iflag, mode, ks, kt);
+}
+
+__global__ void k_make_children_segs(int64_t n, RefSegs R, const int* flag, const int* offs,
+                                     int base, double* centers, int* ilevel, int* iparent,
+                                     int* nchild, int* ichild, int* ses, int* set, int* iflag,
+                                     Keys ks, Keys kt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  const int j = seg_of(R, i);
+  make_children_of(R.lo[j] + (int)(i - R.pos[j]), base + 4 * offs[i], R.half[j], R.lev[j],
+                   centers, ilevel, iparent, nchild, ichild, ses, set, iflag, 1, ks, kt);
 }
 
 __global__ void k_make_children_lv(int64_t ub, const int* dlev, int l, const int* flag,
@@ -511,13 +563,14 @@ __global__ void k_flag2(int64_t n, int lo, double distest, const int* iparent, c
   }
 }
 
-// updateflags (tree_routs2d.f:217-326)
-__global__ void k_updateflags(int64_t n, int lo, double distest, const int* ncoll, const int* coll,
-                              const int* ichild, const int* nchild, const double* centers,
-                              int* iflag) {
+// updateflags (tree_routs2d.f:217-326) over boxes lo1 .. lo1+n1-1, then
+// lo2 .. lo2+(n-n1)-1
+__global__ void k_updateflags(int64_t n, int n1, int lo1, int lo2, double distest,
+                              const int* ncoll, const int* coll, const int* ichild,
+                              const int* nchild, const double* centers, int* iflag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int b = lo + (int)i;
+  const int b = i < n1 ? lo1 + (int)i : lo2 + (int)(i - n1);
   if (iflag[b] != 3) return;
   int f = 0;
   const double bx = centers[2 * b], by = centers[2 * b + 1];
@@ -757,21 +810,25 @@ struct Builder {
                        const std::vector<int>& lev, const std::vector<double>& bsc,
                        std::vector<int>& first, std::vector<int>& last) {
     const size_t K = lo.size();
-    std::vector<int> pos(K + 1, 0);
-    for (size_t k = 0; k < K; ++k) pos[k + 1] = pos[k] + n[k];
+    if (K > (size_t)kRefSeg) throw HipError{hipErrorInvalidValue, "refinement segments"};
+    RefSegs R{};
+    R.k = (int)K;
+    R.pos[0] = 0;
+    for (size_t k = 0; k < K; ++k) {
+      R.pos[k + 1] = R.pos[k] + n[k];
+      R.lo[k] = lo[k];
+      R.lev[k] = lev[k];
+      R.half[k] = 0.5 * bsc[k];
+    }
     first.assign(K, nboxes);
     last.assign(K, nboxes);
-    const int ntot = pos[K];
+    const int ntot = R.pos[K];
     if (ntot == 0) return;
     ensure_flag(ntot);
-    for (size_t k = 0; k < K; ++k)
-      launch1d(k_iflag_pos, n[k], st, lo[k], B.iflag.p, flag.p + pos[k]);
+    launch1d(k_iflag_segs, ntot, st, R, B.iflag.p, flag.p);
     scan_only(ntot);
     if (total.n < K + 1) total.alloc(K + 1, st);
-    std::vector<int> cut(pos);
-    DBuf<int> dcut = upload_ints(cut);
-    hipLaunchKernelGGL(k_seg_totals, dim3(1), dim3(64), 0, st, (int)K, dcut.p, flag.p, offs.p,
-                       total.p);
+    hipLaunchKernelGGL(k_seg_totals, dim3(1), dim3(64), 0, st, R, flag.p, offs.p, total.p);
     std::vector<int> cum(K + 1, 0);
     hip_throw(hipMemcpyAsync(cum.data(), total.p, sizeof(int) * (K + 1), hipMemcpyDeviceToHost,
                              st), "copy");
@@ -782,18 +839,11 @@ struct Builder {
     for (size_t k = 0; k < K; ++k) {
       first[k] = nboxes + 4 * cum[k];
       last[k] = nboxes + 4 * cum[k + 1];
-      if (last[k] > first[k])
-        launch1d(k_make_children, n[k], st, lo[k], flag.p + pos[k], offs.p + pos[k], nboxes,
-                 0.5 * bsc[k], lev[k], B.centers.p, B.ilevel.p, B.iparent.p, B.nchild.p,
-                 B.ichild.p, B.ses.p, B.set.p, B.iflag.p, 1, ks, kt);
     }
+    launch1d(k_make_children_segs, ntot, st, R, flag.p, offs.p, nboxes, B.centers.p,
+             B.ilevel.p, B.iparent.p, B.nchild.p, B.ichild.p, B.ses.p, B.set.p, B.iflag.p, ks,
+             kt);
     nboxes += 4 * nref;
-  }
-  DBuf<int> upload_ints(const std::vector<int>& v) {
-    DBuf<int> d(v.size(), st);
-    hip_throw(hipMemcpyAsync(d.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, st),
-              "copy");
-    return d;  // the caller syncs before v dies
   }
   void ensure_flag(int n) {
     if ((size_t)n > flag.n) {
@@ -1070,10 +1120,10 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
     int added2 = 0;
     for (int l = 2; l <= nlevels - 2; ++l) {
       const double distest = 1.05 * (bs[l] + bs[l + 1]) / 2.0;
-      launch1d(k_updateflags, le[l] - ls[l], st, ls[l], distest, bld.B.ncoll.p, bld.B.coll.p,
-               bld.B.ichild.p, bld.B.nchild.p, bld.B.centers.p, bld.B.iflag.p);
-      launch1d(k_updateflags, te[l] - ts[l], st, ts[l], distest, bld.B.ncoll.p, bld.B.coll.p,
-               bld.B.ichild.p, bld.B.nchild.p, bld.B.centers.p, bld.B.iflag.p);
+      // the original then the tail boxes of level l (their flags are independent)
+      launch1d(k_updateflags, (le[l] - ls[l]) + (te[l] - ts[l]), st, le[l] - ls[l], ls[l], ts[l],
+               distest, bld.B.ncoll.p, bld.B.coll.p, bld.B.ichild.p, bld.B.nchild.p,
+               bld.B.centers.p, bld.B.iflag.p);
       {
         // the original then the tail boxes of level l, in one scan
         std::vector<int> first, last;
