@@ -51,6 +51,7 @@ _FMM_OUT = [_i, _i, _vp, _vp, _vp, _i64, _vp, _i, _vp, _vp, _vp, _vp, _vp]
 SIGNATURES["fmm2d_cfmm2d"] = (_i, [_i, _d] + _FMM_TAIL + _FMM_OUT)
 SIGNATURES["fmm2d_lfmm2d"] = (_i, [_i, _d] + _FMM_TAIL + [_vp] + _FMM_OUT)
 SIGNATURES["fmm2d_hfmm2d"] = (_i, [_i, _d, _d, _d] + _FMM_TAIL + [_vp] + _FMM_OUT)
+SIGNATURES["fmm2d_batch"] = (_i, [_i, _vp, _i])
 
 
 class FmmOpts(C.Structure):
@@ -64,6 +65,19 @@ class FmmOpts(C.Structure):
 
     def timeinfo_array(self):
         return np.array(self.timeinfo[:], dtype=np.float64)
+
+
+class FmmJob(C.Structure):
+    """struct fmm2d_job of include/fmm2d_amd.h (one FMM of fmm2d_batch)."""
+    _fields_ = [("kind", C.c_int32), ("nd", C.c_int32), ("eps", C.c_double),
+                ("zk_re", C.c_double), ("zk_im", C.c_double), ("ns", C.c_int64),
+                ("sources", C.c_void_p), ("ifcharge", C.c_int32), ("ifdipole", C.c_int32),
+                ("charge", C.c_void_p), ("dipstr", C.c_void_p), ("dipvec", C.c_void_p),
+                ("ifpgh", C.c_int32), ("ifpghtarg", C.c_int32), ("pot", C.c_void_p),
+                ("grad", C.c_void_p), ("hess", C.c_void_p), ("nt", C.c_int64),
+                ("targ", C.c_void_p), ("pottarg", C.c_void_p), ("gradtarg", C.c_void_p),
+                ("hesstarg", C.c_void_p), ("opts", C.POINTER(FmmOpts)), ("ier", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 def opts_struct(opts=None):

@@ -1,9 +1,13 @@
 // C-ABI entry points of the FMM drivers, the tree export and the parameter
 // rules.  Declarations and reference citations: include/fmm2d_amd.h.
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <numeric>
+#include <thread>
 #include <vector>
 
 #include "../../include/fmm2d_amd.h"
@@ -64,6 +68,23 @@ CauchyArgs cauchy_args(int nd, double eps, int64_t ns, const double* sources, in
   a.opts = opts;
   return a;
 }
+
+// the batch streams of each device, created on first use and kept (blocks
+// cached by the workspace are keyed by stream)
+std::vector<hipStream_t> batch_streams(int dev, int n) {
+  static std::mutex mu;
+  static std::map<int, std::vector<hipStream_t>> pool;
+  std::lock_guard<std::mutex> lock(mu);
+  std::vector<hipStream_t>& v = pool[dev];
+  while ((int)v.size() < n) {
+    hipStream_t s = nullptr;
+    hip_throw(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    v.push_back(s);
+  }
+  return std::vector<hipStream_t>(v.begin(), v.begin() + n);
+}
+
+int run_job(fmm2d_job& j, hipStream_t st);
 
 struct TreeHandle {
   Tree t;
@@ -276,4 +297,80 @@ int fmm2d_ndiv(double eps, int64_t ns, int64_t nt, int helmholtz, int* ndiv, int
 
 int fmm2d_release_workspace(void) { return ws_release(); }
 
+int fmm2d_batch(int njobs, fmm2d_job* jobs, int nstreams) {
+  if (njobs < 0 || (njobs > 0 && !jobs) || nstreams < 1 || nstreams > 16) return FMM2D_ERR_ARG;
+  if (njobs == 0) return FMM2D_OK;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return FMM2D_ERR_HIP;
+  std::vector<hipStream_t> streams;
+  try {
+    streams = batch_streams(dev, std::min(nstreams, njobs));
+  } catch (const HipError&) {
+    return FMM2D_ERR_HIP;
+  }
+  std::vector<int> order(njobs);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return jobs[a].ns + jobs[a].nt > jobs[b].ns + jobs[b].nt;
+  });
+  std::atomic<int> next{0};
+  std::vector<int> sync_err(streams.size(), 0);
+  auto worker = [&](int w) {
+    (void)hipSetDevice(dev);
+    for (int k; (k = next.fetch_add(1)) < njobs;) {
+      fmm2d_job& j = jobs[order[k]];
+      j.ier = run_job(j, streams[w]);
+    }
+    if (hipStreamSynchronize(streams[w]) != hipSuccess) sync_err[w] = 1;
+  };
+  std::vector<std::thread> th;
+  for (int w = 1; w < (int)streams.size(); ++w) th.emplace_back(worker, w);
+  worker(0);
+  for (auto& t : th) t.join();
+  for (int e : sync_err)
+    if (e) return FMM2D_ERR_HIP;
+  for (int k = 0; k < njobs; ++k)
+    if (jobs[k].ier != 0) return jobs[k].ier;
+  return FMM2D_OK;
+}
+
 }  // extern "C"
+
+namespace {
+
+int run_job(fmm2d_job& j, hipStream_t st) {
+  void* s = reinterpret_cast<void*>(st);
+  auto* ch = static_cast<const fmm2d_cplx*>(j.charge);
+  auto* ds = static_cast<const fmm2d_cplx*>(j.dipstr);
+  auto* p = static_cast<fmm2d_cplx*>(j.pot);
+  auto* g = static_cast<fmm2d_cplx*>(j.grad);
+  auto* h = static_cast<fmm2d_cplx*>(j.hess);
+  auto* pt = static_cast<fmm2d_cplx*>(j.pottarg);
+  auto* gt = static_cast<fmm2d_cplx*>(j.gradtarg);
+  auto* ht = static_cast<fmm2d_cplx*>(j.hesstarg);
+  switch (j.kind) {
+    case 0:
+      return fmm2d_cfmm2d(j.nd, j.eps, j.ns, j.sources, j.ifcharge, ch, j.ifdipole, ds, 0,
+                          j.ifpgh, p, g, h, j.nt, j.targ, j.ifpghtarg, pt, gt, ht, j.opts, s);
+    case 1:
+      return fmm2d_lfmm2d(j.nd, j.eps, j.ns, j.sources, j.ifcharge, ch, j.ifdipole, ds,
+                          j.dipvec, 0, j.ifpgh, p, g, h, j.nt, j.targ, j.ifpghtarg, pt, gt, ht,
+                          j.opts, s);
+    case 2:
+      return fmm2d_rfmm2d(j.nd, j.eps, j.ns, j.sources, j.ifcharge,
+                          static_cast<const double*>(j.charge), j.ifdipole,
+                          static_cast<const double*>(j.dipstr), j.dipvec, 0, j.ifpgh,
+                          static_cast<double*>(j.pot), static_cast<double*>(j.grad),
+                          static_cast<double*>(j.hess), j.nt, j.targ, j.ifpghtarg,
+                          static_cast<double*>(j.pottarg), static_cast<double*>(j.gradtarg),
+                          static_cast<double*>(j.hesstarg), j.opts, s);
+    case 3:
+      return fmm2d_hfmm2d(j.nd, j.eps, j.zk_re, j.zk_im, j.ns, j.sources, j.ifcharge, ch,
+                          j.ifdipole, ds, j.dipvec, 0, j.ifpgh, p, g, h, j.nt, j.targ,
+                          j.ifpghtarg, pt, gt, ht, j.opts, s);
+    default:
+      return FMM2D_ERR_ARG;
+  }
+}
+
+}  // namespace

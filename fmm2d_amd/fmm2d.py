@@ -163,6 +163,104 @@ def _fmm(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
     return res, ier
 
 
+_KIND = {"c": 0, "l": 1, "r": 2, "h": 3}
+
+
+def fmm2d_batch(problems, nstreams=4):
+    """Many independent FMMs in one native call (fmm2d_batch of the C-ABI; the
+    reference has no batched driver).
+
+    problems: list of dicts, each with "kind" ("h", "l", "r" or "c") and the
+    keyword arguments of hfmm2d / lfmm2d / rfmm2d / cfmm2d (eps, zk, sources,
+    charges, dipstr, dipvec, targets, pg, pgt, nd).  When both pg and pgt are
+    set they must be equal, as the single-problem calls require.  The problems
+    run on nstreams streams at once, largest first; returns one Output per
+    problem, in order (ier set on each).
+    """
+    import torch
+    maxpg = {"h": 2, "l": 3, "r": 3, "c": 3}
+    for pb in problems:  # argument checks before any device work
+        kind = pb.get("kind")
+        if kind not in _KIND:
+            raise ValueError(f"unknown kind {kind!r}")
+        pg, pgt = int(pb.get("pg", 0)), int(pb.get("pgt", 0))
+        if not (0 <= pg <= maxpg[kind] and 0 <= pgt <= maxpg[kind]) or (pg == 0 and pgt == 0):
+            raise ValueError("pg, pgt: one of them in 1..%d" % maxpg[kind])
+        if pg and pgt and pg != pgt:
+            raise ValueError("if output is requested at both sources and targets, then pg "
+                             "must be equal to pgt")
+        if pgt and pb.get("targets") is None:
+            raise ValueError("pgt > 0 needs targets")
+        if kind == "h" and pb.get("zk") is None:
+            raise ValueError("hfmm2d needs zk")
+    if not 1 <= int(nstreams) <= 16:
+        raise ValueError("nstreams must be in 1..16")
+    L = nat.lib()
+    jobs, keep = _stage_batch(problems)
+    # inputs were staged on torch's stream; the batch runs on streams of its own
+    torch.cuda.current_stream().synchronize()
+    ier = L.fmm2d_batch(len(problems), jobs, int(nstreams))
+    nat.check(ier, "fmm2d_batch")
+    outs = []
+    for k, (dev, bufs, nd) in enumerate(keep):
+        out = Output()
+        res = {}
+        for name, t in bufs.items():
+            if t is not None:
+                a = nat.fortran_to_numpy(t)
+                res[name] = a[0] if nd == 1 else a
+        _fill(out, res, int(jobs[k].ier))
+        outs.append(out)
+    return outs
+
+
+def _stage_batch(problems):
+    """Device inputs and zeroed outputs of checked problems, and their job
+    structs: (ctypes array of FmmJob, [(inputs, outputs, nd)])."""
+    import torch
+    nat.torch_device()
+    jobs = (nat.FmmJob * max(len(problems), 1))()
+    keep = []
+    for k, pb in enumerate(problems):
+        kind = pb["kind"]
+        nd = int(pb.get("nd", 1))
+        pg, pgt = int(pb.get("pg", 0)), int(pb.get("pgt", 0))
+        targets = pb.get("targets")
+        sources = pb["sources"]
+        ns = _nsrc(sources)
+        charges, dipstr, dipvec = pb.get("charges"), pb.get("dipstr"), pb.get("dipvec")
+        ifc = _check_charges(charges, nd, ns)
+        ifd = _check_dipoles(dipstr, dipvec, nd, ns, need_vec=(kind != "c"))
+        dev = _DevInputs(sources, ns, nd, charges if ifc else None, dipstr if ifd else None,
+                         dipvec, targets if pgt > 0 else None, kind != "c",
+                         real=(kind == "r"))
+        ng = 1 if kind == "c" else 2
+        nh = 1 if kind == "c" else 3
+        odt = torch.float64 if kind == "r" else torch.complex128
+        bufs = {}
+        for flag, n, names in ((pg, ns, ("pot", "grad", "hess")),
+                               (pgt, dev.nt, ("pottarg", "gradtarg", "hesstarg"))):
+            shp = ((nd, n), (nd, n) if ng == 1 else (nd, ng, n),
+                   (nd, n) if nh == 1 else (nd, nh, n))
+            for lvl, (name, sh) in enumerate(zip(names, shp)):
+                bufs[name] = nat.dev_zeros_fortran(sh, odt) if flag > lvl else None
+        j = jobs[k]
+        j.kind, j.nd, j.eps = _KIND[kind], nd, float(pb["eps"])
+        zk = complex(pb.get("zk", 0.0) or 0.0)
+        j.zk_re, j.zk_im = zk.real, zk.imag
+        j.ns, j.sources = ns, nat.ptr(dev.src)
+        j.ifcharge, j.ifdipole = int(dev.charge is not None), int(dev.dipstr is not None)
+        j.charge, j.dipstr = nat.ptr(dev.charge), nat.ptr(dev.dipstr)
+        j.dipvec = nat.ptr(dev.dipvec) if kind != "c" else None
+        j.ifpgh, j.ifpghtarg = pg, pgt
+        j.pot, j.grad, j.hess = (nat.ptr(bufs[n]) for n in ("pot", "grad", "hess"))
+        j.nt, j.targ = dev.nt, nat.ptr(dev.targ)
+        j.pottarg, j.gradtarg, j.hesstarg = (nat.ptr(bufs[n])
+                                             for n in ("pottarg", "gradtarg", "hesstarg"))
+        keep.append((dev, bufs, nd))
+    return jobs, keep
+
+
 def _fmm_dispatch(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
                   maxpg):
     """The reference's branch structure (python/fmm2dpy/fmm2d.py:98-196, 294-454,

@@ -176,6 +176,47 @@ int fmm2d_hfmm2d(int nd, double eps, double zk_re, double zk_im, int64_t ns,
                  fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg, fmm2d_opts* opts, void* stream);
 
 /*
+ * One FMM of a batch (fmm2d_batch): the arguments of fmm2d_cfmm2d (kind 0),
+ * fmm2d_lfmm2d (kind 1), fmm2d_rfmm2d (kind 2: charge, dipstr and the outputs
+ * point to doubles) or fmm2d_hfmm2d (kind 3: zk = zk_re + i zk_im), all device
+ * pointers, iper = 0.  ier receives that call's return code.
+ */
+typedef struct fmm2d_job {
+  int32_t kind, nd;
+  double eps, zk_re, zk_im;
+  int64_t ns;
+  const double* sources;
+  int32_t ifcharge, ifdipole;
+  const void* charge;
+  const void* dipstr;
+  const double* dipvec;
+  int32_t ifpgh, ifpghtarg;
+  void* pot;
+  void* grad;
+  void* hess;
+  int64_t nt;
+  const double* targ;
+  void* pottarg;
+  void* gradtarg;
+  void* hesstarg;
+  fmm2d_opts* opts;
+  int32_t ier, pad;
+} fmm2d_job;
+
+/*
+ * Many independent FMMs in one call.  No counterpart in the reference (whose
+ * drivers take one problem per call); for callers with many small problems of
+ * uneven size, which one at a time leave the GPU idle between launches.
+ * nstreams (1..16) host threads each take the next job, largest (ns + nt)
+ * first, and run it on a stream of its own from a pool the library keeps per
+ * device (default priority, never destroyed), so that several jobs' kernels
+ * and host synchronisations overlap.  Returns when every job's device work is
+ * done: 0, or the first nonzero ier in job order (FMM2D_ERR_ARG for a bad
+ * njobs/jobs/nstreams or kind).
+ */
+int fmm2d_batch(int njobs, fmm2d_job* jobs, int nstreams);
+
+/*
  * Adaptive quadtree (pts_tree_mem + pts_tree_build + pts_tree_sort,
  * common/pts_tree2d.f:34, 406, 1222) over device-resident points, numbered
  * box-for-box as the reference numbers it.  *handle receives an opaque tree;

@@ -113,3 +113,44 @@ def test_reference_fixtures_against_oracle_direct_sum(golden_dir):
     exh = O.direct("h", 1, src, trg, ch, None, None, 2, 1e-16, zk=12.0)
     tagh = "nd1_h_1e-10_1022_"
     assert np.linalg.norm(g[tagh + "pottarg"] - exh["pottarg"]) / np.linalg.norm(exh["pottarg"]) < 1e-9
+
+
+def test_batch_job_layout(tmp_path):
+    """ctypes FmmJob has the field offsets of struct fmm2d_job in the header."""
+    names = [f[0] for f in nat.FmmJob._fields_]
+    src = tmp_path / "job.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "fmm2d_amd.h"\nint main(void){\n'
+        + 'printf("%zu\\n", sizeof(fmm2d_job));\n'
+        + "".join(f'printf("%zu\\n", offsetof(fmm2d_job, {n}));\n' for n in names)
+        + "return 0;}\n")
+    exe = tmp_path / "job"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include",
+                    str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(nat.FmmJob)
+    assert got[1:] == [getattr(nat.FmmJob, n).offset for n in names]
+
+
+def test_batch_argument_checks():
+    """fmm2d_batch rejects bad arguments before any device work; the Python
+    wrapper checks every problem first."""
+    L = nat.lib()
+    jobs = (nat.FmmJob * 2)()
+    assert L.fmm2d_batch(0, None, 4) == 0
+    assert L.fmm2d_batch(1, None, 4) == -1
+    assert L.fmm2d_batch(-1, jobs, 4) == -1
+    assert L.fmm2d_batch(1, jobs, 0) == -1
+    assert L.fmm2d_batch(1, jobs, 17) == -1
+    src = np.zeros((2, 4))
+    ok = dict(kind="l", eps=1e-6, sources=src, charges=np.ones(4), pg=1)
+    with pytest.raises(ValueError, match="unknown kind"):
+        F.fmm2d_batch([ok, dict(ok, kind="x")])
+    with pytest.raises(ValueError, match="pg must be equal to pgt"):
+        F.fmm2d_batch([dict(ok, targets=src, pgt=2)])
+    with pytest.raises(ValueError, match="needs zk"):
+        F.fmm2d_batch([dict(ok, kind="h")])
+    with pytest.raises(ValueError, match="nstreams"):
+        F.fmm2d_batch([ok], nstreams=0)
