@@ -140,6 +140,79 @@ print(json.dumps({{"seconds": time.perf_counter() - t}}))
     return json.loads(r.stdout.strip().splitlines()[-1])["seconds"], None
 
 
+def c4_problems(count, seed=4):
+    """Batched workload (the C4 configuration, reduced to `count` instances):
+    N log-uniform in [1e3, 2e5], points uniform in the unit disk or on the
+    perturbed curve r = 1 + 0.3 cos 5t (1e-4 jitter), Laplace (lfmm2d) and
+    Helmholtz (hfmm2d, zk log-uniform in [1, 300]) alternating, eps in
+    {1e-6, 1e-9}, complex charges ~ CN(0, 1), potential at the sources."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        if rng.random() < 0.5:
+            r = np.sqrt(rng.random(n))
+            t = rng.uniform(0, 2 * np.pi, n)
+            src = np.stack([r * np.cos(t), r * np.sin(t)])
+        else:
+            t = rng.uniform(0, 2 * np.pi, n)
+            r = 1 + 0.3 * np.cos(5 * t)
+            src = np.stack([r * np.cos(t), r * np.sin(t)]) + 1e-4 * rng.standard_normal((2, n))
+        ch = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)
+        pb = dict(kind="h" if k % 2 else "l", eps=float(rng.choice([1e-6, 1e-9])), sources=src,
+                  charges=ch, pg=1)
+        if k % 2:
+            pb["zk"] = float(np.exp(rng.uniform(0, np.log(300.0))))
+        out.append(pb)
+    return out
+
+
+def main_batch(args):
+    """Batched workload (secondary): the C4 mix of independent lfmm2d and
+    hfmm2d problems through fmm2d_batch on 8 streams, against the same
+    problems one after another on one stream.  Rate = points per second of
+    the whole batch, inputs resident on the device."""
+    import torch
+    from fmm2d_amd import _native as nat
+    from fmm2d_amd import fmm2d as F
+    L = nat.lib()
+    count = args.batch
+    probs = c4_problems(count)
+    npts = sum(p["sources"].shape[1] for p in probs)
+    print(f"bench: batch workload, {count} instances, {npts} points", file=sys.stderr,
+          flush=True)
+    jobs, keep = F._stage_batch(probs)
+    torch.cuda.synchronize()
+
+    def run(nstreams):
+        t0 = time.perf_counter()
+        ier = L.fmm2d_batch(count, jobs, nstreams)
+        dt = time.perf_counter() - t0
+        if ier != 0:
+            raise RuntimeError(f"fmm2d_batch returned {ier}")
+        return dt
+
+    for _ in range(args.warmup):
+        run(8)
+    t_seq = min(run(1) for _ in range(args.steps))
+    t_bat = min(run(8) for _ in range(args.steps))
+    print(f"bench[batch]: one stream {t_seq*1e3:.1f} ms, 8 streams {t_bat*1e3:.1f} ms "
+          f"({t_seq / t_bat:.2f}x)", file=sys.stderr, flush=True)
+    line = {
+        "metric": f"batched FMM points/sec, {count} independent lfmm2d/hfmm2d instances "
+                  "(secondary)",
+        "value": round(npts / t_bat, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_bat * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"C4 mix reduced to {count} instances: N log-uniform [1e3, 2e5], "
+                               "disk or perturbed curve, lfmm2d and hfmm2d (zk log-uniform "
+                               "[1, 300]) alternating, eps 1e-6/1e-9, pot at sources",
+                   "instances": count, "points": npts, "streams": 8},
+        "one_stream": {"value": round(npts / t_seq, 1), "ms": round(t_seq * 1e3, 3)},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main_hfmm2d(args):
     """Secondary workload (not the headline metric): hfmm2d, 1e6 sources + 1e6
     targets, zk = 100, eps = 1e-9, charges + dipoles, pot+grad at the targets,
@@ -250,8 +323,10 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run one elementwise copy kernel over MIB MiB (reads and "
                          "writes MIB MiB), the reference kernel for scaling HBM counters")
-    ap.add_argument("--workload", default="lfmm2d", choices=["lfmm2d", "hfmm2d"],
-                    help="lfmm2d: the headline workload; hfmm2d: the secondary Helmholtz one")
+    ap.add_argument("--workload", default="lfmm2d", choices=["lfmm2d", "hfmm2d", "batch"],
+                    help="lfmm2d: the headline workload; hfmm2d: the secondary Helmholtz one; "
+                         "batch: many independent small FMMs through fmm2d_batch")
+    ap.add_argument("--batch", type=int, default=256, help="instances of the batch workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -260,6 +335,8 @@ def main():
 
     if args.workload == "hfmm2d":
         return main_hfmm2d(args)
+    if args.workload == "batch":
+        return main_batch(args)
     import ctypes as C
     import torch
     from fmm2d_amd import _native as nat
