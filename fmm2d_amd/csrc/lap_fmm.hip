@@ -1422,13 +1422,13 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
           tm.mark(3);  // planning
           launch_translate(0, 1, nd, TrBufs{mp32.p, 0, P, loc.p, 0, P}, kSlots, groups,
                            m2l.tlist.p, L.s2.p, d_soff.p, ops.af32, t.ilevel.p, d_band.p,
-                           d_logr.p, st, true, true);
+                           d_logr.p, st, true, true, true);
           tm.mark(13);  // the M2L kernel alone
         } else {
           tm.mark(3);  // planning
           launch_translate(0, 1, nd, TrBufs{mp.p, 0, P, loc.p, 0, P}, kSlots, groups,
                            m2l.tlist.p, L.s2.p, d_soff.p, d_af, t.ilevel.p, d_band.p,
-                           d_logr.p, st, false, true);
+                           d_logr.p, st, false, true, true);
           tm.mark(13);  // the M2L kernel alone
         }
       }

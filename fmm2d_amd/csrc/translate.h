@@ -33,9 +33,16 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 // Group table of one launch, passed by value: group g owns targets
 // tlist[seg0[g]..seg1[g]) and tiles [tile0[g], tile0[g+1]).
 constexpr int kTrMaxGroups = 8;
+// Groups of one launch.  Wave tiles: the first ilv tiles of every group go
+// round-robin over the groups (tile t of group g is wave t * ng + g), the
+// rest group after group from ng * ilv on (group g's from rem0[g]); ilv = 0
+// is group after group throughout.  Interleaving the M2L classes (the four
+// child positions) puts siblings, whose list-2 sources are mostly the same
+// boxes, into neighbouring waves of one workgroup.
 struct TrTable {
-  int ng;
+  int ng, ilv;
   int tile0[kTrMaxGroups + 1];
+  int rem0[kTrMaxGroups + 1];
   int seg0[kTrMaxGroups], seg1[kTrMaxGroups];
   int opbase[kTrMaxGroups], cls[kTrMaxGroups];
 };
@@ -93,9 +100,17 @@ __global__ __launch_bounds__(256) void k_translate(
   const int64_t wave = wv / nrc;
   const int rc0 = (int)(wv - wave * nrc) * RT;
   int g = 0;
-  while (g + 1 < tab.ng && wave >= tab.tile0[g + 1]) ++g;
+  int64_t tl;  // tile within group g
+  if (wave < (int64_t)tab.ilv * tab.ng) {
+    g = (int)(wave % tab.ng);
+    tl = wave / tab.ng;
+  } else {
+    const int64_t w2 = wave - (int64_t)tab.ilv * tab.ng;
+    while (g + 1 < tab.ng && w2 >= tab.rem0[g + 1]) ++g;
+    tl = tab.ilv + (w2 - tab.rem0[g]);
+  }
   const int q = tab.cls[g];
-  const int64_t col = (wave - tab.tile0[g]) * 16 + (lane & 15);
+  const int64_t col = tl * 16 + (lane & 15);
   const int64_t tpos = tab.seg0[g] + col / nd;
   const int d = (int)(col % nd);
   const bool valid = tpos < tab.seg1[g];
@@ -262,7 +277,8 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
                              const std::vector<TrGroup>& groups, const int* tlist,
                              const int* slots, const int* slot_op, const void* Afrag,
                              const int* ilevel, const int2* band, const double* logr,
-                             hipStream_t st, bool fp32 = false, bool slots_by_box = false) {
+                             hipStream_t st, bool fp32 = false, bool slots_by_box = false,
+                             bool interleave = false) {
   const int Pin = io.Pin, Pout = io.Pout;
   const int64_t afs = afrag_stride(Pin, Pout);
   const int RTT = (2 * Pout + 15) / 16, KS = (2 * Pin + 3) / 4;
@@ -284,6 +300,14 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     }
     if (tab.ng == 0) continue;
     tab.tile0[tab.ng] = tiles;
+    tab.ilv = 0;
+    if (interleave) {
+      tab.ilv = tiles;
+      for (int g = 0; g < tab.ng; ++g) tab.ilv = std::min(tab.ilv, tab.tile0[g + 1] - tab.tile0[g]);
+    }
+    tab.rem0[0] = 0;
+    for (int g = 0; g < tab.ng; ++g)
+      tab.rem0[g + 1] = tab.rem0[g] + (tab.tile0[g + 1] - tab.tile0[g]) - tab.ilv;
 #define FMM2D_TR(T, RT, KSU, MODE, LOGT)                                                        \
   {                                                                                             \
     const int64_t nw = (int64_t)tiles * (KSU > 0 ? 1 : (RTT + RT - 1) / RT);                    \
