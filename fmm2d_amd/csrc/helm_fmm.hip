@@ -1000,7 +1000,7 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
                            TrBufs{mp.p + V.off, V.box0, 2 * V.p + 1, loc.p + V.off, V.box0,
                                   2 * V.p + 1},
                            kSlots, groups, m2l.tlist.p, L.s2.p, d_soff.p, opptr(op_m2l[l]),
-                           t.ilevel.p, nullptr, nullptr, st, false, true);
+                           t.ilevel.p, nullptr, nullptr, st, false, true, true);
         }
         tm.mark(13);  // the M2L kernels alone
       }
