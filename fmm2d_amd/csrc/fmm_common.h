@@ -148,21 +148,48 @@ struct SegCursor {
 // possible offsets form a fixed set of 27 (the 6x6 children of the parent's
 // 3x3 neighbourhood minus the 3x3 adjacent ones), so targets of the same q are
 // processed together by the translate kernel, one offset slot after another.
-// class key of every box: child position (0..3) of an M2L target, 4 otherwise
-__global__ void k_m2l_class(int64_t n, const int* ilevel, const int* iparent, const int* ichild,
-                            const int* n2, const int* sse, const int* tse, int ifpgh,
-                            int ifpght, int* key, int* val) {
+__device__ __forceinline__ bool m2l_target(int b, const int* ilevel, const int* n2, const int* sse,
+                                           const int* tse, int ifpgh, int ifpght) {
+  return ilevel[b] >= 2 && n2[b] > 0 && npts_eval(b, sse, tse, ifpgh, ifpght) > 0;
+}
+
+// flag[p] = 1 if some child of box p is an M2L target
+__global__ void k_m2l_parent_flag(int64_t n, const int* ilevel, const int* nchild,
+                                  const int* ichild, const int* n2, const int* sse,
+                                  const int* tse, int ifpgh, int ifpght, int* flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int b = (int)i;
-  int k = 4;
-  if (ilevel[b] >= 2 && n2[b] > 0 && npts_eval(b, sse, tse, ifpgh, ifpght) > 0) {
-    const int dad = iparent[b];
-    k = 0;
-    while (k < 3 && ichild[4 * dad + k] != b) ++k;
+  int f = 0;
+  if (nchild[i] > 0)
+    for (int q = 0; q < 4; ++q) {
+      const int c = ichild[4 * i + q];
+      if (c >= 0 && m2l_target(c, ilevel, n2, sse, tse, ifpgh, ifpght)) f = 1;
+    }
+  flag[i] = f;
+}
+
+// *total = flags set in flag[0..n), from its exclusive scan pos
+__global__ void k_flag_total(int n, const int* flag, const int* pos, int* total) {
+  *total = (n > 0) ? pos[n - 1] + flag[n - 1] : 0;
+}
+
+__global__ void k_m2l_parent_list(int64_t n, const int* flag, const int* pos, int* plist) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) plist[pos[i]] = (int)i;
+}
+
+// tlist[q * np + k] = child q of parent plist[k] if an M2L target, else -1
+__global__ void k_m2l_aligned(int64_t np, const int* plist, const int* ilevel, const int* ichild,
+                              const int* n2, const int* sse, const int* tse, int ifpgh,
+                              int ifpght, int* tlist) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int p = plist[k];
+  for (int q = 0; q < 4; ++q) {
+    const int c = ichild[4 * p + q];
+    tlist[q * np + k] =
+        (c >= 0 && m2l_target(c, ilevel, n2, sse, tse, ifpgh, ifpght)) ? c : -1;
   }
-  key[i] = k;
-  val[i] = b;
 }
 
 // Non-leaf boxes (in box order, hence level order) and their 4-child slots for
@@ -250,20 +277,6 @@ __global__ void k_counts(int64_t n, const int* ilevel, const int* iparent, const
 }
 
 
-// seg[q] = first position of keys (sorted) at or beyond q, one thread per q
-// (launch >= 5 threads)
-__global__ void k_class_bounds(const int* keys, int n, int* seg) {
-  const int q = threadIdx.x;
-  if (q > 4) return;
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) / 2;
-    if (keys[mid] < q) lo = mid + 1;
-    else hi = mid;
-  }
-  seg[q] = lo;
-}
-
 template <typename T> DBuf<T> upload(const std::vector<T>& v, hipStream_t st) {
   DBuf<T> d(v.size(), st);
   if (!v.empty())
@@ -346,27 +359,17 @@ void plan_children(const Tree& t, int pg, int pgt, ChildPlan& cp, hipStream_t st
 }
 
 // seg[q * (nl + 2) + l]: first position in tlist of class q at level >= l
-// one thread per (class q, level l) (launch >= 4 (nl + 2) threads)
-__global__ void k_class_level_bounds(const int* tlist, const int* cseg, const int* lstart,
-                                     int nl, int* seg) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 4 * (nl + 2)) return;
-  const int q = t / (nl + 2), l = t % (nl + 2);
-  int lo = cseg[q], hi = cseg[q + 1];
-  while (lo < hi) {
-    const int mid = (lo + hi) / 2;
-    if (tlist[mid] < lstart[l]) lo = mid + 1;
-    else hi = mid;
-  }
-  seg[t] = lo;
-}
-
-// M2L targets (level >= 2, non-empty list 2, points to evaluate) sorted by
-// child position q, box order within a class (their sources per offset slot
-// are Lists::s2).  cls_seg[q] .. cls_seg[q+1] is class q, and
-// lev_seg[q][l] .. lev_seg[q][l+1] its part at level l.
+// M2L targets (level >= 2, non-empty list 2, points to evaluate) by child
+// position q (their sources per offset slot are Lists::s2).  The classes are
+// aligned by parent: entry k of class q is the q-th child of the k-th parent
+// with an M2L target among its children (parents in box order), or -1 when
+// that child is no target -- so sibling targets, whose list-2 sources are
+// mostly the same boxes, sit at the same position of the four classes (few
+// entries are -1: a refined box's children nearly always all hold points).
+// cls_seg[q] .. cls_seg[q+1] is class q, and lev_seg[q][l] .. lev_seg[q][l+1]
+// its part at level l.
 struct M2LPlan {
-  DBuf<int> tlist, skey;
+  DBuf<int> tlist;
   int cls_seg[5] = {0, 0, 0, 0, 0};
   std::vector<int> lev_seg;  // 4 x (nl + 2)
   int ntg = 0;
@@ -375,32 +378,41 @@ struct M2LPlan {
 
 void plan_m2l(const Tree& t, const Lists& L, int pg, int pgt, M2LPlan& mp, hipStream_t st) {
   const int nb = t.nboxes, nl = t.nlevels;
-  DBuf<int> key(nb, st), val(nb, st), seg(5, st);
-  mp.skey.alloc(nb, st);
-  mp.tlist.alloc(nb, st);
-  launch1d(k_m2l_class, nb, st, t.ilevel.p, t.iparent.p, t.ichild.p, L.n2.p, t.isrcse.p,
-           t.itargse.p, pg, pgt, key.p, val.p);
+  DBuf<int> flag(nb, st), pos(nb + 1, st), cnt(1, st);
+  launch1d(k_m2l_parent_flag, nb, st, t.ilevel.p, t.nchild.p, t.ichild.p, L.n2.p, t.isrcse.p,
+           t.itargse.p, pg, pgt, flag.p);
   size_t bytes = 0;
-  hip_throw(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key.p, mp.skey.p, val.p,
-                                               mp.tlist.p, nb, 0, 3, st), "sort");
+  hip_throw(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag.p, pos.p, nb, st), "scan");
   DBuf<char> tmp(bytes, st);
-  hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, key.p, mp.skey.p, val.p,
-                                               mp.tlist.p, nb, 0, 3, st), "sort");
-  hipLaunchKernelGGL(k_class_bounds, dim3(1), dim3(64), 0, st, mp.skey.p, nb, seg.p);
-  std::vector<int> lsx(t.lstart);
-  lsx.push_back(nb);
-  DBuf<int> d_lsx = upload(lsx, st);
-  DBuf<int> dls(4 * (nl + 2), st);
-  if (4 * (nl + 2) > 256) throw HipError{hipErrorInvalidValue, "tree levels beyond 62"};
-  hipLaunchKernelGGL(k_class_level_bounds, dim3(1), dim3(256), 0, st, mp.tlist.p, seg.p,
-                     d_lsx.p, nl, dls.p);
-  mp.lev_seg.assign(4 * (size_t)(nl + 2), 0);
-  hip_throw(hipMemcpyAsync(mp.cls_seg, seg.p, sizeof(mp.cls_seg), hipMemcpyDeviceToHost, st),
-            "copy");
-  hip_throw(hipMemcpyAsync(mp.lev_seg.data(), dls.p, sizeof(int) * mp.lev_seg.size(),
-                           hipMemcpyDeviceToHost, st), "copy");
+  hip_throw(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, flag.p, pos.p, nb, st), "scan");
+  hipLaunchKernelGGL(k_flag_total, dim3(1), dim3(1), 0, st, nb, flag.p, pos.p, cnt.p);
+  int np = 0;
+  hip_throw(hipMemcpyAsync(&np, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
   hip_throw(hipStreamSynchronize(st), "sync");
-  mp.ntg = mp.cls_seg[4];
+  DBuf<int> plist(std::max(np, 1), st);
+  launch1d(k_m2l_parent_list, nb, st, flag.p, pos.p, plist.p);
+  mp.tlist.alloc(4 * (size_t)std::max(np, 1), st);
+  launch1d(k_m2l_aligned, np, st, plist.p, t.ilevel.p, t.ichild.p, L.n2.p, t.isrcse.p,
+           t.itargse.p, pg, pgt, mp.tlist.p);
+  // targets of level l are children of parents of level l - 1: their rows
+  // start at the first parent at or beyond lstart[l - 1]
+  std::vector<int> lsx(nl + 2, 0);
+  for (int l = 1; l <= nl + 1; ++l) lsx[l] = (l - 1 <= nl) ? t.lstart[l - 1] : nb;
+  lsx[0] = 0;
+  DBuf<int> d_lsx = upload(lsx, st);
+  DBuf<int> dlb(nl + 2, st);
+  if (nl + 2 > 64) throw HipError{hipErrorInvalidValue, "tree levels beyond 62"};
+  hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(64), 0, st, plist.p, np, d_lsx.p, nl,
+                     dlb.p);
+  std::vector<int> lb(nl + 2, 0);
+  hip_throw(hipMemcpyAsync(lb.data(), dlb.p, sizeof(int) * lb.size(), hipMemcpyDeviceToHost,
+                           st), "copy");
+  hip_throw(hipStreamSynchronize(st), "sync");
+  for (int q = 0; q <= 4; ++q) mp.cls_seg[q] = q * np;
+  mp.lev_seg.assign(4 * (size_t)(nl + 2), 0);
+  for (int q = 0; q < 4; ++q)
+    for (int l = 0; l <= nl + 1; ++l) mp.lev_seg[(size_t)q * (nl + 2) + l] = q * np + lb[l];
+  mp.ntg = 4 * np;
 }
 
 // slot -> offset index (dx+3)*7 + (dy+3) for each child position q

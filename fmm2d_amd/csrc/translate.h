@@ -113,8 +113,9 @@ __global__ __launch_bounds__(256) void k_translate(
   const int64_t col = tl * 16 + (lane & 15);
   const int64_t tpos = tab.seg0[g] + col / nd;
   const int d = (int)(col % nd);
-  const bool valid = tpos < tab.seg1[g];
-  const int tb = valid ? tlist[tpos] : -1;
+  // a target list entry may be -1 (no target at that position)
+  const int tb = tpos < tab.seg1[g] ? tlist[tpos] : -1;
+  const bool valid = tb >= 0;
   // slot row of this column's target: by position in tlist, or by box id
   const int64_t srow = slots_by_box ? (int64_t)tb : tpos;
   const int kr = lane >> 4;
