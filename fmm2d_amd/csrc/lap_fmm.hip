@@ -557,6 +557,8 @@ __device__ __forceinline__ void loc_eval(double x, double y, int b, int nd, int 
 }
 
 // Thread per point (any nd): local expansion, then the point's list-3 boxes.
+// The evaluation kernels are the first to write the sorted outputs: they
+// assign, and the near field adds after them.
 template <int PG>
 __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* leaf,
                        const int* ilevel, const int* nterms, const double* rsc,
@@ -573,9 +575,9 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
     for (int k3 = 0; k3 < c3; ++k3)
       mp_eval<PG>(x, y, l3[(int64_t)kMaxList3 * b + k3], nd, d, P, ilevel, nterms, rsc, centers,
                   mp, ap, ag, ah);
-    pot[i * nd + d] += ap;
-    if (PG >= 2) grad[i * nd + d] += ag;
-    if (PG >= 3) hess[i * nd + d] += ah;
+    pot[i * nd + d] = ap;
+    if (PG >= 2) grad[i * nd + d] = ag;
+    if (PG >= 3) hess[i * nd + d] = ah;
   }
 }
 
@@ -642,9 +644,9 @@ __global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double*
   if (!live) return;
   cplx ap, ag, ah;
   loc_eval<PG>(x, y, b, 1, 0, P, ilevel, nterms, rsc, centers, loc, ap, ag, ah);
-  pot[i] += ap + cplx{acc[0][tid], acc[1][tid]};
-  if (PG >= 2) grad[i] += ag + cplx{acc[2][tid], acc[3][tid]};
-  if (PG >= 3) hess[i] += ah + cplx{acc[4][tid], acc[5][tid]};
+  pot[i] = ap + cplx{acc[0][tid], acc[1][tid]};
+  if (PG >= 2) grad[i] = ag + cplx{acc[2][tid], acc[3][tid]};
+  if (PG >= 3) hess[i] = ah + cplx{acc[4][tid], acc[5][tid]};
 }
 
 // ---------------------------------------------------------- near field
@@ -1289,7 +1291,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
         hs(pg >= 3 ? ns * nd : 1, st);
     DBuf<cplx> pt(pgt >= 1 ? nt * nd : 1, st), gt(pgt >= 2 ? nt * nd : 1, st),
         ht(pgt >= 3 ? nt * nd : 1, st);
-    ps.zero(); gs.zero(); hs.zero(); pt.zero(); gt.zero(); ht.zero();
+    // (the outputs need no clearing: list 3 + L2P writes every point first)
     tm.mark(9);
 
     const int* sse = t.isrcse.p;
