@@ -239,3 +239,37 @@ def test_mixed_precision_vs_direct(kind, case):
         if kind == "c" and k in ("pot", "pottarg"):
             a, b = a.real, b.real
         assert _rel(a, b) < 1e-6, (k, _rel(a, b))
+
+
+@pytest.mark.parametrize("kind", ["c", "l"])
+@pytest.mark.parametrize("case", [(1, 0, 2, 2), (1, 1, 3, 3), (0, 1, 2, 0)])
+def test_mixed_precision_coincident_points(kind, case):
+    """Pairs the FP32 near field does not take -- exact duplicate sources,
+    sources 1e-11 apart (|z|^2 below its range), targets on or 1e-11 from a
+    source -- go through the FP64 exclusion rule and form (k_near_slow): the
+    mixed FMM matches the FP64 FMM, whose exclusion rule is the reference's."""
+    rng = np.random.default_rng(33)
+    n0, m = 20000, 60
+    base = rng.random((2, n0))
+    dup = base[:, :m].copy()                                    # exact duplicates
+    near = base[:, m:2 * m] + 1e-11 * np.array([[1.0], [0.5]])  # |z| ~ 1.1e-11
+    src = np.concatenate([base, dup, near], axis=1)
+    n = src.shape[1]
+    trg = np.concatenate([rng.random((2, 500)), base[:, 2 * m:3 * m],
+                          base[:, 3 * m:4 * m] - 1e-11], axis=1)
+    ifc, ifd, pg, pgt = case
+    ch = (rng.standard_normal(n) + (1j * rng.standard_normal(n) if kind == "l" else 0)
+          ) if ifc else None
+    ds = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) if ifd else None
+    dv = rng.standard_normal((2, n)) if (ifd and kind == "l") else None
+    args = (kind, 1e-6, None, src, ch, ds, dv, trg if pgt else None, pg, pgt, 1)
+    mixed, ier = F._fmm(*args, opts={"precision": "mixed"})
+    assert ier == 0
+    fp64, _ = F._fmm(*args)
+    assert set(mixed) == set(fp64)
+    for k in mixed:
+        a, b = mixed[k], fp64[k]
+        assert np.isfinite(a).all(), k
+        if kind == "c" and k in ("pot", "pottarg"):
+            a, b = a.real, b.real
+        assert _rel(a, b) < 1e-6, (k, _rel(a, b))
