@@ -73,6 +73,15 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int kr, int i) { return 4 * kr + i; }
 };
 
+// orders a wavefront's LDS writes before its later LDS reads and vice versa
+// (the wavefront's LDS accesses complete in order; this keeps the compiler
+// from moving them across)
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // T is the arithmetic type of the operator (Afrag) and the source
 // expansions (src, 2 Pin values of T per box and density); the results are
 // added to the FP64 destination expansions.
@@ -122,6 +131,17 @@ __global__ __launch_bounds__(256) void k_translate(
   const int RTT = (2 * Pout + 15) / 16, KS = (2 * Pin + 3) / 4;
   const int2 full_in = int2{0, 2 * Pin}, full_out = int2{0, 2 * Pout};
   const int ob = tab.opbase[g];
+  // FP32 gather mode with 8 k steps (rows of 32 coefficients): per slot the
+  // wavefront first stages the 16 source rows of its columns in LDS
+  // (consecutive lanes load consecutive coefficients of a row, two rows per
+  // load instead of sixteen), then reads the B operands from there.  Measured
+  // slower, and so not used, for 12 k steps (rows straddle the loads) and for
+  // FP64 (twice the LDS, lower occupancy).
+  constexpr bool STAGE = MODE == 0 && KSU == 8 && sizeof(T) == 4;
+  constexpr int SW = STAGE ? 4 * KSU : 1;  // staged coefficients per row
+  __shared__ T sbuf[STAGE ? 4 : 1][STAGE ? 16 * SW : 1];
+  __shared__ int4 sinfo[STAGE ? 4 : 1][16];
+  const int wib = STAGE ? (int)(threadIdx.x >> 6) : 0;
   for (int rc = rc0; rc < (KSU > 0 ? RTT : rc0 + 1); rc += RT) {
     if (MODE == 0) {
       acc_t acc[RT];
@@ -139,9 +159,24 @@ __global__ __launch_bounds__(256) void k_translate(
           klo = bd.x;
           khi = bd.y;
         }
+        if (STAGE) {
+          if (kr == 0) sinfo[wib][lane] = int4{c, klo, c >= 0 ? khi : 0, d};
+          wave_lds_fence();
+#pragma unroll
+          for (int e0 = 0; e0 < 16 * SW; e0 += 64) {
+            const int e = e0 + lane, row = e / SW, k = e - row * SW;
+            const int4 in = sinfo[wib][row];
+            T v = T(0);
+            if (k >= in.y && k < in.z)
+              v = src[((int64_t)(in.x - sbox0) * nd + in.w) * 2 * Pin + k];
+            sbuf[wib][e] = v;
+          }
+          wave_lds_fence();
+        }
         auto step = [&](int ks) {
           const int k = 4 * ks + kr;
-          const T b = (k >= klo && k < khi) ? M[k] : T(0);
+          const T b = STAGE ? sbuf[wib][(lane & 15) * SW + k]
+                            : ((k >= klo && k < khi) ? M[k] : T(0));
           if (LOGT && rc == 0 && ks == 0 && lane < 32) m0 += b;
 #pragma unroll
           for (int r = 0; r < RT; ++r)
@@ -155,6 +190,7 @@ __global__ __launch_bounds__(256) void k_translate(
         } else {
           for (int ks = 0; ks < KS; ++ks) step(ks);
         }
+        if (STAGE) wave_lds_fence();  // the next slot overwrites the rows
       }
       if (valid) {
         const int2 bd = band ? band[ilevel[tb]] : full_out;
