@@ -759,9 +759,14 @@ void launch_hnear(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd, c
 }
 
 // LDS batch for P2M/P2L: (p + 2) cylinder values and a unit phase per source
+// sources per LDS batch of the P2M / list-4 kernels (one wavefront per box):
+// 16 keep a workgroup's LDS near 11 KB at order 40, so that ~14 wavefronts
+// fit on a CU (64 took ~32 KB and left 5); a leaf holds few sources, and
+// more take several batches
+constexpr int kHBatch = 16;
 int lds_batch(int p) {
   const int per = (p + 3) * (int)sizeof(cplx);
-  return std::max(1, std::min(kW, 32768 / per));
+  return std::max(1, std::min(kHBatch, 32768 / per));
 }
 
 }  // namespace
