@@ -208,37 +208,42 @@ __device__ void accumulate_batch(int nd, int p, cplx zk, double s, const cplx* G
   }
 }
 
+// P2M in two steps.  First, one thread per source: the row of scaled J_n(k r)
+// / s^n, n = 0..p+1, about the centre of its leaf (its level's p and s) and
+// e^{-i a}, into G (rows of pstride values) and U; the rows depend only on the
+// source, and a thread per source keeps every lane busy (a wavefront per leaf
+// had one lane per source of a leaf of ~10).
+__global__ void k_hp2m_rows(int64_t n, cplx zk, const HLev* lev, const int* leaf,
+                            const int* ilevel, const double* centers, const double* xs,
+                            int pstride, cplx* G, cplx* U) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = leaf[i];
+  const int l = ilevel[b];
+  const HLev L = lev[l];
+  if (l < 2 || !L.on) return;
+  const double dx = xs[2 * i] - centers[2 * b], dy = xs[2 * i + 1] - centers[2 * b + 1];
+  const double r = hypot(dx, dy);
+  scaled_j(zk * r, L.s, L.p + 1, G + i * pstride);
+  U[i] = (r > 0.0) ? cplx{dx / r, -dy / r} : mkc(1.0);
+}
+
+// Then one wavefront per leaf sums its sources' rows into the multipole
+// (lane n: coefficients n and -n).
 template <int IFC, int IFD>
-__global__ __launch_bounds__(kW) void k_hp2m(int nd, cplx zk, const HLev* lev, int batch,
-                                             const int* ilevel, const int* nchild,
-                                             const double* centers, const int* sse,
-                                             const double* xs, const cplx* chs, const cplx* dps,
-                                             const double* dvs, cplx* mp) {
-  extern __shared__ cplx sh[];
+__global__ __launch_bounds__(kW) void k_hp2m_g(int nd, cplx zk, const HLev* lev,
+                                               const int* ilevel, const int* nchild,
+                                               const int* sse, int pstride, const cplx* G,
+                                               const cplx* U, const cplx* chs, const cplx* dps,
+                                               const double* dvs, cplx* mp) {
   const int b = blockIdx.x;
   const int l = ilevel[b];
   const HLev L = lev[l];
   if (l < 2 || !L.on || nchild[b] != 0) return;
   const int s0 = sse[2 * b], s1 = sse[2 * b + 1];
   if (s1 <= s0) return;
-  const int p = L.p, stride = p + 2;
-  cplx* G = sh;
-  cplx* U = sh + (int64_t)batch * stride;
-  const double cx = centers[2 * b], cy = centers[2 * b + 1];
-  cplx* out = hexp(mp, L, b, nd, 0);
-  for (int j0 = s0; j0 < s1; j0 += batch) {
-    const int nbat = min(batch, s1 - j0);
-    for (int jj = threadIdx.x; jj < nbat; jj += kW) {
-      const int j = j0 + jj;
-      const double dx = xs[2 * j] - cx, dy = xs[2 * j + 1] - cy;
-      const double r = hypot(dx, dy);
-      scaled_j(zk * r, L.s, p + 1, G + (int64_t)jj * stride);
-      U[jj] = (r > 0.0) ? cplx{dx / r, -dy / r} : mkc(1.0);
-    }
-    __syncthreads();
-    accumulate_batch<IFC, IFD, 0>(nd, p, zk, L.s, G, U, nbat, stride, j0, chs, dps, dvs, out);
-    __syncthreads();
-  }
+  accumulate_batch<IFC, IFD, 0>(nd, L.p, zk, L.s, G + (int64_t)s0 * pstride, U + s0, s1 - s0,
+                                pstride, s0, chs, dps, dvs, hexp(mp, L, b, nd, 0));
 }
 
 template <int IFC, int IFD>
@@ -865,14 +870,20 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     for (int v : nterms) pmax = std::max(pmax, v);
     const int batch = lds_batch(pmax);
     const size_t lds = (size_t)batch * (pmax + 3) * sizeof(cplx);
-    // ---- step 1: P2M
+    // ---- step 1: P2M (rows of the sources, then their sums per leaf)
+    {
+      const int pstride = pmax + 2;
+      DBuf<cplx> gJ((size_t)ns * pstride, st), uJ(ns, st);
+      launch1d(k_hp2m_rows, ns, st, zk, d_lev.p, leaf_s.p, t.ilevel.p, t.centers.p, xs.p,
+               pstride, gJ.p, uJ.p);
 #define FMM2D_P2M(C, D)                                                                       \
-  hipLaunchKernelGGL((k_hp2m<C, D>), dim3(nb), dim3(kW), lds, st, nd, zk, d_lev.p, batch,     \
-                     t.ilevel.p, t.nchild.p, t.centers.p, sse, xs.p, chs.p, dps.p, dvs.p, mp.p)
-    if (a.ifcharge && !a.ifdipole) FMM2D_P2M(1, 0);
-    else if (!a.ifcharge && a.ifdipole) FMM2D_P2M(0, 1);
-    else FMM2D_P2M(1, 1);
+  hipLaunchKernelGGL((k_hp2m_g<C, D>), dim3(nb), dim3(kW), 0, st, nd, zk, d_lev.p, t.ilevel.p, \
+                     t.nchild.p, sse, pstride, gJ.p, uJ.p, chs.p, dps.p, dvs.p, mp.p)
+      if (a.ifcharge && !a.ifdipole) FMM2D_P2M(1, 0);
+      else if (!a.ifcharge && a.ifdipole) FMM2D_P2M(0, 1);
+      else FMM2D_P2M(1, 1);
 #undef FMM2D_P2M
+    }
     hip_throw(hipGetLastError(), "p2m");
     tm.mark(0);
     // ---- step 2: list 4
