@@ -152,9 +152,10 @@ __global__ void k_gather_dipvec(int64_t n, int nd, const int* perm, const double
 }
 
 // ---------------------------------------------------------- P2M / P2L
-// One 64-lane wavefront per box.  Sources are taken in batches: lane j
-// computes the cylinder functions of source j into LDS (J/s^n for P2M, H s^n
-// for P2L), then lane n adds the batch's contribution to coefficients +-n:
+// The cylinder functions of each source (J/s^n about its leaf's centre for
+// P2M, H s^n about the target box's centre for P2L) are computed first, one
+// thread per source or (box, source) pair, into rows G; then one 64-lane
+// wavefront per box sums its rows, lane n adding to coefficients +-n:
 //   charge  C_n += q G_n e^{-i n a},  C_{-n} += q (-1)^n G_n e^{i n a}
 //   dipole  C_n += q (k/2) [(v1 - i v2) g_{n-1} f_lo - (v1 + i v2) g_{n+1} f_hi]
 //           (g_j = G_|j| sign(j)^j e^{-i j a}; f_lo, f_hi powers of s from the
@@ -246,40 +247,78 @@ __global__ __launch_bounds__(kW) void k_hp2m_g(int nd, cplx zk, const HLev* lev,
                                 pstride, s0, chs, dps, dvs, hexp(mp, L, b, nd, 0));
 }
 
-template <int IFC, int IFD>
-__global__ __launch_bounds__(kW) void k_hp2l(int nd, cplx zk, const HLev* lev, int batch,
-                                             const int* ilevel, const double* centers,
-                                             const int* n4, const int* l4, const int* sse,
-                                             const int* tse, int ifpgh, int ifpght,
-                                             const double* xs, const cplx* chs, const cplx* dps,
-                                             const double* dvs, cplx* loc) {
-  extern __shared__ cplx sh[];
-  const int b = blockIdx.x;
+// List 4 in three steps, like P2M.  cnt[b] = sources of the list-4 boxes of
+// a box b that takes list-4 work, 0 otherwise.
+__global__ void k_hp2l_count(int64_t nb, const HLev* lev, const int* ilevel, const int* n4,
+                             const int* l4, const int* sse, const int* tse, int ifpgh,
+                             int ifpght, int* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int b = (int)i;
   const int l = ilevel[b];
-  const HLev L = lev[l];
-  const int cnt = n4[b];
-  if (l < 2 || !L.on || cnt == 0 || npts_eval(b, sse, tse, ifpgh, ifpght) == 0) return;
-  const int p = L.p, stride = p + 2;
-  cplx* G = sh;
-  cplx* U = sh + (int64_t)batch * stride;
-  const double cx = centers[2 * b], cy = centers[2 * b + 1];
+  int s = 0;
+  if (l >= 2 && lev[l].on && npts_eval(b, sse, tse, ifpgh, ifpght) > 0)
+    for (int k = 0; k < n4[b]; ++k) {
+      const int c = l4[(int64_t)kMaxList4 * b + k];
+      s += sse[2 * c + 1] - sse[2 * c];
+    }
+  cnt[b] = s;
+}
+
+// One thread per (box b, list-4 source j) pair t, pairs numbered box after
+// box (off = exclusive scan of cnt) and in list order within a box: the row
+// of H_n(k r) s^n, n = 0..p+1, about b's centre and e^{-i a}, into G (rows of
+// pstride values) and U.
+__global__ void k_hp2l_rows(int64_t np, int nb, cplx zk, const HLev* lev, const int* ilevel,
+                            const double* centers, const int* l4, const int* sse,
+                            const int* off, const double* xs, int pstride, cplx* G, cplx* U) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  int lo = 0, hi = nb - 1;  // last box with off[b] <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const int b = lo;
+  int r = (int)(t - off[b]), j = 0;
+  for (int k = 0; k < kMaxList4; ++k) {
+    const int c = l4[(int64_t)kMaxList4 * b + k];
+    const int sz = sse[2 * c + 1] - sse[2 * c];
+    if (r < sz) {
+      j = sse[2 * c] + r;
+      break;
+    }
+    r -= sz;
+  }
+  const HLev L = lev[ilevel[b]];
+  const double dx = xs[2 * j] - centers[2 * b], dy = xs[2 * j + 1] - centers[2 * b + 1];
+  const double rr = hypot(dx, dy);
+  scaled_h(zk * rr, L.s, L.p + 1, G + t * pstride);
+  U[t] = cplx{dx / rr, -dy / rr};
+}
+
+// One wavefront per box with list-4 work sums its pairs' rows into its local
+// expansion, list-4 box after list-4 box (their sources are contiguous).
+template <int IFC, int IFD>
+__global__ __launch_bounds__(kW) void k_hp2l_g(int nd, cplx zk, const HLev* lev,
+                                               const int* ilevel, const int* n4, const int* l4,
+                                               const int* sse, const int* cnt, const int* off,
+                                               int pstride, const cplx* G, const cplx* U,
+                                               const cplx* chs, const cplx* dps,
+                                               const double* dvs, cplx* loc) {
+  const int b = blockIdx.x;
+  if (cnt[b] == 0) return;
+  const HLev L = lev[ilevel[b]];
   cplx* out = hexp(loc, L, b, nd, 0);
-  for (int k = 0; k < cnt; ++k) {
+  int64_t pos = off[b];
+  for (int k = 0; k < n4[b]; ++k) {
     const int c = l4[(int64_t)kMaxList4 * b + k];
     const int c0 = sse[2 * c], c1 = sse[2 * c + 1];
-    for (int j0 = c0; j0 < c1; j0 += batch) {
-      const int nbat = min(batch, c1 - j0);
-      for (int jj = threadIdx.x; jj < nbat; jj += kW) {
-        const int j = j0 + jj;
-        const double dx = xs[2 * j] - cx, dy = xs[2 * j + 1] - cy;
-        const double r = hypot(dx, dy);
-        scaled_h(zk * r, L.s, p + 1, G + (int64_t)jj * stride);
-        U[jj] = cplx{dx / r, -dy / r};
-      }
-      __syncthreads();
-      accumulate_batch<IFC, IFD, 1>(nd, p, zk, L.s, G, U, nbat, stride, j0, chs, dps, dvs, out);
-      __syncthreads();
-    }
+    if (c1 <= c0) continue;
+    accumulate_batch<IFC, IFD, 1>(nd, L.p, zk, L.s, G + pos * pstride, U + pos, c1 - c0,
+                                  pstride, c0, chs, dps, dvs, out);
+    pos += c1 - c0;
   }
 }
 
@@ -763,17 +802,6 @@ void launch_hnear(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd, c
   }
 }
 
-// LDS batch for P2M/P2L: (p + 2) cylinder values and a unit phase per source
-// sources per LDS batch of the P2M / list-4 kernels (one wavefront per box):
-// 16 keep a workgroup's LDS near 11 KB at order 40, so that ~14 wavefronts
-// fit on a CU (64 took ~32 KB and left 5); a leaf holds few sources, and
-// more take several batches
-constexpr int kHBatch = 16;
-int lds_batch(int p) {
-  const int per = (p + 3) * (int)sizeof(cplx);
-  return std::max(1, std::min(kHBatch, 32768 / per));
-}
-
 }  // namespace
 
 int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
@@ -868,8 +896,6 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     const int* tse = t.itargse.p;
     int pmax = 0;
     for (int v : nterms) pmax = std::max(pmax, v);
-    const int batch = lds_batch(pmax);
-    const size_t lds = (size_t)batch * (pmax + 3) * sizeof(cplx);
     // ---- step 1: P2M (rows of the sources, then their sums per leaf)
     {
       const int pstride = pmax + 2;
@@ -886,15 +912,34 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     }
     hip_throw(hipGetLastError(), "p2m");
     tm.mark(0);
-    // ---- step 2: list 4
+    // ---- step 2: list 4 (pair counts, rows of the pairs, their sums per box)
+    {
+      DBuf<int> cnt4(nb, st), off4(nb, st), tot4(1, st);
+      launch1d(k_hp2l_count, nb, st, d_lev.p, t.ilevel.p, L.n4.p, L.l4.p, sse, tse, pg, pgt,
+               cnt4.p);
+      size_t bytes = 0;
+      hip_throw(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt4.p, off4.p, nb, st), "scan");
+      DBuf<char> tmp(bytes, st);
+      hip_throw(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, cnt4.p, off4.p, nb, st), "scan");
+      hipLaunchKernelGGL(k_flag_total, dim3(1), dim3(1), 0, st, nb, cnt4.p, off4.p, tot4.p);
+      int np4 = 0;
+      hip_throw(hipMemcpyAsync(&np4, tot4.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
+      hip_throw(hipStreamSynchronize(st), "sync");
+      if (np4 > 0) {
+        const int pstride = pmax + 2;
+        DBuf<cplx> gH((size_t)np4 * pstride, st), uH(np4, st);
+        launch1d(k_hp2l_rows, np4, st, nb, zk, d_lev.p, t.ilevel.p, t.centers.p, L.l4.p, sse,
+                 off4.p, xs.p, pstride, gH.p, uH.p);
 #define FMM2D_P2L(C, D)                                                                       \
-  hipLaunchKernelGGL((k_hp2l<C, D>), dim3(nb), dim3(kW), lds, st, nd, zk, d_lev.p, batch,     \
-                     t.ilevel.p, t.centers.p, L.n4.p, L.l4.p, sse, tse, pg, pgt, xs.p, chs.p, \
-                     dps.p, dvs.p, loc.p)
-    if (a.ifcharge && !a.ifdipole) FMM2D_P2L(1, 0);
-    else if (!a.ifcharge && a.ifdipole) FMM2D_P2L(0, 1);
-    else FMM2D_P2L(1, 1);
+  hipLaunchKernelGGL((k_hp2l_g<C, D>), dim3(nb), dim3(kW), 0, st, nd, zk, d_lev.p, t.ilevel.p, \
+                     L.n4.p, L.l4.p, sse, cnt4.p, off4.p, pstride, gH.p, uH.p, chs.p, dps.p,  \
+                     dvs.p, loc.p)
+        if (a.ifcharge && !a.ifdipole) FMM2D_P2L(1, 0);
+        else if (!a.ifcharge && a.ifdipole) FMM2D_P2L(0, 1);
+        else FMM2D_P2L(1, 1);
 #undef FMM2D_P2L
+      }
+    }
     hip_throw(hipGetLastError(), "p2l");
     tm.mark(1);
 
