@@ -329,8 +329,9 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="instances of the batch workload")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              "decomposition is described in DESIGN.md and not measured.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); a multi-GPU "
+              "decomposition (leaf ranges in path order per GPU, halo exchange of expansions) "
+              "is not built and not measured.", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "hfmm2d":
