@@ -169,7 +169,7 @@ def c4_problems(count, seed=4):
 
 def main_batch(args):
     """Batched workload (secondary): the C4 mix of independent lfmm2d and
-    hfmm2d problems through fmm2d_batch on 8 streams, against the same
+    hfmm2d problems through fmm2d_batch on --streams streams, against the same
     problems one after another on one stream.  Rate = points per second of
     the whole batch, inputs resident on the device."""
     import torch
@@ -192,11 +192,12 @@ def main_batch(args):
             raise RuntimeError(f"fmm2d_batch returned {ier}")
         return dt
 
+    ns_ = args.streams
     for _ in range(args.warmup):
-        run(8)
+        run(ns_)
     t_seq = min(run(1) for _ in range(args.steps))
-    t_bat = min(run(8) for _ in range(args.steps))
-    print(f"bench[batch]: one stream {t_seq*1e3:.1f} ms, 8 streams {t_bat*1e3:.1f} ms "
+    t_bat = min(run(ns_) for _ in range(args.steps))
+    print(f"bench[batch]: one stream {t_seq*1e3:.1f} ms, {ns_} streams {t_bat*1e3:.1f} ms "
           f"({t_seq / t_bat:.2f}x)", file=sys.stderr, flush=True)
     line = {
         "metric": f"batched FMM points/sec, {count} independent lfmm2d/hfmm2d instances "
@@ -207,7 +208,7 @@ def main_batch(args):
         "config": {"workload": f"C4 mix reduced to {count} instances: N log-uniform [1e3, 2e5], "
                                "disk or perturbed curve, lfmm2d and hfmm2d (zk log-uniform "
                                "[1, 300]) alternating, eps 1e-6/1e-9, pot at sources",
-                   "instances": count, "points": npts, "streams": 8},
+                   "instances": count, "points": npts, "streams": ns_},
         "one_stream": {"value": round(npts / t_seq, 1), "ms": round(t_seq * 1e3, 3)},
     }
     print(json.dumps(line), flush=True)
@@ -327,6 +328,7 @@ def main():
                     help="lfmm2d: the headline workload; hfmm2d: the secondary Helmholtz one; "
                          "batch: many independent small FMMs through fmm2d_batch")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batch workload")
+    ap.add_argument("--streams", type=int, default=8, help="streams of the batch workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); a multi-GPU "
