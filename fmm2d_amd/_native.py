@@ -30,10 +30,13 @@ SIGNATURES["fmm2d_tree_build"] = (_i, [_vp, _i64, _vp, _i64, _i, _i, _i, C.POINT
                                          C.POINTER(_i), C.POINTER(_vp), _vp])
 SIGNATURES["fmm2d_tree_get"] = (_i, [_vp] * 22)
 SIGNATURES["fmm2d_tree_free"] = (None, [_vp])
+SIGNATURES["fmm2d_tree_leaf_histogram"] = (_i, [_vp, _i64, _vp, _i64, _i, _i, _i, _vp, _vp,
+                                                _vp])
 SIGNATURES["fmm2d_l2dterms"] = (_i, [_d, C.POINTER(_i)])
 SIGNATURES["fmm2d_h2dterms"] = (_i, [_d, _d, _d, _d, C.POINTER(_i)])
 SIGNATURES["fmm2d_ndiv"] = (_i, [_d, _i64, _i64, _i, C.POINTER(_i), C.POINTER(_i)])
 SIGNATURES["fmm2d_release_workspace"] = (_i, [])
+SIGNATURES["fmm2d_set_workspace_limit"] = (_i, [_i64])
 SIGNATURES["fmm2d_rfmm2d"] = (_i, [_i, _d, _i64, _vp, _i, _vp, _i, _vp, _vp, _i, _i, _vp, _vp,
                                    _vp, _i64, _vp, _i, _vp, _vp, _vp, _vp, _vp])
 SIGNATURES["fmm2d_r2d_direct"] = (_i, [_i, _vp, _i64, _i, _vp, _i, _vp, _vp, _vp, _i64, _i,

@@ -190,10 +190,39 @@ int fmm2d_tree_build(const double* sources, int64_t ns, const double* targ, int6
   } catch (const HipError& e) {
     delete h;
     return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    delete h;
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    delete h;
+    return FMM2D_ERR_HIP;
   }
   if (nlevels) *nlevels = h->t.nlevels;
   if (nboxes) *nboxes = h->t.nboxes;
   *handle = h;
+  return FMM2D_OK;
+}
+
+int fmm2d_tree_leaf_histogram(const double* sources, int64_t ns, const double* targ, int64_t nt,
+                              int idivflag, int ndiv, int nbins, double* hist, double* boxsize0,
+                              void* stream) {
+  if (!sources || ns < 1 || nt < 0 || (nt > 0 && !targ) || ndiv < 1 || idivflag < 0 ||
+      idivflag > 2 || nbins < 1 || nbins > kLeafHistMax || !hist)
+    return FMM2D_ERR_ARG;
+  try {
+    keep_pool_memory();
+    const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    Tree t;
+    build_tree(sources, ns, targ, nt, ndiv, idivflag, 51, t, st);
+    leaf_level_histogram(t, nbins, hist, st);
+    if (boxsize0) *boxsize0 = t.boxsize[0];
+  } catch (const HipError& e) {
+    return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    return FMM2D_ERR_HIP;
+  }
   return FMM2D_OK;
 }
 
@@ -261,7 +290,11 @@ int fmm2d_tree_get(void* handle, int* laddr, int* ilevel, int* iparent, int* nch
         }
       }
     }
-  } catch (const HipError& e) {
+  } catch (const HipError&) {
+    return FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
     return FMM2D_ERR_HIP;
   }
   return FMM2D_OK;
@@ -297,6 +330,12 @@ int fmm2d_ndiv(double eps, int64_t ns, int64_t nt, int helmholtz, int* ndiv, int
 
 int fmm2d_release_workspace(void) { return ws_release(); }
 
+int fmm2d_set_workspace_limit(int64_t bytes) {
+  if (bytes < 0) return FMM2D_ERR_ARG;
+  ws_set_limit((size_t)bytes);
+  return FMM2D_OK;
+}
+
 int fmm2d_batch(int njobs, fmm2d_job* jobs, int nstreams) {
   if (njobs < 0 || (njobs > 0 && !jobs) || nstreams < 1 || nstreams > 16) return FMM2D_ERR_ARG;
   if (njobs == 0) return FMM2D_OK;
@@ -319,7 +358,13 @@ int fmm2d_batch(int njobs, fmm2d_job* jobs, int nstreams) {
     (void)hipSetDevice(dev);
     for (int k; (k = next.fetch_add(1)) < njobs;) {
       fmm2d_job& j = jobs[order[k]];
-      j.ier = run_job(j, streams[w]);
+      try {
+        j.ier = run_job(j, streams[w]);
+      } catch (const std::bad_alloc&) {
+        j.ier = FMM2D_ERR_ALLOC;
+      } catch (...) {
+        j.ier = FMM2D_ERR_HIP;
+      }
     }
     if (hipStreamSynchronize(streams[w]) != hipSuccess) sync_err[w] = 1;
   };

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <new>
 #include <cmath>
 
 #define FMM2D_HD __host__ __device__ __forceinline__
@@ -71,6 +72,22 @@ enum {
   FMM2D_ERR_HIP = -2,
   FMM2D_ERR_UNSUPPORTED = -3,
 };
+
+// Bounds of the per-level expansion orders a caller may pass in
+// fmm2d_opts.nterms (the reference's rules give at most ~50 for Laplace at
+// eps = 1e-16; Helmholtz orders grow with zk times the box size).
+constexpr int kMaxNtermsLaplace = 200;
+constexpr int kMaxNtermsHelmholtz = 2000;
+
+// Checks fmm2d_opts.nterms before any device work: FMM2D_ERR_ARG for a
+// negative count or a missing array, FMM2D_ERR_NTERMS for an order outside
+// [lo, hi], else FMM2D_OK.
+static inline int check_nterms_override(int n, const int32_t* v, int lo, int hi) {
+  if (n < 0 || (n > 0 && !v)) return FMM2D_ERR_ARG;
+  for (int i = 0; i < n; ++i)
+    if (v[i] < lo || v[i] > hi) return FMM2D_ERR_NTERMS;
+  return FMM2D_OK;
+}
 
 #define FMM2D_HIP_TRY(expr)                                                   \
   do {                                                                        \

@@ -26,6 +26,9 @@ void* ws_alloc(size_t bytes, hipStream_t st);
 void ws_free(void* p, hipStream_t st);
 // Free every cached block of the current device (after a device sync).
 int ws_release();
+// Cap the bytes the cache holds per device (cached + in use); 0 = no cap.  A
+// request beyond it first frees the cached blocks, then fails as out of memory.
+void ws_set_limit(size_t bytes);
 
 template <typename T> struct DBuf {
   T* p = nullptr;

@@ -15,11 +15,17 @@
 //   |z| >= 17          Hankel's asymptotic expansion (A&S 9.2.7), summed until
 //                      the terms fall below 1e-17 relative.
 //
-// Accuracy (checked against mpmath): <= 3e-15 relative on the closed upper
-// half-plane, which covers every Im(zk) >= 0.  Below the real axis the rule
-// is used for arg z >= -0.3; deeper in the lower half-plane (exponentially
-// growing kernels) the series (|z| < 8) or the asymptotic sum is used, with
-// lower accuracy for 8 <= |z| < 17.
+// Below the real axis (Im zk < 0: exponentially growing kernels) the rule
+// above is used for arg z >= -0.3; deeper in the fourth quadrant the series
+// (|z| < 5), Hankel's integral along the ray arg t = -pi/4, which keeps the
+// integrand's branch point t = 2iz away from the path (|z| < 17), or the
+// asymptotic expansion; the third quadrant by the continuation formula
+// (A&S 9.1.36 with J = (H1 + H2)/2 and H2(w) = conj H1(conj w))
+//   H_n(z) = (-1)^n [2 H_n(-z) + conj H_n(-conj z)],
+// both of whose arguments lie in the first or the fourth quadrant.
+//
+// Accuracy (checked against mpmath): <= 5e-15 relative on the closed upper
+// half-plane, which covers every Im(zk) >= 0, and <= 1e-14 below it.
 #pragma once
 #include "common.h"
 #include "gl_tables.h"
@@ -143,9 +149,32 @@ FMM2D_HD void hank01_asymptotic(cplx z, cplx* h0, cplx* h1) {
   *h1 = p1 * s1;
 }
 
-// H0^(1)(z), H1^(1)(z).  z must be nonzero.
-FMM2D_HD void hank01(cplx z, cplx* h0, cplx* h1) {
-  double r = absc(z);
+// Hankel's integral along the ray t = s (1 - i), for the fourth quadrant below
+// arg z = -0.3: the same nodes x_k with weights w_k e^{i x_k} (1 - i)^{a+1}.
+FMM2D_HD void hank01_laguerre_rot(cplx z, cplx* h0, cplx* h1) {
+  static constexpr double gl_m_x[FMM2D_GL_N] = {FMM2D_GL_M_X};
+  static constexpr double gl_p_x[FMM2D_GL_N] = {FMM2D_GL_P_X};
+  static constexpr double gl_m_wr[FMM2D_GL_N] = {FMM2D_GL_M_WRRE};
+  static constexpr double gl_m_wi[FMM2D_GL_N] = {FMM2D_GL_M_WRIM};
+  static constexpr double gl_p_wr[FMM2D_GL_N] = {FMM2D_GL_P_WRRE};
+  static constexpr double gl_p_wi[FMM2D_GL_N] = {FMM2D_GL_P_WRIM};
+  cplx c = divc(cplx{0.0, 0.5}, z) * cplx{1.0, -1.0};  // i (1 - i) / (2z)
+  cplx s0 = mkc(0.0), s1 = mkc(0.0);
+#pragma unroll 4
+  for (int k = 0; k < FMM2D_GL_N; ++k) {
+    cplx u0 = cplx{1.0 + gl_m_x[k] * c.re, gl_m_x[k] * c.im};
+    cplx u1 = cplx{1.0 + gl_p_x[k] * c.re, gl_p_x[k] * c.im};
+    s0 += cplx{gl_m_wr[k], gl_m_wi[k]} * invc(sqrtc(u0));
+    s1 += cplx{gl_p_wr[k], gl_p_wi[k]} * sqrtc(u1);
+  }
+  cplx p = hank_prefactor(z);
+  *h0 = p * s0;
+  *h1 = cplx{p.im, -p.re} * s1;
+}
+
+// The ascending series, the quadrature and the asymptotic expansion for z in
+// the closed upper half-plane or with arg z >= -0.3.
+FMM2D_HD void hank01_upper(cplx z, double r, cplx* h0, cplx* h1) {
   // The series cancels like exp(|z| + Im z) relative to |H|: near the
   // positive imaginary axis it hands over to the quadrature already at 1.75.
   if (r < 1.75 || (r < 2.5 && fabs(z.re) >= z.im)) {
@@ -155,17 +184,50 @@ FMM2D_HD void hank01(cplx z, cplx* h0, cplx* h1) {
       hank01_series_real(z.re, h0, h1, nterm);
     else
       hank01_series(z, h0, h1, nterm);
-  } else if (z.im >= 0.0 || atan2(z.im, z.re) >= -0.3) {
-    if (r < 17.0)
-      hank01_laguerre(z, h0, h1);
-    else
-      hank01_asymptotic(z, h0, h1);
+  } else if (r < 17.0) {
+    hank01_laguerre(z, h0, h1);
   } else {
-    if (r < 8.0)
-      hank01_series(z, h0, h1, 26);  // 16^26/(26!)^2 < 1e-21
-    else
-      hank01_asymptotic(z, h0, h1);
+    hank01_asymptotic(z, h0, h1);
   }
+}
+
+// z in the fourth quadrant (Re z >= 0 > Im z).
+FMM2D_HD void hank01_quad4(cplx z, double r, cplx* h0, cplx* h1) {
+  if (atan2(z.im, z.re) >= -0.3)
+    hank01_upper(z, r, h0, h1);
+  else if (r < 5.0)
+    hank01_series(z, h0, h1, 26);  // (6.25)^26/(26!)^2 < 1e-32
+  else if (r < 17.0)
+    hank01_laguerre_rot(z, h0, h1);
+  else
+    hank01_asymptotic(z, h0, h1);
+}
+
+// Below the real axis: out of line, so that the kernels whose arguments stay
+// in the upper half-plane (every Im zk >= 0) carry only the call.
+static __host__ __device__ __noinline__ void hank01_lower(cplx z, double r, cplx* h0, cplx* h1) {
+  if (z.re >= 0.0) {
+    hank01_quad4(z, r, h0, h1);
+    return;
+  }
+  if (r < 1.75) {  // no cancellation below the axis at this size
+    hank01_series(z, h0, h1, r < 0.5 ? 8 : (r < 1.2 ? 10 : 14));
+    return;
+  }
+  cplx a0, a1, b0, b1;
+  hank01_upper(cplx{-z.re, -z.im}, r, &a0, &a1);  // first quadrant
+  hank01_quad4(cplx{-z.re, z.im}, r, &b0, &b1);   // fourth quadrant
+  *h0 = cplx{2.0 * a0.re + b0.re, 2.0 * a0.im - b0.im};
+  *h1 = cplx{-(2.0 * a1.re + b1.re), -(2.0 * a1.im - b1.im)};
+}
+
+// H0^(1)(z), H1^(1)(z).  z must be nonzero.
+FMM2D_HD void hank01(cplx z, cplx* h0, cplx* h1) {
+  double r = absc(z);
+  if (z.im >= 0.0)
+    hank01_upper(z, r, h0, h1);
+  else
+    hank01_lower(z, r, h0, h1);
 }
 
 }  // namespace fmm2d

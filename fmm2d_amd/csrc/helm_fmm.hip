@@ -41,6 +41,7 @@
 #include "helm_ops.h"
 #include "lap_ops.h"
 #include "p2p_math.h"
+#include "scan.h"
 #include "translate.h"
 #include "tree.h"
 
@@ -211,12 +212,22 @@ __device__ void accumulate_batch(int nd, int p, cplx zk, double s, const cplx* G
 
 // P2M in two steps.  First, one thread per source: the row of scaled J_n(k r)
 // / s^n, n = 0..p+1, about the centre of its leaf (its level's p and s) and
-// e^{-i a}, into G (rows of pstride values) and U; the rows depend only on the
-// source, and a thread per source keeps every lane busy (a wavefront per leaf
-// had one lane per source of a leaf of ~10).
+// e^{-i a}, into G and U; the rows depend only on the source, and a thread per
+// source keeps every lane busy (a wavefront per leaf had one lane per source of
+// a leaf of ~10).  Row i starts at G + roff[i] and holds p+2 values of its
+// leaf's level (roff = exclusive scan of k_hp2m_rowlen), so G holds exactly
+// the rows that are used, whatever the orders of the coarse levels.
+__global__ void k_hp2m_rowlen(int64_t n, const HLev* lev, const int* leaf, const int* ilevel,
+                              int64_t* len) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int l = ilevel[leaf[i]];
+  len[i] = (l >= 2 && lev[l].on) ? lev[l].p + 2 : 0;
+}
+
 __global__ void k_hp2m_rows(int64_t n, cplx zk, const HLev* lev, const int* leaf,
                             const int* ilevel, const double* centers, const double* xs,
-                            int pstride, cplx* G, cplx* U) {
+                            const int64_t* roff, cplx* G, cplx* U) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = leaf[i];
@@ -225,7 +236,7 @@ __global__ void k_hp2m_rows(int64_t n, cplx zk, const HLev* lev, const int* leaf
   if (l < 2 || !L.on) return;
   const double dx = xs[2 * i] - centers[2 * b], dy = xs[2 * i + 1] - centers[2 * b + 1];
   const double r = hypot(dx, dy);
-  scaled_j(zk * r, L.s, L.p + 1, G + i * pstride);
+  scaled_j(zk * r, L.s, L.p + 1, G + roff[i]);
   U[i] = (r > 0.0) ? cplx{dx / r, -dy / r} : mkc(1.0);
 }
 
@@ -234,17 +245,17 @@ __global__ void k_hp2m_rows(int64_t n, cplx zk, const HLev* lev, const int* leaf
 template <int IFC, int IFD>
 __global__ __launch_bounds__(kW) void k_hp2m_g(int nd, cplx zk, const HLev* lev,
                                                const int* ilevel, const int* nchild,
-                                               const int* sse, int pstride, const cplx* G,
-                                               const cplx* U, const cplx* chs, const cplx* dps,
-                                               const double* dvs, cplx* mp) {
+                                               const int* sse, const int64_t* roff,
+                                               const cplx* G, const cplx* U, const cplx* chs,
+                                               const cplx* dps, const double* dvs, cplx* mp) {
   const int b = blockIdx.x;
   const int l = ilevel[b];
   const HLev L = lev[l];
   if (l < 2 || !L.on || nchild[b] != 0) return;
   const int s0 = sse[2 * b], s1 = sse[2 * b + 1];
   if (s1 <= s0) return;
-  accumulate_batch<IFC, IFD, 0>(nd, L.p, zk, L.s, G + (int64_t)s0 * pstride, U + s0, s1 - s0,
-                                pstride, s0, chs, dps, dvs, hexp(mp, L, b, nd, 0));
+  accumulate_batch<IFC, IFD, 0>(nd, L.p, zk, L.s, G + roff[s0], U + s0, s1 - s0, L.p + 2, s0,
+                                chs, dps, dvs, hexp(mp, L, b, nd, 0));
 }
 
 // List 4 in three steps, like P2M.  cnt[b] = sources of the list-4 boxes of
@@ -265,13 +276,22 @@ __global__ void k_hp2l_count(int64_t nb, const HLev* lev, const int* ilevel, con
   cnt[b] = s;
 }
 
+// Row storage of a box's pairs: cnt[b] rows of p+2 values of b's level.
+__global__ void k_hp2l_rowlen(int64_t nb, const HLev* lev, const int* ilevel, const int* cnt,
+                              int64_t* len) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  len[b] = (int64_t)cnt[b] * (lev[ilevel[b]].p + 2);
+}
+
 // One thread per (box b, list-4 source j) pair t, pairs numbered box after
 // box (off = exclusive scan of cnt) and in list order within a box: the row
-// of H_n(k r) s^n, n = 0..p+1, about b's centre and e^{-i a}, into G (rows of
-// pstride values) and U.
+// of H_n(k r) s^n, n = 0..p+1, about b's centre and e^{-i a}, into G (b's rows
+// from G + roff[b], p+2 values each) and U.
 __global__ void k_hp2l_rows(int64_t np, int nb, cplx zk, const HLev* lev, const int* ilevel,
                             const double* centers, const int* l4, const int* sse,
-                            const int* off, const double* xs, int pstride, cplx* G, cplx* U) {
+                            const int* off, const double* xs, const int64_t* roff, cplx* G,
+                            cplx* U) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= np) return;
   int lo = 0, hi = nb - 1;  // last box with off[b] <= t
@@ -294,7 +314,7 @@ __global__ void k_hp2l_rows(int64_t np, int nb, cplx zk, const HLev* lev, const 
   const HLev L = lev[ilevel[b]];
   const double dx = xs[2 * j] - centers[2 * b], dy = xs[2 * j + 1] - centers[2 * b + 1];
   const double rr = hypot(dx, dy);
-  scaled_h(zk * rr, L.s, L.p + 1, G + t * pstride);
+  scaled_h(zk * rr, L.s, L.p + 1, G + roff[b] + (t - off[b]) * (L.p + 2));
   U[t] = cplx{dx / rr, -dy / rr};
 }
 
@@ -304,19 +324,22 @@ template <int IFC, int IFD>
 __global__ __launch_bounds__(kW) void k_hp2l_g(int nd, cplx zk, const HLev* lev,
                                                const int* ilevel, const int* n4, const int* l4,
                                                const int* sse, const int* cnt, const int* off,
-                                               int pstride, const cplx* G, const cplx* U,
-                                               const cplx* chs, const cplx* dps,
+                                               const int64_t* roff, const cplx* G,
+                                               const cplx* U, const cplx* chs, const cplx* dps,
                                                const double* dvs, cplx* loc) {
   const int b = blockIdx.x;
   if (cnt[b] == 0) return;
   const HLev L = lev[ilevel[b]];
+  const int pstride = L.p + 2;
   cplx* out = hexp(loc, L, b, nd, 0);
-  int64_t pos = off[b];
+  const int64_t pos0 = off[b];
+  int64_t pos = pos0;
+  const cplx* Gb = G + roff[b];  // row of pair pos at Gb + (pos - pos0) * pstride
   for (int k = 0; k < n4[b]; ++k) {
     const int c = l4[(int64_t)kMaxList4 * b + k];
     const int c0 = sse[2 * c], c1 = sse[2 * c + 1];
     if (c1 <= c0) continue;
-    accumulate_batch<IFC, IFD, 1>(nd, L.p, zk, L.s, G + pos * pstride, U + pos, c1 - c0,
+    accumulate_batch<IFC, IFD, 1>(nd, L.p, zk, L.s, Gb + (pos - pos0) * pstride, U + pos, c1 - c0,
                                   pstride, c0, chs, dps, dvs, out);
     pos += c1 - c0;
   }
@@ -820,6 +843,10 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
   if (!a.ifcharge && !a.ifdipole) return FMM2D_OK;
   const fmm2d_opts* o = a.opts;
   if (o && o->precision != 0) return FMM2D_ERR_UNSUPPORTED;
+  if (o) {
+    const int e = check_nterms_override(o->n_nterms, o->nterms, 1, kMaxNtermsHelmholtz);
+    if (e != FMM2D_OK) return e;
+  }
   try {
     double tinfo[16] = {0};
     Timer tm(o != nullptr, st);
@@ -854,7 +881,7 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     std::vector<long double> sl(nl + 1);
     int64_t ncoef = 0;
     for (int l = 0; l <= nl; ++l) {
-      if (nterms[l] < 1 || nterms[l] > 2000) return FMM2D_ERR_NTERMS;
+      if (nterms[l] < 1 || nterms[l] > kMaxNtermsHelmholtz) return FMM2D_ERR_NTERMS;
       HLev& v = lev[l];
       sl[l] = std::min(absk * (long double)t.boxsize[l] / (2.0L * (long double)kPi), 1.0L);
       v.s = (double)sl[l];
@@ -894,17 +921,22 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
 
     const int* sse = t.isrcse.p;
     const int* tse = t.itargse.p;
-    int pmax = 0;
-    for (int v : nterms) pmax = std::max(pmax, v);
     // ---- step 1: P2M (rows of the sources, then their sums per leaf)
     {
-      const int pstride = pmax + 2;
-      DBuf<cplx> gJ((size_t)ns * pstride, st), uJ(ns, st);
+      DBuf<int64_t> roff(ns, st), rtot(1, st);
+      launch1d(k_hp2m_rowlen, ns, st, d_lev.p, leaf_s.p, t.ilevel.p, roff.p);
+      exclusive_scan(roff.p, roff.p, ns, rtot.p, st);
+      int64_t nrow = 0;
+      hip_throw(hipMemcpyAsync(&nrow, rtot.p, sizeof(int64_t), hipMemcpyDeviceToHost, st),
+                "copy");
+      hip_throw(hipStreamSynchronize(st), "sync");
+      DBuf<cplx> gJ(std::max<int64_t>(nrow, 1), st), uJ(ns, st);
       launch1d(k_hp2m_rows, ns, st, zk, d_lev.p, leaf_s.p, t.ilevel.p, t.centers.p, xs.p,
-               pstride, gJ.p, uJ.p);
+               (const int64_t*)roff.p, gJ.p, uJ.p);
 #define FMM2D_P2M(C, D)                                                                       \
   hipLaunchKernelGGL((k_hp2m_g<C, D>), dim3(nb), dim3(kW), 0, st, nd, zk, d_lev.p, t.ilevel.p, \
-                     t.nchild.p, sse, pstride, gJ.p, uJ.p, chs.p, dps.p, dvs.p, mp.p)
+                     t.nchild.p, sse, (const int64_t*)roff.p, gJ.p, uJ.p, chs.p, dps.p, dvs.p,  \
+                     mp.p)
       if (a.ifcharge && !a.ifdipole) FMM2D_P2M(1, 0);
       else if (!a.ifcharge && a.ifdipole) FMM2D_P2M(0, 1);
       else FMM2D_P2M(1, 1);
@@ -915,25 +947,26 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     // ---- step 2: list 4 (pair counts, rows of the pairs, their sums per box)
     {
       DBuf<int> cnt4(nb, st), off4(nb, st), tot4(1, st);
+      DBuf<int64_t> roff4(nb, st), rtot4(1, st);
       launch1d(k_hp2l_count, nb, st, d_lev.p, t.ilevel.p, L.n4.p, L.l4.p, sse, tse, pg, pgt,
                cnt4.p);
-      size_t bytes = 0;
-      hip_throw(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt4.p, off4.p, nb, st), "scan");
-      DBuf<char> tmp(bytes, st);
-      hip_throw(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, cnt4.p, off4.p, nb, st), "scan");
-      hipLaunchKernelGGL(k_flag_total, dim3(1), dim3(1), 0, st, nb, cnt4.p, off4.p, tot4.p);
+      exclusive_scan(cnt4.p, off4.p, nb, tot4.p, st);
+      launch1d(k_hp2l_rowlen, nb, st, d_lev.p, t.ilevel.p, cnt4.p, roff4.p);
+      exclusive_scan(roff4.p, roff4.p, nb, rtot4.p, st);
       int np4 = 0;
+      int64_t nrow4 = 0;
       hip_throw(hipMemcpyAsync(&np4, tot4.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
+      hip_throw(hipMemcpyAsync(&nrow4, rtot4.p, sizeof(int64_t), hipMemcpyDeviceToHost, st),
+                "copy");
       hip_throw(hipStreamSynchronize(st), "sync");
       if (np4 > 0) {
-        const int pstride = pmax + 2;
-        DBuf<cplx> gH((size_t)np4 * pstride, st), uH(np4, st);
+        DBuf<cplx> gH(std::max<int64_t>(nrow4, 1), st), uH(np4, st);
         launch1d(k_hp2l_rows, np4, st, nb, zk, d_lev.p, t.ilevel.p, t.centers.p, L.l4.p, sse,
-                 off4.p, xs.p, pstride, gH.p, uH.p);
+                 off4.p, xs.p, (const int64_t*)roff4.p, gH.p, uH.p);
 #define FMM2D_P2L(C, D)                                                                       \
   hipLaunchKernelGGL((k_hp2l_g<C, D>), dim3(nb), dim3(kW), 0, st, nd, zk, d_lev.p, t.ilevel.p, \
-                     L.n4.p, L.l4.p, sse, cnt4.p, off4.p, pstride, gH.p, uH.p, chs.p, dps.p,  \
-                     dvs.p, loc.p)
+                     L.n4.p, L.l4.p, sse, cnt4.p, off4.p, (const int64_t*)roff4.p, gH.p, uH.p, \
+                     chs.p, dps.p, dvs.p, loc.p)
         if (a.ifcharge && !a.ifdipole) FMM2D_P2L(1, 0);
         else if (!a.ifcharge && a.ifdipole) FMM2D_P2L(0, 1);
         else FMM2D_P2L(1, 1);
@@ -1166,6 +1199,10 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
   } catch (const HipError& e) {
     fprintf(stderr, "fmm2d_amd: %s failed: %s\n", e.what, hipGetErrorString(e.err));
     return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    return FMM2D_ERR_HIP;
   }
 }
 

@@ -1216,6 +1216,10 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
   if (!a.ifcharge && !a.ifdipole) return FMM2D_OK;
   const fmm2d_opts* o = a.opts;
   if (o && (o->precision < 0 || o->precision > 1)) return FMM2D_ERR_ARG;
+  if (o) {
+    const int e = check_nterms_override(o->n_nterms, o->nterms, 2, kMaxNtermsLaplace);
+    if (e != FMM2D_OK) return e;
+  }
   // precision 1: FP32 M2L on MFMA and FP32 near-field kernels with FP64 sums;
   // tree, expansions, P2M/L2P, M2M/L2L stay FP64
   const bool mixed = o && o->precision == 1;
@@ -1548,6 +1552,10 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
   } catch (const HipError& e) {
     fprintf(stderr, "fmm2d_amd: %s failed: %s\n", e.what, hipGetErrorString(e.err));
     return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    return FMM2D_ERR_HIP;
   }
 }
 

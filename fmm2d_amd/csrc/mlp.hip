@@ -147,6 +147,10 @@ int fmm2d_mlp_forward(int nin, int nout, const float* params, int batch, const f
     hip_throw(hipGetLastError(), "mlp forward");
   } catch (const HipError& e) {
     return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    return FMM2D_ERR_HIP;
   }
   return FMM2D_OK;
 }
@@ -186,6 +190,10 @@ int fmm2d_mlp_loss_grad(int nin, int nout, const float* params, int batch, const
     hip_throw(hipGetLastError(), "mlp loss/grad");
   } catch (const HipError& e) {
     return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    return FMM2D_ERR_HIP;
   }
   return FMM2D_OK;
 }

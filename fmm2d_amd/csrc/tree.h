@@ -63,4 +63,9 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
 // Lists 1-4 of computelists (common/tree_routs2d.f:486-710), as sets.
 void build_lists(const Tree& t, Lists& l, hipStream_t st);
 
+// hist[l] (host, nbins <= kLeafHistMax values) = points, sources plus targets,
+// held by the leaves of level l; the last bin also takes every deeper level.
+constexpr int kLeafHistMax = 64;
+void leaf_level_histogram(const Tree& t, int nbins, double* hist, hipStream_t st);
+
 }  // namespace fmm2d

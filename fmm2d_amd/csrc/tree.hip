@@ -1193,4 +1193,36 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   t.coll = std::move(bld.B.coll);
 }
 
+// Points (sources + targets) held by the leaves of each level, levels beyond
+// nbins - 1 counted in the last bin: per-block counts in LDS, then one global
+// atomic per bin and block (integer counts, so the result does not depend on
+// the order of the additions).
+__global__ void k_leaf_hist(int64_t nb, const int* nchild, const int* ilevel, const int* ses,
+                            const int* set, int nbins, unsigned long long* hist) {
+  __shared__ unsigned long long h[kLeafHistMax];
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
+  __syncthreads();
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb && nchild[b] == 0) {
+    const int n = (ses[2 * b + 1] - ses[2 * b]) + (set[2 * b + 1] - set[2 * b]);
+    if (n > 0) atomicAdd(&h[min(ilevel[b], nbins - 1)], (unsigned long long)n);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x)
+    if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+void leaf_level_histogram(const Tree& t, int nbins, double* hist, hipStream_t st) {
+  if (nbins < 1 || nbins > kLeafHistMax) throw HipError{hipErrorInvalidValue, "histogram bins"};
+  DBuf<unsigned long long> d(nbins, st);
+  d.zero();
+  launch1d(k_leaf_hist, t.nboxes, st, t.nchild.p, t.ilevel.p, t.isrcse.p, t.itargse.p, nbins,
+           d.p);
+  std::vector<unsigned long long> h(nbins);
+  hip_throw(hipMemcpyAsync(h.data(), d.p, sizeof(unsigned long long) * nbins,
+                           hipMemcpyDeviceToHost, st), "copy");
+  hip_throw(hipStreamSynchronize(st), "sync");
+  for (int k = 0; k < nbins; ++k) hist[k] = (double)h[k];
+}
+
 }  // namespace fmm2d

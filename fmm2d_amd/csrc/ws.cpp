@@ -30,6 +30,8 @@ struct Owner {
 std::mutex g_mu;
 std::map<Key, std::multimap<size_t, void*>> g_free;  // cached blocks by size
 std::unordered_map<void*, Owner> g_owner;            // every block we hold
+std::map<int, size_t> g_held;                        // bytes held per device
+size_t g_limit = 0;                                  // 0: no limit
 
 size_t round_size(size_t b) {
   const size_t mib = size_t(1) << 20;
@@ -47,6 +49,7 @@ void drop_cached(int dev) {
     for (auto& kv : it->second) {
       (void)hipFree(kv.second);
       g_owner.erase(kv.second);
+      g_held[dev] -= kv.first;
     }
     it = g_free.erase(it);
   }
@@ -67,6 +70,12 @@ void* ws_alloc(size_t bytes, hipStream_t st) {
     fl.erase(it);
     return p;
   }
+  if (g_limit && g_held[dev] + want > g_limit) {
+    // over the caller's cap: give the cached blocks back first
+    (void)hipDeviceSynchronize();
+    drop_cached(dev);
+    if (g_held[dev] + want > g_limit) throw HipError{hipErrorOutOfMemory, "workspace limit"};
+  }
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, want);
   if (e == hipErrorOutOfMemory) {
@@ -78,6 +87,7 @@ void* ws_alloc(size_t bytes, hipStream_t st) {
   }
   hip_throw(e, "hipMalloc");
   g_owner[p] = Owner{key, want};
+  g_held[dev] += want;
   return p;
 }
 
@@ -97,6 +107,11 @@ int ws_release() {
   if (hipDeviceSynchronize() != hipSuccess) return FMM2D_ERR_HIP;
   drop_cached(dev);
   return FMM2D_OK;
+}
+
+void ws_set_limit(size_t bytes) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_limit = bytes;
 }
 
 }  // namespace fmm2d

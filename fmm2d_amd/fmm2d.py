@@ -158,8 +158,14 @@ def _fmm(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
             continue
         a = nat.fortran_to_numpy(t)
         res[name] = a[0] if nd == 1 else a
-    if opts is not None and "timeinfo" in opts:
-        opts["timeinfo"][:] = o.timeinfo_array()
+    if opts is not None:
+        # per-call diagnostics written back into the caller's dict
+        if "timeinfo" in opts:
+            opts["timeinfo"][:] = o.timeinfo_array()
+        opts["nlevels"], opts["nboxes"] = int(o.nlevels), int(o.nboxes)
+        if o.count_work:
+            opts["work"] = np.array(o.work[:], dtype=np.float64)
+            opts["near_pairs"], opts["m2l_pairs"] = int(o.near_pairs), int(o.m2l_pairs)
     return res, ier
 
 

@@ -242,6 +242,19 @@ int fmm2d_tree_get(void* handle, int* laddr, int* ilevel, int* iparent, int* nch
 
 void fmm2d_tree_free(void* handle);
 
+/*
+ * Leaf occupancy by level of the tree fmm2d_tree_build builds (the reference's
+ * pts_tree_build, common/pts_tree2d.f:406, with leaf capacity ndiv), computed
+ * on the device: hist[l] (HOST array of nbins <= 64 doubles) = the number of
+ * points, sources plus targets, held by the leaves of level l, the last bin
+ * taking every deeper level; *boxsize0 (HOST, may be NULL) = the root box
+ * size.  No reference counterpart: these are the input features of the
+ * parameter proposer (below), which the reference does not have.
+ */
+int fmm2d_tree_leaf_histogram(const double* sources, int64_t ns, const double* targ, int64_t nt,
+                              int idivflag, int ndiv, int nbins, double* hist, double* boxsize0,
+                              void* stream);
+
 /* Expansion-order and leaf-size rules on the host:
  *   l2dterms  laplace/l2dterms.f:38     h2dterms helmholtz/h2dterms.f:35
  *   lndiv2d   laplace/lndiv2d.f         hndiv2d  helmholtz/hndiv2d.f */
@@ -258,6 +271,16 @@ int fmm2d_ndiv(double eps, int64_t ns, int64_t nt, int helmholtz, int* ndiv, int
  * called or the process ends.
  */
 int fmm2d_release_workspace(void);
+
+/*
+ * Cap the device workspace the library holds per device (cached and in use),
+ * in bytes; 0 = no cap (the default).  No reference counterpart: the reference
+ * returns ier = 4 when a Fortran ALLOCATE of its workspace fails
+ * (laplace/cfmm2d.f:251-262, helmholtz/hfmm2d.f:354-371); with a cap, a call
+ * whose workspace would exceed it first returns the cached blocks and then
+ * fails with that same code, FMM2D_ERR_ALLOC (4), for callers sharing the GPU.
+ */
+int fmm2d_set_workspace_limit(int64_t bytes);
 
 /*
  * Parameter-proposer network (no reference counterpart: the reference picks

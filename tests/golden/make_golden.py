@@ -15,6 +15,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "..", ".."))
 from oracle import ref_binding as R  # noqa: E402
+sys.path.insert(0, HERE)
+import inputs as I  # noqa: E402
 
 
 def direct_fixtures():
@@ -60,6 +62,10 @@ def hankel_fixture():
             zs.append(r * np.exp(1j * th))
     for r in (0.5, 2.0, 5.0, 30.0):
         for th in (-0.05, -0.2):
+            zs.append(r * np.exp(1j * th))
+    # deeper in the lower half-plane (Im zk < 0): every regime of the evaluator
+    for r in (0.4, 1.5, 2.2, 4.0, 6.5, 9.0, 13.0, 16.5, 18.0, 25.0):
+        for th in (-0.5, -1.0, -np.pi / 2, -2.0, -2.6, -3.1):
             zs.append(r * np.exp(1j * th))
     z = np.array(zs, dtype=np.complex128)
     h0r, h1r = R.hank103(z)
@@ -269,8 +275,59 @@ def rfmm_fixtures():
     np.savez_compressed(os.path.join(HERE, "rfmm_ref.npz"), **out)
 
 
+# The bench workload's point distribution at a size the reference runs in
+# seconds: lfmm2d with real charges at eps 1e-6, default leaf capacity and
+# leaf capacity 37 (lfmm2d_ndiv, laplace/lfmm2d_ndiv.f:29), and rfmm2d.
+HEADLINE = dict(n=120000, seed=31, nsample=3000, sample_seed=32, eps=1e-6, ndiv=37)
+
+
+def headline_fixtures():
+    """Reference lfmm2d / lfmm2d_ndiv / rfmm2d / rfmm2d_ndiv outputs (pot, grad at
+    the sources, pg = 2) and the reference direct sum, at sampled sources."""
+    h = HEADLINE
+    src, q = I.clustered(h["n"], h["seed"])
+    idx = np.sort(np.random.default_rng(h["sample_seed"]).choice(h["n"], h["nsample"],
+                                                                 replace=False))
+    out = {"input_sha256": np.array(I.digest(src, q)), "idx": idx}
+    eps, nd = h["eps"], h["ndiv"]
+    runs = {
+        "l_default": R.lfmm2d(nd=1, eps=eps, sources=src, charges=q + 0j, ifpgh=2),
+        "l_ndiv": R.lfmm2d(nd=1, eps=eps, sources=src, charges=q + 0j, ifpgh=2, ndiv=nd),
+        "r_default": R.rfmm2d(nd=1, eps=eps, sources=src, charges=q, ifpgh=2),
+        "r_ndiv": R.rfmm2d(nd=1, eps=eps, sources=src, charges=q, ifpgh=2, ndiv=nd),
+    }
+    for tag, r in runs.items():
+        assert r["ier"] == 0, (tag, r["ier"])
+        out[tag + "_pot"] = r["pot"][0, idx]
+        out[tag + "_grad"] = r["grad"][0][:, idx]
+    d = R.direct("l", 1, src, src[:, idx], charges=q + 0j, pgt=2, thresh=1e-16)
+    out["direct_pot"] = d["pottarg"][0]
+    out["direct_grad"] = d["gradtarg"][0]
+    np.savez_compressed(os.path.join(HERE, "headline_ref.npz"), **out)
+
+
+# The Helmholtz workload (hfmm2d, zk = 100, eps 1e-9, charges and dipoles,
+# pot+grad at separate targets) at 1e4 sources + 1e4 targets.
+HELM_C2 = dict(n=10000, seed=2, zk=100.0, eps=1e-9, nsample=2000, sample_seed=3)
+
+
+def helm_c2_fixture():
+    """Reference hfmm2d outputs (pottarg, gradtarg at sampled targets)."""
+    h = HELM_C2
+    src, trg, ch, ds, dv = I.helmholtz_st(h["n"], h["seed"])
+    idx = np.sort(np.random.default_rng(h["sample_seed"]).choice(h["n"], h["nsample"],
+                                                                 replace=False))
+    r = R.hfmm2d(zk=h["zk"] + 0j, nd=1, eps=h["eps"], sources=src, charges=ch[None],
+                 dipstr=ds[None], dipvec=dv[None], targets=trg, ifpgh=0, ifpghtarg=2)
+    assert r["ier"] == 0
+    out = {"input_sha256": np.array(I.digest(src, trg, ch, ds, dv)), "idx": idx,
+           "pottarg": r["pottarg"][0, idx], "gradtarg": r["gradtarg"][0][:, idx]}
+    np.savez_compressed(os.path.join(HERE, "helm_c2_ref.npz"), **out)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm", "hfmm", "lreal", "rfmm"]
+    which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm", "hfmm", "lreal", "rfmm",
+                             "headline", "helmc2"]
     if "direct" in which:
         direct_fixtures()
     if "hankel" in which:
@@ -285,5 +342,9 @@ if __name__ == "__main__":
         lfmm_real_fixtures()
     if "rfmm" in which:
         rfmm_fixtures()
+    if "headline" in which:
+        headline_fixtures()
+    if "helmc2" in which:
+        helm_c2_fixture()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))

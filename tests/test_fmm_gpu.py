@@ -273,3 +273,45 @@ def test_mixed_precision_coincident_points(kind, case):
         if kind == "c" and k in ("pot", "pottarg"):
             a, b = a.real, b.real
         assert _rel(a, b) < 1e-6, (k, _rel(a, b))
+
+
+def test_hfmm2d_zk100_matches_reference(golden_dir):
+    """The Helmholtz workload's parameters -- zk = 100, eps 1e-9, charges and
+    dipoles, pot+grad at separate targets -- at 1e4 sources + 1e4 targets
+    against the reference's hfmm2d (tests/golden/helm_c2_ref.npz, written by
+    make_golden.py:helm_c2_fixture from the seeded generator)."""
+    from golden import inputs as I
+    g = np.load(f"{golden_dir}/helm_c2_ref.npz")
+    src, trg, ch, ds, dv = I.helmholtz_st(10000, 2)
+    assert I.digest(src, trg, ch, ds, dv) == str(g["input_sha256"])
+    res, ier = F._fmm("h", 1e-9, 100.0 + 0j, src, ch, ds, dv, trg, 0, 2, 1)
+    assert ier == 0
+    idx = g["idx"]
+    assert _rel(res["pottarg"][idx], g["pottarg"]) < 1e-12
+    assert _rel(res["gradtarg"][:, idx], g["gradtarg"]) < 1e-12
+
+
+def test_error_codes_ier4_ier13():
+    """ier = 13 when no expansion order meets eps (h2dterms, helmholtz/h2dterms.f:
+    92-93: zk = 1e5 on the unit square) and ier = 4 when the workspace cannot be
+    had (a workspace cap below what the call needs); the next call without the
+    cap runs normally."""
+    from fmm2d_amd import _native as nat
+    rng = np.random.default_rng(8)
+    src = rng.random((2, 3000))
+    ch = rng.standard_normal(3000) + 1j * rng.standard_normal(3000)
+    _, ier = F._fmm("h", 1e-6, 1e5, src, ch, None, None, None, 1, 0, 1)
+    assert ier == 13
+    L = nat.lib()
+    nat.lib().fmm2d_release_workspace()
+    assert L.fmm2d_set_workspace_limit(1 << 16) == 0
+    try:
+        _, ier = F._fmm("l", 1e-6, None, src, ch, None, None, None, 2, 0, 1)
+        assert ier == 4
+        _, ier = F._fmm("h", 1e-6, 10.0, src, ch, None, None, None, 2, 0, 1)
+        assert ier == 4
+    finally:
+        assert L.fmm2d_set_workspace_limit(0) == 0
+    res, ier = F._fmm("l", 1e-6, None, src, ch, None, None, None, 2, 0, 1)
+    assert ier == 0 and np.all(np.isfinite(res["pot"]))
+    assert L.fmm2d_set_workspace_limit(-1) == -1

@@ -154,3 +154,26 @@ def test_batch_argument_checks():
         F.fmm2d_batch([dict(ok, kind="h")])
     with pytest.raises(ValueError, match="nstreams"):
         F.fmm2d_batch([ok], nstreams=0)
+
+
+def test_library_rules_match_reference():
+    """The library's host rules (fmm2d_l2dterms, fmm2d_h2dterms, fmm2d_ndiv) give
+    the reference's values, including h2dterms' ier = 13 when no order up to
+    50000 meets eps (helmholtz/h2dterms.f:92-93: nterms = 10001)."""
+    from oracle import ref_binding as R
+    if not R.available():
+        pytest.skip("reference library not built (oracle/ref.mk)")
+    L = nat.lib()
+    v = ctypes.c_int(0)
+    for eps in (1e-3, 0.5e-6, 1e-6, 1e-9, 1e-12, 1e-15):
+        assert L.fmm2d_l2dterms(eps, ctypes.byref(v)) == 0
+        assert v.value == R.l2dterms(eps)[0], eps
+        for ns in (100, 10 ** 5, 10 ** 7):
+            nd, idf = ctypes.c_int(0), ctypes.c_int(0)
+            assert L.fmm2d_ndiv(eps, ns, 0, 0, ctypes.byref(nd), ctypes.byref(idf)) == 0
+            assert (nd.value, idf.value) == R.ndiv(eps, ns, 0), (eps, ns)
+    for bs, zk, eps in ((1.0, 10.0, 1e-6), (0.25, 100.0 + 2.0j, 1e-9), (2.6, 500.0, 1e-6),
+                        (1.0, 6e4, 1e-6), (1.0, 1e5, 1e-6)):
+        ier = L.fmm2d_h2dterms(bs, complex(zk).real, complex(zk).imag, eps, ctypes.byref(v))
+        assert (v.value, ier) == R.h2dterms(bs, zk, eps), (bs, zk, eps)
+    assert R.h2dterms(1.0, 1e5, 1e-6) == (10001, 13)
