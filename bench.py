@@ -8,11 +8,16 @@ Workload (one "step" = one full lfmm2d call, tree build and unsort included):
   timed region is K back-to-back calls bracketed by device synchronisation.
 
 Pipelines, all on the same inputs: FP64; mixed precision (FP32 M2L on MFMA and
-FP32 near field with FP64 sums); mixed precision with the learned proposer's
-expansion orders and leaf capacity (proposed outside the timed region).  The
-reported value is the fastest pipeline whose relative L2 error of pot+grad
-against the FP64 direct sum at 1e4 sampled sources is within eps; every
-pipeline's rate and error are in "pipelines".
+FP32 near field with FP64 sums); and both again with the learned proposer's
+expansion orders and leaf capacity.  A learned step is end to end: the
+proposal (leaf-occupancy histogram of the default tree made on the device,
+the network on the device), the FMM with the proposed parameters, and the
+proposer's acceptance test (FP64 direct sum at --validate sampled sources,
+fallback to the reference's rules if the error exceeds eps) are all inside
+the timed region.  The reported value is the mixed learned pipeline when its
+relative L2 error of pot+grad against the FP64 direct sum at 1e4 sampled
+sources is within eps and it beats FP64 (else mixed, else FP64); every
+pipeline's rate, error and FMM-call-alone time are in "pipelines".
 
 Also reported: per-stage times from HIP events recorded by the library on the
 call's stream, the roofline fractions of the near-field and M2L kernels
@@ -21,7 +26,7 @@ HBM traffic from rocprofv3 counters (profiles/), and the reference library
 timed on the host cores on a bounded sample.
 
   python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--n 10000000]
-                  [--precision mixed|fp64] [--no-learned] [--no-cpu]
+                  [--precision mixed|fp64] [--no-learned] [--no-cpu] [--validate 256]
 """
 import argparse
 import json
@@ -77,9 +82,36 @@ def c3_points(n, seed=3):
     return pts, q  # pts (n,2) = Fortran (2,n)
 
 
+def cpu_threads():
+    """Host threads for the reference: the OpenMP share this process is given
+    (OMP_NUM_THREADS, 16 per GPU on the GPU hosts), else every core it may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def host_cpu():
+    """CPU model, logical CPUs of the machine and of this process's affinity mask."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(n_sample, threads):
     """Reference lfmm2d (oracle/_ref, built from the reference sources) on the
-    host cores, in a child process so its OpenMP runtime is its own."""
+    host cores, in a child process so its OpenMP runtime is its own.  Returns
+    (seconds, error message, path of an .npz with its pot and grad)."""
+    import tempfile
+    out = os.path.join(tempfile.gettempdir(), f"bench_ref_{os.getpid()}.npz")
     code = f"""
 import sys, time, json
 sys.path.insert(0, {ROOT!r})
@@ -90,17 +122,51 @@ pts, q = c3_points({n_sample})
 src = np.ascontiguousarray(pts.T)
 R.lfmm2d(nd=1, eps=1e-6, sources=src[:, :1000], charges=q[:1000] + 0j, ifpgh=2)
 t = time.perf_counter()
-R.lfmm2d(nd=1, eps=1e-6, sources=src, charges=q + 0j, ifpgh=2)
+o = R.lfmm2d(nd=1, eps=1e-6, sources=src, charges=q + 0j, ifpgh=2)
 dt = time.perf_counter() - t
+np.savez({out!r}, pot=o["pot"][0], grad=o["grad"][0])
 print(json.dumps({{"seconds": dt}}))
 """
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                        timeout=600)
     if r.returncode != 0:
-        return None, r.stderr[-400:]
+        return None, r.stderr[-400:], None
     dt = json.loads(r.stdout.strip().splitlines()[-1])["seconds"]
-    return dt, None
+    return dt, None, out
+
+
+def compare_with_reference(L, nat, n_sample, ref_path, head, proposer):
+    """Relative L2 difference (pot and grad) between this library's FMM, run as
+    the reported pipeline (its precision; for a learned pipeline, its own
+    proposal for the sample) on the sample the reference ran on (the
+    workload's generator at n_sample points), and the reference's lfmm2d
+    output on it."""
+    import torch
+    dev = nat.torch_device()
+    with np.load(ref_path) as z:
+        rp, rg = z["pot"], z["grad"]
+    os.remove(ref_path)
+    m = n_sample
+    pts, q = c3_points(m)
+    src = torch.from_numpy(np.ascontiguousarray(pts)).to(dev)
+    chg = torch.from_numpy(q.astype(np.complex128)).to(dev)
+    pot = torch.zeros(m, dtype=torch.complex128, device=dev)
+    grad = torch.zeros((m, 2), dtype=torch.complex128, device=dev)
+    base = {"precision": "fp64" if head.startswith("fp64") else "mixed"}
+    if head.endswith("+learned") and proposer is not None:
+        prop = proposer.propose_device("l", nat.ptr(src), m, EPS)
+        base.update(nterms=prop["nterms"], ndiv=prop["ndiv"])
+    o = nat.opts_struct(base)
+    ier = L.fmm2d_lfmm2d(1, EPS, m, nat.ptr(src), 1, nat.ptr(chg), 0, None, None, 0, 2,
+                         nat.ptr(pot), nat.ptr(grad), None, 0, None, 0, None, None, None,
+                         nat.byref_opts(o), nat.stream_ptr())
+    if ier != 0:
+        raise RuntimeError(f"fmm2d_lfmm2d returned {ier}")
+    gp, gg = pot.cpu().numpy(), grad.cpu().numpy().T  # (2, m) like the reference's grad
+    num = np.sum(np.abs(gp - rp) ** 2) + np.sum(np.abs(gg - rg) ** 2)
+    den = np.sum(np.abs(rp) ** 2) + np.sum(np.abs(rg) ** 2)
+    return float(np.sqrt(num / den))
 
 
 def c2_inputs(n, seed=2):
@@ -117,7 +183,8 @@ def c2_inputs(n, seed=2):
 
 
 def cpu_baseline_c2(n_sample, threads):
-    """Reference hfmm2d on the first n_sample sources and targets of c2_inputs."""
+    """Reference hfmm2d on c2_inputs(n_sample): n_sample sources and targets of
+    the workload's generator."""
     code = f"""
 import sys, time, json
 sys.path.insert(0, {ROOT!r})
@@ -278,14 +345,14 @@ def main_hfmm2d(args):
     achieved = work[3] / t_m2l / 1e12 if t_m2l > 0 else 0.0
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         ns_cpu = 100_000
         secs, errmsg = cpu_baseline_c2(ns_cpu, threads)
         if secs is not None:
             cpu = {"value": round(ns_cpu / secs, 1), "unit": "targets/s", "cores": threads,
                    "kind": "reference",
-                   "sample": f"reference hfmm2d (oracle/_ref) on the first {ns_cpu} sources "
-                             f"and targets of the same generator; {secs:.2f} s"}
+                   "sample": f"reference hfmm2d (oracle/_ref) on {ns_cpu} sources and "
+                             f"{ns_cpu} targets of the same generator (seed 2); {secs:.2f} s"}
         else:
             print("bench: cpu baseline failed: " + str(errmsg), file=sys.stderr)
     line = {
@@ -319,6 +386,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nerr", type=int, default=10_000)
+    ap.add_argument("--validate", type=int, default=256,
+                    help="sampled sources of the learned pipelines' in-step acceptance test")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-parameter pipeline")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -382,18 +451,67 @@ def main():
     assert ier == 0
     den = float(torch.sqrt((pex.abs() ** 2).sum() + (gex.abs() ** 2).sum()))
 
-    def run(label, base):
-        """warm-up, K timed calls, one counting call; returns the measurements"""
-        precision = label
-        opts = nat.opts_struct(base)
+    # sampled FP64 direct sums that validate a learned proposal inside the
+    # step (the proposer's acceptance test: relative L2 error <= eps)
+    nval = min(args.validate, n)
+    rng_v = np.random.default_rng(777)
+    iv = torch.from_numpy(np.sort(rng_v.choice(n, size=nval, replace=False))).to(dev)
+    tsel_v = src[iv].contiguous() if nval else None
+    pv = torch.zeros(max(nval, 1), dtype=torch.complex128, device=dev)
+    gv = torch.zeros((max(nval, 1), 2), dtype=torch.complex128, device=dev)
+    proposer = None
+    if args.precision == "mixed" and not args.no_learned:
+        from fmm2d_amd import proposer as PR
+        proposer = PR.Proposer()  # network weights to HBM once, like any model load
+
+    def learned_step(base, st):
+        """One end-to-end learned call: features from the device leaf histogram,
+        the network, the FMM with the proposed orders / leaf capacity, and the
+        sampled-direct-sum acceptance test with fallback to the reference rules."""
+        prop = proposer.propose_device("l", nat.ptr(src), n, EPS)
+        o = nat.opts_struct(dict(base, nterms=prop["nterms"], ndiv=prop["ndiv"]))
+        step(o)
+        st["proposal"], st["opts"] = prop, o
+        if nval:
+            pv.zero_()
+            gv.zero_()
+            ier = L.fmm2d_l2d_direct(1, nat.ptr(src), n, 1, nat.ptr(chg), 0, None, None,
+                                     nat.ptr(tsel_v), nval, 2, nat.ptr(pv), nat.ptr(gv), None,
+                                     1e-16, stream)
+            if ier != 0:
+                raise RuntimeError(f"fmm2d_l2d_direct returned {ier}")
+            num = ((pot[iv] - pv).abs() ** 2).sum() + ((grad[iv] - gv).abs() ** 2).sum()
+            den_v = (pv.abs() ** 2).sum() + (gv.abs() ** 2).sum()
+            e = float(torch.sqrt(num / den_v))
+            st["val_err"] = e
+            if e > EPS:  # rejected: the reference's rules
+                o = nat.opts_struct(base)
+                step(o)
+                st["opts"] = o
+                st["fallbacks"] = st.get("fallbacks", 0) + 1
+        return st["opts"]
+
+    def run(label, base, learned=False):
+        """warm-up, K timed calls, the FMM call alone, one counting call.
+        learned: every timed step makes its own proposal and validates it."""
+        st = {}
+        if learned:
+            def one():
+                return learned_step(base, st)
+        else:
+            opts0 = nat.opts_struct(base)
+
+            def one():
+                step(opts0)
+                return opts0
         for _ in range(args.warmup):
-            step(opts)
+            one()
         torch.cuda.synchronize()
         stage = np.zeros(16)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            step(opts)
-            stage += np.array(opts.timeinfo[:])
+            o = one()
+            stage += np.array(o.timeinfo[:])
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         stage /= args.steps
@@ -401,37 +519,50 @@ def main():
         num = (pot[it] - pex).abs() ** 2
         num = float(torch.sqrt(num.sum() + ((grad[it] - gex).abs() ** 2).sum()))
         relerr = num / den
-        # one more (untimed) call with the work counters on: algorithmic flops
-        copts = nat.opts_struct(dict(base, count_work=1))
+        # the FMM call alone (no proposal, no validation) with the last step's
+        # parameters, then one more with the work counters on
+        fbase = dict(base)
+        if learned:
+            fbase.update(nterms=st["proposal"]["nterms"], ndiv=st["proposal"]["ndiv"])
+        of = nat.opts_struct(fbase)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step(of)
+        torch.cuda.synchronize()
+        dt_fmm = (time.perf_counter() - t1) / args.steps
+        copts = nat.opts_struct(dict(fbase, count_work=1))
         step(copts)
         torch.cuda.synchronize()
         work = np.array(copts.work[:])
-        print(f"bench[{precision}]: {dt*1e3:.1f} ms/step, levels {opts.nlevels}, boxes "
-              f"{opts.nboxes}, near pairs {copts.near_pairs}, m2l pairs {copts.m2l_pairs}, "
-              f"rel L2 err {relerr:.3e}", file=sys.stderr, flush=True)
+        o = st.get("opts", of) if learned else of
+        print(f"bench[{label}]: {dt*1e3:.2f} ms/step (FMM call alone {dt_fmm*1e3:.2f} ms), "
+              f"levels {o.nlevels}, boxes {o.nboxes}, near pairs {copts.near_pairs}, m2l pairs "
+              f"{copts.m2l_pairs}, rel L2 err {relerr:.3e}"
+              + (f", validation err {st.get('val_err', float('nan')):.2e}, fallbacks "
+                 f"{st.get('fallbacks', 0)}" if learned else ""), file=sys.stderr, flush=True)
         for k, name in STAGES.items():
-            print(f"bench[{precision}]:   {name:12s} {stage[k]*1e3:9.3f} ms", file=sys.stderr)
-        return {"dt": dt, "stage": stage, "work": work, "relerr": relerr,
-                "nlevels": int(opts.nlevels), "nboxes": int(opts.nboxes)}
+            print(f"bench[{label}]:   {name:12s} {stage[k]*1e3:9.3f} ms", file=sys.stderr)
+        return {"dt": dt, "dt_fmm": dt_fmm, "stage": stage, "work": work, "relerr": relerr,
+                "nlevels": int(o.nlevels), "nboxes": int(o.nboxes),
+                "proposal": st.get("proposal"), "fallbacks": st.get("fallbacks", 0),
+                "val_err": st.get("val_err")}
 
     runs = {"fp64": run("fp64", {"precision": "fp64"})}
     proposal = None
     if args.precision == "mixed":
         runs["mixed"] = run("mixed", {"precision": "mixed"})
-        if not args.no_learned:
-            # learned expansion orders / leaf capacity (outside the timed region;
-            # the proposer builds the default tree for its features)
-            from fmm2d_amd import proposer as PR
-            t0 = time.perf_counter()
-            proposal = PR.Proposer().propose("l", np.ascontiguousarray(pts.T), EPS)
-            t_prop = time.perf_counter() - t0
-            print(f"bench: proposal ndiv {proposal['ndiv']}, nterms {proposal['nterms'][:6]}..."
-                  f" ({t_prop:.2f} s)", file=sys.stderr, flush=True)
-            runs["mixed+learned"] = run("mixed+learned",
-                                        {"precision": "mixed", "nterms": proposal["nterms"],
-                                         "ndiv": proposal["ndiv"]})
-    # the headline: the fastest pipeline whose error against the FP64 direct
-    # sum meets eps (mixed+learned, mixed, then FP64)
+        if proposer is not None:
+            # learned expansion orders / leaf capacity, proposed inside every
+            # timed step from the device-resident points; FP64 and mixed
+            runs["fp64+learned"] = run("fp64+learned", {"precision": "fp64"}, learned=True)
+            runs["mixed+learned"] = run("mixed+learned", {"precision": "mixed"}, learned=True)
+            proposal = runs["mixed+learned"]["proposal"]
+            print(f"bench: proposal ndiv {proposal['ndiv']}, nterms {proposal['nterms'][:6]}...",
+                  file=sys.stderr, flush=True)
+    # the headline: the learned mixed-precision pipeline when its error against
+    # the FP64 direct sum meets eps and it is faster than FP64, else the mixed
+    # pipeline under the same conditions, else FP64
     head = "fp64"
     for cand in ("mixed+learned", "mixed"):
         if cand in runs and runs[cand]["relerr"] <= EPS and runs[cand]["dt"] < runs[head]["dt"]:
@@ -467,16 +598,20 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         print(f"bench: reference CPU baseline on {args.cpu_sample} points, {threads} threads",
               file=sys.stderr, flush=True)
-        secs, errmsg = cpu_baseline(args.cpu_sample, threads)
+        secs, errmsg, ref_out = cpu_baseline(args.cpu_sample, threads)
         if secs is not None:
             cpu = {"value": round(args.cpu_sample / secs, 1), "unit": "points/s",
                    "cores": threads, "kind": "reference",
-                   "sample": f"reference lfmm2d (oracle/_ref, flang -O3) on the first "
-                             f"{args.cpu_sample} points of the same generator, eps 1e-6, pg 2; "
-                             f"{secs:.2f} s"}
+                   "sample": f"reference lfmm2d (oracle/_ref, flang -O3, OpenMP) on "
+                             f"{args.cpu_sample} points of the same generator (seed 3), eps "
+                             f"1e-6, pg 2; {secs:.2f} s",
+                   "host": host_cpu()}
+            if ref_out is not None:
+                cpu["gpu_vs_reference_rel_l2"] = compare_with_reference(
+                    L, nat, args.cpu_sample, ref_out, head, proposer)
         else:
             print("bench: cpu baseline failed: " + str(errmsg), file=sys.stderr)
 
@@ -507,7 +642,13 @@ def main():
     for k, r in runs.items():
         line.setdefault("pipelines", {})[k] = {
             "value": round(n / r["dt"], 1), "ms_per_step": round(r["dt"] * 1e3, 3),
+            "ms_fmm_call_alone": round(r["dt_fmm"] * 1e3, 3),
             "rel_l2_err": r["relerr"], "vs_fp64": round(f["dt"] / r["dt"], 4)}
+        if r["proposal"] is not None:
+            line["pipelines"][k].update(
+                includes="proposal (device leaf histogram + network) + FMM + acceptance "
+                         f"test ({args.validate} sampled FP64 direct sums)",
+                validation_err=r["val_err"], fallbacks=r["fallbacks"])
     if proposal is not None:
         line["proposal"] = {"ndiv": proposal["ndiv"], "nterms": proposal["nterms"][:8]}
     print(json.dumps(line), flush=True)

@@ -70,6 +70,26 @@ def tree_features(sources, targets, eps, zk, kind, ndiv):
     return f.astype(np.float32), tr["boxsize"]
 
 
+def device_features(src_ptr, ns, targ_ptr, nt, eps, zk, kind, ndiv, stream=None):
+    """Feature vector (NIN floats) and the root box size for device-resident
+    points (sources(2,ns), targets(2,nt) as device pointers): the same
+    features as tree_features, with the leaf histogram made on the device
+    (fmm2d_tree_leaf_histogram) -- no tree is copied to the host."""
+    import ctypes as C
+    L = nat.lib()
+    hist = np.zeros(NLEV, dtype=np.float64)
+    bs0 = C.c_double(0.0)
+    nat.check(L.fmm2d_tree_leaf_histogram(src_ptr, int(ns), targ_ptr, int(nt), 0, int(ndiv),
+                                          NLEV, hist.ctypes.data, C.byref(bs0),
+                                          nat.stream_ptr() if stream is None else stream),
+              "tree_leaf_histogram")
+    hist = hist / max(1.0, hist.sum())
+    kz = 0.0 if kind != "h" else abs(complex(zk)) * bs0.value
+    f = np.concatenate([[np.log10(ns + nt) / 7.0, np.log10(1.0 + kz) / 3.0,
+                         -np.log10(eps) / 15.0, 1.0 if kind == "h" else 0.0], hist])
+    return f.astype(np.float32), bs0.value
+
+
 def decode(y, p_default, ndiv_default, margin=0.0):
     """Network output -> (nterms per level, ndiv); margin is taken off the
     predicted order reductions (a proposal's safety allowance)."""
@@ -115,6 +135,20 @@ class Proposer:
         y = self.predict(f[None, :])[0]
         # orders for the levels of a tree up to 4 levels deeper than the default one
         bsx = list(bs) + [bs[-1] / 2 ** k for k in range(1, 5)]
+        p0 = default_nterms(kind, eps, bsx, zk)
+        nterms, ndiv = decode(y, p0, nd0, margin=MARGIN)
+        return {"nterms": nterms, "ndiv": ndiv}
+
+    def propose_device(self, kind, src_ptr, ns, eps, targ_ptr=None, nt=0, zk=None,
+                       nlevels_max=52):
+        """propose() for device-resident points: features from the device leaf
+        histogram, the network on the device, orders for levels
+        0..nlevels_max-1 (the drivers read the entries of their tree's
+        levels)."""
+        nd0 = default_ndiv(eps, ns, nt)
+        f, bs0 = device_features(src_ptr, ns, targ_ptr, nt, eps, zk, kind, nd0)
+        y = self.predict(f[None, :])[0]
+        bsx = [bs0 / 2.0 ** l for l in range(nlevels_max)]
         p0 = default_nterms(kind, eps, bsx, zk)
         nterms, ndiv = decode(y, p0, nd0, margin=MARGIN)
         return {"nterms": nterms, "ndiv": ndiv}
