@@ -26,6 +26,7 @@ SIGNATURES = {
                               _vp, _vp, _d, _vp]),
     "fmm2d_hank103": (_i, [_vp, _i64, _vp, _vp, _vp]),
 }
+SIGNATURES["fmm2d_l2d_direct_noalloc"] = SIGNATURES["fmm2d_l2d_direct"]
 SIGNATURES["fmm2d_tree_build"] = (_i, [_vp, _i64, _vp, _i64, _i, _i, _i, C.POINTER(_i),
                                          C.POINTER(_i), C.POINTER(_vp), _vp])
 SIGNATURES["fmm2d_tree_get"] = (_i, [_vp] * 22)

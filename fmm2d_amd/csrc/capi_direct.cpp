@@ -14,7 +14,7 @@ int run_direct(int family, int nd, double zr, double zi, const double* sources, 
                int ifcharge, const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dipstr,
                const double* dipvec, const double* targ, int64_t nt, int ifpgh,
                fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, double thresh,
-               void* stream) {
+               void* stream, bool noalloc = false) {
   DirectArgs a;
   a.family = family;
   a.nd = nd;
@@ -37,7 +37,7 @@ int run_direct(int family, int nd, double zr, double zi, const double* sources, 
     return FMM2D_ERR_ARG;
   if (!pot || (ifpgh >= 2 && !grad) || (ifpgh >= 3 && !hess)) return FMM2D_ERR_ARG;
   if ((ns > 0 && !sources) || (nt > 0 && !targ)) return FMM2D_ERR_ARG;
-  return direct_eval(a, reinterpret_cast<hipStream_t>(stream));
+  return direct_eval(a, reinterpret_cast<hipStream_t>(stream), noalloc);
 }
 }  // namespace
 
@@ -60,6 +60,15 @@ int fmm2d_l2d_direct(int nd, const double* sources, int64_t ns, int ifcharge,
                      void* stream) {
   return run_direct(1, nd, 0.0, 0.0, sources, ns, ifcharge, charge, ifdipole, dipstr, dipvec,
                     targ, nt, ifpgh, pot, grad, hess, thresh, stream);
+}
+
+int fmm2d_l2d_direct_noalloc(int nd, const double* sources, int64_t ns, int ifcharge,
+                             const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dipstr,
+                             const double* dipvec, const double* targ, int64_t nt, int ifpgh,
+                             fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess,
+                             double thresh, void* stream) {
+  return run_direct(1, nd, 0.0, 0.0, sources, ns, ifcharge, charge, ifdipole, dipstr, dipvec,
+                    targ, nt, ifpgh, pot, grad, hess, thresh, stream, true);
 }
 
 int fmm2d_h2d_direct(int nd, double zk_re, double zk_im, const double* sources, int64_t ns,

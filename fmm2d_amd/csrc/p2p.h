@@ -21,7 +21,10 @@ struct DirectArgs {
   cplx *pot, *grad, *hess;  // incremented
 };
 
-int direct_eval(const DirectArgs& a, hipStream_t st);
+// noalloc: split partial results go to a static per-device array instead of
+// stream-ordered allocations (fewer splits when they do not fit it), and no
+// allocating call is made
+int direct_eval(const DirectArgs& a, hipStream_t st, bool noalloc = false);
 
 // r2d_direct{c,d,cd}{p,g,h} (laplace/rlapkernels2d.f): the Laplace kernels on
 // real densities with real outputs.  The formulas are those of l2d_direct

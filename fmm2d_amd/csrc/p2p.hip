@@ -8,6 +8,8 @@
 // too few target blocks to fill the 256 CUs the source range is split across
 // blocks; each split writes a private copy of the outputs and a second kernel
 // adds the copies into the caller's buffers in split order (deterministic).
+#include <mutex>
+
 #include "devbuf.h"
 #include "p2p.h"
 #include "p2p_math.h"
@@ -90,18 +92,29 @@ __global__ __launch_bounds__(kTile) void direct_kernel(
   cplx* P = pot + blockIdx.y * split_stride_pot;
   cplx* G = grad + blockIdx.y * split_stride_grad;
   cplx* H = hess + blockIdx.y * split_stride_hess;
+  // split copies (stride != 0) are written whole by exactly one block each
+  // (target block x, split y, density chunk z), so they are assigned and need
+  // no clearing; the caller's buffers are incremented
+  const bool assign = split_stride_pot != 0;
 #pragma unroll
   for (int d = 0; d < NDT; ++d) {
     if (d0 + d >= nd) break;
     const int dd = d0 + d;
-    P[(int64_t)nd * it + dd] += acc.pot[d];
+    cplx* q = &P[(int64_t)nd * it + dd];
+    *q = assign ? acc.pot[d] : *q + acc.pot[d];
     if (PG >= 2) {
 #pragma unroll
-      for (int c = 0; c < NG; ++c) G[(int64_t)nd * (NG * it + c) + dd] += acc.grad[d][c];
+      for (int c = 0; c < NG; ++c) {
+        q = &G[(int64_t)nd * (NG * it + c) + dd];
+        *q = assign ? acc.grad[d][c] : *q + acc.grad[d][c];
+      }
     }
     if (PG >= 3) {
 #pragma unroll
-      for (int c = 0; c < NH; ++c) H[(int64_t)nd * (NH * it + c) + dd] += acc.hess[d][c];
+      for (int c = 0; c < NH; ++c) {
+        q = &H[(int64_t)nd * (NH * it + c) + dd];
+        *q = assign ? acc.hess[d][c] : *q + acc.hess[d][c];
+      }
     }
   }
 }
@@ -140,7 +153,35 @@ static void dispatch_cd(const DirectArgs& a, dim3 grid, hipStream_t st, int64_t 
   else dispatch_pg<FAM, 1, 1, NDT>(a.pg, grid, st, a, sl, p, g, h, sp, sg, sh);
 }
 
-int direct_eval(const DirectArgs& a, hipStream_t st) {
+// Split copies of the no-allocation entry point (fmm2d_l2d_direct_noalloc):
+// kScratchCplx complex values (32 MiB) per device, held until the process
+// ends.  Host code never uses the name `scratch` as an address: every device's
+// copy is reached through the pointer hipGetSymbolAddress returned for it.
+constexpr int64_t kScratchCplx = int64_t(1) << 21;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) cplx scratch[kScratchCplx];
+
+namespace {
+std::mutex g_scratch_mu;
+cplx* g_scratch_ptr[kMaxDevices] = {};
+
+// this device's copy of `scratch` (looked up once per device, with it current)
+int scratch_ptr(cplx** out) {
+  int dev = 0;
+  FMM2D_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return FMM2D_ERR_ARG;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    FMM2D_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<cplx*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return FMM2D_OK;
+}
+}  // namespace
+
+int direct_eval(const DirectArgs& a, hipStream_t st, bool noalloc) {
   if (a.nd < 1 || a.ns < 0 || a.nt < 0 || a.pg < 1 || a.pg > 3) return FMM2D_ERR_ARG;
   if (!a.ifcharge && !a.ifdipole) return FMM2D_ERR_ARG;
   if (a.family < CAUCHY || a.family > HELM) return FMM2D_ERR_ARG;
@@ -148,6 +189,11 @@ int direct_eval(const DirectArgs& a, hipStream_t st) {
   const int NDT = (a.nd == 1) ? 1 : 2;
   const int nchunk = (a.nd + NDT - 1) / NDT;
   const int64_t tblocks = ceil_div(a.nt, kTile);
+  const int NG = (a.family == CAUCHY) ? 1 : 2, NH = (a.family == CAUCHY) ? 1 : 3;
+  const int64_t np = (int64_t)a.nd * a.nt;
+  const int64_t ng = (a.pg >= 2) ? np * NG : 0;
+  const int64_t nh = (a.pg >= 3) ? np * NH : 0;
+  const int64_t per = np + ng + nh;  // complex values of one split copy
   // aim for >= 2048 blocks (8 per CU) before splitting the source range
   int64_t nsplit = 1;
   const int64_t want = 2048;
@@ -155,18 +201,21 @@ int direct_eval(const DirectArgs& a, hipStream_t st) {
   const int64_t max_split = (a.ns + kTile - 1) / kTile;
   if (nsplit > max_split) nsplit = max_split;
   if (nsplit > 65535) nsplit = 65535;
+  // without allocation the split copies must fit the static array: fewer
+  // splits, and none when one copy alone does not fit
+  if (noalloc && nsplit > 1 && per * nsplit > kScratchCplx) nsplit = kScratchCplx / per;
+  if (nsplit < 1) nsplit = 1;
   const int64_t split_len = ((a.ns + nsplit - 1) / nsplit + kTile - 1) / kTile * kTile;
   nsplit = (a.ns + split_len - 1) / split_len;
-  const int NG = (a.family == CAUCHY) ? 1 : 2, NH = (a.family == CAUCHY) ? 1 : 3;
-  const int64_t np = (int64_t)a.nd * a.nt;
-  const int64_t ng = (a.pg >= 2) ? np * NG : 0;
-  const int64_t nh = (a.pg >= 3) ? np * NH : 0;
   cplx *p = a.pot, *g = a.grad, *h = a.hess;
   cplx* work = nullptr;
   if (nsplit > 1) {
-    const int64_t per = np + ng + nh;
-    FMM2D_HIP_TRY(hipMallocAsync((void**)&work, sizeof(cplx) * per * nsplit, st));
-    FMM2D_HIP_TRY(hipMemsetAsync(work, 0, sizeof(cplx) * per * nsplit, st));
+    if (noalloc) {
+      const int e = scratch_ptr(&work);
+      if (e != FMM2D_OK) return e;
+    } else {
+      FMM2D_HIP_TRY(hipMallocAsync((void**)&work, sizeof(cplx) * per * nsplit, st));
+    }
     p = work;
     g = work + np * nsplit;
     h = work + (np + ng) * nsplit;
@@ -189,7 +238,7 @@ int direct_eval(const DirectArgs& a, hipStream_t st) {
     if (ng) hipLaunchKernelGGL(add_splits_kernel, dim3(ceil_div(ng, 256)), dim3(256), 0, st, a.grad, g, ng, (int)nsplit);
     if (nh) hipLaunchKernelGGL(add_splits_kernel, dim3(ceil_div(nh, 256)), dim3(256), 0, st, a.hess, h, nh, (int)nsplit);
     FMM2D_HIP_TRY(hipGetLastError());
-    FMM2D_HIP_TRY(hipFreeAsync(work, st));
+    if (!noalloc) FMM2D_HIP_TRY(hipFreeAsync(work, st));
   }
   return FMM2D_OK;
 }

@@ -96,6 +96,25 @@ int fmm2d_l2d_direct(int nd, const double* sources, int64_t ns, int ifcharge,
                      void* stream);
 
 /*
+ * fmm2d_l2d_direct without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and increments as fmm2d_l2d_direct.  The partial results
+ * of a source range split across workgroups go to a static 32 MiB array per
+ * device (2^21 complex values) instead of a stream-ordered allocation; a call
+ * whose split copies would not fit runs with fewer splits (none when one copy,
+ * nd * nt * (1 + 2 [grad] + 3 [hess]) values, exceeds it), so no size is
+ * refused and no allocating routine is called.  Every call on a device shares
+ * that device's array: calls that overlap on one device (different streams or
+ * host threads) race and must be serialised by the caller.
+ */
+int fmm2d_l2d_direct_noalloc(int nd, const double* sources, int64_t ns, int ifcharge,
+                             const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dipstr,
+                             const double* dipvec, const double* targ, int64_t nt, int ifpgh,
+                             fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess,
+                             double thresh, void* stream);
+
+/*
  * Direct evaluation of the real Laplace kernels, INCREMENTS pot/grad/hess:
  *   replaces laplace/rlapkernels2d.f r2d_direct{c,d,cd}{p,g,h} (f2py: r2ddir,
  *   python/fmm2dpy/fmm2d.py:1370), real densities and outputs, the same
