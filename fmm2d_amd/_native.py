@@ -56,6 +56,8 @@ SIGNATURES["fmm2d_cfmm2d"] = (_i, [_i, _d] + _FMM_TAIL + _FMM_OUT)
 SIGNATURES["fmm2d_lfmm2d"] = (_i, [_i, _d] + _FMM_TAIL + [_vp] + _FMM_OUT)
 SIGNATURES["fmm2d_hfmm2d"] = (_i, [_i, _d, _d, _d] + _FMM_TAIL + [_vp] + _FMM_OUT)
 SIGNATURES["fmm2d_batch"] = (_i, [_i, _vp, _i])
+SIGNATURES["fmm2d_hfmm2d_mps"] = (_i, [_i, _d, _d, _d, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                                       _vp, _vp])
 
 
 class FmmOpts(C.Structure):

@@ -175,6 +175,27 @@ int fmm2d_hfmm2d(int nd, double eps, double zk_re, double zk_im, int64_t ns,
   return helmholtz_fmm(a, reinterpret_cast<hipStream_t>(stream));
 }
 
+int fmm2d_hfmm2d_mps(int nd, double eps, double zk_re, double zk_im, int64_t nmpole,
+                     const double* cmpole, const double* rmpole, const int* mterms,
+                     int64_t ntot, const fmm2d_cplx* mpole, const int64_t* impole,
+                     fmm2d_cplx* local, fmm2d_opts* opts, void* stream) {
+  HelmMpsArgs a;
+  a.nd = nd;
+  a.eps = eps;
+  a.zk = cplx{zk_re, zk_im};
+  a.n = nmpole;
+  a.ntot = ntot;
+  a.centers = cmpole;
+  a.rscale = rmpole;
+  a.mterms = mterms;
+  a.mpole = reinterpret_cast<const cplx*>(mpole);
+  a.impole = impole;
+  a.local = reinterpret_cast<cplx*>(local);
+  a.opts = opts;
+  keep_pool_memory();
+  return helmholtz_mps(a, reinterpret_cast<hipStream_t>(stream));
+}
+
 int fmm2d_tree_build(const double* sources, int64_t ns, const double* targ, int64_t nt,
                      int idivflag, int ndiv, int nlmax, int* nlevels, int* nboxes,
                      void** handle, void* stream) {

@@ -825,6 +825,342 @@ void launch_hnear(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd, c
   }
 }
 
+// ---------------------------------------------------- multipole sources (mps)
+// hfmm2d_mps (helmholtz/hfmm2d_mps.f): the sources are multipole expansions,
+// each with its own centre, scale and order, and the outputs are local
+// expansions about the same centres.  A wavefront applies one translation
+// between two expansions of arbitrary centres, scales and orders -- h2dmpmp
+// (kind 0), h2dmploc (kind 1), h2dlocloc (kind 2), helmrouts2d.f:967, 1221,
+// 1096.  In unscaled coefficients all three are the truncated Graf theorem
+//   out_i += sum_j in_j F_{i-j}(k r) e^{-i (i-j) a},  F = J (kinds 0, 2), H (1),
+// (r, a + pi) the polar form of c_out - c_in and F_{-n} = (-1)^n F_n.  Stored
+// multipoles are true / s^|n| and stored locals true * s^|n|; the cylinder
+// functions come scaled by the input's scale s1 (J_n / s1^n, H_n s1^n, as
+// jbessel2d / h2dall), and each term takes the single power of s1 and s2 that
+// gives the output's scaling (one exp2 per term, so no partial product
+// overflows where the scaled result does not).
+struct MpsExp {
+  double x, y, s;  // centre, scale
+  int p;           // order
+  int pad;
+  int64_t off;     // 0-based index of coefficient (density 0, n = 0) in mpole / local
+};
+
+constexpr int kMpsW = 64;
+constexpr size_t kMpsLds = 64 * 1024;  // dynamic LDS of the mps translation kernels
+
+// ft: 2 N + 2 complex of LDS, N = p1 + p2; in(n, d) = in[n sn + d sd], n = -p1..p1.
+__device__ void mps_translate(int kind, cplx zk, int nd, const cplx* __restrict__ in, int64_t isn,
+                              int64_t isd, double x1, double y1, double s1, int p1,
+                              cplx* __restrict__ out, int64_t osn, int64_t osd, double x2,
+                              double y2, double s2, int p2, cplx* ft) {
+  const int N = p1 + p2;
+  const double dx = x2 - x1, dy = y2 - y1;
+  const double r = sqrt(dx * dx + dy * dy);
+  const double th = atan2(dy, dx) - kPi;
+  __syncthreads();  // the previous translation's reads of ft are done
+  if (threadIdx.x == 0) {
+    const cplx z = zk * r;
+    if (kind == 1) scaled_h(z, s1, N > 1 ? N : 1, ft + N);
+    else scaled_j(z, s1, N > 1 ? N : 1, ft + N);
+  }
+  __syncthreads();
+  for (int n = threadIdx.x; n <= N; n += kMpsW) {
+    const cplx f = ft[N + n];
+    double sn, cs;
+    sincos(n * th, &sn, &cs);
+    ft[N + n] = f * cplx{cs, -sn};
+    if (n > 0) {
+      const cplx g = f * cplx{cs, sn};
+      ft[N - n] = (n & 1) ? cplx{-g.re, -g.im} : g;
+    }
+  }
+  __syncthreads();
+  const double a = (kind == 1) ? -1.0 : 1.0, b = (kind == 2) ? -1.0 : 1.0;
+  const double c = (kind == 0) ? -1.0 : 1.0;
+  const double l1 = log2(s1), l2 = log2(s2);
+  for (int i = (int)threadIdx.x - p2; i <= p2; i += kMpsW) {
+    const double ei = c * (double)abs(i) * l2;
+    for (int d = 0; d < nd; ++d) {
+      cplx acc = mkc(0.0);
+      for (int j = -p1; j <= p1; ++j) {
+        const double e = (a * (double)abs(i - j) + b * (double)abs(j)) * l1 + ei;
+        acc = acc + (in[j * isn + d * isd] * ft[N + i - j]) * exp2(e);
+      }
+      cplx* o = out + i * osn + d * osd;
+      *o = *o + acc;
+    }
+  }
+}
+
+__global__ void k_mps_gather(int64_t n, const int* perm, const double* cen, const double* rs,
+                             const int* mt, const int64_t* imp, MpsExp* e) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = perm[i];
+  MpsExp v;
+  v.x = cen[2 * j];
+  v.y = cen[2 * j + 1];
+  v.s = rs[j];
+  v.p = mt[j];
+  v.pad = 0;
+  v.off = imp[j] - 1;
+  e[i] = v;
+}
+
+__global__ void k_mps_pmax(int64_t n, const int* mt, int* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicMax(out, mt[i]);
+  atomicMin(out + 1, mt[i]);
+}
+
+// Step 1 (hfmm2d_mps.f:505-530): each leaf's expansions shifted to the leaf's
+// multipole (h2dmpmp), one wavefront per box.
+__global__ __launch_bounds__(kMpsW) void k_mps_p2m(int nd, cplx zk, const HLev* lev,
+                                                   const int* ilevel, const int* nchild,
+                                                   const int* se, const double* centers,
+                                                   const MpsExp* ex, const cplx* mpole,
+                                                   cplx* mp) {
+  extern __shared__ cplx ft[];
+  const int b = blockIdx.x;
+  const int l = ilevel[b];
+  if (l < 2 || !lev[l].on || nchild[b] != 0) return;
+  const HLev V = lev[l];
+  cplx* o = hexp(mp, V, b, nd, 0);
+  for (int e = se[2 * b]; e < se[2 * b + 1]; ++e) {
+    const MpsExp E = ex[e];
+    mps_translate(0, zk, nd, mpole + E.off + (int64_t)E.p * nd, nd, 1, E.x, E.y, E.s, E.p, o, 1,
+                  2 * V.p + 1, centers[2 * b], centers[2 * b + 1], V.s, V.p, ft);
+  }
+}
+
+// Step 2 (hfmm2d_mps.f:535-565): list-4 expansions into the box's local
+// (h2dmploc), one wavefront per box.
+__global__ __launch_bounds__(kMpsW) void k_mps_p2l(int nd, cplx zk, const HLev* lev,
+                                                   const int* ilevel, const int* n4,
+                                                   const int* l4, const int* se,
+                                                   const double* centers, const MpsExp* ex,
+                                                   const cplx* mpole, cplx* loc) {
+  extern __shared__ cplx ft[];
+  const int b = blockIdx.x;
+  const int l = ilevel[b];
+  if (l < 2 || !lev[l].on || n4[b] == 0) return;
+  const HLev V = lev[l];
+  cplx* o = hexp(loc, V, b, nd, 0);
+  for (int k = 0; k < n4[b]; ++k) {
+    const int jb = l4[(int64_t)b * kMaxList4 + k];
+    for (int e = se[2 * jb]; e < se[2 * jb + 1]; ++e) {
+      const MpsExp E = ex[e];
+      mps_translate(1, zk, nd, mpole + E.off + (int64_t)E.p * nd, nd, 1, E.x, E.y, E.s, E.p, o,
+                    1, 2 * V.p + 1, centers[2 * b], centers[2 * b + 1], V.s, V.p, ft);
+    }
+  }
+}
+
+// Steps 6-7 (hfmm2d_mps.f:880-945): one wavefront per expansion clears its
+// local, adds the list-3 boxes' multipoles (h2dmploc; the reference gates
+// them on the list-3 level, ilev + 1) and its leaf's local (h2dlocloc).
+__global__ __launch_bounds__(kMpsW) void k_mps_far(int64_t n, int nd, cplx zk, const HLev* lev,
+                                                   int nl, const int* leaf, const int* ilevel,
+                                                   const int* n3, const int* l3,
+                                                   const double* centers, const MpsExp* ex,
+                                                   const cplx* mp, const cplx* loc,
+                                                   cplx* local) {
+  extern __shared__ cplx ft[];
+  const int64_t e = blockIdx.x;
+  const MpsExp E = ex[e];
+  cplx* o = local + E.off + (int64_t)E.p * nd;
+  for (int k = threadIdx.x; k < nd * (2 * E.p + 1); k += kMpsW) local[E.off + k] = mkc(0.0);
+  const int b = leaf[e];
+  const int l = ilevel[b];
+  if (l >= 1 && l <= nl - 1 && lev[l + 1].on) {
+    for (int k = 0; k < n3[b]; ++k) {
+      const int jb = l3[(int64_t)b * kMaxList3 + k];
+      const HLev W = lev[ilevel[jb]];
+      mps_translate(1, zk, nd, hexp(mp, W, jb, nd, 0), 1, 2 * W.p + 1, centers[2 * jb],
+                    centers[2 * jb + 1], W.s, W.p, o, nd, 1, E.x, E.y, E.s, E.p, ft);
+    }
+  }
+  const HLev V = lev[l];
+  if (V.on)
+    mps_translate(2, zk, nd, hexp(loc, V, b, nd, 0), 1, 2 * V.p + 1, centers[2 * b],
+                  centers[2 * b + 1], V.s, V.p, o, nd, 1, E.x, E.y, E.s, E.p, ft);
+}
+
+// Step 8 (hfmm2d_mps.f:960-1000): the expansions of the leaf's list-1 boxes
+// whose centres are farther than thresh (so not the expansion itself),
+// h2dmploc, one wavefront per expansion.
+__global__ __launch_bounds__(kMpsW) void k_mps_near(int64_t n, int nd, cplx zk, double thresh,
+                                                    const int* leaf, const int* n1,
+                                                    const int* l1, const int* se,
+                                                    const MpsExp* ex, const cplx* mpole,
+                                                    cplx* local) {
+  extern __shared__ cplx ft[];
+  const int64_t e = blockIdx.x;
+  const MpsExp E = ex[e];
+  cplx* o = local + E.off + (int64_t)E.p * nd;
+  const int b = leaf[e];
+  for (int k = 0; k < n1[b]; ++k) {
+    const int jb = l1[(int64_t)b * kMaxList1 + k];
+    for (int j = se[2 * jb]; j < se[2 * jb + 1]; ++j) {
+      const MpsExp S = ex[j];
+      const double dx = S.x - E.x, dy = S.y - E.y;
+      const double d = sqrt(dx * dx + dy * dy);
+      if (!(d > thresh)) continue;
+      mps_translate(1, zk, nd, mpole + S.off + (int64_t)S.p * nd, nd, 1, S.x, S.y, S.s, S.p, o,
+                    nd, 1, E.x, E.y, E.s, E.p, ft);
+    }
+  }
+}
+
+// Box-to-box passes shared by hfmm2d and hfmm2d_mps (hfmm2d.f:1021-1300,
+// hfmm2d_mps.f:520-760): translation operators for every level, then M2M
+// (finest parents first), M2L level by level and L2L (coarsest first), on the
+// per-level multipole and local blocks.  pg / pgt say which boxes hold
+// evaluation points (sources / targets), as the planners need.
+void helm_box_passes(const Tree& t, const Lists& L, const std::vector<HLev>& lev,
+                     const std::vector<long double>& sl, const std::vector<int>& nterms,
+                     int nd, cplx zk, int pg, int pgt, cplx* mpp, cplx* locp, Timer& tm,
+                     hipStream_t st) {
+  const int nl = t.nlevels;
+  // ---- translation operators, level by level
+  OpTables T;
+  std::vector<int> op_m2l(nl + 1, -1), op_m2m(nl + 1, -1), op_l2l(nl + 1, -1);
+  const lc k(zk.re, zk.im);
+  for (int l = 1; l <= nl; ++l) {
+    const long double bs = t.boxsize[l];
+    const int p = nterms[l];
+    if (l >= 2 && lev[l].on) {
+      // M2L: source - target = (dx, dy) bs, max(|dx|,|dy|) >= 2 (h2dmploc,
+      // helmrouts2d.f:1221-1380)
+      op_m2l[l] = (int)T.desc.size();
+      const int w1 = push_powers(T, sl[l], 2 * p);
+      const int w2 = push_powers(T, 1.0L, p);
+      std::vector<std::vector<lc>> hr(19);
+      for (int dx = -3; dx <= 3; ++dx)
+        for (int dy = -3; dy <= 3; ++dy) {
+          const int r2 = dx * dx + dy * dy;
+          if (std::abs(dx) <= 1 && std::abs(dy) <= 1) {
+            add_op(T, p, p, 0, w1, w2, 0, 0, 0);  // never referenced: zero table slot
+            T.desc.back().g = -1;
+            continue;
+          }
+          if (hr[r2].empty())
+            hr[r2] = host_scaled_h(k * (bs * std::sqrt((long double)r2)), sl[l], 2 * p);
+          const int g = push_table(T, hr[r2], 2 * p, std::atan2((long double)dy, (long double)dx));
+          add_op(T, p, p, g, w1, w2, -1, 1, 1);
+        }
+    }
+    if (l <= nl - 1 && lev[l].on) {
+      // M2M into level l from children at l + 1 (h2dmpmp, helmrouts2d.f:967-1090) and
+      // L2L from level l to its children (h2dlocloc, helmrouts2d.f:1096-1215);
+      // child - parent = (sx, sy) bs / 4
+      const int pc = nterms[l + 1];
+      const int mm = p + pc;
+      const lc z = k * (bs * std::sqrt(2.0L) / 4.0L);
+      std::vector<lc> J = bessel_j(z, std::max(mm, 1));
+      std::vector<lc> Jc(J), Jp(J);  // scaled by the child's / parent's s^-m
+      for (int m = 0; m <= mm; ++m) {
+        Jc[m] = J[m] / std::pow(sl[l + 1], (long double)m);
+        Jp[m] = J[m] / std::pow(sl[l], (long double)m);
+      }
+      op_m2m[l] = (int)T.desc.size();
+      {
+        const int w1 = push_powers(T, sl[l + 1], 2 * std::max(p, pc));
+        const int w2 = push_powers(T, sl[l + 1] / sl[l], p);
+        for (int q = 0; q < 4; ++q) {
+          const long double sx = (q & 1) ? 1.0L : -1.0L, sy = (q & 2) ? 1.0L : -1.0L;
+          const int g = push_table(T, Jc, mm, std::atan2(sy, sx));
+          add_op(T, pc, p, g, w1, w2, 1, 1, -1);
+        }
+      }
+      op_l2l[l] = (int)T.desc.size();
+      {
+        const int w1 = push_powers(T, sl[l], 2 * std::max(p, pc));
+        const int w2 = push_powers(T, sl[l + 1] / sl[l], pc);
+        for (int q = 0; q < 4; ++q) {
+          const long double sx = (q & 1) ? 1.0L : -1.0L, sy = (q & 2) ? 1.0L : -1.0L;
+          const int g = push_table(T, Jp, mm, std::atan2(-sy, -sx));
+          add_op(T, p, pc, g, w1, w2, 1, -1, 1);
+        }
+      }
+    }
+  }
+  DBuf<double> d_af(std::max<int64_t>(T.af_total, 1), st);
+  {
+    // drop the never-referenced slots from the build list
+    std::vector<OpDesc> build;
+    int64_t maxaf = 0;
+    for (const OpDesc& od : T.desc)
+      if (od.g >= 0) {
+        build.push_back(od);
+        maxaf = std::max(maxaf, afrag_stride(2 * od.pin + 1, 2 * od.pout + 1));
+      }
+    if (!build.empty()) {
+      DBuf<OpDesc> d_desc = upload(build, st);
+      DBuf<cplx> d_g = upload(T.g, st);
+      DBuf<double> d_w = upload(T.w, st);
+      const unsigned gx = (unsigned)std::min<int64_t>((maxaf + 255) / 256, 64);
+      hipLaunchKernelGGL(k_build_ops, dim3(gx, (unsigned)build.size()), dim3(256), 0, st,
+                         d_desc.p, d_g.p, d_w.p, d_af.p);
+      hip_throw(hipGetLastError(), "build_ops");
+      hip_throw(hipStreamSynchronize(st), "sync");
+    }
+  }
+  auto opptr = [&](int first) { return d_af.p + T.desc[first].af; };
+  tm.mark(12);
+
+  ChildPlan cp;
+  plan_children(t, pg, pgt, cp, st);
+  DBuf<int> d_op4 = upload(std::vector<int>{0, 1, 2, 3}, st);
+  // ---- step 3: M2M, finest parents first (hfmm2d.f:1021-1100)
+  for (int l = nl - 1; l >= 1; --l) {
+    if (op_m2m[l] < 0) continue;
+    const HLev &P = lev[l], &C = lev[l + 1];
+    launch_translate(0, 0, nd,
+                     TrBufs{mpp + C.off, C.box0, 2 * C.p + 1, mpp + P.off, P.box0,
+                            2 * P.p + 1},
+                     4, {TrGroup{cp.lev_pos[l], cp.lev_pos[l + 1], 0, 0}}, cp.list.p, cp.up.p,
+                     d_op4.p, opptr(op_m2m[l]), t.ilevel.p, nullptr, nullptr, st);
+  }
+  tm.mark(2);
+  // ---- step 4: M2L level by level, targets grouped by child position
+  {
+    M2LPlan m2l;
+    plan_m2l(t, L, pg, pgt, m2l, st);
+    if (m2l.ntg > 0) {
+      DBuf<int> d_soff = upload(m2l_slot_offsets(), st);
+      tm.mark(3);  // planning
+      for (int l = 2; l <= nl; ++l) {
+        if (op_m2l[l] < 0) continue;
+        std::vector<TrGroup> groups;
+        for (int q = 0; q < 4; ++q)
+          groups.push_back(TrGroup{m2l.lev(q, l, nl), m2l.lev(q, l + 1, nl), 0, q});
+        const HLev& V = lev[l];
+        launch_translate(0, 0, nd,
+                         TrBufs{mpp + V.off, V.box0, 2 * V.p + 1, locp + V.off, V.box0,
+                                2 * V.p + 1},
+                         kSlots, groups, m2l.tlist.p, L.s2.p, d_soff.p, opptr(op_m2l[l]),
+                         t.ilevel.p, nullptr, nullptr, st, false, true, true);
+      }
+      tm.mark(13);  // the M2L kernels alone
+    }
+  }
+  tm.mark(3);
+  // ---- step 5: L2L, coarsest first (hfmm2d.f:1225-1300)
+  for (int l = 1; l <= nl - 1; ++l) {
+    if (op_l2l[l] < 0) continue;
+    const HLev &P = lev[l], &C = lev[l + 1];
+    launch_translate(1, 0, nd,
+                     TrBufs{locp + P.off, P.box0, 2 * P.p + 1, locp + C.off, C.box0,
+                            2 * C.p + 1},
+                     4, {TrGroup{cp.lev_pos[l], cp.lev_pos[l + 1], 0, 0}}, cp.list.p,
+                     cp.down.p, d_op4.p, opptr(op_l2l[l]), t.ilevel.p, nullptr, nullptr, st);
+  }
+  tm.mark(4);
+}
+
 }  // namespace
 
 int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
@@ -976,141 +1312,7 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     hip_throw(hipGetLastError(), "p2l");
     tm.mark(1);
 
-    // ---- translation operators, level by level
-    OpTables T;
-    std::vector<int> op_m2l(nl + 1, -1), op_m2m(nl + 1, -1), op_l2l(nl + 1, -1);
-    const lc k(zk.re, zk.im);
-    for (int l = 1; l <= nl; ++l) {
-      const long double bs = t.boxsize[l];
-      const int p = nterms[l];
-      if (l >= 2 && lev[l].on) {
-        // M2L: source - target = (dx, dy) bs, max(|dx|,|dy|) >= 2 (h2dmploc,
-        // helmrouts2d.f:1221-1380)
-        op_m2l[l] = (int)T.desc.size();
-        const int w1 = push_powers(T, sl[l], 2 * p);
-        const int w2 = push_powers(T, 1.0L, p);
-        std::vector<std::vector<lc>> hr(19);
-        for (int dx = -3; dx <= 3; ++dx)
-          for (int dy = -3; dy <= 3; ++dy) {
-            const int r2 = dx * dx + dy * dy;
-            if (std::abs(dx) <= 1 && std::abs(dy) <= 1) {
-              add_op(T, p, p, 0, w1, w2, 0, 0, 0);  // never referenced: zero table slot
-              T.desc.back().g = -1;
-              continue;
-            }
-            if (hr[r2].empty())
-              hr[r2] = host_scaled_h(k * (bs * std::sqrt((long double)r2)), sl[l], 2 * p);
-            const int g = push_table(T, hr[r2], 2 * p, std::atan2((long double)dy, (long double)dx));
-            add_op(T, p, p, g, w1, w2, -1, 1, 1);
-          }
-      }
-      if (l <= nl - 1 && lev[l].on) {
-        // M2M into level l from children at l + 1 (h2dmpmp, helmrouts2d.f:967-1090) and
-        // L2L from level l to its children (h2dlocloc, helmrouts2d.f:1096-1215);
-        // child - parent = (sx, sy) bs / 4
-        const int pc = nterms[l + 1];
-        const int mm = p + pc;
-        const lc z = k * (bs * std::sqrt(2.0L) / 4.0L);
-        std::vector<lc> J = bessel_j(z, std::max(mm, 1));
-        std::vector<lc> Jc(J), Jp(J);  // scaled by the child's / parent's s^-m
-        for (int m = 0; m <= mm; ++m) {
-          Jc[m] = J[m] / std::pow(sl[l + 1], (long double)m);
-          Jp[m] = J[m] / std::pow(sl[l], (long double)m);
-        }
-        op_m2m[l] = (int)T.desc.size();
-        {
-          const int w1 = push_powers(T, sl[l + 1], 2 * std::max(p, pc));
-          const int w2 = push_powers(T, sl[l + 1] / sl[l], p);
-          for (int q = 0; q < 4; ++q) {
-            const long double sx = (q & 1) ? 1.0L : -1.0L, sy = (q & 2) ? 1.0L : -1.0L;
-            const int g = push_table(T, Jc, mm, std::atan2(sy, sx));
-            add_op(T, pc, p, g, w1, w2, 1, 1, -1);
-          }
-        }
-        op_l2l[l] = (int)T.desc.size();
-        {
-          const int w1 = push_powers(T, sl[l], 2 * std::max(p, pc));
-          const int w2 = push_powers(T, sl[l + 1] / sl[l], pc);
-          for (int q = 0; q < 4; ++q) {
-            const long double sx = (q & 1) ? 1.0L : -1.0L, sy = (q & 2) ? 1.0L : -1.0L;
-            const int g = push_table(T, Jp, mm, std::atan2(-sy, -sx));
-            add_op(T, p, pc, g, w1, w2, 1, -1, 1);
-          }
-        }
-      }
-    }
-    DBuf<double> d_af(std::max<int64_t>(T.af_total, 1), st);
-    {
-      // drop the never-referenced slots from the build list
-      std::vector<OpDesc> build;
-      int64_t maxaf = 0;
-      for (const OpDesc& od : T.desc)
-        if (od.g >= 0) {
-          build.push_back(od);
-          maxaf = std::max(maxaf, afrag_stride(2 * od.pin + 1, 2 * od.pout + 1));
-        }
-      if (!build.empty()) {
-        DBuf<OpDesc> d_desc = upload(build, st);
-        DBuf<cplx> d_g = upload(T.g, st);
-        DBuf<double> d_w = upload(T.w, st);
-        const unsigned gx = (unsigned)std::min<int64_t>((maxaf + 255) / 256, 64);
-        hipLaunchKernelGGL(k_build_ops, dim3(gx, (unsigned)build.size()), dim3(256), 0, st,
-                           d_desc.p, d_g.p, d_w.p, d_af.p);
-        hip_throw(hipGetLastError(), "build_ops");
-        hip_throw(hipStreamSynchronize(st), "sync");
-      }
-    }
-    auto opptr = [&](int first) { return d_af.p + T.desc[first].af; };
-    tm.mark(12);
-
-    ChildPlan cp;
-    plan_children(t, pg, pgt, cp, st);
-    DBuf<int> d_op4 = upload(std::vector<int>{0, 1, 2, 3}, st);
-    // ---- step 3: M2M, finest parents first (hfmm2d.f:1021-1100)
-    for (int l = nl - 1; l >= 1; --l) {
-      if (op_m2m[l] < 0) continue;
-      const HLev &P = lev[l], &C = lev[l + 1];
-      launch_translate(0, 0, nd,
-                       TrBufs{mp.p + C.off, C.box0, 2 * C.p + 1, mp.p + P.off, P.box0,
-                              2 * P.p + 1},
-                       4, {TrGroup{cp.lev_pos[l], cp.lev_pos[l + 1], 0, 0}}, cp.list.p, cp.up.p,
-                       d_op4.p, opptr(op_m2m[l]), t.ilevel.p, nullptr, nullptr, st);
-    }
-    tm.mark(2);
-    // ---- step 4: M2L level by level, targets grouped by child position
-    {
-      M2LPlan m2l;
-      plan_m2l(t, L, pg, pgt, m2l, st);
-      if (m2l.ntg > 0) {
-        DBuf<int> d_soff = upload(m2l_slot_offsets(), st);
-        tm.mark(3);  // planning
-        for (int l = 2; l <= nl; ++l) {
-          if (op_m2l[l] < 0) continue;
-          std::vector<TrGroup> groups;
-          for (int q = 0; q < 4; ++q)
-            groups.push_back(TrGroup{m2l.lev(q, l, nl), m2l.lev(q, l + 1, nl), 0, q});
-          const HLev& V = lev[l];
-          launch_translate(0, 0, nd,
-                           TrBufs{mp.p + V.off, V.box0, 2 * V.p + 1, loc.p + V.off, V.box0,
-                                  2 * V.p + 1},
-                           kSlots, groups, m2l.tlist.p, L.s2.p, d_soff.p, opptr(op_m2l[l]),
-                           t.ilevel.p, nullptr, nullptr, st, false, true, true);
-        }
-        tm.mark(13);  // the M2L kernels alone
-      }
-    }
-    tm.mark(3);
-    // ---- step 5: L2L, coarsest first (hfmm2d.f:1225-1300)
-    for (int l = 1; l <= nl - 1; ++l) {
-      if (op_l2l[l] < 0) continue;
-      const HLev &P = lev[l], &C = lev[l + 1];
-      launch_translate(1, 0, nd,
-                       TrBufs{loc.p + P.off, P.box0, 2 * P.p + 1, loc.p + C.off, C.box0,
-                              2 * C.p + 1},
-                       4, {TrGroup{cp.lev_pos[l], cp.lev_pos[l + 1], 0, 0}}, cp.list.p,
-                       cp.down.p, d_op4.p, opptr(op_l2l[l]), t.ilevel.p, nullptr, nullptr, st);
-    }
-    tm.mark(4);
+    helm_box_passes(t, L, lev, sl, nterms, nd, zk, pg, pgt, mp.p, loc.p, tm, st);
     // ---- steps 6+7: list 3 and L2P
     auto eval = [&](int p_, int64_t n, const double* xy, const int* lf, cplx* P_, cplx* G_,
                     cplx* H_) {
@@ -1194,6 +1396,132 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
         ow->near_pairs = near;
         ow->m2l_pairs = m2l_pairs;
       }
+    }
+    return FMM2D_OK;
+  } catch (const HipError& e) {
+    fprintf(stderr, "fmm2d_amd: %s failed: %s\n", e.what, hipGetErrorString(e.err));
+    return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    return FMM2D_ERR_HIP;
+  }
+}
+
+
+int helmholtz_mps(const HelmMpsArgs& a, hipStream_t st) {
+  if (a.nd < 1 || a.n < 1 || a.n > INT32_MAX || a.ntot < 1) return FMM2D_ERR_ARG;
+  if (!a.centers || !a.rscale || !a.mterms || !a.mpole || !a.impole || !a.local)
+    return FMM2D_ERR_ARG;
+  if (a.zk.re == 0.0 && a.zk.im == 0.0) return FMM2D_ERR_ARG;
+  const fmm2d_opts* o = a.opts;
+  if (o && o->precision != 0) return FMM2D_ERR_UNSUPPORTED;
+  if (o) {
+    const int e = check_nterms_override(o->n_nterms, o->nterms, 1, kMaxNtermsHelmholtz);
+    if (e != FMM2D_OK) return e;
+  }
+  try {
+    double tinfo[16] = {0};
+    Timer tm(o != nullptr, st);
+    const int64_t n = a.n;
+    const int nd = a.nd;
+    const cplx zk = a.zk;
+    // orders of the expansions (the LDS of the translation kernels holds
+    // 2 (p1 + p2) + 2 coefficients)
+    int pm[2] = {0, 0};
+    {
+      DBuf<int> d(2, st);
+      const int init[2] = {INT32_MIN, INT32_MAX};
+      hip_throw(hipMemcpyAsync(d.p, init, sizeof(init), hipMemcpyHostToDevice, st), "copy");
+      launch1d(k_mps_pmax, n, st, a.mterms, d.p);
+      hip_throw(hipMemcpyAsync(pm, d.p, sizeof(pm), hipMemcpyDeviceToHost, st), "copy");
+      hip_throw(hipStreamSynchronize(st), "sync");
+    }
+    if (pm[1] < 0) return FMM2D_ERR_ARG;
+    // ---- tree over the centres, one expansion per leaf by default
+    // (hfmm2d_mps.f:100-150: idivflag 0, ndiv 1; hfmm2d_mps_ndiv.f: ndiv given)
+    int ndiv = (o && o->ndiv > 0) ? o->ndiv : 1;
+    Tree t;
+    build_tree(a.centers, n, nullptr, 0, ndiv, 0, 51, t, st);
+    Lists L;
+    build_lists(t, L, st);
+    tm.mark(8);
+    const int nl = t.nlevels, nb = t.nboxes;
+    // ---- per-level orders and scales (hfmm2d_mps.f:190-200)
+    std::vector<int> nterms(nl + 1);
+    for (int l = 0; l <= nl; ++l) {
+      int ier = 0;
+      nterms[l] = h2dterms(t.boxsize[l], zk, a.eps, &ier);
+      if (ier) return 13;
+    }
+    if (o && o->n_nterms > 0 && o->nterms)
+      for (int l = 0; l <= nl; ++l) nterms[l] = o->nterms[std::min(l, o->n_nterms - 1)];
+    const long double absk = std::hypot((long double)zk.re, (long double)zk.im);
+    std::vector<HLev> lev(nl + 1);
+    std::vector<long double> sl(nl + 1);
+    int64_t ncoef = 0;
+    int pbox = 0;
+    for (int l = 0; l <= nl; ++l) {
+      if (nterms[l] < 1 || nterms[l] > kMaxNtermsHelmholtz) return FMM2D_ERR_NTERMS;
+      HLev& v = lev[l];
+      sl[l] = std::min(absk * (long double)t.boxsize[l] / (2.0L * (long double)kPi), 1.0L);
+      v.s = (double)sl[l];
+      v.p = nterms[l];
+      v.box0 = t.lstart[l];
+      v.off = ncoef;
+      v.on = (zk.im * t.boxsize[l] < kZkiUpBound) ? 1 : 0;
+      const int nbl = ((l < nl) ? t.lstart[l + 1] : nb) - t.lstart[l];
+      ncoef += (int64_t)nbl * nd * (2 * nterms[l] + 1);
+      pbox = std::max(pbox, nterms[l]);
+    }
+    const int nmax = 2 * std::max(pm[0], pbox);  // largest p1 + p2 of a translation
+    const size_t lds = sizeof(cplx) * (2 * (size_t)nmax + 2);
+    if (lds > kMpsLds) return FMM2D_ERR_UNSUPPORTED;
+    DBuf<HLev> d_lev = upload(lev, st);
+    DBuf<MpsExp> ex(n, st);
+    launch1d(k_mps_gather, n, st, t.isrc.p, a.centers, a.rscale, a.mterms, a.impole, ex.p);
+    DBuf<int> leaf(n, st);
+    launch1d(k_leaf_of, nb, st, t.nchild.p, t.isrcse.p, leaf.p);
+    DBuf<cplx> mp(ncoef, st), loc(ncoef, st);
+    mp.zero();
+    loc.zero();
+    tm.mark(9);
+    const int* se = t.isrcse.p;
+    // ---- step 1: expansions into their leaves' multipoles
+    hipLaunchKernelGGL(k_mps_p2m, dim3(nb), dim3(kMpsW), lds, st, nd, zk, d_lev.p, t.ilevel.p,
+                       t.nchild.p, se, t.centers.p, ex.p, a.mpole, mp.p);
+    hip_throw(hipGetLastError(), "mps p2m");
+    tm.mark(0);
+    // ---- step 2: list-4 expansions into box locals
+    hipLaunchKernelGGL(k_mps_p2l, dim3(nb), dim3(kMpsW), lds, st, nd, zk, d_lev.p, t.ilevel.p,
+                       L.n4.p, L.l4.p, se, t.centers.p, ex.p, a.mpole, loc.p);
+    hip_throw(hipGetLastError(), "mps p2l");
+    tm.mark(1);
+    // ---- steps 3-5: M2M, M2L, L2L between boxes (every box holding an
+    // expansion needs its local: pg = 1)
+    helm_box_passes(t, L, lev, sl, nterms, nd, zk, 1, 0, mp.p, loc.p, tm, st);
+    // ---- steps 6-7: list-3 multipoles and the leaf's local at every centre
+    hipLaunchKernelGGL(k_mps_far, dim3((unsigned)n), dim3(kMpsW), lds, st, n, nd, zk, d_lev.p,
+                       nl, leaf.p, t.ilevel.p, L.n3.p, L.l3.p, t.centers.p, ex.p, mp.p, loc.p,
+                       a.local);
+    hip_throw(hipGetLastError(), "mps far");
+    tm.mark(6);
+    // ---- step 8: list-1 expansions, thresh = |zk| boxsize(0) 2^-51 (hfmm2d_mps.f:962)
+    const int ifnear = o ? o->ifnear : 1;
+    if (ifnear) {
+      const double thresh = std::hypot(zk.re, zk.im) * t.boxsize[0] * std::ldexp(1.0, -51);
+      hipLaunchKernelGGL(k_mps_near, dim3((unsigned)n), dim3(kMpsW), lds, st, n, nd, zk, thresh,
+                         leaf.p, L.n1.p, L.l1.p, se, ex.p, a.mpole, a.local);
+      hip_throw(hipGetLastError(), "mps near");
+    }
+    tm.mark(7);
+    hip_throw(hipStreamSynchronize(st), "sync");
+    if (o) {
+      fmm2d_opts* ow = const_cast<fmm2d_opts*>(o);
+      tm.collect(tinfo);
+      for (int kk = 0; kk < 16; ++kk) ow->timeinfo[kk] = tinfo[kk];
+      ow->nlevels = nl;
+      ow->nboxes = nb;
     }
     return FMM2D_OK;
   } catch (const HipError& e) {

@@ -554,6 +554,49 @@ def comperr(*, ntest, out, outex, pg=0, pgt=0, nd=1, cauchy=0, bh=0):
 
 
 # --------------------------------------------------------------------------
+# Helmholtz FMM on multipole sources
+def hfmm2d_mps(*, eps, zk, centers, rscales, nterms, mpole, impole=None, nd=1, ndiv=None):
+    """Local expansions at every centre due to all other multipole expansions
+    (helmholtz/hfmm2d_mps.f:19; with ndiv, helmholtz/hfmm2d_mps_ndiv.f:70 --
+    ndiv = number of expansions evaluates every pair directly).
+
+    centers (2, n), rscales (n,), nterms (n,) orders; mpole: flat complex array
+    holding expansion i's coefficients (nd, -nterms[i]..nterms[i]) in Fortran
+    order (density fastest) from offset impole[i] (0-based; default: the
+    expansions one after the other).  Returns local, same layout as mpole
+    (entries outside the expansions' blocks are zero).  The reference has no
+    Python wrapper for this routine; the arguments mirror its Fortran ones.
+    """
+    import torch
+    cen = np.asarray(centers, dtype=np.float64)
+    assert cen.shape[0] == 2, "The first dimension of centers must be 2"
+    n = cen.shape[1]
+    nt_ = np.asarray(nterms, dtype=np.int32).reshape(n)
+    rs = np.asarray(rscales, dtype=np.float64).reshape(n)
+    mp = np.asarray(mpole, dtype=np.complex128).ravel()
+    if impole is None:
+        impole = np.concatenate([[0], np.cumsum(nd * (2 * nt_[:-1].astype(np.int64) + 1))])
+    imp = np.asarray(impole, dtype=np.int64).reshape(n)
+    assert np.all(imp >= 0) and np.all(imp + nd * (2 * nt_.astype(np.int64) + 1) <= mp.size), \
+        "Every expansion must lie inside mpole"
+    dev = nat.torch_device()
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+         for k, v in dict(cen=cen.T, rs=rs, nt=nt_, mp=mp, imp=imp + 1).items()}
+    loc = torch.zeros_like(d["mp"])
+    o = nat.opts_struct({"ndiv": ndiv} if ndiv is not None else None)
+    zk = complex(zk)
+    L = nat.lib()
+    ier = L.fmm2d_hfmm2d_mps(nd, float(eps), zk.real, zk.imag, n, nat.ptr(d["cen"]),
+                             nat.ptr(d["rs"]), nat.ptr(d["nt"]), mp.size, nat.ptr(d["mp"]),
+                             nat.ptr(d["imp"]), nat.ptr(loc), nat.byref_opts(o),
+                             nat.stream_ptr())
+    nat.check(ier, "hfmm2d_mps")
+    if ier != 0:
+        raise RuntimeError(f"fmm2d_amd: hfmm2d_mps returned ier = {ier}")
+    return loc.cpu().numpy()
+
+
+# --------------------------------------------------------------------------
 # families of the reference that are outside this library's scope
 def _out_of_scope(name):
     def f(*args, **kwargs):

@@ -195,6 +195,30 @@ int fmm2d_hfmm2d(int nd, double eps, double zk_re, double zk_im, int64_t ns,
                  fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg, fmm2d_opts* opts, void* stream);
 
 /*
+ * Helmholtz FMM whose sources are multipole expansions; OVERWRITES the local
+ * expansions' blocks of local.
+ *   replaces helmholtz/hfmm2d_mps.f:19 hfmm2d_mps(nd,eps,zk,nmpole,cmpole,
+ *   rmpole,mterms,ntot,mpole,impole,local,ier) and, through opts->ndiv,
+ *   helmholtz/hfmm2d_mps_ndiv.f:70 hfmm2d_mps_ndiv (ndiv = nmpole evaluates
+ *   every pair directly, as hfmm2d_mps_direct).
+ * Expansion i has centre cmpole(2,i), scale rmpole(i), order mterms(i) and
+ * coefficients mpole(nd, -mterms(i):mterms(i)) starting at the 1-based offset
+ * impole(i) of mpole (complex values, density fastest), meaning
+ *   u(x) = sum_n mpole_n rmpole^|n| H_n(k r) e^{i n t}.
+ * local receives, in the same layout and scaling (local_n rmpole^-|n|
+ * J_n(k r) e^{i n t}), the local expansion at each centre of all the other
+ * expansions (centres closer than |zk| boxsize 2^-51 are skipped).  ntot is
+ * the length of mpole and local.  Default leaf capacity 1 (the reference's);
+ * opts->ndiv, ifnear and nterms apply; precision must be 0.  Returns 13 when
+ * h2dterms finds no order, FMM2D_ERR_UNSUPPORTED when an order exceeds the
+ * translation kernels' limit (p_source + p_target <= 2047).
+ */
+int fmm2d_hfmm2d_mps(int nd, double eps, double zk_re, double zk_im, int64_t nmpole,
+                     const double* cmpole, const double* rmpole, const int* mterms,
+                     int64_t ntot, const fmm2d_cplx* mpole, const int64_t* impole,
+                     fmm2d_cplx* local, fmm2d_opts* opts, void* stream);
+
+/*
  * One FMM of a batch (fmm2d_batch): the arguments of fmm2d_cfmm2d (kind 0),
  * fmm2d_lfmm2d (kind 1), fmm2d_rfmm2d (kind 2: charge, dipstr and the outputs
  * point to doubles) or fmm2d_hfmm2d (kind 3: zk = zk_re + i zk_im), all device

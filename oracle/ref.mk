@@ -17,7 +17,7 @@ LAP    := cauchykernels2d cfmm2d cfmm2d_ndiv cfmm2dwrap cfmm2dwrap_vec l2dterms 
           lapkernels2d laprouts2d lfmm2d lfmm2d_ndiv lfmm2dwrap lfmm2dwrap_vec \
           lndiv2d rfmm2d rfmm2d_ndiv rfmm2dwrap rfmm2dwrap_vec rlapkernels2d
 HELM   := h2dcommon h2dterms helmkernels2d helmrouts2d hfmm2d hfmm2d_ndiv \
-          hfmm2dwrap hfmm2dwrap_vec hndiv2d wideband2d
+          hfmm2dwrap hfmm2dwrap_vec hndiv2d wideband2d hfmm2d_mps hfmm2d_mps_ndiv
 
 OBJS := $(addprefix $(OUT)/obj/c_,$(addsuffix .o,$(COMMON))) \
         $(OUT)/obj/c_cumsum.o \

@@ -353,3 +353,45 @@ def lists(tr):
     L.computelists_(*args)
     return dict(nlist1=ns[0], list1=ls[0], nlist2=ns[1], list2=ls[1],
                 nlist3=ns[2], list3=ls[2], nlist4=ns[3], list4=ls[3])
+
+
+def h2dformmpc(zk, rscale, sources, charges, center, nterms):
+    """Multipole expansion of point charges (helmholtz/helmrouts2d.f h2dformmpc),
+    nd = 1: coefficients n = -nterms..nterms."""
+    L = lib()
+    sources = np.asarray(sources, dtype=np.float64)
+    ns = sources.shape[1]
+    src, psrc = _farr(sources, np.float64)
+    ch, pch = _farr(np.asarray(charges, dtype=np.complex128).reshape(1, ns), np.complex128)
+    cen, pcen = _farr(np.asarray(center, dtype=np.float64), np.float64)
+    zka, pzk = _z(zk)
+    mp, pmp = _out((2 * nterms + 1,))
+    L.h2dformmpc_(_i(1), pzk, _d(rscale), psrc, _i(ns), pch, pcen, _i(nterms), pmp)
+    return mp
+
+
+def hfmm2d_mps(nd, eps, zk, centers, rscales, nterms, mpole, impole, ndiv=None):
+    """helmholtz/hfmm2d_mps.f:19 hfmm2d_mps (or hfmm2d_mps_ndiv.f:70
+    hfmm2d_mps_ndiv with the given ndiv; ndiv = nmpole is the reference's
+    all-pairs evaluation, hfmm2d_mps_direct).  impole is 1-based (Fortran);
+    mpole and the returned local are flat complex arrays of length ntot."""
+    L = lib()
+    centers = np.asarray(centers, dtype=np.float64)
+    n = centers.shape[1]
+    cen, pcen = _farr(centers, np.float64)
+    rs, prs = _farr(np.asarray(rscales, dtype=np.float64), np.float64)
+    mt, pmt = _farr(np.asarray(nterms, dtype=np.int32), np.int32)
+    im, pim = _farr(np.asarray(impole, dtype=np.int32), np.int32)
+    mp, pmp = _farr(np.asarray(mpole, dtype=np.complex128), np.complex128)
+    ntot = mp.shape[0]
+    loc, ploc = _out((ntot,))
+    zka, pzk = _z(zk)
+    ier = C.c_int(0)
+    if ndiv is None:
+        L.hfmm2d_mps_(_i(nd), _d(eps), pzk, _i(n), pcen, prs, pmt, _i(ntot), pmp, pim, ploc,
+                      C.byref(ier))
+    else:
+        tinfo = np.zeros(8)
+        L.hfmm2d_mps_ndiv_(_i(nd), _d(eps), pzk, _i(n), pcen, prs, pmt, _i(ntot), pmp, pim, ploc,
+                           _i(ndiv), tinfo.ctypes.data_as(C.c_void_p), C.byref(ier))
+    return loc, ier.value

@@ -325,9 +325,101 @@ def helm_c2_fixture():
     np.savez_compressed(os.path.join(HERE, "helm_c2_ref.npz"), **out)
 
 
+def mps_cases():
+    """Inputs of the hfmm2d_mps fixtures: per case (name, nd, eps, zk, centers
+    (2,n), rscales, nterms, mpole, impole (0-based), sources (2,n) and charges
+    (nd,n) of the point charges the expansions were formed from).
+
+    grid: the reference's own test (test/helmholtz/test_hfmm2d_mps.f90): a 10x10
+    grid of sources h = 1/11 apart, centres shifted by h/100 in x, zk = 2 pi,
+    order 30, rscale = |zk| h/100, eps 0.5e-9 -- here with random complex charges
+    at every source.  cluster: 300 centres (two Gaussian clusters and a uniform
+    background, centres at least 0.01 apart), nd = 2, zk = 40 + 0.5i, orders
+    6..20, one source per centre at a random offset below 1e-3, eps 1e-8,
+    expansions stored in shuffled order with gaps between them."""
+    rng = np.random.default_rng(11)
+    cases = []
+    n1 = 10
+    h = 1.0 / (n1 + 1)
+    src = np.array([[h * i, h * j] for i in range(1, n1 + 1) for j in range(1, n1 + 1)]).T
+    cen = src.copy()
+    cen[0] += h / 100
+    zk = 2 * np.pi + 0j
+    n = src.shape[1]
+    rs = np.full(n, min(abs(zk) * h / 100, 1.0))
+    nt = np.full(n, 30, dtype=np.int32)
+    ch = (rng.standard_normal((1, n)) + 1j * rng.standard_normal((1, n)))
+    cases.append(("grid", 1, 0.5e-9, zk, cen, rs, nt, ch, src))
+    n = 300
+    a = 0.3 + 0.05 * rng.standard_normal((2, 4000))
+    b = np.array([[0.7], [0.6]]) + 0.08 * rng.standard_normal((2, 4000))
+    c = rng.random((2, 2000))
+    pool = np.concatenate([a, b, c], axis=1)[:, rng.permutation(10000)]
+    keep = []
+    for k in range(pool.shape[1]):  # centres at least 0.01 apart
+        if all(np.hypot(*(pool[:, k] - pool[:, j])) >= 0.01 for j in keep):
+            keep.append(k)
+        if len(keep) == n:
+            break
+    cen = pool[:, keep]
+    off = rng.standard_normal((2, n))
+    off *= (1e-3 * rng.random(n)) / np.linalg.norm(off, axis=0)
+    src = cen + off
+    zk = 40.0 + 0.5j
+    rs = np.minimum(abs(zk) * np.linalg.norm(off, axis=0) * 1.5, 1.0)
+    nt = rng.integers(6, 21, n).astype(np.int32)
+    ch = (rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n)))
+    cases.append(("cluster", 2, 1e-8, zk, cen, rs, nt, ch, src))
+    out = []
+    for name, nd, eps, zk, cen, rs, nt, ch, src in cases:
+        n = cen.shape[1]
+        blocks = []
+        for i in range(n):
+            per = [R.h2dformmpc(zk, rs[i], src[:, i:i + 1], ch[d, i:i + 1], cen[:, i], int(nt[i]))
+                   for d in range(nd)]
+            blocks.append(np.stack(per, axis=1).ravel())  # (2m+1, nd) -> density fastest
+        if name == "cluster":
+            order = rng.permutation(n)
+            gaps = rng.integers(0, 3, n)
+        else:
+            order = np.arange(n)
+            gaps = np.zeros(n, dtype=int)
+        imp = np.zeros(n, dtype=np.int64)
+        pos = 0
+        for k in order:
+            pos += gaps[k]
+            imp[k] = pos
+            pos += blocks[k].size
+        mp = np.zeros(pos + 2, np.complex128)
+        for k in range(n):
+            mp[imp[k]:imp[k] + blocks[k].size] = blocks[k]
+        out.append((name, nd, eps, zk, cen, rs, nt, mp, imp, src, ch))
+    return out
+
+
+def mps_fixtures():
+    """Reference hfmm2d_mps outputs (FMM, and hfmm2d_mps_ndiv with ndiv = n:
+    every pair directly) for mps_cases(), with the inputs (the expansions come
+    from the reference's h2dformmpc, so the test reads them from here)."""
+    out = {}
+    for name, nd, eps, zk, cen, rs, nt, mp, imp, src, ch in mps_cases():
+        n = cen.shape[1]
+        loc, ier = R.hfmm2d_mps(nd, eps, zk, cen, rs, nt, mp, imp + 1)
+        assert ier == 0
+        locd, ier = R.hfmm2d_mps(nd, eps, zk, cen, rs, nt, mp, imp + 1, ndiv=n)
+        assert ier == 0
+        out[name + "_input_sha256"] = np.array(I.digest(cen, rs, nt, mp, imp))
+        for key, val in dict(nd=nd, eps=eps, zk=zk, centers=cen, rscales=rs, nterms=nt,
+                             mpole=mp, impole=imp, sources=src, charges=ch).items():
+            out[f"{name}_{key}"] = np.asarray(val)
+        out[name + "_local"] = loc
+        out[name + "_local_direct"] = locd
+    np.savez_compressed(os.path.join(HERE, "mps_ref.npz"), **out)
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm", "hfmm", "lreal", "rfmm",
-                             "headline", "helmc2"]
+                             "headline", "helmc2", "mps"]
     if "direct" in which:
         direct_fixtures()
     if "hankel" in which:
@@ -346,5 +438,7 @@ if __name__ == "__main__":
         headline_fixtures()
     if "helmc2" in which:
         helm_c2_fixture()
+    if "mps" in which:
+        mps_fixtures()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))
