@@ -33,6 +33,7 @@ SIGNATURES["fmm2d_tree_get"] = (_i, [_vp] * 22)
 SIGNATURES["fmm2d_tree_free"] = (None, [_vp])
 SIGNATURES["fmm2d_tree_leaf_histogram"] = (_i, [_vp, _i64, _vp, _i64, _i, _i, _i, _vp, _vp,
                                                 _vp])
+SIGNATURES["fmm2d_key_leaf_histogram"] = (_i, [_vp, _i64, _vp, _i64, _i, _i, _vp, _vp, _vp])
 SIGNATURES["fmm2d_l2dterms"] = (_i, [_d, C.POINTER(_i)])
 SIGNATURES["fmm2d_h2dterms"] = (_i, [_d, _d, _d, _d, C.POINTER(_i)])
 SIGNATURES["fmm2d_ndiv"] = (_i, [_d, _i64, _i64, _i, C.POINTER(_i), C.POINTER(_i)])
@@ -123,6 +124,12 @@ def lib():
             raise NativeLibraryError(
                 f"{LIBPATH} is missing: build it with `make -C fmm2d_amd/csrc` "
                 "(there is no CPU fallback)")
+        # torch bundles its own HIP runtime under the same soname as ROCm's
+        # (libamdhip64.so.7); whichever is loaded first serves the whole
+        # process.  Importing torch first makes this library bind to torch's
+        # runtime, the one that owns the device buffers handed to it -- loaded
+        # the other way round, the library's HIP calls find no device.
+        import torch  # noqa: F401
         L = C.CDLL(LIBPATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name, None)

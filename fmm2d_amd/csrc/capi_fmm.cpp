@@ -247,6 +247,26 @@ int fmm2d_tree_leaf_histogram(const double* sources, int64_t ns, const double* t
   return FMM2D_OK;
 }
 
+int fmm2d_key_leaf_histogram(const double* sources, int64_t ns, const double* targ, int64_t nt,
+                             int ndiv, int nbins, double* hist, double* boxsize0, void* stream) {
+  if (!sources || ns < 1 || nt < 0 || (nt > 0 && !targ) || ndiv < 1 || nbins < 1 ||
+      nbins > kLeafHistMax || !hist || ns > INT32_MAX || nt > INT32_MAX)
+    return FMM2D_ERR_ARG;
+  try {
+    keep_pool_memory();
+    key_leaf_histogram(sources, ns, targ, nt, ndiv, nbins, hist, boxsize0,
+                       reinterpret_cast<hipStream_t>(stream));
+  } catch (const HipError& e) {
+    fprintf(stderr, "fmm2d_amd: %s failed: %s\n", e.what, hipGetErrorString(e.err));
+    return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    return FMM2D_ERR_HIP;
+  }
+  return FMM2D_OK;
+}
+
 int fmm2d_tree_get(void* handle, int* laddr, int* ilevel, int* iparent, int* nchild,
                    int* ichild, int* ncoll, int* coll, double* centers, double* boxsize,
                    int* isrc, int* isrcse, int* itarg, int* itargse, int* nlist1, int* list1,

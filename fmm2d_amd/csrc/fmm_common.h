@@ -5,12 +5,12 @@
 // Everything here has internal linkage (anonymous namespace), so each driver
 // translation unit carries its own copy of the kernels.
 #pragma once
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <vector>
 
 #include "devbuf.h"
+#include "scan.h"
 #include "tree.h"
 
 namespace fmm2d {
@@ -169,13 +169,10 @@ __global__ void k_m2l_parent_flag(int64_t n, const int* ilevel, const int* nchil
 }
 
 // *total = flags set in flag[0..n), from its exclusive scan pos
-__global__ void k_flag_total(int n, const int* flag, const int* pos, int* total) {
-  *total = (n > 0) ? pos[n - 1] + flag[n - 1] : 0;
-}
-
-__global__ void k_m2l_parent_list(int64_t n, const int* flag, const int* pos, int* plist) {
+// out[pos[i]] = i for every flagged i (pos = exclusive scan of flag)
+__global__ void k_compact_flagged(int64_t n, const int* flag, const int* pos, int* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && flag[i]) plist[pos[i]] = (int)i;
+  if (i < n && flag[i]) out[pos[i]] = (int)i;
 }
 
 // tlist[q * np + k] = child q of parent plist[k] if an M2L target, else -1
@@ -195,11 +192,10 @@ __global__ void k_m2l_aligned(int64_t np, const int* plist, const int* ilevel, c
 // Non-leaf boxes (in box order, hence level order) and their 4-child slots for
 // M2M (a child without sources is skipped, laprouts2d.f via cfmm2d.f:962-983)
 // and L2L (all children when the parent has points to evaluate, cfmm2d.f:1042-1076).
-__global__ void k_nonleaf_flag(int64_t n, const int* nchild, int* flag, int* iota) {
+__global__ void k_nonleaf_flag(int64_t n, const int* nchild, int* flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   flag[i] = nchild[i] > 0 ? 1 : 0;
-  iota[i] = (int)i;
 }
 
 __global__ void k_child_slots(int64_t n, const int* list, const int* ichild, const int* sse,
@@ -332,14 +328,14 @@ void plan_children(const Tree& t, int pg, int pgt, ChildPlan& cp, hipStream_t st
   const int nb = t.nboxes, nl = t.nlevels;
   cp.list.alloc(nb, st);
   cp.lev_pos.assign(nl + 2, 0);
-  DBuf<int> flag(nb, st), iota(nb, st), cnt(1, st), dlb(nl + 2, st);
-  launch1d(k_nonleaf_flag, nb, st, t.nchild.p, flag.p, iota.p);
-  size_t bytes = 0;
-  hip_throw(hipcub::DeviceSelect::Flagged(nullptr, bytes, iota.p, flag.p, cp.list.p, cnt.p, nb,
-                                          st), "select");
-  DBuf<char> tmp(bytes, st);
-  hip_throw(hipcub::DeviceSelect::Flagged(tmp.p, bytes, iota.p, flag.p, cp.list.p, cnt.p, nb,
-                                          st), "select");
+  DBuf<int> flag(nb, st), cnt(1, st), dlb(nl + 2, st);
+  launch1d(k_nonleaf_flag, nb, st, t.nchild.p, flag.p);
+  {
+    // stream compaction of the flagged box ids (exclusive scan, then scatter)
+    DBuf<int> pos(nb, st);
+    exclusive_scan(flag.p, pos.p, nb, cnt.p, st);
+    launch1d(k_compact_flagged, nb, st, (const int*)flag.p, (const int*)pos.p, cp.list.p);
+  }
   int nnl = 0;
   hip_throw(hipMemcpyAsync(&nnl, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
   hip_throw(hipStreamSynchronize(st), "sync");
@@ -381,16 +377,12 @@ void plan_m2l(const Tree& t, const Lists& L, int pg, int pgt, M2LPlan& mp, hipSt
   DBuf<int> flag(nb, st), pos(nb + 1, st), cnt(1, st);
   launch1d(k_m2l_parent_flag, nb, st, t.ilevel.p, t.nchild.p, t.ichild.p, L.n2.p, t.isrcse.p,
            t.itargse.p, pg, pgt, flag.p);
-  size_t bytes = 0;
-  hip_throw(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag.p, pos.p, nb, st), "scan");
-  DBuf<char> tmp(bytes, st);
-  hip_throw(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, flag.p, pos.p, nb, st), "scan");
-  hipLaunchKernelGGL(k_flag_total, dim3(1), dim3(1), 0, st, nb, flag.p, pos.p, cnt.p);
+  exclusive_scan(flag.p, pos.p, nb, cnt.p, st);
   int np = 0;
   hip_throw(hipMemcpyAsync(&np, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
   hip_throw(hipStreamSynchronize(st), "sync");
   DBuf<int> plist(std::max(np, 1), st);
-  launch1d(k_m2l_parent_list, nb, st, flag.p, pos.p, plist.p);
+  launch1d(k_compact_flagged, nb, st, (const int*)flag.p, (const int*)pos.p, plist.p);
   mp.tlist.alloc(4 * (size_t)std::max(np, 1), st);
   launch1d(k_m2l_aligned, np, st, plist.p, t.ilevel.p, t.ichild.p, L.n2.p, t.isrcse.p,
            t.itargse.p, pg, pgt, mp.tlist.p);

@@ -539,8 +539,8 @@ __global__ __launch_bounds__(kHEvB) void k_heval_b(int64_t n, cplx zk, const HLe
                                                    const cplx* loc, cplx* pot, cplx* grad,
                                                    cplx* hess) {
   constexpr int NC = NComp<PG>::v;
-  using Scan = hipcub::BlockScan<int, kHEvB>;
-  __shared__ typename Scan::TempStorage scan_tmp;
+  static_assert(kHEvB == kScanThreads, "block_exclusive_scan spans the workgroup");
+  __shared__ int scan_tmp[kScanWaves + 1];
   __shared__ int off[kHEvB];
   __shared__ int sb[kHEvB];
   __shared__ double sx[kHEvB], sy[kHEvB];
@@ -557,7 +557,7 @@ __global__ __launch_bounds__(kHEvB) void k_heval_b(int64_t n, cplx zk, const HLe
 #pragma unroll
   for (int a = 0; a < 2 * NC; ++a) acc[a][tid] = 0.0;
   int o, m;
-  Scan(scan_tmp).ExclusiveSum(cnt, o, m);
+  o = block_exclusive_scan(cnt, scan_tmp, &m);
   off[tid] = o;
   __syncthreads();
   for (int k = tid; k < m; k += kHEvB) {

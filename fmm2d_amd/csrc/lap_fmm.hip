@@ -26,12 +26,12 @@
 #include <mutex>
 #include <vector>
 
-#include <hipcub/hipcub.hpp>
 
 #include "devbuf.h"
 #include "fmm_common.h"
 #include "lap_ops.h"
 #include "p2p_math.h"
+#include "scan.h"
 #include "translate.h"
 #include "tree.h"
 
@@ -446,12 +446,12 @@ __global__ __launch_bounds__(kP2lWaves * kW) void k_p2l_w(
           cplx* row = sh + lane * P;
           // z0 = source - centre of a well-separated box (no overflow guards)
           const double zx = xs[2 * j] - cx, zy = xs[2 * j + 1] - cy;
-          const double rr = zx * zx + zy * zy, ri = 1.0 / rr;
+          const double rr = zx * zx + zy * zy, ri = fast_rcp(rr);
           const cplx zi = cplx{zx * ri, -zy * ri};
           const cplx u = r * zi;
           if (IFD) e = -(zi * dps[j * nd + d]);
           cplx t0 = e;
-          if (IFC) t0 += (0.5 * log(rr)) * q;
+          if (IFC) t0 += (0.5 * fast_log(rr)) * q;
           row[0] = t0;
           cplx v = mkc(1.0);  // u^m
           for (int m = 1; m <= p; ++m) {
@@ -528,7 +528,7 @@ __device__ __forceinline__ void mp_eval(double x, double y, int c, int nd, int d
   const cplx m0 = M[0];
   cplx g, g1, g2;
   horner<PG>(M, pc, true, u, g, g1, g2);
-  ap += (0.5 * log(rr)) * m0 + g;
+  ap += (0.5 * fast_log(rr)) * m0 + g;
   if (PG >= 2) {
     const cplx u2 = u * u;
     ag += (u * rci) * m0 - (u2 * rci) * g1;
@@ -597,8 +597,8 @@ __global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double*
                                                  const int* l3, const cplx* mp,
                                                  const cplx* loc, cplx* pot, cplx* grad,
                                                  cplx* hess) {
-  using Scan = hipcub::BlockScan<int, kEvB>;
-  __shared__ typename Scan::TempStorage scan_tmp;
+  static_assert(kEvB == kScanThreads, "block_exclusive_scan spans the workgroup");
+  __shared__ int scan_tmp[kScanWaves + 1];
   __shared__ int off[kEvB];
   __shared__ int sb[kEvB];
   __shared__ double sx[kEvB], sy[kEvB];
@@ -615,7 +615,7 @@ __global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double*
 #pragma unroll
   for (int a = 0; a < 2 * PG; ++a) acc[a][tid] = 0.0;
   int o, m;
-  Scan(scan_tmp).ExclusiveSum(cnt, o, m);
+  o = block_exclusive_scan(cnt, scan_tmp, &m);
   off[tid] = o;
   __syncthreads();
   for (int k = tid; k < m; k += kEvB) {
@@ -674,11 +674,11 @@ __global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double*
 template <int IFC, int IFD, int PG, int NDT, int RE>
 __device__ __forceinline__ void pair_c64(double dx, double dy, double rr, const SrcStr<NDT>& s,
                                          Acc<CAUCHY, PG, NDT>& a) {
-  const double ri = 1.0 / rr;
+  const double ri = fast_rcp(rr);
   const cplx zinv = cplx{dx * ri, -dy * ri};  // 1/z
   const cplx z2 = -(zinv * zinv);             // -1/z^2
   const cplx z3 = (-2.0) * (z2 * zinv);       // 2/z^3
-  const double lg = IFC ? 0.5 * log(rr) : 0.0;
+  const double lg = IFC ? 0.5 * fast_log(rr) : 0.0;
 #pragma unroll
   for (int d = 0; d < NDT; ++d) {
     if (IFC) {
@@ -1317,10 +1317,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     if (P <= kW) {
       DBuf<int> flag(nb, st), pos(nb, st);
       launch1d(k_p2l_flag, nb, st, t.ilevel.p, L.n4.p, sse, tse, pg, pgt, flag.p);
-      size_t sbytes = 0;
-      hip_throw(hipcub::DeviceScan::ExclusiveSum(nullptr, sbytes, flag.p, pos.p, nb, st), "scan");
-      DBuf<char> stmp(sbytes, st);
-      hip_throw(hipcub::DeviceScan::ExclusiveSum(stmp.p, sbytes, flag.p, pos.p, nb, st), "scan");
+      exclusive_scan(flag.p, pos.p, nb, (int*)nullptr, st);
       launch1d(k_p2l_list, nb, st, flag.p, pos.p, p2x_list.p, p2x_cnt.p);
       int dev = 0, ncu = 0, per_cu = 0;
       hip_throw(hipGetDevice(&dev), "device");

@@ -13,6 +13,7 @@
 // skip_pair().
 #pragma once
 #include "common.h"
+#include "fastmath.h"
 #include "hankel.h"
 
 namespace fmm2d {
@@ -75,11 +76,12 @@ FMM2D_HD void pair_cauchy(double dx, double dy, const SrcStr<NDT>& s, double thr
   double absz = 0.0;
   if (!IFC) absz = hypot(dx, dy);
   if (skip_pair<CAUCHY, IFC, IFD, PG>(rr, thresh, thresh2, absz)) return;
-  cplx zinv = cplx{dx / rr, -dy / rr};  // 1/z
-  cplx z2 = -(zinv * zinv);              // -1/z^2
-  cplx z3 = (-2.0) * (z2 * zinv);         // 2/z^3
+  const double rinv = fast_rcp(rr);
+  cplx zinv = cplx{dx * rinv, -dy * rinv};  // 1/z
+  cplx z2 = -(zinv * zinv);                  // -1/z^2
+  cplx z3 = (-2.0) * (z2 * zinv);             // 2/z^3
   double lg = 0.0;
-  if (IFC) lg = 0.5 * log(rr);
+  if (IFC) lg = 0.5 * fast_log(rr);
 #pragma unroll
   for (int d = 0; d < NDT; ++d) {
     if (IFC) {
@@ -102,12 +104,12 @@ FMM2D_HD void pair_laplace(double dx, double dy, const SrcStr<NDT>& s, double th
   double dx2 = dx * dx, dy2 = dy * dy;
   double rr = dx2 + dy2;
   if (skip_pair<LAPLACE, IFC, IFD, PG>(rr, thresh, thresh2, 0.0)) return;
-  double rinv = 1.0 / rr;
+  double rinv = fast_rcp(rr);
   double rinv2 = rinv * rinv;
   double px = dx * rinv, py = dy * rinv;  // charge gradient kernel
   double hxx = (rr - 2.0 * dx2) * rinv2, hxy = -2.0 * dx * dy * rinv2, hyy = (rr - 2.0 * dy2) * rinv2;
   double lg = 0.0;
-  if (IFC) lg = 0.5 * log(rr);
+  if (IFC) lg = 0.5 * fast_log(rr);
   // dipole third-derivative kernel
   double rinv3 = rinv2 * rinv;
   double t1 = -2.0 * dx * (dx2 - 3.0 * dy2) * rinv3;

@@ -68,4 +68,10 @@ void build_lists(const Tree& t, Lists& l, hipStream_t st);
 constexpr int kLeafHistMax = 64;
 void leaf_level_histogram(const Tree& t, int nbins, double* hist, hipStream_t st);
 
+// hist[l] as above for the tree refined while a box holds more than ndiv
+// sources, without level restriction, computed from the sorted path keys
+// only (no tree is built); *boxsize0 = the root box size.
+void key_leaf_histogram(const double* src, int64_t ns, const double* trg, int64_t nt, int ndiv,
+                        int nbins, double* hist, double* boxsize0, hipStream_t st);
+
 }  // namespace fmm2d

@@ -13,15 +13,16 @@
 //
 // GPU formulation: one thread per point for classification into children
 // (y - cy < 0 then x - cx < 0, as sort_pts_to_children 659-754), one thread
-// per box for every box-level step, hipCUB scans where the reference uses
-// cumsum, and the final point order as the points of each leaf (a contiguous
+// per box for every box-level step, device prefix sums (scan.h) where the
+// reference uses cumsum, and the final point order as the points of each leaf (a contiguous
 // range of the path order) in input order -- which is exactly the order the
 // reference's recursive stable partition produces.  Box-level steps run level
 // by level where the reference's loops carry a dependency between levels.
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
+#include "radix_sort.h"
+#include "scan.h"
 #include "tree.h"
 
 namespace fmm2d {
@@ -641,11 +642,11 @@ __global__ void k_reorg_apply(int64_t n, const int* map, const double* c0, const
 // inside each leaf, which the reference keeps in input order: label every
 // point (in input order) with its leaf's first position, then stable-sort.
 __global__ void k_leaf_label(int64_t n, const int* nchild, const int* se, const int* perm,
-                             int* label) {
+                             unsigned* label) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n || nchild[b] != 0) return;
   const int s = se[2 * b];
-  for (int i = s; i < se[2 * b + 1]; ++i) label[perm[i]] = s;
+  for (int i = s; i < se[2 * b + 1]; ++i) label[perm[i]] = (unsigned)s;
 }
 
 // The same order without the sort: inside each leaf (a contiguous range of
@@ -727,17 +728,6 @@ __global__ void k_iota(int64_t n, int* v) {
 }
 
 // ------------------------------------------------------------ host helpers
-template <typename K, typename V>
-void sort_pairs(const K* kin, K* kout, const V* vin, V* vout, int64_t n, int bits,
-                hipStream_t st) {
-  size_t bytes = 0;
-  hip_throw(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0,
-                                               bits, st), "sort");
-  DBuf<char> tmp(bytes, st);
-  hip_throw(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kin, kout, vin, vout, (int)n, 0,
-                                               bits, st), "sort");
-}
-
 // Points of one set (sources or targets) sorted by path key.
 struct PathKeys {
   int64_t n = 0;
@@ -759,14 +749,14 @@ struct PathKeys {
     DBuf<int> iota(n, st);
     launch1d(k_path_keys, n, st, xy, cx, cy, half, d1, d2, u1.p, u2.p, iota.p);
     if (d2 == 0) {
-      sort_pairs(u1.p, k1.p, iota.p, perm.p, n, 2 * d1, st);
+      radix_sort(u1.p, k1.p, iota.p, perm.p, n, 2 * d1, st);
     } else {
       // lexicographic (key1, key2): stable LSD passes, key2 first
       DBuf<unsigned long long> t2(n, st), t1(n, st);
       DBuf<int> p1(n, st);
-      sort_pairs(u2.p, t2.p, iota.p, p1.p, n, 2 * d2, st);
+      radix_sort(u2.p, t2.p, iota.p, p1.p, n, 2 * d2, st);
       launch1d(k_gather_by<unsigned long long>, n, st, p1.p, u1.p, t1.p);
-      sort_pairs(t1.p, k1.p, p1.p, perm.p, n, 2 * d1, st);
+      radix_sort(t1.p, k1.p, p1.p, perm.p, n, 2 * d1, st);
       launch1d(k_gather_by<unsigned long long>, n, st, perm.p, u2.p, k2.p);
     }
   }
@@ -777,7 +767,6 @@ struct Builder {
   Boxes B;
   int nboxes = 0;
   DBuf<int> flag, offs, total;
-  DBuf<char> scan_tmp;  // hipcub scan scratch, kept across scans
   Keys ks{}, kt{};
   // the centre comparisons of k_box_ij are exact when the domain is not far
   // from the origin relative to its size
@@ -785,12 +774,7 @@ struct Builder {
 
   explicit Builder(hipStream_t s) : st(s) { total.alloc(64, s); }
 
-  void scan_only(int n) {
-    size_t bytes = 0;
-    hip_throw(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag.p, offs.p, n, st), "scan");
-    if (bytes > scan_tmp.n) scan_tmp.alloc(std::max(bytes, 2 * scan_tmp.n), st);
-    hip_throw(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, bytes, flag.p, offs.p, n, st), "scan");
-  }
+  void scan_only(int n) { exclusive_scan(flag.p, offs.p, n, (int*)nullptr, st); }
   // exclusive scan of flag[0..n) into offs, returns the total
   int scan(int n) {
     if (n <= 0) return 0;
@@ -1174,13 +1158,14 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   for (int k = 0; k < 2; ++k) {
     if (npts[k] == 0 || !hbig[k]) continue;
     // a leaf too large for k_leaf_sort: stable sort of all points by leaf
-    DBuf<int> label(npts[k], st), iota(npts[k], st), lsorted(npts[k], st);
+    DBuf<unsigned> label(npts[k], st), lsorted(npts[k], st);
+    DBuf<int> iota(npts[k], st);
     launch1d(k_leaf_label, nb, st, bld.B.nchild.p, se[k], pk[k].perm.p, label.p);
     int bits = 1;
     while ((int64_t(1) << bits) <= npts[k]) ++bits;
     // values: input indices 0..n-1 in input order
     launch1d(k_iota, npts[k], st, iota.p);
-    sort_pairs(label.p, lsorted.p, iota.p, perm[k]->p, npts[k], bits, st);
+    radix_sort(label.p, lsorted.p, iota.p, perm[k]->p, npts[k], bits, st);
   }
   t.isrcse = std::move(bld.B.ses);
   t.itargse = std::move(bld.B.set);
@@ -1210,6 +1195,113 @@ __global__ void k_leaf_hist(int64_t nb, const int* nchild, const int* ilevel, co
   __syncthreads();
   for (int k = threadIdx.x; k < nbins; k += blockDim.x)
     if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+// ---------------------------------------------------- key leaf histogram
+// Leaf level of every point in the tree refined while a box holds more than
+// ndiv sources (pts_tree_build's rule, common/pts_tree2d.f:551-631, without
+// the level restriction of pts_tree_fix_lr), from the sorted source path
+// keys alone: with w(j) = common levels of keys j and j + ndiv, the level-l box
+// of a point holds more than ndiv sources iff some window j..j+ndiv of sorted
+// sources lies in it -- for a source at sorted position i, iff w(j) >= l for a
+// j in [i - ndiv, i]; for a target whose key sorts at position pos, iff
+// min(w(j), common levels(key_j, target)) >= l for a j in [pos - ndiv - 1, pos].
+// The leaf level is one more than the largest such l (0 if the root is a leaf).
+__device__ __forceinline__ int key_common_levels(unsigned long long a, unsigned long long b,
+                                                 int d1) {
+  const unsigned long long x = a ^ b;
+  if (x == 0) return d1;
+  return (__clzll((long long)x) - (64 - 2 * d1)) >> 1;
+}
+
+__global__ void k_window_levels(int64_t m, int ndiv, int d1, const unsigned long long* keys,
+                                unsigned char* w) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < m) w[j] = (unsigned char)key_common_levels(keys[j], keys[j + ndiv], d1);
+}
+
+__global__ void k_key_leaf_hist(int64_t n, int64_t ns, int ndiv, int d1,
+                                const unsigned long long* __restrict__ skeys,
+                                const unsigned char* __restrict__ w,
+                                const unsigned long long* __restrict__ tkeys, int nbins,
+                                unsigned long long* hist) {
+  __shared__ unsigned long long h[kLeafHistMax];
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    int lev = 0;
+    if (ns > ndiv) {
+      const int64_t last = ns - ndiv - 1;  // last window start
+      int best = 0;
+      if (i < ns) {
+        for (int64_t j = max((int64_t)0, i - ndiv); j <= min(i, last); ++j)
+          best = max(best, (int)w[j]);
+      } else {
+        const unsigned long long t = tkeys[i - ns];
+        int64_t lo = 0, hi = ns;  // pos = first sorted source key >= t
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (skeys[mid] < t) lo = mid + 1;
+          else hi = mid;
+        }
+        for (int64_t j = max((int64_t)0, lo - ndiv - 1); j <= min(lo, last); ++j)
+          best = max(best, min((int)w[j], key_common_levels(skeys[j], t, d1)));
+      }
+      lev = best + 1;
+    }
+    atomicAdd(&h[min(lev, nbins - 1)], 1ull);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x)
+    if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+void key_leaf_histogram(const double* src, int64_t ns, const double* trg, int64_t nt, int ndiv,
+                        int nbins, double* hist, double* boxsize0, hipStream_t st) {
+  if (nbins < 1 || nbins > kLeafHistMax) throw HipError{hipErrorInvalidValue, "histogram bins"};
+  double bb[4];
+  {
+    const int nb = (int)std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, ceil_div(ns + nt, 256)));
+    DBuf<double> part(4 * nb, st), out(4, st);
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, st, src, ns, trg, nt, part.p);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, st, part.p, nb, out.p);
+    hip_throw(hipMemcpyAsync(bb, out.p, sizeof(bb), hipMemcpyDeviceToHost, st), "copy");
+    hip_throw(hipStreamSynchronize(st), "sync");
+  }
+  const double bs = std::max(bb[1] - bb[0], bb[3] - bb[2]);
+  if (boxsize0) *boxsize0 = bs;
+  const int d1 = kD1;
+  // half[l] = boxsize(l) / 2 with boxsize(l + 1) = boxsize(l) / 2, as in build_tree
+  std::vector<double> half(d1 + 2);
+  double bsl = bs;
+  for (int l = 0; l < (int)half.size(); ++l) {
+    half[l] = 0.5 * bsl;
+    bsl = bsl / 2;
+  }
+  DBuf<double> dhalf(half.size(), st);
+  hip_throw(hipMemcpyAsync(dhalf.p, half.data(), sizeof(double) * half.size(),
+                           hipMemcpyHostToDevice, st), "copy");
+  const double cx = (bb[0] + bb[1]) / 2, cy = (bb[2] + bb[3]) / 2;
+  DBuf<unsigned long long> k0(ns, st), ks(ns, st), kt(std::max<int64_t>(nt, 1), st);
+  launch1d(k_path_keys, ns, st, src, cx, cy, (const double*)dhalf.p, d1, 0, k0.p,
+           (unsigned long long*)nullptr, (int*)nullptr);
+  if (nt > 0)
+    launch1d(k_path_keys, nt, st, trg, cx, cy, (const double*)dhalf.p, d1, 0, kt.p,
+             (unsigned long long*)nullptr, (int*)nullptr);
+  radix_sort(k0.p, ks.p, (const int*)nullptr, (int*)nullptr, ns, 2 * d1, st);
+  const int64_t m = ns > ndiv ? ns - ndiv : 0;
+  DBuf<unsigned char> w(std::max<int64_t>(m, 1), st);
+  launch1d(k_window_levels, m, st, ndiv, d1, (const unsigned long long*)ks.p, w.p);
+  DBuf<unsigned long long> d(nbins, st);
+  d.zero();
+  launch1d(k_key_leaf_hist, ns + nt, st, ns, ndiv, d1, (const unsigned long long*)ks.p,
+           (const unsigned char*)w.p, (const unsigned long long*)kt.p, nbins, d.p);
+  std::vector<unsigned long long> h(nbins);
+  hip_throw(hipMemcpyAsync(h.data(), d.p, sizeof(unsigned long long) * nbins,
+                           hipMemcpyDeviceToHost, st), "copy");
+  hip_throw(hipStreamSynchronize(st), "sync");
+  for (int k = 0; k < nbins; ++k) hist[k] = (double)h[k];
 }
 
 void leaf_level_histogram(const Tree& t, int nbins, double* hist, hipStream_t st) {

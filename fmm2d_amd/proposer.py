@@ -10,7 +10,8 @@ FP64 direct sum on sampled points exceeds eps is rejected.
 
 Inputs (20): log10 N / 7, log10(1 + |zk| boxsize0) / 3, -log10 eps / 15, a
 Helmholtz flag, and the fraction of points held by leaves at levels 0..15 of
-the tree built with the default leaf capacity.
+the tree refined with the default leaf capacity (no level restriction;
+computed on the device from the sorted path keys).
 Outputs (17): r_l, l = 0..15, the fractional reduction of level l's default
 order (nterms_l = max(2, round(p_l (1 - r_l))), r_l clipped to [0, 0.7]), and
 log2 of the leaf capacity relative to the default one (clipped to [-1.5, 3]).
@@ -53,36 +54,36 @@ def default_nterms(kind, eps, boxsize, zk=None):
 
 
 def tree_features(sources, targets, eps, zk, kind, ndiv):
-    """Feature vector (NIN floats) and the default tree's box sizes."""
-    from .tree import build_tree
-    src = np.asarray(sources, dtype=np.float64)
-    ns = src.shape[1]
-    nt = 0 if targets is None else np.asarray(targets).shape[1]
-    tr = build_tree(src, targets, ndiv=ndiv)
-    leaf = tr["nchild"] == 0
-    cnt = (tr["isrcse"][1] - tr["isrcse"][0] + 1) + (tr["itargse"][1] - tr["itargse"][0] + 1)
-    lev = np.minimum(tr["ilevel"], NLEV - 1)
-    hist = np.bincount(lev[leaf], weights=cnt[leaf], minlength=NLEV)[:NLEV]
-    hist = hist / max(1.0, hist.sum())
-    kz = 0.0 if kind != "h" else abs(complex(zk)) * float(tr["boxsize"][0])
-    f = np.concatenate([[np.log10(ns + nt) / 7.0, np.log10(1.0 + kz) / 3.0,
-                         -np.log10(eps) / 15.0, 1.0 if kind == "h" else 0.0], hist])
-    return f.astype(np.float32), tr["boxsize"]
+    """Feature vector (NIN floats) and the root box size for host points
+    (copied to the device; see device_features)."""
+    import torch
+    dev = nat.torch_device()
+    src = torch.from_numpy(np.ascontiguousarray(np.asarray(sources, dtype=np.float64).T)).to(dev)
+    ns = src.shape[0]
+    trg, nt = None, 0
+    if targets is not None:
+        trg = torch.from_numpy(np.ascontiguousarray(np.asarray(targets, dtype=np.float64).T)).to(
+            dev)
+        nt = trg.shape[0]
+    return device_features(nat.ptr(src), ns, nat.ptr(trg), nt, eps, zk, kind, ndiv)
 
 
 def device_features(src_ptr, ns, targ_ptr, nt, eps, zk, kind, ndiv, stream=None):
     """Feature vector (NIN floats) and the root box size for device-resident
-    points (sources(2,ns), targets(2,nt) as device pointers): the same
-    features as tree_features, with the leaf histogram made on the device
-    (fmm2d_tree_leaf_histogram) -- no tree is copied to the host."""
+    points (sources(2,ns), targets(2,nt) as device pointers): log10 N / 7,
+    log10(1 + |zk| boxsize0) / 3, -log10 eps / 15, a Helmholtz flag, and the
+    fraction of points held by the leaves of levels 0..15 of the tree refined
+    while a box holds more than ndiv sources (fmm2d_key_leaf_histogram: from
+    the sorted path keys, no tree built and nothing copied to the host but the
+    16 counts)."""
     import ctypes as C
     L = nat.lib()
     hist = np.zeros(NLEV, dtype=np.float64)
     bs0 = C.c_double(0.0)
-    nat.check(L.fmm2d_tree_leaf_histogram(src_ptr, int(ns), targ_ptr, int(nt), 0, int(ndiv),
-                                          NLEV, hist.ctypes.data, C.byref(bs0),
-                                          nat.stream_ptr() if stream is None else stream),
-              "tree_leaf_histogram")
+    nat.check(L.fmm2d_key_leaf_histogram(src_ptr, int(ns), targ_ptr, int(nt), int(ndiv), NLEV,
+                                         hist.ctypes.data, C.byref(bs0),
+                                         nat.stream_ptr() if stream is None else stream),
+              "key_leaf_histogram")
     hist = hist / max(1.0, hist.sum())
     kz = 0.0 if kind != "h" else abs(complex(zk)) * bs0.value
     f = np.concatenate([[np.log10(ns + nt) / 7.0, np.log10(1.0 + kz) / 3.0,
@@ -127,17 +128,18 @@ class Proposer:
         return y.cpu().numpy()
 
     def propose(self, kind, sources, eps, targets=None, zk=None):
-        """dict(nterms=[...per level of the proposed tree...], ndiv=k)."""
-        ns = np.asarray(sources).shape[1]
-        nt = 0 if targets is None else np.asarray(targets).shape[1]
-        nd0 = default_ndiv(eps, ns, nt)
-        f, bs = tree_features(sources, targets, eps, zk, kind, nd0)
-        y = self.predict(f[None, :])[0]
-        # orders for the levels of a tree up to 4 levels deeper than the default one
-        bsx = list(bs) + [bs[-1] / 2 ** k for k in range(1, 5)]
-        p0 = default_nterms(kind, eps, bsx, zk)
-        nterms, ndiv = decode(y, p0, nd0, margin=MARGIN)
-        return {"nterms": nterms, "ndiv": ndiv}
+        """dict(nterms=[...per level...], ndiv=k) for host points."""
+        import torch
+        dev = nat.torch_device()
+        src = torch.from_numpy(np.ascontiguousarray(np.asarray(sources, dtype=np.float64).T)).to(
+            dev)
+        trg, nt = None, 0
+        if targets is not None:
+            trg = torch.from_numpy(np.ascontiguousarray(
+                np.asarray(targets, dtype=np.float64).T)).to(dev)
+            nt = trg.shape[0]
+        return self.propose_device(kind, nat.ptr(src), src.shape[0], eps,
+                                   targ_ptr=nat.ptr(trg), nt=nt, zk=zk)
 
     def propose_device(self, kind, src_ptr, ns, eps, targ_ptr=None, nt=0, zk=None,
                        nlevels_max=52):

@@ -48,7 +48,10 @@ def make_points(rng, n, dist):
     return np.stack([r * np.cos(t), r * np.sin(t)]) + 1e-4 * rng.standard_normal((2, n))
 
 
-def run_problem(rng):
+def problem_inputs(rng):
+    """One random problem of the training distribution: (kind, dist, n, eps, zk,
+    sources, charges, sampled indices).  Every random draw of a problem is
+    made here, in this order, so a seed replays the same problems."""
     kind = "h" if rng.random() < 0.5 else "l"
     dist = ["uniform", "cluster", "curve"][int(rng.integers(0, 3))]
     n = int(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
@@ -57,12 +60,16 @@ def run_problem(rng):
     src = make_points(rng, n, dist)
     ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     idx = np.sort(rng.choice(n, NSAMPLE, replace=False))
+    return kind, dist, n, eps, zk, src, ch, idx
+
+
+def run_problem(rng):
+    kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng)
     (p, g, _), _ = F._direct(kind, zk, src, src[:, idx], ch, None, None, 2, 1, 1e-16)
     ex = np.concatenate([p[0], g[0].ravel()])
     nd0 = PR.default_ndiv(eps, n, 0)
-    feats, bs = PR.tree_features(src, None, eps, zk, kind, nd0)
-    bsx = list(bs) + [bs[-1] / 2 ** k for k in range(1, 5)]
-    p0 = PR.default_nterms(kind, eps, bsx, zk)
+    feats, bs0 = PR.tree_features(src, None, eps, zk, kind, nd0)
+    p0 = PR.default_nterms(kind, eps, [bs0 / 2.0 ** l for l in range(52)], zk)
     rows = []
     for e in E_GRID:
         for r in R_GRID:
@@ -122,6 +129,22 @@ def fit(feats, targets, steps=3000, lr=3e-3, seed=0):
     return p.cpu().numpy(), hist
 
 
+def refeature(problems, seed):
+    """Recompute the features of stored problems by replaying their inputs
+    from the generator seed (checks that each replayed problem is the stored
+    one); the measured tuples and targets do not depend on the features."""
+    rng = np.random.default_rng(seed)
+    for i, pb in enumerate(problems):
+        kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng)
+        if (kind, dist, n, eps) != (pb["kind"], pb["dist"], pb["n"], pb["eps"]):
+            raise ValueError(f"problem {i}: replay gives {(kind, dist, n, eps)}, stored "
+                             f"{(pb['kind'], pb['dist'], pb['n'], pb['eps'])}")
+        nd0 = PR.default_ndiv(eps, n, 0)
+        feats, _ = PR.tree_features(src, None, eps, zk, kind, nd0)
+        pb["features"] = feats.tolist()
+        print(f"refeature {i}: {kind} {dist} n={n}", flush=True)
+
+
 def targets_of(problems):
     y = np.zeros((len(problems), PR.NOUT), dtype=np.float32)
     for i, pb in enumerate(problems):
@@ -137,10 +160,16 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--out", default=PR.WEIGHTS)
     ap.add_argument("--data", default=None, help="reuse tuples from this JSON file")
+    ap.add_argument("--refeature", action="store_true",
+                    help="with --data: replay the stored problems from --seed (the seed they "
+                         "were generated with), recompute their features with the current "
+                         "feature function and refit; the targets are kept")
     args = ap.parse_args()
     if args.data:
         with open(args.data) as f:
             problems = json.load(f)
+        if args.refeature:
+            refeature(problems, args.seed)
     else:
         rng = np.random.default_rng(args.seed)
         problems = []

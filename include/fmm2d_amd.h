@@ -298,6 +298,18 @@ int fmm2d_tree_leaf_histogram(const double* sources, int64_t ns, const double* t
                               int idivflag, int ndiv, int nbins, double* hist, double* boxsize0,
                               void* stream);
 
+/*
+ * The same histogram for the tree that refines every box holding more than
+ * ndiv SOURCES (the refinement rule of pts_tree_build, common/pts_tree2d.f:
+ * 551-631, with idivflag 0) without the level restriction of pts_tree_fix_lr,
+ * computed from the sorted path keys of the points alone -- no tree is built,
+ * which makes it several times cheaper than fmm2d_tree_leaf_histogram.  Levels
+ * deeper than 20 count in the last bin (nbins <= 21 is exact).  No reference
+ * counterpart: the parameter proposer's input features.
+ */
+int fmm2d_key_leaf_histogram(const double* sources, int64_t ns, const double* targ, int64_t nt,
+                             int ndiv, int nbins, double* hist, double* boxsize0, void* stream);
+
 /* Expansion-order and leaf-size rules on the host:
  *   l2dterms  laplace/l2dterms.f:38     h2dterms helmholtz/h2dterms.f:35
  *   lndiv2d   laplace/lndiv2d.f         hndiv2d  helmholtz/hndiv2d.f */

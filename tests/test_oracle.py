@@ -192,3 +192,28 @@ def test_mlp_oracle_closed_forms():
     g = np.array([2.0, -3.0, 0.5])
     p2, m, v = M.adam(np.zeros(3), g, np.zeros(3), np.zeros(3), 0.1, 0.9, 0.999, 0.0, 1)
     assert np.allclose(p2, -0.1 * np.sign(g), rtol=1e-12)
+
+
+def test_leaf_histogram_restatements_agree():
+    """The key-window restatement of the proposer's leaf histogram (the
+    device's algorithm) equals box-by-box subdivision, with and without
+    targets, on uniform, clustered and curve sets and leaf capacities 1..40;
+    a planted off-by-one window fails."""
+    from oracle import leaf_hist_ref as H
+    rng = np.random.default_rng(0)
+    for trial in range(18):
+        ns = int(rng.integers(1, 2500))
+        nt = int(rng.integers(0, 400)) if trial % 2 else 0
+        if trial % 3 == 0:
+            src = rng.random((2, ns))
+        elif trial % 3 == 1:
+            src = 0.5 + 0.01 * rng.standard_normal((2, ns))
+        else:
+            t = rng.uniform(0, 2 * np.pi, ns)
+            src = np.stack([np.cos(t), np.sin(t)]) + 1e-4 * rng.standard_normal((2, ns))
+        trg = rng.random((2, nt)) if nt else None
+        ndiv = int(rng.integers(1, 41))
+        assert np.array_equal(H.windows(src, trg, ndiv), H.brute(src, trg, ndiv)), trial
+    # a window one source short (ndiv - 1) refines boxes of exactly ndiv sources
+    src = rng.random((2, 3000))
+    assert not np.array_equal(H.windows(src, None, 14), H.brute(src, None, 15))
