@@ -38,6 +38,7 @@
 
 #include "devbuf.h"
 #include "fmm_common.h"
+#include "hankel32.h"
 #include "helm_ops.h"
 #include "lap_ops.h"
 #include "p2p_math.h"
@@ -663,6 +664,147 @@ __global__ void k_hnear(int64_t n, int nd, int d0, cplx zk, const double* xy, co
   }
 }
 
+// Mixed precision (real wavenumber): the pair is formed in FP64 (displacement,
+// |z|^2, r, and the phase of H inside hank01_f32), the Hankel pair and the
+// kernel products in FP32, summed in FP32 over at most 32 sources and folded
+// into FP64.  Pairs with k r < 1e-4 -- the point's own pair, duplicates and
+// near-duplicates, where 1/z and the threshold rule matter -- take the exact
+// FP64 formula and exclusion rule out of line.
+template <int IFC, int IFD, int PG, int NDT> struct AccH32 {
+  c32 pot[NDT];
+  c32 grad[NDT][2];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int d = 0; d < NDT; ++d) {
+      pot[d] = c32{0.f, 0.f};
+      grad[d][0] = grad[d][1] = c32{0.f, 0.f};
+    }
+  }
+};
+
+template <int IFC, int IFD, int PG, int NDT>
+__device__ __noinline__ void pair_helm_slow(double dx, double dy, cplx zk,
+                                            const SrcStr<NDT>& s, double thresh,
+                                            Acc<HELM, PG, NDT>& a) {
+  pair_helm<IFC, IFD, PG, NDT>(dx, dy, zk, s, thresh, a);
+}
+
+template <int IFC, int IFD, int PG, int NDT>
+__device__ __forceinline__ void pair_helm_f32(double dx, double dy, double rr, double k,
+                                              const SrcStr<NDT>& s,
+                                              AccH32<IFC, IFD, PG, NDT>& f) {
+  const double r = sqrt(rr);
+  c32 h0, h1;
+  hank01_f32(k * r, &h0, &h1);
+  const float rinv = 1.0f / (float)r, kf = (float)k, fx = (float)dx, fy = (float)dy;
+  const float k4 = 0.25f * kf * rinv;
+  if (IFC) {
+    const c32 p0{-0.25f * h0.im, 0.25f * h0.re};  // (i/4) H0
+    const c32 cdd{k4 * h1.im, -k4 * h1.re};       // -(ik/4) H1 / r
+#pragma unroll
+    for (int d = 0; d < NDT; ++d) {
+      const c32 c{(float)s.charge[d].re, (float)s.charge[d].im};
+      f.pot[d] = f.pot[d] + p0 * c;
+      if (PG >= 2) {
+        const c32 cd = cdd * c;
+        f.grad[d][0] = f.grad[d][0] + fx * cd;
+        f.grad[d][1] = f.grad[d][1] + fy * cd;
+      }
+    }
+  }
+  if (IFD) {
+    const c32 cdd{-k4 * h1.im, k4 * h1.re};  // (ik/4) H1 / r
+    c32 hf1{0.f, 0.f}, hf2{0.f, 0.f}, hf3{0.f, 0.f};
+    const float kk4 = -0.25f * kf * kf;      // -(k^2) i/4 = (0, kk4)
+    if (PG >= 2) {
+      const float zi = rinv / kf;
+      const c32 h2 = (2.0f * zi) * h1 - h0;
+      const float ct = fx * rinv, st = fy * rinv;
+      hf1 = (ct * ct - 0.5f) * h2 - 0.5f * h0;
+      hf2 = (ct * st) * h2;
+      hf3 = (st * st - 0.5f) * h2 - 0.5f * h0;
+    }
+#pragma unroll
+    for (int d = 0; d < NDT; ++d) {
+      const c32 ds{(float)s.dipstr[d].re, (float)s.dipstr[d].im};
+      const float v1 = (float)s.dv[d][0], v2 = (float)s.dv[d][1];
+      f.pot[d] = f.pot[d] + (fx * v1 + fy * v2) * (cdd * ds);
+      if (PG >= 2) {
+        const c32 cd{-kk4 * ds.im, kk4 * ds.re};
+        const c32 hxx = cd * hf1, hxy = cd * hf2, hyy = cd * hf3;
+        f.grad[d][0] = f.grad[d][0] + v1 * hxx + v2 * hxy;
+        f.grad[d][1] = f.grad[d][1] + v1 * hxy + v2 * hyy;
+      }
+    }
+  }
+}
+
+template <int IFC, int IFD, int PG, int NDT>
+__device__ __forceinline__ void fold_h32(AccH32<IFC, IFD, PG, NDT>& f, Acc<HELM, PG, NDT>& a) {
+#pragma unroll
+  for (int d = 0; d < NDT; ++d) {
+    a.pot[d] += cplx{(double)f.pot[d].re, (double)f.pot[d].im};
+    if (PG >= 2) {
+      a.grad[d][0] += cplx{(double)f.grad[d][0].re, (double)f.grad[d][0].im};
+      a.grad[d][1] += cplx{(double)f.grad[d][1].re, (double)f.grad[d][1].im};
+    }
+  }
+  f.zero();
+}
+
+// k_hnear with the mixed-precision pair (real zk, PG <= 2)
+template <int IFC, int IFD, int PG, int NDT>
+__global__ void k_hnear32(int64_t n, int nd, int d0, cplx zk, const double* xy, const int* leaf,
+                          const int2* seg, const double* xs, const cplx* chs, const cplx* dps,
+                          const double* dvs, double thresh, cplx* pot, cplx* grad, cplx* hess) {
+  static_assert(PG <= 2, "hfmm2d has no Hessians");
+  const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = leaf[i];
+  const double x = xy[2 * i], y = xy[2 * i + 1];
+  const double k = zk.re;
+  const double rr_slow = 1e-8 / (k * k);  // k r < 1e-4
+  Acc<HELM, PG, NDT> a;
+  a.zero();
+  AccH32<IFC, IFD, PG, NDT> f;
+  f.zero();
+  SegCursor sc;
+  const int T = sc.init(seg + (int64_t)kSegRow * b);
+  for (int t = 0; t < T; ++t) {
+    const int j = sc.j;
+    SrcStr<NDT> s;
+#pragma unroll
+    for (int dd = 0; dd < NDT; ++dd) {
+      const bool ok = d0 + dd < nd;
+      const int dc = ok ? d0 + dd : d0;
+      if (IFC) s.charge[dd] = ok ? chs[(int64_t)j * nd + dc] : mkc(0.0);
+      if (IFD) {
+        s.dipstr[dd] = ok ? dps[(int64_t)j * nd + dc] : mkc(0.0);
+        s.dv[dd][0] = dvs[(2 * (int64_t)j) * nd + dc];
+        s.dv[dd][1] = dvs[(2 * (int64_t)j + 1) * nd + dc];
+      }
+    }
+    const double dx = x - xs[2 * (int64_t)j], dy = y - xs[2 * (int64_t)j + 1];
+    const double rr = fma(dx, dx, dy * dy);
+    if (rr < rr_slow) pair_helm_slow<IFC, IFD, PG, NDT>(dx, dy, zk, s, thresh, a);
+    else pair_helm_f32<IFC, IFD, PG, NDT>(dx, dy, rr, k, s, f);
+    if ((t & 31) == 31) fold_h32<IFC, IFD, PG, NDT>(f, a);
+    sc.j = j + 1;
+    sc.step();
+  }
+  fold_h32<IFC, IFD, PG, NDT>(f, a);
+#pragma unroll
+  for (int dd = 0; dd < NDT; ++dd) {
+    if (d0 + dd >= nd) break;
+    const int dc = d0 + dd;
+    pot[i * nd + dc] += a.pot[dd];
+    if (PG >= 2) {
+      grad[(2 * i) * nd + dc] += a.grad[dd][0];
+      grad[(2 * i + 1) * nd + dc] += a.grad[dd][1];
+    }
+  }
+}
+
 // ---------------------------------------------------------- operators
 // Operator o: entry (i, j) (output i, input j, |i| <= pout, |j| <= pin) is
 //   G[i - j] * w1[sa |i-j| + sb |j| + sc |i|] * w2[|i|]
@@ -676,6 +818,11 @@ struct OpDesc {
   int sa, sb, sc;
   int pad;
 };
+
+__global__ void k_to_f32(int64_t n, const double* in, float* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
 
 __global__ void k_build_ops(const OpDesc* ops, const cplx* gt, const double* wt, double* af) {
   const OpDesc o = ops[blockIdx.y];
@@ -782,8 +929,14 @@ template <int IFC, int IFD, int NDT>
 void launch_hnear_f(int pg, int64_t n, hipStream_t st, int nd, int d0, cplx zk, const double* xy,
                     const int* leaf, const int2* seg, const double* xs,
                     const cplx* chs, const cplx* dps, const double* dvs, double thresh, cplx* p,
-                    cplx* g, cplx* h) {
-  if (pg == 1)
+                    cplx* g, cplx* h, bool f32) {
+  if (f32 && pg == 1)
+    launch1d_xcd(k_hnear32<IFC, IFD, 1, NDT>, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps,
+                 dvs, thresh, p, g, h);
+  else if (f32)
+    launch1d_xcd(k_hnear32<IFC, IFD, 2, NDT>, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps,
+                 dvs, thresh, p, g, h);
+  else if (pg == 1)
     launch1d_xcd(k_hnear<IFC, IFD, 1, NDT>, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps,
              dvs, thresh, p, g, h);
   else if (pg == 2)
@@ -798,30 +951,32 @@ template <int NDT>
 void launch_hnear_n(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd, int d0, cplx zk,
                     const double* xy, const int* leaf, const int2* seg,
                     const double* xs, const cplx* chs, const cplx* dps, const double* dvs,
-                    double thresh, cplx* p, cplx* g, cplx* h) {
+                    double thresh, cplx* p, cplx* g, cplx* h, bool f32) {
   if (ifc && !ifd)
     launch_hnear_f<1, 0, NDT>(pg, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs, thresh,
-                              p, g, h);
+                              p, g, h, f32);
   else if (!ifc && ifd)
     launch_hnear_f<0, 1, NDT>(pg, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs, thresh,
-                              p, g, h);
+                              p, g, h, f32);
   else
     launch_hnear_f<1, 1, NDT>(pg, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs, thresh,
-                              p, g, h);
+                              p, g, h, f32);
 }
 
+// f32: the mixed-precision pair (real zk and pg <= 2 only)
 void launch_hnear(int pg, int ifc, int ifd, int64_t n, hipStream_t st, int nd, cplx zk,
                   const double* xy, const int* leaf, const int2* seg,
                   const double* xs, const cplx* chs, const cplx* dps, const double* dvs,
-                  double thresh, cplx* p, cplx* g, cplx* h) {
+                  double thresh, cplx* p, cplx* g, cplx* h, bool f32 = false) {
   const int NDT = (nd == 1) ? 1 : 2;
+  f32 = f32 && zk.im == 0.0 && pg <= 2;
   for (int d0 = 0; d0 < nd; d0 += NDT) {
     if (NDT == 1)
       launch_hnear_n<1>(pg, ifc, ifd, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs,
-                        thresh, p, g, h);
+                        thresh, p, g, h, f32);
     else
       launch_hnear_n<2>(pg, ifc, ifd, n, st, nd, d0, zk, xy, leaf, seg, xs, chs, dps, dvs,
-                        thresh, p, g, h);
+                        thresh, p, g, h, f32);
   }
 }
 
@@ -1018,11 +1173,14 @@ __global__ __launch_bounds__(kMpsW) void k_mps_near(int64_t n, int nd, cplx zk, 
 // hfmm2d_mps.f:520-760): translation operators for every level, then M2M
 // (finest parents first), M2L level by level and L2L (coarsest first), on the
 // per-level multipole and local blocks.  pg / pgt say which boxes hold
-// evaluation points (sources / targets), as the planners need.
+// evaluation points (sources / targets), as the planners need.  mixed: the M2L
+// runs on v_mfma_f32_16x16x4f32 with FP32 copies of the operators (the
+// swizzled A-fragment layout is the same for both element types) and of the
+// multipoles, accumulating in FP32 and adding into the FP64 locals.
 void helm_box_passes(const Tree& t, const Lists& L, const std::vector<HLev>& lev,
                      const std::vector<long double>& sl, const std::vector<int>& nterms,
                      int nd, cplx zk, int pg, int pgt, cplx* mpp, cplx* locp, Timer& tm,
-                     hipStream_t st) {
+                     hipStream_t st, bool mixed = false) {
   const int nl = t.nlevels;
   // ---- translation operators, level by level
   OpTables T;
@@ -1109,6 +1267,8 @@ void helm_box_passes(const Tree& t, const Lists& L, const std::vector<HLev>& lev
     }
   }
   auto opptr = [&](int first) { return d_af.p + T.desc[first].af; };
+  DBuf<float> af32(mixed ? std::max<int64_t>(T.af_total, 1) : 0, st);
+  if (mixed) launch1d(k_to_f32, T.af_total, st, (const double*)d_af.p, af32.p);
   tm.mark(12);
 
   ChildPlan cp;
@@ -1131,6 +1291,11 @@ void helm_box_passes(const Tree& t, const Lists& L, const std::vector<HLev>& lev
     plan_m2l(t, L, pg, pgt, m2l, st);
     if (m2l.ntg > 0) {
       DBuf<int> d_soff = upload(m2l_slot_offsets(), st);
+      // FP32 copy of every level's multipoles (mixed precision)
+      const int64_t ncoef = lev[nl].off + (int64_t)(t.nboxes - t.lstart[nl]) * nd *
+                                              (2 * lev[nl].p + 1);
+      DBuf<float> mp32(mixed ? 2 * std::max<int64_t>(ncoef, 1) : 0, st);
+      if (mixed) launch1d(k_to_f32, 2 * ncoef, st, (const double*)mpp, mp32.p);
       tm.mark(3);  // planning
       for (int l = 2; l <= nl; ++l) {
         if (op_m2l[l] < 0) continue;
@@ -1138,11 +1303,19 @@ void helm_box_passes(const Tree& t, const Lists& L, const std::vector<HLev>& lev
         for (int q = 0; q < 4; ++q)
           groups.push_back(TrGroup{m2l.lev(q, l, nl), m2l.lev(q, l + 1, nl), 0, q});
         const HLev& V = lev[l];
-        launch_translate(0, 0, nd,
-                         TrBufs{mpp + V.off, V.box0, 2 * V.p + 1, locp + V.off, V.box0,
-                                2 * V.p + 1},
-                         kSlots, groups, m2l.tlist.p, L.s2.p, d_soff.p, opptr(op_m2l[l]),
-                         t.ilevel.p, nullptr, nullptr, st, false, true, true);
+        if (mixed)
+          launch_translate(0, 0, nd,
+                           TrBufs{mp32.p + 2 * V.off, V.box0, 2 * V.p + 1, locp + V.off,
+                                  V.box0, 2 * V.p + 1},
+                           kSlots, groups, m2l.tlist.p, L.s2.p, d_soff.p,
+                           af32.p + T.desc[op_m2l[l]].af, t.ilevel.p, nullptr, nullptr, st,
+                           true, true, true);
+        else
+          launch_translate(0, 0, nd,
+                           TrBufs{mpp + V.off, V.box0, 2 * V.p + 1, locp + V.off, V.box0,
+                                  2 * V.p + 1},
+                           kSlots, groups, m2l.tlist.p, L.s2.p, d_soff.p, opptr(op_m2l[l]),
+                           t.ilevel.p, nullptr, nullptr, st, false, true, true);
       }
       tm.mark(13);  // the M2L kernels alone
     }
@@ -1178,7 +1351,9 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
   if (a.zk.re == 0.0 && a.zk.im == 0.0) return FMM2D_ERR_ARG;
   if (!a.ifcharge && !a.ifdipole) return FMM2D_OK;
   const fmm2d_opts* o = a.opts;
-  if (o && o->precision != 0) return FMM2D_ERR_UNSUPPORTED;
+  if (o && (o->precision < 0 || o->precision > 1)) return FMM2D_ERR_ARG;
+  // precision 1: FP32 M2L on MFMA and FP32 near field with FP64 sums
+  const bool mixed = o && o->precision == 1;
   if (o) {
     const int e = check_nterms_override(o->n_nterms, o->nterms, 1, kMaxNtermsHelmholtz);
     if (e != FMM2D_OK) return e;
@@ -1312,7 +1487,7 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     hip_throw(hipGetLastError(), "p2l");
     tm.mark(1);
 
-    helm_box_passes(t, L, lev, sl, nterms, nd, zk, pg, pgt, mp.p, loc.p, tm, st);
+    helm_box_passes(t, L, lev, sl, nterms, nd, zk, pg, pgt, mp.p, loc.p, tm, st, mixed);
     // ---- steps 6+7: list 3 and L2P
     auto eval = [&](int p_, int64_t n, const double* xy, const int* lf, cplx* P_, cplx* G_,
                     cplx* H_) {
@@ -1332,10 +1507,10 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
       launch1d(k_near_segs, t.nboxes, st, L.n1.p, L.l1.p, sse, seg.p);
       if (pg > 0)
         launch_hnear(pg, a.ifcharge, a.ifdipole, ns, st, nd, zk, xs.p, leaf_s.p, seg.p, xs.p,
-                     chs.p, dps.p, dvs.p, thresh, ps.p, gs.p, hs.p);
+                     chs.p, dps.p, dvs.p, thresh, ps.p, gs.p, hs.p, mixed);
       if (pgt > 0)
         launch_hnear(pgt, a.ifcharge, a.ifdipole, nt, st, nd, zk, xt.p, leaf_t.p, seg.p, xs.p,
-                     chs.p, dps.p, dvs.p, thresh, pt.p, gt.p, ht.p);
+                     chs.p, dps.p, dvs.p, thresh, pt.p, gt.p, ht.p, mixed);
     }
     tm.mark(7);
     // ---- unsort
