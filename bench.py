@@ -376,6 +376,140 @@ def main_hfmm2d(args):
     print(json.dumps(line), flush=True)
 
 
+def c5_points(n, seed=5):
+    """Proposer evaluation set: half on the perturbed closed curve r = 1 + 0.3 cos 5t
+    (1e-4 jitter), half in 16 Gaussian clusters (sigma log-uniform in [1e-3, 1e-2])
+    centred on the unit circle; complex charges CN(0, 1)."""
+    rng = np.random.default_rng(seed)
+    m = n // 2
+    t = rng.uniform(0, 2 * np.pi, m)
+    r = 1 + 0.3 * np.cos(5 * t)
+    curve = np.stack([r * np.cos(t), r * np.sin(t)]) + 1e-4 * rng.standard_normal((2, m))
+    a = rng.uniform(0, 2 * np.pi, 16)
+    cen = np.stack([np.cos(a), np.sin(a)])
+    lab = rng.integers(0, 16, n - m)
+    sig = np.exp(rng.uniform(np.log(1e-3), np.log(1e-2), 16))
+    cl = cen[:, lab] + sig[lab] * rng.standard_normal((2, n - m))
+    pts = np.concatenate([curve, cl], axis=1)
+    ch = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)
+    return np.ascontiguousarray(pts.T), ch
+
+
+def main_c5(args):
+    """The learned proposer's evaluation configuration: hfmm2d, N = 2e6 points on a
+    curve plus clusters, zk = 500, eps = 1e-6, pot+grad at the sources.  Pipelines:
+    the reference's rules in FP64 and in mixed precision, and mixed precision with
+    the proposer's orders and leaf capacity, end to end (proposal, FMM and the
+    256-point acceptance test, whose direct sums run on a second stream)."""
+    import torch
+    from fmm2d_amd import _native as nat
+    from fmm2d_amd import proposer as PR
+    L = nat.lib()
+    dev = nat.torch_device()
+    n = args.n if args.n != 10_000_000 else 2_000_000
+    eps, zk = 1e-6, 500.0
+    print(f"bench: c5 workload, {n} points, zk={zk}", file=sys.stderr, flush=True)
+    pts, q = c5_points(n)
+    src = torch.from_numpy(pts).to(dev)
+    chg = torch.from_numpy(q).to(dev)
+    pot = torch.zeros(n, dtype=torch.complex128, device=dev)
+    grad = torch.zeros((n, 2), dtype=torch.complex128, device=dev)
+
+    def step(o):
+        ier = L.fmm2d_hfmm2d(1, eps, zk, 0.0, n, nat.ptr(src), 1, nat.ptr(chg), 0, None, None, 0,
+                             2, nat.ptr(pot), nat.ptr(grad), None, 0, None, 0, None, None, None,
+                             nat.byref_opts(o), nat.stream_ptr())
+        if ier != 0:
+            raise RuntimeError(f"fmm2d_hfmm2d returned {ier}")
+
+    def sample_err(idx_t, pex, gex):
+        num = ((pot[idx_t] - pex).abs() ** 2).sum() + ((grad[idx_t] - gex).abs() ** 2).sum()
+        return float(torch.sqrt(num / ((pex.abs() ** 2).sum() + (gex.abs() ** 2).sum())))
+
+    def direct_at(idx_t, stream=None):
+        m = len(idx_t)
+        tsel = src[idx_t].contiguous()
+        pex = torch.zeros(m, dtype=torch.complex128, device=dev)
+        gex = torch.zeros((m, 2), dtype=torch.complex128, device=dev)
+        ier = L.fmm2d_h2d_direct(1, zk, 0.0, nat.ptr(src), n, 1, nat.ptr(chg), 0, None, None,
+                                 nat.ptr(tsel), m, 2, nat.ptr(pex), nat.ptr(gex), None, 1e-16,
+                                 nat.stream_ptr())
+        assert ier == 0
+        return pex, gex
+
+    rng = np.random.default_rng(4321)
+    it = torch.from_numpy(np.sort(rng.choice(n, min(args.nerr, n), replace=False))).to(dev)
+    pex, gex = direct_at(it)
+    iv = torch.from_numpy(np.sort(np.random.default_rng(777).choice(n, args.validate,
+                                                                     replace=False))).to(dev)
+    proposer = PR.Proposer()
+    side = torch.cuda.Stream()
+    state = {}
+
+    def learned():
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            pv, gv = direct_at(iv)
+        prop = PR.Proposer.propose_device(proposer, "h", nat.ptr(src), n, eps, zk=zk)
+        o = nat.opts_struct({"precision": "mixed", "nterms": prop["nterms"], "ndiv": prop["ndiv"]})
+        step(o)
+        torch.cuda.current_stream().wait_stream(side)
+        e = sample_err(iv, pv, gv)
+        state.update(proposal=prop, val_err=e)
+        if e > eps:
+            o = nat.opts_struct({"precision": "mixed"})
+            step(o)
+            state["fallbacks"] = state.get("fallbacks", 0) + 1
+        return o
+
+    pipelines = {}
+    for label, fn in (("fp64", None), ("mixed", None), ("mixed+learned", learned)):
+        if fn is None:
+            o0 = nat.opts_struct({"precision": label})
+
+            def fn(o0=o0):
+                step(o0)
+                return o0
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stage = np.zeros(16)
+        for _ in range(args.steps):
+            o = fn()
+            stage += np.array(o.timeinfo[:])
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        stage /= args.steps
+        err = sample_err(it, pex, gex)
+        pipelines[label] = {"value": round(n / dt, 1), "ms_per_step": round(dt * 1e3, 3),
+                            "rel_l2_err": err, "nlevels": int(o.nlevels), "nboxes": int(o.nboxes),
+                            "stage_ms": {name: round(stage[k] * 1e3, 4)
+                                         for k, name in STAGES.items()}}
+        if label == "mixed+learned":
+            pipelines[label].update(proposal={"ndiv": state["proposal"]["ndiv"],
+                                              "nterms": state["proposal"]["nterms"][:12]},
+                                    validation_err=state["val_err"],
+                                    fallbacks=state.get("fallbacks", 0))
+        print(f"bench[c5 {label}]: {dt*1e3:.2f} ms/step, rel L2 err {err:.3e}, levels "
+              f"{o.nlevels}, boxes {o.nboxes}", file=sys.stderr, flush=True)
+    line = {
+        "metric": "hfmm2d points/sec, N=2e6 curve+cluster, zk=500, eps 1e-6: learned proposer "
+                  "vs default rules (secondary)",
+        "value": pipelines["mixed+learned"]["value"], "unit": "points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": pipelines["mixed+learned"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "hfmm2d N=2e6, half on r = 1 + 0.3 cos 5t, half in 16 Gaussian "
+                               "clusters; zk=500, eps=1e-6, charges CN(0,1), pot+grad at sources",
+                   "n": n, "zk": zk, "eps": eps},
+        "pipelines": pipelines,
+        "gain_vs_default_rules": round(pipelines["mixed"]["ms_per_step"] /
+                                       pipelines["mixed+learned"]["ms_per_step"], 4),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -395,7 +529,7 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run one elementwise copy kernel over MIB MiB (reads and "
                          "writes MIB MiB), the reference kernel for scaling HBM counters")
-    ap.add_argument("--workload", default="lfmm2d", choices=["lfmm2d", "hfmm2d", "batch"],
+    ap.add_argument("--workload", default="lfmm2d", choices=["lfmm2d", "hfmm2d", "batch", "c5"],
                     help="lfmm2d: the headline workload; hfmm2d: the secondary Helmholtz one; "
                          "batch: many independent small FMMs through fmm2d_batch")
     ap.add_argument("--batch", type=int, default=256, help="instances of the batch workload")
@@ -411,6 +545,8 @@ def main():
         return main_hfmm2d(args)
     if args.workload == "batch":
         return main_batch(args)
+    if args.workload == "c5":
+        return main_c5(args)
     import ctypes as C
     import torch
     from fmm2d_amd import _native as nat

@@ -91,16 +91,23 @@ __global__ __launch_bounds__(256) void k_translate(
     const int* tlist, const int* slots, int slots_by_box, const int* slot_op, const T* Afrag,
     int64_t afs,
     int sbox0, int dbox0, const int* ilevel, const int2* band, const double* logr, const T* src,
-    cplx* dst) {
+    cplx* dst, int ssplit, int ksplit) {
   typedef typename Mfma<T>::acc_t acc_t;
   // Workgroups are dispatched round-robin over the 8 XCDs; block b works on
   // logical block (b % 8) * per_xcd + b / 8 so that each XCD walks a
   // contiguous range of targets, whose neighbouring source boxes then stay
   // in that XCD's L2.
   const int64_t lblk = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-  const int64_t wv = lblk * 4 + (threadIdx.x >> 6);
-  if (wv >= nwaves) return;
+  const int64_t wv0 = lblk * 4 + (threadIdx.x >> 6);
+  if (wv0 >= nwaves) return;
   const int lane = threadIdx.x & 63;
+  // Split launches (few targets, long operators): wavefront wv0 also takes
+  // one of ssplit groups of slots and one of ksplit ranges of k steps, and
+  // adds its partial products with atomics.
+  const int nsk = ssplit * ksplit;
+  const int sk = (int)(wv0 % nsk);
+  const int64_t wv = wv0 / nsk;
+  const bool split = nsk > 1;
   // With the k loop not unrolled (long operators), the chunks of RT row tiles
   // go to separate wavefronts: wavefront wv works on tile wv / nrc, chunk
   // wv % nrc (the rows are disjoint outputs).
@@ -142,13 +149,22 @@ __global__ __launch_bounds__(256) void k_translate(
   __shared__ T sbuf[STAGE ? 4 : 1][STAGE ? 16 * SW : 1];
   __shared__ int4 sinfo[STAGE ? 4 : 1][16];
   const int wib = STAGE ? (int)(threadIdx.x >> 6) : 0;
+  const int sg = sk / ksplit, kg = sk - sg * ksplit;
+  const int s_lo = (int)((int64_t)nslots * sg / ssplit);
+  const int s_hi = (int)((int64_t)nslots * (sg + 1) / ssplit);
+  const int k_lo = (int)((int64_t)KS * kg / ksplit);
+  const int k_hi = (int)((int64_t)KS * (kg + 1) / ksplit);
+  auto put = [&](double* a, double v) {
+    if (split) atomicAdd(a, v);
+    else *a += v;
+  };
   for (int rc = rc0; rc < (KSU > 0 ? RTT : rc0 + 1); rc += RT) {
     if (MODE == 0) {
       acc_t acc[RT];
 #pragma unroll
       for (int r = 0; r < RT; ++r) acc[r] = acc_t{0, 0, 0, 0};
       T m0 = 0;
-      for (int s = 0; s < nslots; ++s) {
+      for (int s = s_lo; s < s_hi; ++s) {
         const int c = valid ? slots[srow * nslots + s] : -1;
         const T* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
         const T* M = nullptr;
@@ -188,7 +204,7 @@ __global__ __launch_bounds__(256) void k_translate(
           for (int ks = 0; ks < KSU; ++ks)
             if (ks < KS) step(ks);
         } else {
-          for (int ks = 0; ks < KS; ++ks) step(ks);
+          for (int ks = k_lo; ks < k_hi; ++ks) step(ks);
         }
         if (STAGE) wave_lds_fence();  // the next slot overwrites the rows
       }
@@ -203,7 +219,7 @@ __global__ __launch_bounds__(256) void k_translate(
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int row = 16 * (rc + r) + Mfma<T>::row(kr, i);
-            if (rc + r < RTT && row >= bd.x && row < bd.y) L[row] += (double)acc[r][i];
+            if (rc + r < RTT && row >= bd.x && row < bd.y) put(&L[row], (double)acc[r][i]);
           }
       }
     } else {
@@ -224,7 +240,7 @@ __global__ __launch_bounds__(256) void k_translate(
           bv[ks] = (ks < KS && k >= klo && k < khi) ? M[k] : T(0);
         }
       }
-      for (int s = 0; s < nslots; ++s) {
+      for (int s = s_lo; s < s_hi; ++s) {
         const int c = valid ? slots[srow * nslots + s] : -1;
         const T* A = Afrag + (int64_t)(ob + slot_op[q * nslots + s]) * afs;
         acc_t acc[RT];
@@ -241,7 +257,7 @@ __global__ __launch_bounds__(256) void k_translate(
           for (int ks = 0; ks < (KSU > 0 ? KSU : 1); ++ks)
             if (ks < KS) step(ks, bv[ks]);
         } else {
-          for (int ks = 0; ks < KS; ++ks) {
+          for (int ks = k_lo; ks < k_hi; ++ks) {
             const int k = 4 * ks + kr;
             step(ks, (k >= klo && k < khi) ? M[k] : T(0));
           }
@@ -254,7 +270,7 @@ __global__ __launch_bounds__(256) void k_translate(
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
               const int row = 16 * (rc + r) + Mfma<T>::row(kr, i);
-              if (rc + r < RTT && row >= bd.x && row < bd.y) L[row] += (double)acc[r][i];
+              if (rc + r < RTT && row >= bd.x && row < bd.y) put(&L[row], (double)acc[r][i]);
             }
         }
       }
@@ -310,6 +326,10 @@ struct TrBufs {
 // launch; launches are stream-ordered, so groups must not depend on each other).
 // fp32: operators (Afrag) and sources are float, accumulation FP32 on
 // v_mfma_f32_16x16x4f32 (gather mode only); otherwise FP64.
+// Launches of long operators (k loop not unrolled) with fewer wavefronts than
+// this split their slots and k steps over more wavefronts (atomic adds).
+constexpr int64_t kTrSplitWaves = 4096;
+
 inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int nslots,
                              const std::vector<TrGroup>& groups, const int* tlist,
                              const int* slots, const int* slot_op, const void* Afrag,
@@ -347,13 +367,21 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
       tab.rem0[g + 1] = tab.rem0[g] + (tab.tile0[g + 1] - tab.tile0[g]) - tab.ilv;
 #define FMM2D_TR(T, RT, KSU, MODE, LOGT)                                                        \
   {                                                                                             \
-    const int64_t nw = (int64_t)tiles * (KSU > 0 ? 1 : (RTT + RT - 1) / RT);                    \
+    const int64_t nw0 = (int64_t)tiles * (KSU > 0 ? 1 : (RTT + RT - 1) / RT);                   \
+    int ssplit = 1, ksplit = 1;                                                                 \
+    if (KSU == 0 && !LOGT && nw0 < kTrSplitWaves) {                                             \
+      ssplit = (int)std::min<int64_t>(nslots, (kTrSplitWaves + nw0 - 1) / nw0);                 \
+      ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(KS / 16, (kTrSplitWaves + nw0 *     \
+                                        ssplit - 1) / (nw0 * ssplit)));                        \
+    }                                                                                           \
+    const int64_t nw = nw0 * ssplit * ksplit;                                                   \
     const int64_t nblk = (nw + 3) / 4;                                                          \
     const int per_xcd = (int)((nblk + 7) / 8);                                                  \
     hipLaunchKernelGGL((k_translate<T, RT, KSU, MODE, LOGT>), dim3((unsigned)(8 * per_xcd)),    \
                        dim3(256), 0, st, nd, Pin, Pout, nslots, nw, per_xcd, tab, tlist, slots, \
                        (int)slots_by_box, slot_op, static_cast<const T*>(Afrag), afs, io.sbox0, \
-                       io.dbox0, ilevel, band, logr, static_cast<const T*>(io.src), io.dst);    \
+                       io.dbox0, ilevel, band, logr, static_cast<const T*>(io.src), io.dst,     \
+                       ssplit, ksplit);                                                         \
   }
 #define FMM2D_TR_RT(T, MODE, LOGT)                 \
   switch (KS <= 4 * RTT ? RTT : 0) {               \
