@@ -15,6 +15,18 @@
 //   |z| >= 17          Hankel's asymptotic expansion (A&S 9.2.7), summed until
 //                      the terms fall below 1e-17 relative.
 //
+// Real positive z = x (every real wavenumber) takes its own path, hank01_real:
+//   x < 4              the ascending series as four polynomials of degree 16
+//                      in q = x^2/4 (Horner's rule; coefficients exact
+//                      rationals rounded once) and one logarithm;
+//   x >= 4             H_nu(x) = sqrt(2/(pi x)) e^{i (x - nu pi/2 - pi/4)} M_nu(4/x),
+//                      the amplitude M_nu as polynomials of degree 20 in s = 4/x
+//                      (Chebyshev interpolants computed with mpmath; error about
+//                      one unit in the last place of |M| ~ 1), so a pair costs a
+//                      reciprocal, a square root, one sincos and 84 fused
+//                      multiply-adds with no data-dependent loop.
+//                      Both tables: hank64_tables.h (gen_hank64.py).
+//
 // Below the real axis (Im zk < 0: exponentially growing kernels) the rule
 // above is used for arg z >= -0.3; deeper in the fourth quadrant the series
 // (|z| < 5), Hankel's integral along the ray arg t = -pi/4, which keeps the
@@ -28,7 +40,9 @@
 // half-plane, which covers every Im(zk) >= 0, and <= 1e-14 below it.
 #pragma once
 #include "common.h"
+#include "fastmath.h"
 #include "gl_tables.h"
+#include "hank64_tables.h"
 
 namespace fmm2d {
 
@@ -221,13 +235,93 @@ static __host__ __device__ __noinline__ void hank01_lower(cplx z, double r, cplx
   *h1 = cplx{-(2.0 * a1.re + b1.re), -(2.0 * a1.im - b1.im)};
 }
 
-// H0^(1)(z), H1^(1)(z).  z must be nonzero.
-FMM2D_HD void hank01(cplx z, cplx* h0, cplx* h1) {
+// The coefficient rows of hank64_tables.h: NSER series rows (a, b, c, d) then
+// NFIT + 1 amplitude rows (P0, Q0, P1, Q1).
+constexpr int kH64Rows = FMM2D_H64_NSER + FMM2D_H64_NFIT + 1;
+#if defined(__HIP_DEVICE_COMPILE__)
+static __constant__ double kH64Tab[4 * kH64Rows] = {FMM2D_H64_SER, FMM2D_H64_AMP};
+#endif
+
+// H0^(1)(x), H1^(1)(x) for real x > 0.  On the device the coefficients are
+// read with scalar loads from a constant table whose address passes through an
+// empty asm statement: the compiler cannot hoist them out of the caller's loop
+// (as literal constants, all 152 were hoisted into vector registers ahead of
+// the loop, which then ran at one wave per SIMD or spilled).
+FMM2D_HD void hank01_real(double x, cplx* h0, cplx* h1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __attribute__((address_space(4))) double* tab =
+      (const __attribute__((address_space(4))) double*)kH64Tab;
+  asm volatile("" : "+s"(tab));
+#else
+  static constexpr double tab[4 * kH64Rows] = {FMM2D_H64_SER, FMM2D_H64_AMP};
+#endif
+  if (x < FMM2D_H64_X1) {
+    const double q = 0.25 * (x * x);
+    constexpr int n = FMM2D_H64_NSER - 1;
+    double a = tab[4 * n], b = tab[4 * n + 1], c = tab[4 * n + 2], d = tab[4 * n + 3];
+#pragma unroll
+    for (int k = n - 1; k >= 0; --k) {
+      a = fma(a, q, tab[4 * k]);
+      b = fma(b, q, tab[4 * k + 1]);
+      c = fma(c, q, tab[4 * k + 2]);
+      d = fma(d, q, tab[4 * k + 3]);
+    }
+    const double xh = 0.5 * x;
+    const double J1 = xh * b;
+    const double L = fast_log(xh) + kEuler;
+    const double Y0 = (2.0 / kPi) * fma(L, a, c);
+    const double Y1 = (2.0 / kPi) * (L * J1) - (2.0 / kPi) * fast_rcp(x) - (1.0 / kPi) * (xh * d);
+    *h0 = cplx{a, Y0};
+    *h1 = cplx{J1, Y1};
+    return;
+  }
+  const double t = fast_rcp(x);
+  const double s = FMM2D_H64_X1 * t;
+  constexpr int o = 4 * FMM2D_H64_NSER, n = FMM2D_H64_NFIT;
+  double p0 = tab[o + 4 * n], q0 = tab[o + 4 * n + 1], p1 = tab[o + 4 * n + 2],
+         q1 = tab[o + 4 * n + 3];
+#pragma unroll
+  for (int k = n - 1; k >= 0; --k) {
+    p0 = fma(p0, s, tab[o + 4 * k]);
+    q0 = fma(q0, s, tab[o + 4 * k + 1]);
+    p1 = fma(p1, s, tab[o + 4 * k + 2]);
+    q1 = fma(q1, s, tab[o + 4 * k + 3]);
+  }
+  // sqrt(2/(pi x)) e^{i (x - pi/4)}: e^{i x} first, then the fixed rotation,
+  // so the phase keeps the accuracy of sincos(x)
+  double sn, cs;
+  sincos(x, &sn, &cs);
+  const double m = sqrt((2.0 / kPi) * t) * 0.70710678118654752440;
+  const cplx e{m * (cs + sn), m * (sn - cs)};
+  *h0 = e * cplx{p0, q0};
+  const cplx m1 = e * cplx{p1, q1};
+  *h1 = cplx{m1.im, -m1.re};  // times e^{-i pi/2}
+}
+
+// The general evaluator (any nonzero z; used for z off the positive real
+// axis): out of line, so that kernels of real wavenumbers, whose arguments
+// never come here, carry only the call and keep their registers for the real
+// path.
+static __host__ __device__ __noinline__ void hank01_complex(cplx z, cplx* h0, cplx* h1) {
   double r = absc(z);
   if (z.im >= 0.0)
     hank01_upper(z, r, h0, h1);
   else
     hank01_lower(z, r, h0, h1);
+}
+
+// H0^(1)(z), H1^(1)(z).  z must be nonzero.
+FMM2D_HD void hank01(cplx z, cplx* h0, cplx* h1) {
+  if (z.im == 0.0 && z.re > 0.0)
+    hank01_real(z.re, h0, h1);
+  else
+    hank01_complex(z, h0, h1);
+}
+
+// hank01 as a call, for callers outside the pair loops (one evaluation per
+// expansion): their kernels then keep no registers for the real path.
+static __host__ __device__ __noinline__ void hank01_ool(cplx z, cplx* h0, cplx* h1) {
+  hank01(z, h0, h1);
 }
 
 }  // namespace fmm2d

@@ -131,7 +131,7 @@ __device__ void scaled_j(cplx z, double s, int m, cplx* out) {
 // H_0..H_m(z) s^n into out[0..m] (m >= 1), one thread.
 __device__ void scaled_h(cplx z, double s, int m, cplx* out) {
   cplx h0, h1;
-  hank01(z, &h0, &h1);
+  hank01_ool(z, &h0, &h1);
   out[0] = h0;
   h1 = h1 * s;
   out[1] = h1;
@@ -459,7 +459,7 @@ __device__ void eval_mpole(const Deriv& D, cplx z, double s, cplx w, cplx* res) 
   const int nc = D.p + PG - 1;
   constexpr int NC = NComp<PG>::v;
   cplx h0, h1;
-  hank01(z, &h0, &h1);
+  hank01_ool(z, &h0, &h1);
   h1 = h1 * s;
   const cplx zi = invc(z);
   cplx e = mkc(1.0);
@@ -478,7 +478,7 @@ __device__ void eval_mpole(const Deriv& D, cplx z, double s, cplx w, cplx* res) 
 // One thread per evaluation point: local expansion of its leaf and
 // multipoles of the leaf's list-3 boxes (hfmm2d.f:1310-1520).
 template <int PG>
-__global__ void k_heval(int64_t n, int nd, cplx zk, const HLev* lev, const double* xy,
+__global__ __launch_bounds__(256) void k_heval(int64_t n, int nd, cplx zk, const HLev* lev, const double* xy,
                         const int* leaf, const int* ilevel, const double* centers, const int* n3,
                         const int* l3, const cplx* mp, const cplx* loc, cplx* pot, cplx* grad,
                         cplx* hess) {
@@ -617,7 +617,7 @@ __global__ __launch_bounds__(kHEvB) void k_heval_b(int64_t n, cplx zk, const HLe
 // row (k_near_segs), so lanes of different leaves in one wavefront stay busy
 // until their own totals run out.
 template <int IFC, int IFD, int PG, int NDT>
-__global__ void k_hnear(int64_t n, int nd, int d0, cplx zk, const double* xy, const int* leaf,
+__global__ __launch_bounds__(256) void k_hnear(int64_t n, int nd, int d0, cplx zk, const double* xy, const int* leaf,
                         const int2* seg, const double* xs, const cplx* chs, const cplx* dps,
                         const double* dvs, double thresh, cplx* pot, cplx* grad, cplx* hess) {
   const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
@@ -686,7 +686,7 @@ template <int IFC, int IFD, int PG, int NDT>
 __device__ __noinline__ void pair_helm_slow(double dx, double dy, cplx zk,
                                             const SrcStr<NDT>& s, double thresh,
                                             Acc<HELM, PG, NDT>& a) {
-  pair_helm<IFC, IFD, PG, NDT>(dx, dy, zk, s, thresh, a);
+  pair_helm<IFC, IFD, PG, NDT, true>(dx, dy, zk, s, thresh, a);
 }
 
 template <int IFC, int IFD, int PG, int NDT>

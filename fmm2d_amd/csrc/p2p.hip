@@ -5,7 +5,8 @@
 // helmholtz/helmkernels2d.f -- O(ns*nt) double loops that INCREMENT pot/grad/
 // hess.  Here: one thread per target, sources streamed through LDS in tiles of
 // 256, per-thread register accumulation over NDT densities.  When there are
-// too few target blocks to fill the 256 CUs the source range is split across
+// too few target blocks to fill the 256 CUs the four waves of a block share
+// 64 targets and split each source tile, and the source range is split across
 // blocks; each split writes a private copy of the outputs and a second kernel
 // adds the copies into the caller's buffers in split order (deterministic).
 #include <mutex>
@@ -27,7 +28,12 @@ template <int FAM, int IFC, int IFD, int NDT> struct TileSmem {
 
 // Output layout (reference, column-major): pot(nd,n); Cauchy grad(nd,n),
 // hess(nd,n); Laplace/Helmholtz grad(nd,2,n), hess(nd,3,n).
-template <int FAM, int IFC, int IFD, int PG, int NDT>
+//
+// WS waves share one block's kTile/WS targets: wave w takes the w-th part of
+// every source tile and the parts are added in LDS in wave order at the end
+// (WS = 4 for launches with few targets, where a block of 256 targets would
+// leave most lanes idle).
+template <int FAM, int IFC, int IFD, int PG, int NDT, int WS>
 __global__ __launch_bounds__(kTile) void direct_kernel(
     int nd, cplx zk, const double* __restrict__ src, int64_t ns, int64_t split_len,
     const cplx* __restrict__ charge, const cplx* __restrict__ dipstr,
@@ -35,8 +41,11 @@ __global__ __launch_bounds__(kTile) void direct_kernel(
     double thresh, cplx* __restrict__ pot, cplx* __restrict__ grad, cplx* __restrict__ hess,
     int64_t split_stride_pot, int64_t split_stride_grad, int64_t split_stride_hess) {
   constexpr int NG = FamDims<FAM>::NG, NH = FamDims<FAM>::NH;
+  constexpr int TT = kTile / WS;  // targets per block
   __shared__ TileSmem<FAM, IFC, IFD, NDT> sm;
-  const int64_t it = (int64_t)blockIdx.x * kTile + threadIdx.x;
+  __shared__ cplx red[WS > 1 ? (WS - 1) * TT : 1];
+  const int lane = (int)threadIdx.x % TT, w = (int)threadIdx.x / TT;
+  const int64_t it = (int64_t)blockIdx.x * TT + lane;
   const int d0 = blockIdx.z * NDT;
   const int64_t s_begin = (int64_t)blockIdx.y * split_len;
   const int64_t s_end = min(ns, s_begin + split_len);
@@ -66,8 +75,9 @@ __global__ __launch_bounds__(kTile) void direct_kernel(
     }
     __syncthreads();
     if (active) {
-#pragma unroll 2
-      for (int k = 0; k < nload; ++k) {
+      const int kb = w * TT, ke = min(nload, kb + TT);
+#pragma unroll(FAM == HELM ? 1 : 2)
+      for (int k = kb; k < ke; ++k) {
         SrcStr<NDT> s;
 #pragma unroll
         for (int d = 0; d < NDT; ++d) {
@@ -87,6 +97,31 @@ __global__ __launch_bounds__(kTile) void direct_kernel(
           pair_helm<IFC, IFD, PG, NDT>(dx, dy, zk, s, thresh, acc);
       }
     }
+  }
+  if constexpr (WS > 1) {
+    // wave 0 adds the other waves' parts in wave order
+    auto fold = [&](cplx& v) {
+      __syncthreads();
+      if (w > 0) red[(w - 1) * TT + lane] = v;
+      __syncthreads();
+      if (w == 0) {
+#pragma unroll
+        for (int u = 0; u < WS - 1; ++u) v += red[u * TT + lane];
+      }
+    };
+#pragma unroll
+    for (int d = 0; d < NDT; ++d) {
+      fold(acc.pot[d]);
+      if (PG >= 2) {
+#pragma unroll
+        for (int c = 0; c < NG; ++c) fold(acc.grad[d][c]);
+      }
+      if (PG >= 3) {
+#pragma unroll
+        for (int c = 0; c < NH; ++c) fold(acc.hess[d][c]);
+      }
+    }
+    if (w > 0) return;
   }
   if (!active) return;
   cplx* P = pot + blockIdx.y * split_stride_pot;
@@ -130,27 +165,33 @@ __global__ void add_splits_kernel(cplx* __restrict__ out, const cplx* __restrict
 }
 
 template <int FAM, int IFC, int IFD, int PG, int NDT>
-static void launch_direct(dim3 grid, hipStream_t st, const DirectArgs& a, int64_t split_len,
-                          cplx* pot, cplx* grad, cplx* hess, int64_t sp, int64_t sg, int64_t sh) {
-  hipLaunchKernelGGL((direct_kernel<FAM, IFC, IFD, PG, NDT>), grid, dim3(kTile), 0, st, a.nd,
-                     a.zk, a.src, a.ns, split_len, a.charge, a.dipstr, a.dipvec, a.targ, a.nt,
-                     a.thresh, pot, grad, hess, sp, sg, sh);
+static void launch_direct(int ws, dim3 grid, hipStream_t st, const DirectArgs& a,
+                          int64_t split_len, cplx* pot, cplx* grad, cplx* hess, int64_t sp,
+                          int64_t sg, int64_t sh) {
+  if (NDT == 1 && ws == 4)
+    hipLaunchKernelGGL((direct_kernel<FAM, IFC, IFD, PG, NDT, 4>), grid, dim3(kTile), 0, st,
+                       a.nd, a.zk, a.src, a.ns, split_len, a.charge, a.dipstr, a.dipvec, a.targ,
+                       a.nt, a.thresh, pot, grad, hess, sp, sg, sh);
+  else
+    hipLaunchKernelGGL((direct_kernel<FAM, IFC, IFD, PG, NDT, 1>), grid, dim3(kTile), 0, st,
+                       a.nd, a.zk, a.src, a.ns, split_len, a.charge, a.dipstr, a.dipvec, a.targ,
+                       a.nt, a.thresh, pot, grad, hess, sp, sg, sh);
 }
 
 template <int FAM, int IFC, int IFD, int NDT>
-static void dispatch_pg(int pg, dim3 grid, hipStream_t st, const DirectArgs& a, int64_t sl,
+static void dispatch_pg(int ws, dim3 grid, hipStream_t st, const DirectArgs& a, int64_t sl,
                         cplx* p, cplx* g, cplx* h, int64_t sp, int64_t sg, int64_t sh) {
-  if (pg == 1) launch_direct<FAM, IFC, IFD, 1, NDT>(grid, st, a, sl, p, g, h, sp, sg, sh);
-  else if (pg == 2) launch_direct<FAM, IFC, IFD, 2, NDT>(grid, st, a, sl, p, g, h, sp, sg, sh);
-  else launch_direct<FAM, IFC, IFD, 3, NDT>(grid, st, a, sl, p, g, h, sp, sg, sh);
+  if (a.pg == 1) launch_direct<FAM, IFC, IFD, 1, NDT>(ws, grid, st, a, sl, p, g, h, sp, sg, sh);
+  else if (a.pg == 2) launch_direct<FAM, IFC, IFD, 2, NDT>(ws, grid, st, a, sl, p, g, h, sp, sg, sh);
+  else launch_direct<FAM, IFC, IFD, 3, NDT>(ws, grid, st, a, sl, p, g, h, sp, sg, sh);
 }
 
 template <int FAM, int NDT>
-static void dispatch_cd(const DirectArgs& a, dim3 grid, hipStream_t st, int64_t sl, cplx* p,
-                        cplx* g, cplx* h, int64_t sp, int64_t sg, int64_t sh) {
-  if (a.ifcharge && !a.ifdipole) dispatch_pg<FAM, 1, 0, NDT>(a.pg, grid, st, a, sl, p, g, h, sp, sg, sh);
-  else if (!a.ifcharge && a.ifdipole) dispatch_pg<FAM, 0, 1, NDT>(a.pg, grid, st, a, sl, p, g, h, sp, sg, sh);
-  else dispatch_pg<FAM, 1, 1, NDT>(a.pg, grid, st, a, sl, p, g, h, sp, sg, sh);
+static void dispatch_cd(int ws, const DirectArgs& a, dim3 grid, hipStream_t st, int64_t sl,
+                        cplx* p, cplx* g, cplx* h, int64_t sp, int64_t sg, int64_t sh) {
+  if (a.ifcharge && !a.ifdipole) dispatch_pg<FAM, 1, 0, NDT>(ws, grid, st, a, sl, p, g, h, sp, sg, sh);
+  else if (!a.ifcharge && a.ifdipole) dispatch_pg<FAM, 0, 1, NDT>(ws, grid, st, a, sl, p, g, h, sp, sg, sh);
+  else dispatch_pg<FAM, 1, 1, NDT>(ws, grid, st, a, sl, p, g, h, sp, sg, sh);
 }
 
 // Split copies of the no-allocation entry point (fmm2d_l2d_direct_noalloc):
@@ -188,7 +229,9 @@ int direct_eval(const DirectArgs& a, hipStream_t st, bool noalloc) {
   if (a.nt == 0 || a.ns == 0) return FMM2D_OK;
   const int NDT = (a.nd == 1) ? 1 : 2;
   const int nchunk = (a.nd + NDT - 1) / NDT;
-  const int64_t tblocks = ceil_div(a.nt, kTile);
+  // few targets (one density): four waves share each block's 64 targets
+  const int ws = (NDT == 1 && ceil_div(a.nt, kTile) < 2048) ? 4 : 1;
+  const int64_t tblocks = ceil_div(a.nt, kTile / ws);
   const int NG = (a.family == CAUCHY) ? 1 : 2, NH = (a.family == CAUCHY) ? 1 : 3;
   const int64_t np = (int64_t)a.nd * a.nt;
   const int64_t ng = (a.pg >= 2) ? np * NG : 0;
@@ -223,13 +266,13 @@ int direct_eval(const DirectArgs& a, hipStream_t st, bool noalloc) {
   const int64_t sp = (nsplit > 1) ? np : 0, sg = (nsplit > 1) ? ng : 0, sh = (nsplit > 1) ? nh : 0;
   dim3 grid((unsigned)tblocks, (unsigned)nsplit, (unsigned)nchunk);
   if (NDT == 1) {
-    if (a.family == CAUCHY) dispatch_cd<CAUCHY, 1>(a, grid, st, split_len, p, g, h, sp, sg, sh);
-    else if (a.family == LAPLACE) dispatch_cd<LAPLACE, 1>(a, grid, st, split_len, p, g, h, sp, sg, sh);
-    else dispatch_cd<HELM, 1>(a, grid, st, split_len, p, g, h, sp, sg, sh);
+    if (a.family == CAUCHY) dispatch_cd<CAUCHY, 1>(ws, a, grid, st, split_len, p, g, h, sp, sg, sh);
+    else if (a.family == LAPLACE) dispatch_cd<LAPLACE, 1>(ws, a, grid, st, split_len, p, g, h, sp, sg, sh);
+    else dispatch_cd<HELM, 1>(ws, a, grid, st, split_len, p, g, h, sp, sg, sh);
   } else {
-    if (a.family == CAUCHY) dispatch_cd<CAUCHY, 2>(a, grid, st, split_len, p, g, h, sp, sg, sh);
-    else if (a.family == LAPLACE) dispatch_cd<LAPLACE, 2>(a, grid, st, split_len, p, g, h, sp, sg, sh);
-    else dispatch_cd<HELM, 2>(a, grid, st, split_len, p, g, h, sp, sg, sh);
+    if (a.family == CAUCHY) dispatch_cd<CAUCHY, 2>(ws, a, grid, st, split_len, p, g, h, sp, sg, sh);
+    else if (a.family == LAPLACE) dispatch_cd<LAPLACE, 2>(ws, a, grid, st, split_len, p, g, h, sp, sg, sh);
+    else dispatch_cd<HELM, 2>(ws, a, grid, st, split_len, p, g, h, sp, sg, sh);
   }
   FMM2D_HIP_TRY(hipGetLastError());
   if (nsplit > 1) {

@@ -141,17 +141,25 @@ FMM2D_HD void pair_laplace(double dx, double dy, const SrcStr<NDT>& s, double th
 }
 
 // ---------------------------------------------------------------- Helmholtz
-template <int IFC, int IFD, int PG, int NDT>
+// OOL: evaluate the Hankel pair by a call to the general evaluator (for rarely
+// taken paths: their kernels then reserve neither registers nor scalar
+// registers for the table-driven real path).
+template <int IFC, int IFD, int PG, int NDT, bool OOL = false>
 FMM2D_HD void pair_helm(double dx, double dy, cplx zk, const SrcStr<NDT>& s, double thresh,
                         Acc<HELM, PG, NDT>& a) {
   double rr = dx * dx + dy * dy;
   double r = sqrt(rr);
   cplx z = zk * r;
-  if (skip_pair<HELM, IFC, IFD, PG>(rr, thresh, 0.0, absc(z))) return;
+  // |z| of a real wavenumber is |Re z| exactly (what hypot returns for it)
+  const double absz = (zk.im == 0.0) ? fabs(z.re) : absc(z);
+  if (skip_pair<HELM, IFC, IFD, PG>(rr, thresh, 0.0, absz)) return;
   cplx h0, h1;
-  hank01(z, &h0, &h1);
+  if constexpr (OOL)
+    hank01_complex(z, &h0, &h1);
+  else
+    hank01(z, &h0, &h1);
   const cplx ik4 = cplx{0.0, 0.25};
-  const double rinv = 1.0 / r;
+  const double rinv = fast_rcp(r);
   const cplx kq = zk * ik4;  // i k / 4
   if (IFC) {
     const cplx p0 = h0 * ik4;

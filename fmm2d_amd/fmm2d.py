@@ -116,24 +116,39 @@ def _fmm(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
     densities and outputs).  pg/pgt are the
     evaluation flags at sources/targets (0 = none); targets may be None.
     """
-    import torch
     ns = _nsrc(sources)
-    need_vec = kind != "c"
     dev = _DevInputs(sources, ns, nd, charges, dipstr, dipvec,
-                     targets if pgt > 0 else None, need_vec, real=(kind == "r"))
-    nt = dev.nt
+                     targets if pgt > 0 else None, kind != "c", real=(kind == "r"))
+    bufs, ier = _fmm_device(kind, eps, zk, dev, pg, pgt, nd, opts)
+    res = {}
+    for name, t in bufs.items():
+        if t is None:
+            continue
+        a = nat.fortran_to_numpy(t)
+        res[name] = a[0] if nd == 1 else a
+    return res, ier
+
+
+def _fmm_device(kind, eps, zk, dev, pg, pgt, nd, opts=None, bufs=None):
+    """The native call of _fmm on inputs already on the device (a _DevInputs);
+    returns (dict of device output tensors in the C-ABI layout, ier).  bufs:
+    output tensors of an earlier call with the same shapes, reused (the driver
+    overwrites them)."""
+    ns, nt = dev.ns, dev.nt
     ng = 1 if kind == "c" else 2
     nh = 1 if kind == "c" else 3
 
     def shapes(n):
         return ((nd, n), (nd, n) if ng == 1 else (nd, ng, n), (nd, n) if nh == 1 else (nd, nh, n))
 
-    bufs = {}
-    for flag, n, names in ((pg, ns, ("pot", "grad", "hess")),
-                           (pgt, nt, ("pottarg", "gradtarg", "hesstarg"))):
-        for lvl, (name, shp) in enumerate(zip(names, shapes(n))):
-            odt = torch.float64 if kind == "r" else torch.complex128
-            bufs[name] = nat.dev_zeros_fortran(shp, odt) if flag > lvl else None
+    if bufs is None:
+        import torch
+        bufs = {}
+        for flag, n, names in ((pg, ns, ("pot", "grad", "hess")),
+                               (pgt, nt, ("pottarg", "gradtarg", "hesstarg"))):
+            for lvl, (name, shp) in enumerate(zip(names, shapes(n))):
+                odt = torch.float64 if kind == "r" else torch.complex128
+                bufs[name] = nat.dev_zeros_fortran(shp, odt) if flag > lvl else None
     L = nat.lib()
     ifc = 1 if dev.charge is not None else 0
     ifd = 1 if dev.dipstr is not None else 0
@@ -152,12 +167,6 @@ def _fmm(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
           "r": L.fmm2d_rfmm2d}[kind]
     ier = fn(*args)
     nat.check(ier, kind + "fmm2d")
-    res = {}
-    for name, t in bufs.items():
-        if t is None:
-            continue
-        a = nat.fortran_to_numpy(t)
-        res[name] = a[0] if nd == 1 else a
     if opts is not None:
         # per-call diagnostics written back into the caller's dict
         if "timeinfo" in opts:
@@ -166,7 +175,7 @@ def _fmm(kind, eps, zk, sources, charges, dipstr, dipvec, targets, pg, pgt, nd,
         if o.count_work:
             opts["work"] = np.array(o.work[:], dtype=np.float64)
             opts["near_pairs"], opts["m2l_pairs"] = int(o.near_pairs), int(o.m2l_pairs)
-    return res, ier
+    return bufs, ier
 
 
 _KIND = {"c": 0, "l": 1, "r": 2, "h": 3}

@@ -1,19 +1,24 @@
 """Generate training tuples with the FMM itself and train the proposer on the GPU.
 
-  python -m fmm2d_amd.train_proposer --problems 160 --out fmm2d_amd/data/proposer.npz
+  python -m fmm2d_amd.train_proposer --problems 200 --seed 21 --out tuples_a.npz
+  python -m fmm2d_amd.train_proposer --data a_tuples.json,b_tuples.json --out proposer.npz
 
 For each random problem (Laplace lfmm2d or Helmholtz hfmm2d; points uniform in
-a square, 4-64 Gaussian clusters or on the curve r = 1 + 0.3 cos 5t; N log-uniform
-in [1e4, 1e7]; zk log-uniform in [1, 300]; eps in {1e-4, 1e-6, 1e-9}) the FMM
-runs on a grid of parameters -- order reduction r in {0, .1, ..., .5} at every
-level and leaf capacity 2^e times the default, e in {-1, 0, 1, 2, 3} -- and each
-run's GPU time
-(the driver's own event timing, timeinfo[11]) and relative L2 error of
-pot+grad against the FP64 direct sum at 256 sampled sources are recorded.
-The target of a problem is the fastest setting whose error is at most eps/2.
-The network is then fitted by the library's device kernels (Adam on the
-weighted squared error).  Every tuple (features, parameters, error, time) is
-saved next to the weights.
+a square, 4-64 Gaussian clusters, on the curve r = 1 + 0.3 cos 5t, or half on
+that curve and half in 16 clusters on the unit circle; N log-uniform in
+[1e4, 1e7]; zk log-uniform in [1, ZKMAX]; eps in {1e-4, 1e-6, 1e-9}) the FMM
+runs on a grid of parameters -- order reduction r_f at the levels whose
+default order is below 40, r_c at the levels at or above it (the
+high-frequency levels of Helmholtz problems; Laplace orders never reach 40),
+and leaf capacity 2^e times the default; mixed precision for eps >= 1e-6,
+FP64 below, as the proposals are used -- and each run's GPU time (the
+driver's own event timing, timeinfo[11], the minimum of REPEATS calls after a
+warm-up) and relative L2 error of pot+grad against the FP64 direct sum at 256
+sampled sources are recorded.  The target of a problem is the fastest setting
+whose error is at most eps/2: per level, r_c where the default order is at
+least 40 and r_f elsewhere, and e.  The network is then fitted by the
+library's device kernels (Adam on the squared error).  Every tuple (features,
+parameters, error, time) is saved next to the weights.
 """
 import argparse
 import json
@@ -26,9 +31,13 @@ from . import fmm2d as F
 from . import proposer as PR
 
 R_GRID = [0.0, 0.1, 0.2, 0.3, 0.4, 0.5]
+RC_GRID = [0.0, 0.2, 0.4]  # reduction at levels whose default order is >= P_HF
+P_HF = 40
 E_GRID = [-1, 0, 1, 2, 3]
 NSAMPLE = 256
-REPEATS = 1  # timed calls per setting (after one warm-up call); the minimum is kept
+REPEATS = 2  # timed calls per setting (after one warm-up call); the minimum is kept
+ZKMAX = 1000.0
+DISTS = ["uniform", "cluster", "curve", "curvecluster"]
 
 
 def make_points(rng, n, dist):
@@ -43,57 +52,95 @@ def make_points(rng, n, dist):
         m = n // 10
         p[:, :m] = rng.random((2, m))
         return p
+    if dist == "curvecluster":
+        m = n // 2
+        t = rng.uniform(0, 2 * np.pi, m)
+        r = 1 + 0.3 * np.cos(5 * t)
+        curve = np.stack([r * np.cos(t), r * np.sin(t)]) + 1e-4 * rng.standard_normal((2, m))
+        a = rng.uniform(0, 2 * np.pi, 16)
+        lab = rng.integers(0, 16, n - m)
+        sig = np.exp(rng.uniform(np.log(1e-3), np.log(1e-2), 16))
+        cl = np.stack([np.cos(a), np.sin(a)])[:, lab] + sig[lab] * rng.standard_normal((2, n - m))
+        return np.concatenate([curve, cl], axis=1)
     t = rng.uniform(0, 2 * np.pi, n)
     r = 1 + 0.3 * np.cos(5 * t)
     return np.stack([r * np.cos(t), r * np.sin(t)]) + 1e-4 * rng.standard_normal((2, n))
 
 
-def problem_inputs(rng):
+def problem_inputs(rng, dists=("uniform", "cluster", "curve"), zkmax=300.0):
     """One random problem of the training distribution: (kind, dist, n, eps, zk,
     sources, charges, sampled indices).  Every random draw of a problem is
-    made here, in this order, so a seed replays the same problems."""
+    made here, in this order, so a seed replays the same problems (the
+    defaults are those of the first shipped tuple set)."""
     kind = "h" if rng.random() < 0.5 else "l"
-    dist = ["uniform", "cluster", "curve"][int(rng.integers(0, 3))]
+    dist = list(dists)[int(rng.integers(0, len(dists)))]
     n = int(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
     eps = float([1e-4, 1e-6, 1e-9][int(rng.integers(0, 3))])
-    zk = complex(np.exp(rng.uniform(0, np.log(300.0)))) if kind == "h" else None
+    zk = complex(np.exp(rng.uniform(0, np.log(zkmax)))) if kind == "h" else None
     src = make_points(rng, n, dist)
     ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     idx = np.sort(rng.choice(n, NSAMPLE, replace=False))
     return kind, dist, n, eps, zk, src, ch, idx
 
 
-def run_problem(rng):
-    kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng)
+def precision_for(eps):
+    """The arithmetic a proposal is used with: mixed precision (FP32 M2L and near
+    field) down to eps = 1e-6, FP64 below."""
+    return "mixed" if eps >= 1e-6 else "fp64"
+
+
+def run_problem(rng, dists=DISTS, zkmax=ZKMAX):
+    kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng, dists, zkmax)
     (p, g, _), _ = F._direct(kind, zk, src, src[:, idx], ch, None, None, 2, 1, 1e-16)
     ex = np.concatenate([p[0], g[0].ravel()])
     nd0 = PR.default_ndiv(eps, n, 0)
     feats, bs0 = PR.tree_features(src, None, eps, zk, kind, nd0)
     p0 = PR.default_nterms(kind, eps, [bs0 / 2.0 ** l for l in range(52)], zk)
+    hf = [l for l in range(PR.NLEV) if p0[l] >= P_HF]
+    prec = precision_for(eps)
+    # the inputs stay on the device for every run of the grid; only the
+    # sampled outputs come back
+    import torch
+    dev = F._DevInputs(src, n, 1, ch, None, None, None, True)
+    sel = torch.from_numpy(idx).to(dev.src.device)
+    bufs = None
     rows = []
     for e in E_GRID:
-        for r in R_GRID:
-            y = np.zeros(PR.NOUT)
-            y[:PR.NLEV] = r
-            y[PR.NLEV] = e
-            nterms, ndiv = PR.decode(y, p0, nd0)
-            opts = {"nterms": nterms, "ndiv": ndiv, "timeinfo": np.zeros(16)}
-            F._fmm(kind, eps, zk, src, ch, None, None, None, 2, 0, 1, opts=dict(opts))
-            best = None
-            for _ in range(REPEATS):
-                res, ier = F._fmm(kind, eps, zk, src, ch, None, None, None, 2, 0, 1, opts=opts)
-                t = float(opts["timeinfo"][11])
-                best = t if best is None else min(best, t)
-            got = np.concatenate([res["pot"][idx], res["grad"][:, idx].ravel()])
-            err = float(np.linalg.norm(got - ex) / np.linalg.norm(ex))
-            rows.append({"r": r, "e": e, "time": best, "err": err, "ier": int(ier)})
+        for rc in (RC_GRID if hf else [None]):
+            for r in R_GRID:
+                y = np.zeros(PR.NOUT)
+                y[:PR.NLEV] = r
+                if rc is not None:
+                    y[hf] = rc
+                y[PR.NLEV] = e
+                nterms, ndiv = PR.decode(y, p0, nd0)
+                opts = {"nterms": nterms, "ndiv": ndiv, "precision": prec,
+                        "timeinfo": np.zeros(16)}
+                bufs, _ = F._fmm_device(kind, eps, zk, dev, 2, 0, 1, dict(opts), bufs)
+                best = None
+                for _ in range(REPEATS):
+                    bufs, ier = F._fmm_device(kind, eps, zk, dev, 2, 0, 1, opts, bufs)
+                    t = float(opts["timeinfo"][11])
+                    best = t if best is None else min(best, t)
+                # device layouts: pot (ns, 1), grad (ns, 2, 1)
+                pot = bufs["pot"][sel, 0].cpu().numpy()
+                grad = bufs["grad"][sel, :, 0].cpu().numpy().T
+                got = np.concatenate([pot, grad.ravel()])
+                err = float(np.linalg.norm(got - ex) / np.linalg.norm(ex))
+                rows.append({"r": r, "rc": rc, "e": e, "time": best, "err": err,
+                             "ier": int(ier)})
     ok = [w for w in rows if w["err"] <= 0.5 * eps and w["ier"] == 0]
-    tgt = min(ok, key=lambda w: w["time"]) if ok else {"r": 0.0, "e": 0}
-    default = [w for w in rows if w["r"] == 0.0 and w["e"] == 0][0]
-    return {"kind": kind, "dist": dist, "n": n, "eps": eps,
+    tgt = min(ok, key=lambda w: w["time"]) if ok else {"r": 0.0, "rc": None, "e": 0}
+    default = [w for w in rows if w["r"] == 0.0 and w["e"] == 0 and not w["rc"]][0]
+    target = [0.0] * PR.NOUT
+    for l in range(PR.NLEV):
+        target[l] = tgt["rc"] if (tgt["rc"] is not None and l in hf) else tgt["r"]
+    target[PR.NLEV] = tgt["e"]
+    return {"kind": kind, "dist": dist, "n": n, "eps": eps, "gen": [list(dists), zkmax],
+            "precision": prec,
             "zk": None if zk is None else [zk.real, zk.imag], "features": feats.tolist(),
-            "grid": rows, "target": [tgt["r"], tgt["e"]], "default": default,
-            "chosen": tgt}
+            "grid": rows, "target": [tgt["r"], tgt["e"]], "target_vec": target,
+            "default": default, "chosen": tgt}
 
 
 def fit(feats, targets, steps=3000, lr=3e-3, seed=0):
@@ -135,7 +182,8 @@ def refeature(problems, seed):
     one); the measured tuples and targets do not depend on the features."""
     rng = np.random.default_rng(seed)
     for i, pb in enumerate(problems):
-        kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng)
+        gen = pb.get("gen", [("uniform", "cluster", "curve"), 300.0])
+        kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng, gen[0], gen[1])
         if (kind, dist, n, eps) != (pb["kind"], pb["dist"], pb["n"], pb["eps"]):
             raise ValueError(f"problem {i}: replay gives {(kind, dist, n, eps)}, stored "
                              f"{(pb['kind'], pb['dist'], pb['n'], pb['eps'])}")
@@ -148,8 +196,11 @@ def refeature(problems, seed):
 def targets_of(problems):
     y = np.zeros((len(problems), PR.NOUT), dtype=np.float32)
     for i, pb in enumerate(problems):
-        y[i, :PR.NLEV] = pb["target"][0]
-        y[i, PR.NLEV] = pb["target"][1]
+        if "target_vec" in pb:
+            y[i] = pb["target_vec"]
+        else:  # one reduction at every level
+            y[i, :PR.NLEV] = pb["target"][0]
+            y[i, PR.NLEV] = pb["target"][1]
     return y
 
 
@@ -160,14 +211,18 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--out", default=PR.WEIGHTS)
     ap.add_argument("--data", default=None, help="reuse tuples from this JSON file")
+    ap.add_argument("--tuples-only", action="store_true",
+                    help="generate and save the tuples without fitting")
     ap.add_argument("--refeature", action="store_true",
                     help="with --data: replay the stored problems from --seed (the seed they "
                          "were generated with), recompute their features with the current "
                          "feature function and refit; the targets are kept")
     args = ap.parse_args()
     if args.data:
-        with open(args.data) as f:
-            problems = json.load(f)
+        problems = []
+        for path in args.data.split(","):
+            with open(path) as f:
+                problems += json.load(f)
         if args.refeature:
             refeature(problems, args.seed)
     else:
@@ -181,6 +236,11 @@ def main():
                   f"target r={pb['target'][0]} e={pb['target'][1]} "
                   f"t={pb['chosen'].get('time', 0):.4f}s default={pb['default']['time']:.4f}s "
                   f"({time.time() - t0:.0f}s)", flush=True)
+    if args.tuples_only:
+        with open(args.out.replace(".npz", "_tuples.json"), "w") as f:
+            json.dump(problems, f)
+        print("saved tuples", flush=True)
+        return
     feats = np.array([pb["features"] for pb in problems], dtype=np.float32)
     params, hist = fit(feats, targets_of(problems), steps=args.steps)
     for s, l in hist[-3:]:

@@ -1,12 +1,13 @@
 """The trained proposer (fmm2d_amd/data/proposer.npz) on problems it was not
 trained on: its proposals meet eps against the FP64 direct sum and, summed
-over the problems, run faster than the reference's nterms / ndiv rules."""
+over the problems, run faster than the reference's nterms / ndiv rules (both in
+the arithmetic a proposal is used with: mixed precision down to eps = 1e-6)."""
 import numpy as np
 import pytest
 
 from fmm2d_amd import fmm2d as F
 from fmm2d_amd import proposer as PR
-from fmm2d_amd.train_proposer import make_points
+from fmm2d_amd.train_proposer import make_points, precision_for
 
 pytestmark = pytest.mark.gpu
 
@@ -35,12 +36,14 @@ def test_proposals_meet_eps_and_beat_default_rules():
         src = make_points(rng, n, dist)
         ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
         zkc = None if zk is None else complex(zk)
-        res, info = PR.fmm_validated(kind, eps, src, ch, zk=zkc, proposer=prop)
+        # the arithmetic the proposals are trained for and used with
+        prec = {"precision": precision_for(eps)}
+        res, info = PR.fmm_validated(kind, eps, src, ch, zk=zkc, proposer=prop, opts=prec)
         assert info["used"] == "proposal", info
         assert info["err"] <= eps, info
         p = info["proposal"]
-        tp = _time(kind, eps, zkc, src, ch, {"nterms": p["nterms"], "ndiv": p["ndiv"]})
-        td = _time(kind, eps, zkc, src, ch, {})
+        tp = _time(kind, eps, zkc, src, ch, dict(prec, nterms=p["nterms"], ndiv=p["ndiv"]))
+        td = _time(kind, eps, zkc, src, ch, prec)
         print(f"{kind} {dist} n={n} eps={eps:.0e}: proposal {tp*1e3:.2f} ms "
               f"(err {info['err']:.2e}), default {td*1e3:.2f} ms")
         t_prop += tp
