@@ -57,6 +57,13 @@ SIGNATURES["fmm2d_cfmm2d"] = (_i, [_i, _d] + _FMM_TAIL + _FMM_OUT)
 SIGNATURES["fmm2d_lfmm2d"] = (_i, [_i, _d] + _FMM_TAIL + [_vp] + _FMM_OUT)
 SIGNATURES["fmm2d_hfmm2d"] = (_i, [_i, _d, _d, _d] + _FMM_TAIL + [_vp] + _FMM_OUT)
 SIGNATURES["fmm2d_batch"] = (_i, [_i, _vp, _i])
+SIGNATURES["fmm2d_bhfmm2d"] = (_i, [_i, _d] + _FMM_TAIL + _FMM_OUT)
+SIGNATURES["fmm2d_bh2d_direct"] = (_i, [_i, _vp, _i64, _i, _vp, _i, _vp, _vp, _i64, _i, _vp,
+                                        _vp, _d, _vp])
+SIGNATURES["fmm2d_stfmm2d"] = (_i, [_i, _d, _i64, _vp, _i, _vp, _i, _vp, _vp, _i, _vp, _vp,
+                                    _vp, _i64, _vp, _i, _vp, _vp, _vp, _vp, _vp])
+SIGNATURES["fmm2d_st2d_direct"] = (_i, [_i, _vp, _i, _vp, _i, _vp, _vp, _i64, _vp, _i64, _vp,
+                                        _vp, _vp, _d, _vp])
 SIGNATURES["fmm2d_hfmm2d_mps"] = (_i, [_i, _d, _d, _d, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                                        _vp, _vp])
 

@@ -2,6 +2,7 @@
 // Declarations and reference citations: include/fmm2d_amd.h.
 #include "../../include/fmm2d_amd.h"
 
+#include "bh_st.h"
 #include "common.h"
 #include "p2p.h"
 
@@ -86,6 +87,51 @@ int fmm2d_r2d_direct(int nd, const double* sources, int64_t ns, int ifcharge, co
                      double thresh, void* stream) {
   return direct_eval_real(nd, sources, ns, ifcharge, charge, ifdipole, dipstr, dipvec, targ, nt,
                           ifpgh, thresh, pot, grad, hess, reinterpret_cast<hipStream_t>(stream));
+}
+
+int fmm2d_bh2d_direct(int nd, const double* sources, int64_t ns, int ifcharge,
+                      const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dip,
+                      const double* targ, int64_t nt, int ifpgh, fmm2d_cplx* vel,
+                      fmm2d_cplx* grad, double thresh, void* stream) {
+  if ((ns > 0 && !sources) || (nt > 0 && !targ)) return FMM2D_ERR_ARG;
+  BhDirectArgs a{};
+  a.nd = nd;
+  a.src = sources;
+  a.ns = ns;
+  a.ifcharge = ifcharge;
+  a.ifdipole = ifdipole;
+  a.charge = reinterpret_cast<const cplx*>(charge);
+  a.dip = reinterpret_cast<const cplx*>(dip);
+  a.targ = targ;
+  a.nt = nt;
+  a.pg = ifpgh;
+  a.vel = reinterpret_cast<cplx*>(vel);
+  a.grad = reinterpret_cast<cplx*>(grad);
+  a.thresh = thresh;
+  return bh_direct(a, reinterpret_cast<hipStream_t>(stream));
+}
+
+int fmm2d_st2d_direct(int nd, const double* sources, int ifstoklet, const double* stoklet,
+                      int istress, const double* strslet, const double* strsvec, int64_t ns,
+                      const double* targ, int64_t nt, double* pot, double* pre, double* grad,
+                      double thresh, void* stream) {
+  if ((ns > 0 && !sources) || (nt > 0 && !targ)) return FMM2D_ERR_ARG;
+  StDirectArgs a{};
+  a.nd = nd;
+  a.src = sources;
+  a.ns = ns;
+  a.ifstoklet = ifstoklet;
+  a.stoklet = stoklet;
+  a.istress = istress;
+  a.strslet = strslet;
+  a.strsvec = strsvec;
+  a.targ = targ;
+  a.nt = nt;
+  a.pot = pot;
+  a.pre = pre;
+  a.grad = grad;
+  a.thresh = thresh;
+  return st_direct(a, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/fmm2d_amd.h"
+#include "bh_st.h"
 #include "helm_ops.h"
 #include "lap_ops.h"
 #include "tree.h"
@@ -96,6 +97,66 @@ struct TreeHandle {
 }  // namespace
 
 extern "C" {
+
+int fmm2d_bhfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
+                  const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dip, int iper,
+                  int ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, int64_t nt,
+                  const double* targ, int ifpghtarg, fmm2d_cplx* pottarg,
+                  fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg, fmm2d_opts* opts, void* stream) {
+  (void)hess;
+  (void)hesstarg;
+  if (iper != 0) return FMM2D_ERR_UNSUPPORTED;
+  BhArgs a{};
+  a.nd = nd;
+  a.eps = eps;
+  a.ns = ns;
+  a.src = sources;
+  a.ifcharge = ifcharge;
+  a.charge = reinterpret_cast<const cplx*>(charge);
+  a.ifdipole = ifdipole;
+  a.dip = reinterpret_cast<const cplx*>(dip);
+  a.ifpgh = ifpgh;
+  a.pot = reinterpret_cast<cplx*>(pot);
+  a.grad = reinterpret_cast<cplx*>(grad);
+  a.nt = nt;
+  a.targ = targ;
+  a.ifpghtarg = ifpghtarg;
+  a.pottarg = reinterpret_cast<cplx*>(pottarg);
+  a.gradtarg = reinterpret_cast<cplx*>(gradtarg);
+  a.opts = opts;
+  keep_pool_memory();
+  return bh_fmm(a, reinterpret_cast<hipStream_t>(stream));
+}
+
+int fmm2d_stfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifstoklet,
+                  const double* stoklet, int ifstrslet, const double* strslet,
+                  const double* strsvec, int ifppreg, double* pot, double* pre, double* grad,
+                  int64_t nt, const double* targ, int ifppregtarg, double* pottarg,
+                  double* pretarg, double* gradtarg, fmm2d_opts* opts, void* stream) {
+  StArgs a{};
+  a.nd = nd;
+  a.eps = eps;
+  a.ns = ns;
+  a.src = sources;
+  a.ifstoklet = ifstoklet;
+  a.stoklet = stoklet;
+  a.ifstrslet = ifstrslet;
+  a.strslet = strslet;
+  a.strsvec = strsvec;
+  a.ifppreg = ifppreg;
+  a.pot = pot;
+  a.pre = pre;
+  a.grad = grad;
+  a.nt = nt;
+  a.targ = targ;
+  a.ifppregtarg = ifppregtarg;
+  a.pottarg = pottarg;
+  a.pretarg = pretarg;
+  a.gradtarg = gradtarg;
+  a.opts = opts;
+  keep_pool_memory();
+  return st_fmm(a, reinterpret_cast<hipStream_t>(stream));
+}
 
 int fmm2d_cfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
                  const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dipstr, int iper,

@@ -606,17 +606,287 @@ def hfmm2d_mps(*, eps, zk, centers, rscales, nterms, mpole, impole=None, nd=1, n
 
 
 # --------------------------------------------------------------------------
-# families of the reference that are outside this library's scope
-def _out_of_scope(name):
-    def f(*args, **kwargs):
-        raise NotImplementedError(
-            f"{name} is not provided by fmm2d_amd (only the Laplace (complex and real), "
-            "Cauchy and Helmholtz FMMs and direct evaluators are)")
-    f.__name__ = name
-    return f
+# Biharmonic and Stokes sums (python/fmm2dpy/fmm2d.py:966-1248, 1730-2016)
+def _bh_inputs(charges, dipoles, nd, ns):
+    """The reference's checks and reshapes (python/fmm2dpy/fmm2d.py:1031-1054)."""
+    ifcharge = ifdipole = 0
+    if charges is not None:
+        if nd == 1 and ns > 1:
+            assert charges.shape[0] == 2 and charges.shape[1] == ns, \
+                "Charges must be of shape [2, number of source]"
+        if nd == 1 and ns == 1:
+            assert charges.shape[0] == 2, "Charges must of shape [2, number of sources]"
+        if nd > 1:
+            assert charges.shape[0] == nd and charges.shape[1] == 2 and charges.shape[2] == ns, \
+                "Charges must be of shape [nd,2,ns] where nd is number of densities, and ns is number of sources"
+        ifcharge = 1
+        charges = np.reshape(charges, [nd, 2, ns])
+    else:
+        charges = np.zeros([nd, 2, ns], dtype="complex")
+    if dipoles is not None:
+        if nd == 1 and ns > 1:
+            assert dipoles.shape[0] == 3 and dipoles.shape[1] == ns, \
+                "Dipole must of shape [3, number of sources]"
+        if nd == 1 and ns == 1:
+            assert dipoles.shape[0] == 3, "Dipole must of shape [3, number of sources]"
+        if nd > 1:
+            assert dipoles.shape[0] == nd and dipoles.shape[1] == 3 and dipoles.shape[2] == ns, \
+                "Dipole must of shape [nd,2, number of sources]"
+        dipoles = np.reshape(dipoles, [nd, 3, ns])
+        ifdipole = 1
+    else:
+        dipoles = np.zeros([nd, 3, ns], dtype="complex")
+    return ifcharge, charges, ifdipole, dipoles
 
 
-bhfmm2d = _out_of_scope("bhfmm2d")
-stfmm2d = _out_of_scope("stfmm2d")
-bh2ddir = _out_of_scope("bh2ddir")
-st2ddir = _out_of_scope("st2ddir")
+def bhfmm2d(*, eps, sources, charges=None, dipoles=None, targets=None, pg=0, pgt=0, nd=1):
+    """Biharmonic FMM (python/fmm2dpy/fmm2d.py:966-1085; biharmonic/bhfmm2d.f:1).
+
+    vel(z) = sum_j 2 c_{j,1} log|z-z_j| + c_{j,2} (z-z_j)/conj(z-z_j) + d_{j,1}/(z-z_j)
+             + d_{j,3}/conj(z-z_j) + d_{j,2} (z-z_j)/conj(z-z_j)^2;
+    grad (3 components): d/dz vel projected on z, on zbar, and d/dzbar vel.
+    """
+    out = Output()
+    ns = _nsrc(sources)
+    if pg == 0 and pgt == 0:
+        print("Nothing to compute, set either pg or pgt to non-zero")
+        return out
+    ifcharge, charges, ifdipole, dipoles = _bh_inputs(charges, dipoles, nd, ns)
+    if targets is not None:
+        assert targets.shape[0] == 2, "The first dimension of targets must be 2"
+        nt = np.shape(targets)[1]
+    else:
+        targets = np.zeros([2, 0], dtype="double")
+        nt = 0
+    import torch
+    src = nat.to_dev_fortran(np.reshape(sources, (2, ns)), np.float64)
+    ch = nat.to_dev_fortran(charges, np.complex128)
+    dp = nat.to_dev_fortran(dipoles, np.complex128)
+    trg = nat.to_dev_fortran(targets, np.float64) if nt > 0 else None
+    pot = nat.dev_zeros_fortran((nd, ns), torch.complex128)
+    grad = nat.dev_zeros_fortran((nd, 3, ns), torch.complex128)
+    pt = nat.dev_zeros_fortran((nd, max(nt, 1)), torch.complex128)
+    gt = nat.dev_zeros_fortran((nd, 3, max(nt, 1)), torch.complex128)
+    ier = nat.lib().fmm2d_bhfmm2d(nd, float(eps), ns, nat.ptr(src), ifcharge, nat.ptr(ch),
+                                  ifdipole, nat.ptr(dp), 0, int(pg), nat.ptr(pot),
+                                  nat.ptr(grad), None, nt, nat.ptr(trg), int(pgt) if nt else 0,
+                                  nat.ptr(pt), nat.ptr(gt), None, None, nat.stream_ptr())
+    nat.check(ier, "bhfmm2d")
+    out.pot = nat.fortran_to_numpy(pot)
+    out.grad = nat.fortran_to_numpy(grad)
+    out.pottarg = nat.fortran_to_numpy(pt)[:, :nt]
+    out.gradtarg = nat.fortran_to_numpy(gt)[:, :, :nt]
+    out.ier = ier
+    out.hess = None
+    out.hesstarg = None
+    if nd == 1:
+        if pgt > 0:
+            out.pottarg = out.pottarg.reshape(nt,)
+        if pgt == 2:
+            out.gradtarg = out.gradtarg.reshape(3, nt)
+        if pg > 0:
+            out.pot = out.pot.reshape(ns,)
+        if pg == 2:
+            out.grad = out.grad.reshape(3, ns)
+    if pg < 2:
+        out.grad = None
+    if pgt < 2:
+        out.gradtarg = None
+    return out
+
+
+def bh2ddir(*, sources, targets, charges=None, dipoles=None, pgt=0, nd=1, thresh=1e-16):
+    """Biharmonic direct evaluation at targets (python/fmm2dpy/fmm2d.py:1730-1843;
+    biharmonic/bhkernels2d.f:26-410)."""
+    out = Output()
+    ns = _nsrc(sources)
+    if pgt == 0:
+        print("Nothing to compute, set either pg or pgt to non-zero")
+        return out
+    ifcharge, charges, ifdipole, dipoles = _bh_inputs(charges, dipoles, nd, ns)
+    assert targets.shape[0] == 2, "The first dimension of targets must be 2"
+    nt = targets.shape[1]
+    if not (ifcharge or ifdipole) or pgt not in (1, 2):
+        return out
+    import torch
+    src = nat.to_dev_fortran(np.reshape(sources, (2, ns)), np.float64)
+    ch = nat.to_dev_fortran(charges, np.complex128)
+    dp = nat.to_dev_fortran(dipoles, np.complex128)
+    trg = nat.to_dev_fortran(targets, np.float64)
+    vel = nat.dev_zeros_fortran((nd, nt), torch.complex128)
+    grad = nat.dev_zeros_fortran((nd, 3, nt), torch.complex128)
+    ier = nat.lib().fmm2d_bh2d_direct(nd, nat.ptr(src), ns, ifcharge, nat.ptr(ch), ifdipole,
+                                      nat.ptr(dp), nat.ptr(trg), nt, int(pgt), nat.ptr(vel),
+                                      nat.ptr(grad), float(thresh), nat.stream_ptr())
+    nat.check(ier, "bh2d_direct")
+    out.pottarg = nat.fortran_to_numpy(vel)
+    if pgt == 2:
+        out.gradtarg = nat.fortran_to_numpy(grad)
+    if nd == 1:
+        out.pottarg = out.pottarg.reshape(nt,)
+        if pgt == 2:
+            out.gradtarg = out.gradtarg.reshape(3, nt,)
+    return out
+
+
+def _st_inputs(sources, stoklet, strslet, strsvec, targets, nd):
+    """The reference's checks and reshapes (python/fmm2dpy/fmm2d.py:1186-1228)."""
+    assert sources.shape[0] == 2, "The first dimension of sources must be 2"
+    ns = _nsrc(sources)
+    if targets is not None:
+        assert targets.shape[0] == 2, "The first dimension of targets must be 2"
+        nt = targets.shape[1] if np.size(np.shape(targets)) == 2 else 1
+    else:
+        targets = np.zeros([2, 0], dtype="double")
+        nt = 0
+    ifstoklet = ifstrslet = 0
+    if stoklet is not None:
+        if nd == 1 and ns > 1:
+            assert stoklet.shape[0] == 2 and stoklet.shape[1] == ns, \
+                "stoklet vectors must be of shape [2,number of sources]"
+        if nd == 1 and ns == 1:
+            assert stoklet.shape[0] == 2, "stoklet vectors must be of shape [2,number of sources]"
+        if nd > 1:
+            assert stoklet.shape[0] == nd and stoklet.shape[1] == 2 and stoklet.shape[2] == ns, \
+                "stoklet vectors must be of shape [nd,2,ns] where nd is number of densities, and ns is number of sources"
+        stoklet = np.reshape(stoklet, [nd, 2, ns])
+        ifstoklet = 1
+    else:
+        stoklet = np.zeros([nd, 2, ns], dtype="double")
+    if strslet is not None and strsvec is not None:
+        if nd == 1 and ns > 1:
+            assert strslet.shape[0] == 2 and strslet.shape[1] == ns, \
+                "strslet vectors must be of shape [2,number of sources]"
+            assert strsvec.shape[0] == 2 and strsvec.shape[1] == ns, \
+                "strsvec vectors must be of shape [2,number of sources]"
+        if nd == 1 and ns == 1:
+            assert strslet.shape[0] == 2, "strslet vectors must be of shape [2,number of sources]"
+            assert strsvec.shape[0] == 2, "strsvec vectors must be of shape [2,number of sources]"
+        if nd > 1:
+            assert strslet.shape[0] == nd and strslet.shape[1] == 2 and strslet.shape[2] == ns, \
+                "strslet vectors must be of shape [nd,2,ns] where nd is number of densities, and ns is number of sources"
+            assert strsvec.shape[0] == nd and strsvec.shape[1] == 2 and strsvec.shape[2] == ns, \
+                "strsvec vectors must be of shape [nd,2,ns] where nd is number of densities, and ns is number of sources"
+        strslet = np.reshape(strslet, [nd, 2, ns])
+        strsvec = np.reshape(strsvec, [nd, 2, ns])
+        ifstrslet = 1
+    else:
+        strslet = np.zeros([nd, 2, ns], dtype="double")
+        strsvec = np.zeros([nd, 2, ns], dtype="double")
+    return ns, nt, targets, ifstoklet, stoklet, ifstrslet, strslet, strsvec
+
+
+def _st_nothing(ifppreg, ifppregtarg, stoklet, strslet, strsvec):
+    """The reference's early returns (python/fmm2dpy/fmm2d.py:1172-1184); True when
+    there is nothing to compute."""
+    if ifppreg == 0 and ifppregtarg == 0:
+        print("Nothing to compute, set either ifppreg or ifppregtarg to non-zero")
+        return True
+    if stoklet is None and strslet is None and strsvec is None:
+        print("Nothing to compute, set either stoklet or strslet+strsvec to non-None")
+        return True
+    if (strslet is not None) != (strsvec is not None):
+        print("strslet and strsvec mush be both None or both not None")
+        return True
+    return False
+
+
+def _st_trim(out, ifppreg, ifppregtarg):
+    """python/fmm2dpy/fmm2d.py:1233-1248 / 2003-2014."""
+    if ifppreg < 3:
+        out.grad = None
+    if ifppregtarg < 3:
+        out.gradtarg = None
+    if ifppreg < 2:
+        out.pre = None
+    if ifppregtarg < 2:
+        out.pretarg = None
+    if ifppreg < 1:
+        out.pot = None
+    if ifppregtarg < 1:
+        out.pottarg = None
+    return out
+
+
+def stfmm2d(*, eps, sources, stoklet=None, strslet=None, strsvec=None, targets=None,
+            ifppreg=0, ifppregtarg=0, nd=1):
+    """Stokes FMM (python/fmm2dpy/fmm2d.py:1087-1248; stokes/stfmm2d.f:27).
+
+    u_i(x) = sum_m G_ij(x, y_m) sigma_m,j + T_ijk(x, y_m) mu_m,j nu_m,k with
+    G_ij = r_i r_j/(2 r^2) - delta_ij log(r)/2, T_ijk = -2 r_i r_j r_k / r^4;
+    pressure and velocity gradient on request.  Outputs keep the reference's
+    Fortran shapes pot (nd,2,n), pre (nd,n), grad (nd,2,2,n).
+    """
+    out = Output()
+    if _st_nothing(ifppreg, ifppregtarg, stoklet, strslet, strsvec):
+        return out
+    ns, nt, targets, ifs, stoklet, ift, strslet, strsvec = _st_inputs(
+        sources, stoklet, strslet, strsvec, targets, nd)
+    import torch
+    f64 = torch.float64
+    src = nat.to_dev_fortran(np.reshape(sources, (2, ns)), np.float64)
+    sk = nat.to_dev_fortran(stoklet, np.float64)
+    sl = nat.to_dev_fortran(strslet, np.float64)
+    sv = nat.to_dev_fortran(strsvec, np.float64)
+    trg = nat.to_dev_fortran(targets, np.float64) if nt > 0 else None
+    n1 = max(nt, 1)
+    pot, pre = nat.dev_zeros_fortran((nd, 2, ns), f64), nat.dev_zeros_fortran((nd, ns), f64)
+    grad = nat.dev_zeros_fortran((nd, 2, 2, ns), f64)
+    pt, prt = nat.dev_zeros_fortran((nd, 2, n1), f64), nat.dev_zeros_fortran((nd, n1), f64)
+    gt = nat.dev_zeros_fortran((nd, 2, 2, n1), f64)
+    ier = nat.lib().fmm2d_stfmm2d(nd, float(eps), ns, nat.ptr(src), ifs, nat.ptr(sk), ift,
+                                  nat.ptr(sl), nat.ptr(sv), int(ifppreg), nat.ptr(pot),
+                                  nat.ptr(pre), nat.ptr(grad), nt, nat.ptr(trg),
+                                  int(ifppregtarg) if nt else 0, nat.ptr(pt), nat.ptr(prt),
+                                  nat.ptr(gt), None, nat.stream_ptr())
+    nat.check(ier, "stfmm2d")
+    out.pot, out.pre, out.grad = (nat.fortran_to_numpy(pot), nat.fortran_to_numpy(pre),
+                                  nat.fortran_to_numpy(grad))
+    out.pottarg = nat.fortran_to_numpy(pt)[..., :nt]
+    out.pretarg = nat.fortran_to_numpy(prt)[..., :nt]
+    out.gradtarg = nat.fortran_to_numpy(gt)[..., :nt]
+    out.ier = ier
+    return _st_trim(out, ifppreg, ifppregtarg)
+
+
+def st2ddir(*, eps, sources, stoklet=None, strslet=None, strsvec=None, targets=None,
+            ifppreg=0, ifppregtarg=0, nd=1, thresh=1e-16):
+    """Stokes direct evaluation (python/fmm2dpy/fmm2d.py:1845-2015;
+    stokes/stokkernels2d.f:24, 169) at the sources (the self term skipped by
+    thresh) and at the targets.
+
+    Two deliberate differences from the reference wrapper: the "nothing to
+    compute" test uses ``is None`` (the reference's ``stoklet == None`` raises
+    for an array argument), and the evaluation at the sources counts the
+    sources (the reference passes the number of targets there, so it returns
+    empty source outputs when targets is None).
+    """
+    out = Output()
+    if _st_nothing(ifppreg, ifppregtarg, stoklet, strslet, strsvec):
+        return out
+    ns, nt, targets, ifs, stoklet, ift, strslet, strsvec = _st_inputs(
+        sources, stoklet, strslet, strsvec, targets, nd)
+    import torch
+    f64 = torch.float64
+    L = nat.lib()
+    src = nat.to_dev_fortran(np.reshape(sources, (2, ns)), np.float64)
+    sk = nat.to_dev_fortran(stoklet, np.float64)
+    sl = nat.to_dev_fortran(strslet, np.float64)
+    sv = nat.to_dev_fortran(strsvec, np.float64)
+
+    def run(trg_np, n):
+        pot, pre = nat.dev_zeros_fortran((nd, 2, n), f64), nat.dev_zeros_fortran((nd, n), f64)
+        grad = nat.dev_zeros_fortran((nd, 2, 2, n), f64)
+        if n > 0:
+            trg = nat.to_dev_fortran(trg_np, np.float64)
+            ier = L.fmm2d_st2d_direct(nd, nat.ptr(src), ifs, nat.ptr(sk), 1 if ift else 0,
+                                      nat.ptr(sl), nat.ptr(sv), ns, nat.ptr(trg), n,
+                                      nat.ptr(pot), nat.ptr(pre), nat.ptr(grad), float(thresh),
+                                      nat.stream_ptr())
+            nat.check(ier, "st2d_direct")
+        return (nat.fortran_to_numpy(pot), nat.fortran_to_numpy(pre),
+                nat.fortran_to_numpy(grad))
+
+    out.pot, out.pre, out.grad = run(np.reshape(sources, (2, ns)), ns)
+    out.pottarg, out.pretarg, out.gradtarg = run(targets, nt)
+    return _st_trim(out, ifppreg, ifppregtarg)

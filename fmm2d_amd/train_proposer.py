@@ -6,7 +6,7 @@
 For each random problem (Laplace lfmm2d or Helmholtz hfmm2d; points uniform in
 a square, 4-64 Gaussian clusters, on the curve r = 1 + 0.3 cos 5t, or half on
 that curve and half in 16 clusters on the unit circle; N log-uniform in
-[1e4, 1e7]; zk log-uniform in [1, ZKMAX]; eps in {1e-4, 1e-6, 1e-9}) the FMM
+[1e4, nmax] (1e7 by default); zk log-uniform in [1, ZKMAX]; eps in {1e-4, 1e-6, 1e-9}) the FMM
 runs on a grid of parameters -- order reduction r_f at the levels whose
 default order is below 40, r_c at the levels at or above it (the
 high-frequency levels of Helmholtz problems; Laplace orders never reach 40),
@@ -67,14 +67,14 @@ def make_points(rng, n, dist):
     return np.stack([r * np.cos(t), r * np.sin(t)]) + 1e-4 * rng.standard_normal((2, n))
 
 
-def problem_inputs(rng, dists=("uniform", "cluster", "curve"), zkmax=300.0):
+def problem_inputs(rng, dists=("uniform", "cluster", "curve"), zkmax=300.0, nmax=1e7):
     """One random problem of the training distribution: (kind, dist, n, eps, zk,
     sources, charges, sampled indices).  Every random draw of a problem is
     made here, in this order, so a seed replays the same problems (the
     defaults are those of the first shipped tuple set)."""
     kind = "h" if rng.random() < 0.5 else "l"
     dist = list(dists)[int(rng.integers(0, len(dists)))]
-    n = int(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
+    n = int(np.exp(rng.uniform(np.log(1e4), np.log(nmax))))
     eps = float([1e-4, 1e-6, 1e-9][int(rng.integers(0, 3))])
     zk = complex(np.exp(rng.uniform(0, np.log(zkmax)))) if kind == "h" else None
     src = make_points(rng, n, dist)
@@ -89,8 +89,8 @@ def precision_for(eps):
     return "mixed" if eps >= 1e-6 else "fp64"
 
 
-def run_problem(rng, dists=DISTS, zkmax=ZKMAX):
-    kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng, dists, zkmax)
+def run_problem(rng, dists=DISTS, zkmax=ZKMAX, nmax=1e7):
+    kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng, dists, zkmax, nmax)
     (p, g, _), _ = F._direct(kind, zk, src, src[:, idx], ch, None, None, 2, 1, 1e-16)
     ex = np.concatenate([p[0], g[0].ravel()])
     nd0 = PR.default_ndiv(eps, n, 0)
@@ -136,7 +136,7 @@ def run_problem(rng, dists=DISTS, zkmax=ZKMAX):
     for l in range(PR.NLEV):
         target[l] = tgt["rc"] if (tgt["rc"] is not None and l in hf) else tgt["r"]
     target[PR.NLEV] = tgt["e"]
-    return {"kind": kind, "dist": dist, "n": n, "eps": eps, "gen": [list(dists), zkmax],
+    return {"kind": kind, "dist": dist, "n": n, "eps": eps, "gen": [list(dists), zkmax, nmax],
             "precision": prec,
             "zk": None if zk is None else [zk.real, zk.imag], "features": feats.tolist(),
             "grid": rows, "target": [tgt["r"], tgt["e"]], "target_vec": target,
@@ -183,7 +183,8 @@ def refeature(problems, seed):
     rng = np.random.default_rng(seed)
     for i, pb in enumerate(problems):
         gen = pb.get("gen", [("uniform", "cluster", "curve"), 300.0])
-        kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng, gen[0], gen[1])
+        nmax = gen[2] if len(gen) > 2 else 1e7
+        kind, dist, n, eps, zk, src, ch, idx = problem_inputs(rng, gen[0], gen[1], nmax)
         if (kind, dist, n, eps) != (pb["kind"], pb["dist"], pb["n"], pb["eps"]):
             raise ValueError(f"problem {i}: replay gives {(kind, dist, n, eps)}, stored "
                              f"{(pb['kind'], pb['dist'], pb['n'], pb['eps'])}")
@@ -211,6 +212,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--out", default=PR.WEIGHTS)
     ap.add_argument("--data", default=None, help="reuse tuples from this JSON file")
+    ap.add_argument("--nmax", type=float, default=1e7,
+                    help="largest problem size (log-uniform from 1e4)")
     ap.add_argument("--tuples-only", action="store_true",
                     help="generate and save the tuples without fitting")
     ap.add_argument("--refeature", action="store_true",
@@ -230,12 +233,15 @@ def main():
         problems = []
         t0 = time.time()
         for i in range(args.problems):
-            problems.append(run_problem(rng))
+            problems.append(run_problem(rng, nmax=args.nmax))
             pb = problems[-1]
             print(f"problem {i}: {pb['kind']} {pb['dist']} n={pb['n']} eps={pb['eps']:.0e} "
                   f"target r={pb['target'][0]} e={pb['target'][1]} "
                   f"t={pb['chosen'].get('time', 0):.4f}s default={pb['default']['time']:.4f}s "
                   f"({time.time() - t0:.0f}s)", flush=True)
+            if args.tuples_only and (i + 1) % 25 == 0:  # keep what a cut-off run has done
+                with open(args.out.replace(".npz", "_tuples.json"), "w") as f:
+                    json.dump(problems, f)
     if args.tuples_only:
         with open(args.out.replace(".npz", "_tuples.json"), "w") as f:
             json.dump(problems, f)

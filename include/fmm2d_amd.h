@@ -358,6 +358,56 @@ int fmm2d_mlp_loss_grad(int nin, int nout, const float* params, int batch, const
 int fmm2d_mlp_adam(int64_t n, float* params, const float* grad, float* m, float* v, float lr,
                    float beta1, float beta2, float eps, int step, void* stream);
 
+/*
+ * Biharmonic sums (complex velocity of Stokes flow / elasticity).
+ *   vel(z) = sum_j 2 c1_j log|z-z_j| + c2_j (z-z_j)/conj(z-z_j) + d1_j/(z-z_j)
+ *            + d3_j/conj(z-z_j) + d2_j (z-z_j)/conj(z-z_j)^2
+ * charge(nd,2,ns) holds (c1, c2), dip(nd,3,ns) holds (d1, d2, d3);
+ * grad(nd,3,n) = (d/dz vel projected on z, projected on zbar, d/dzbar vel).
+ *
+ * fmm2d_bhfmm2d replaces biharmonic/bhfmm2d.f:1 bhfmm2d (same arguments, device
+ * pointers, OVERWRITES pot/grad/pottarg/gradtarg; hess and hesstarg are not
+ * written, as in the reference; iper must be 0).
+ */
+int fmm2d_bhfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifcharge,
+                  const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dip, int iper,
+                  int ifpgh, fmm2d_cplx* pot, fmm2d_cplx* grad, fmm2d_cplx* hess, int64_t nt,
+                  const double* targ, int ifpghtarg, fmm2d_cplx* pottarg,
+                  fmm2d_cplx* gradtarg, fmm2d_cplx* hesstarg, fmm2d_opts* opts, void* stream);
+
+/* bh2d_direct{c,d,cd}{p,g} (biharmonic/bhkernels2d.f:26, 82, 153, 210, 277, 339):
+ * INCREMENTS vel(nd,nt) and, for ifpgh = 2, grad(nd,3,nt); pairs with
+ * |t-s| <= thresh are skipped. */
+int fmm2d_bh2d_direct(int nd, const double* sources, int64_t ns, int ifcharge,
+                      const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dip,
+                      const double* targ, int64_t nt, int ifpgh, fmm2d_cplx* vel,
+                      fmm2d_cplx* grad, double thresh, void* stream);
+
+/*
+ * Stokes sums: Stokeslets stoklet(nd,2,ns) and type-I stresslets strslet,
+ * strsvec (nd,2,ns); velocity pot(nd,2,n), pressure pre(nd,n), velocity
+ * gradient grad(nd,2,2,n) (grad(l,i,j,k): ith component of the gradient of
+ * velocity component j).
+ *
+ * fmm2d_stfmm2d replaces stokes/stfmm2d.f:27 stfmm2d (same arguments, device
+ * pointers; ifppreg / ifppregtarg 1 = velocity, 2 = + pressure, 3 = + gradient;
+ * OVERWRITES the requested outputs).
+ */
+int fmm2d_stfmm2d(int nd, double eps, int64_t ns, const double* sources, int ifstoklet,
+                  const double* stoklet, int ifstrslet, const double* strslet,
+                  const double* strsvec, int ifppreg, double* pot, double* pre, double* grad,
+                  int64_t nt, const double* targ, int ifppregtarg, double* pottarg,
+                  double* pretarg, double* gradtarg, fmm2d_opts* opts, void* stream);
+
+/* st2ddirectstokg / st2ddirectstokstrsg (stokes/stokkernels2d.f:24, 169):
+ * INCREMENTS pot, pre and grad at the targets (each may be NULL, then it is
+ * not computed); istress = 1 adds the type-I stresslets; pairs with
+ * |t-s| < thresh are skipped. */
+int fmm2d_st2d_direct(int nd, const double* sources, int ifstoklet, const double* stoklet,
+                      int istress, const double* strslet, const double* strsvec, int64_t ns,
+                      const double* targ, int64_t nt, double* pot, double* pre, double* grad,
+                      double thresh, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -717,3 +717,103 @@ void orc_tree_free(void* handle) {
   for (int k = 0; k < 4; ++k) { free(t->nl[k]); free(t->ll[k]); }
   free(t);
 }
+
+/* ======================================================================
+ * Biharmonic direct sums (biharmonic/bhkernels2d.f:26-410, bh2d_direct*):
+ * complex velocity vel(nd,nt) and its three derivatives grad(nd,3,nt),
+ * incremented; pairs with |t - s| <= thresh skipped.  charge(nd,2,ns) and
+ * dip(nd,3,ns) are complex (interleaved doubles).
+ * ==================================================================== */
+void orc_bh_direct(int nd, const double* src, int ns, int ifc, const double* charge, int ifd,
+                   const double* dip, const double* targ, int nt, int pg, double thresh,
+                   double* vel, double* grad) {
+  const zc* c = (const zc*)charge;
+  const zc* d = (const zc*)dip;
+  zc* v = (zc*)vel;
+  zc* g = (zc*)grad;
+  for (int j = 0; j < nt; ++j) {
+    const zc zt = targ[2 * j] + I * targ[2 * j + 1];
+    for (int i = 0; i < ns; ++i) {
+      const zc z = zt - (src[2 * i] + I * src[2 * i + 1]);
+      if (cabs(z) <= thresh) continue;
+      const zc z1 = 1.0 / z, z2 = z1 * z1;
+      for (int k = 0; k < nd; ++k) {
+        zc dv = 0, g1 = 0, g2 = 0, g3 = 0;
+        if (ifc) {
+          const zc c1 = c[k + nd * (2 * i)], c2 = c[k + nd * (2 * i + 1)];
+          dv += 2.0 * c1 * log(cabs(z)) + c2 * conj(z1) * z;
+          g1 += c1 * z1;
+          g2 += c2 * conj(z1);
+          g3 += c1 * conj(z1) - c2 * conj(z2) * z;
+        }
+        if (ifd) {
+          const zc d1 = d[k + nd * (3 * i)], d2 = d[k + nd * (3 * i + 1)],
+                   d3 = d[k + nd * (3 * i + 2)];
+          dv += d1 * z1 + d3 * conj(z1) + d2 * conj(z2) * z;
+          g1 -= d1 * z2;
+          g2 += d2 * conj(z2);
+          g3 -= d3 * conj(z2) + 2.0 * d2 * conj(z2 * z1) * z;
+        }
+        v[k + nd * j] += dv;
+        if (pg >= 2) {
+          g[k + nd * (3 * j)] += g1;
+          g[k + nd * (3 * j + 1)] += g2;
+          g[k + nd * (3 * j + 2)] += g3;
+        }
+      }
+    }
+  }
+}
+
+/* ======================================================================
+ * Stokes direct sums (stokes/stokkernels2d.f:24 st2ddirectstokg, :169
+ * st2ddirectstokstrsg): velocity pot(nd,2,nt), pressure pre(nd,nt) and
+ * velocity gradient grad(nd,2,2,nt) (grad(l,i,j,k) = d_i u_j), incremented;
+ * pairs with |t - s|^2 < thresh^2 skipped.  With r = t - s:
+ *   Stokeslet s:  u_i = -s_i log(r^2)/4 + r_i (r.s)/(2 r^2),  p = (r.s)/r^2
+ *   stresslet (mu, nu): u_i = -2 r_i (r.mu)(r.nu)/r^4,
+ *                       p = -4 (r.mu)(r.nu)/r^4 + 2 (mu.nu)/r^2
+ * and the gradients as the reference writes them out.
+ * ==================================================================== */
+void orc_st_direct(int nd, const double* src, int ns, int ifs, const double* stoklet, int ift,
+                   const double* strslet, const double* strsvec, const double* targ, int nt,
+                   double thresh, double* pot, double* pre, double* grad) {
+  const double th2 = thresh * thresh;
+  for (int j = 0; j < nt; ++j) {
+    for (int i = 0; i < ns; ++i) {
+      const double x = targ[2 * j] - src[2 * i], y = targ[2 * j + 1] - src[2 * i + 1];
+      const double r2 = x * x + y * y;
+      if (r2 < th2) continue;
+      const double r4 = r2 * r2, r6 = r4 * r2, rt = x * x - y * y;
+      for (int k = 0; k < nd; ++k) {
+        double* u = pot + k + nd * 2 * j;      /* u[0], u[nd] */
+        double* gr = grad + k + nd * 4 * j;    /* gr[nd*(a + 2 b)] = d_a u_b */
+        if (ifs) {
+          const double s1 = stoklet[k + nd * (2 * i)], s2 = stoklet[k + nd * (2 * i + 1)];
+          const double pl = x * s1 + y * s2, pl2 = x * s2 + y * s1;
+          u[0] += -s1 * log(r2) / 4 + x * pl / r2 / 2;
+          u[nd] += -s2 * log(r2) / 4 + y * pl / r2 / 2;
+          gr[0] += -rt * pl / r4 / 2;
+          gr[nd * 3] += rt * pl / r4 / 2;
+          gr[nd * 1] += (pl2 * rt - s1 * 4 * y * x * x) / 2 / r4;
+          gr[nd * 2] += -(pl2 * rt + s2 * 4 * x * y * y) / 2 / r4;
+          pre[k + nd * j] += pl / r2;
+        }
+        if (ift) {
+          const double m1 = strslet[k + nd * (2 * i)], m2 = strslet[k + nd * (2 * i + 1)];
+          const double n1 = strsvec[k + nd * (2 * i)], n2 = strsvec[k + nd * (2 * i + 1)];
+          const double pl = x * m1 + y * m2, pl2 = x * n1 + y * n2, pv = m1 * n1 + m2 * n2;
+          u[0] += -2 * x * pl * pl2 / r4;
+          u[nd] += -2 * y * pl * pl2 / r4;
+          gr[0] += -2 * pl * pl2 / r4 - 2 * x * m1 * pl2 / r4 - 2 * x * n1 * pl / r4 +
+                   8 * x * x * pl * pl2 / r6;
+          gr[nd * 1] += -2 * x * m2 * pl2 / r4 - 2 * x * n2 * pl / r4 + 8 * x * y * pl * pl2 / r6;
+          gr[nd * 2] += -2 * y * m1 * pl2 / r4 - 2 * y * n1 * pl / r4 + 8 * x * y * pl * pl2 / r6;
+          gr[nd * 3] += -2 * pl * pl2 / r4 - 2 * y * m2 * pl2 / r4 - 2 * y * n2 * pl / r4 +
+                        8 * y * y * pl * pl2 / r6;
+          pre[k + nd * j] += -4 * pl * pl2 / r4 + 2 * pv / r2;
+        }
+      }
+    }
+  }
+}

@@ -16,6 +16,10 @@
  *   orc_direct    direct_ref.npz (all 27 variants x 3 families), single-source
  *                 closed forms, harmonicity / Helmholtz equation by finite
  *                 differences
+ *   orc_bh_direct bh_ref.npz (reference bh2d_direct*, all six variants), the
+ *                 bi-Laplacian and the z / zbar derivatives by finite differences
+ *   orc_st_direct bh_ref.npz (reference st2ddirect*), divergence-free velocity,
+ *                 gradient and Stokes momentum balance by finite differences
  *   orc_hank01    hankel.npz (mpmath values), Wronskian, recurrence
  *   orc_tree_*    tree_ref.npz, bit for bit
  *   orc_l2dterms, orc_lndiv2d   the reference routines (through ref_binding)
@@ -43,6 +47,18 @@ int orc_tree_get(void* handle, int* laddr, int* ilevel, int* iparent, int* nchil
                  int* isrcse, int* itarg, int* itargse, int* nlist1, int* list1, int* nlist2,
                  int* list2, int* nlist3, int* list3, int* nlist4, int* list4);
 void orc_tree_free(void* handle);
+
+/* bh2d_direct{c,d,cd}{p,g} (biharmonic/bhkernels2d.f:26-410): complex
+ * charge(nd,2,ns), dip(nd,3,ns), vel(nd,nt), grad(nd,3,nt), incremented. */
+void orc_bh_direct(int nd, const double* src, int ns, int ifc, const double* charge, int ifd,
+                   const double* dip, const double* targ, int nt, int pg, double thresh,
+                   double* vel, double* grad);
+
+/* st2ddirectstokg / st2ddirectstokstrsg (stokes/stokkernels2d.f:24, 169):
+ * pot(nd,2,nt), pre(nd,nt), grad(nd,2,2,nt), incremented. */
+void orc_st_direct(int nd, const double* src, int ns, int ifs, const double* stoklet, int ift,
+                   const double* strslet, const double* strsvec, const double* targ, int nt,
+                   double thresh, double* pot, double* pre, double* grad);
 
 int orc_l2dterms(double eps);                          /* laplace/l2dterms.f:38 */
 int orc_lndiv2d(double eps, int ns, int nt);           /* laplace/lndiv2d.f */

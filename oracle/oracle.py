@@ -62,6 +62,46 @@ def direct(kind, nd, sources, targets, charges=None, dipstr=None, dipvec=None, p
     return out
 
 
+def bh_direct(nd, sources, targets, charges=None, dipoles=None, pgt=1, thresh=1e-16):
+    """orc_bh_direct: vel (nd,nt), grad (nd,3,nt) from zero; charges (nd,2,ns),
+    dipoles (nd,3,ns) complex."""
+    src = np.asfortranarray(np.asarray(sources, dtype=np.float64))
+    trg = np.asfortranarray(np.asarray(targets, dtype=np.float64))
+    ns, nt = src.shape[1], trg.shape[1]
+    ch = np.asfortranarray(np.reshape(charges, (nd, 2, ns)).astype(np.complex128)) \
+        if charges is not None else np.zeros((nd, 2, 1), np.complex128)
+    dp = np.asfortranarray(np.reshape(dipoles, (nd, 3, ns)).astype(np.complex128)) \
+        if dipoles is not None else np.zeros((nd, 3, 1), np.complex128)
+    vel = np.zeros((nd, nt), np.complex128, order="F")
+    grad = np.zeros((nd, 3, nt), np.complex128, order="F")
+    lib().orc_bh_direct(C.c_int(nd), _p(src), C.c_int(ns), C.c_int(charges is not None),
+                        _p(ch), C.c_int(dipoles is not None), _p(dp), _p(trg), C.c_int(nt),
+                        C.c_int(pgt), C.c_double(thresh), _p(vel), _p(grad))
+    out = dict(pottarg=vel)
+    if pgt >= 2:
+        out["gradtarg"] = grad
+    return out
+
+
+def st_direct(nd, sources, targets, stoklet=None, strslet=None, strsvec=None, thresh=1e-16):
+    """orc_st_direct: pot (nd,2,nt), pre (nd,nt), grad (nd,2,2,nt) from zero."""
+    src = np.asfortranarray(np.asarray(sources, dtype=np.float64))
+    trg = np.asfortranarray(np.asarray(targets, dtype=np.float64))
+    ns, nt = src.shape[1], trg.shape[1]
+
+    def arr(a):
+        return np.asfortranarray(np.reshape(a, (nd, 2, ns)).astype(np.float64)) \
+            if a is not None else np.zeros((nd, 2, 1))
+    sk, sl, sv = arr(stoklet), arr(strslet), arr(strsvec)
+    pot = np.zeros((nd, 2, nt), order="F")
+    pre = np.zeros((nd, nt), order="F")
+    grad = np.zeros((nd, 2, 2, nt), order="F")
+    lib().orc_st_direct(C.c_int(nd), _p(src), C.c_int(ns), C.c_int(stoklet is not None),
+                        _p(sk), C.c_int(strslet is not None), _p(sl), _p(sv), _p(trg),
+                        C.c_int(nt), C.c_double(thresh), _p(pot), _p(pre), _p(grad))
+    return dict(pottarg=pot, pretarg=pre, gradtarg=grad)
+
+
 def hank01(z):
     z = np.asarray(z, dtype=np.complex128).ravel()
     h0 = np.zeros_like(z)

@@ -18,11 +18,15 @@ LAP    := cauchykernels2d cfmm2d cfmm2d_ndiv cfmm2dwrap cfmm2dwrap_vec l2dterms 
           lndiv2d rfmm2d rfmm2d_ndiv rfmm2dwrap rfmm2dwrap_vec rlapkernels2d
 HELM   := h2dcommon h2dterms helmkernels2d helmrouts2d hfmm2d hfmm2d_ndiv \
           hfmm2dwrap hfmm2dwrap_vec hndiv2d wideband2d hfmm2d_mps hfmm2d_mps_ndiv
+BH     := bh2dterms bhfmm2d bhfmm2dwrap bhkernels2d bhndiv2d bhrouts2d
+ST     := stfmm2d stokkernels2d
 
 OBJS := $(addprefix $(OUT)/obj/c_,$(addsuffix .o,$(COMMON))) \
         $(OUT)/obj/c_cumsum.o \
         $(addprefix $(OUT)/obj/l_,$(addsuffix .o,$(LAP))) \
-        $(addprefix $(OUT)/obj/h_,$(addsuffix .o,$(HELM)))
+        $(addprefix $(OUT)/obj/h_,$(addsuffix .o,$(HELM))) \
+        $(addprefix $(OUT)/obj/b_,$(addsuffix .o,$(BH))) \
+        $(addprefix $(OUT)/obj/s_,$(addsuffix .o,$(ST)))
 
 all: $(OUT)/libfmm2d_ref.so
 
@@ -36,6 +40,10 @@ $(OUT)/obj/c_%.o: $(SRC)/common/%.f | $(OUT)/obj
 $(OUT)/obj/l_%.o: $(SRC)/laplace/%.f | $(OUT)/obj
 	$(FLANG) $(FFLAGS) -c $< -o $@
 $(OUT)/obj/h_%.o: $(SRC)/helmholtz/%.f | $(OUT)/obj
+	$(FLANG) $(FFLAGS) -c $< -o $@
+$(OUT)/obj/b_%.o: $(SRC)/biharmonic/%.f | $(OUT)/obj
+	$(FLANG) $(FFLAGS) -c $< -o $@
+$(OUT)/obj/s_%.o: $(SRC)/stokes/%.f | $(OUT)/obj
 	$(FLANG) $(FFLAGS) -c $< -o $@
 
 $(OUT)/libfmm2d_ref.so: $(OBJS)

@@ -395,3 +395,128 @@ def hfmm2d_mps(nd, eps, zk, centers, rscales, nterms, mpole, impole, ndiv=None):
         L.hfmm2d_mps_ndiv_(_i(nd), _d(eps), pzk, _i(n), pcen, prs, pmt, _i(ntot), pmp, pim, ploc,
                            _i(ndiv), tinfo.ctypes.data_as(C.c_void_p), C.byref(ier))
     return loc, ier.value
+
+
+def bh_direct(nd, sources, targets, charges=None, dipoles=None, pgt=1, thresh=1e-16):
+    """Reference bh2d_direct{c,d,cd}{p,g} (biharmonic/bhkernels2d.f:26-410): vel (nd,nt),
+    grad (nd,3,nt) from zero."""
+    L = lib()
+    src, psrc = _farr(sources, np.float64)
+    trg, ptrg = _farr(targets, np.float64)
+    ns, nt = src.shape[1], trg.shape[1]
+    keep = []
+    args = [_i(nd), psrc, _i(ns)]
+    if charges is not None:
+        ch, pch = _farr(np.reshape(charges, (nd, 2, ns)), np.complex128)
+        keep.append(ch)
+        args.append(pch)
+    if dipoles is not None:
+        dp, pdp = _farr(np.reshape(dipoles, (nd, 3, ns)), np.complex128)
+        keep.append(dp)
+        args.append(pdp)
+    vel, pvel = _out((nd, nt))
+    grad, pgrad = _out((nd, 3, nt))
+    args += [ptrg, _i(nt), pvel]
+    if pgt >= 2:
+        args.append(pgrad)
+    args.append(_d(thresh))
+    tag = ("c" if charges is not None else "") + ("d" if dipoles is not None else "") + \
+        "pg"[pgt - 1]
+    getattr(L, f"bh2d_direct{tag}_")(*args)
+    out = dict(pottarg=vel)
+    if pgt >= 2:
+        out["gradtarg"] = grad
+    return out
+
+
+def st_direct(nd, sources, targets, stoklet=None, strslet=None, strsvec=None, thresh=1e-16):
+    """Reference st2ddirectstokg / st2ddirectstokstrsg (stokes/stokkernels2d.f:24, 169):
+    pot (nd,2,nt), pre (nd,nt), grad (nd,2,2,nt) from zero."""
+    L = lib()
+    src, psrc = _farr(sources, np.float64)
+    trg, ptrg = _farr(targets, np.float64)
+    ns, nt = src.shape[1], trg.shape[1]
+
+    def arr(a):
+        return _farr(np.reshape(a, (nd, 2, ns)) if a is not None else np.zeros((nd, 2, ns)),
+                     np.float64)
+    sk, psk = arr(stoklet)
+    sl, psl = arr(strslet)
+    sv, psv = arr(strsvec)
+    pot, ppot = _out((nd, 2, nt), np.float64)
+    pre, ppre = _out((nd, nt), np.float64)
+    grad, pgrad = _out((nd, 2, 2, nt), np.float64)
+    if strslet is None:
+        L.st2ddirectstokg_(_i(nd), psrc, psk, _i(ns), ptrg, _i(nt), ppot, ppre, pgrad,
+                           _d(thresh))
+    else:
+        L.st2ddirectstokstrsg_(_i(nd), psrc, _i(1 if stoklet is not None else 0), psk, _i(1),
+                               psl, psv, _i(ns), ptrg, _i(nt), ppot, ppre, pgrad, _d(thresh))
+    return dict(pottarg=pot, pretarg=pre, gradtarg=grad)
+
+
+def bhfmm2d(nd, eps, sources, charges=None, dipoles=None, targets=None, ifpgh=0,
+            ifpghtarg=0):
+    """Reference bhfmm2d (biharmonic/bhfmm2d.f:1): pot (nd,ns), grad (nd,3,ns), pottarg,
+    gradtarg."""
+    L = lib()
+    src, psrc = _farr(sources, np.float64)
+    ns = src.shape[1]
+    if targets is None:
+        nt = 0
+        trg, ptrg = _out((2, 1), np.float64)
+    else:
+        trg, ptrg = _farr(targets, np.float64)
+        nt = trg.shape[1]
+    ifc = 1 if charges is not None else 0
+    ifd = 1 if dipoles is not None else 0
+    ch, pch = _farr(np.reshape(charges, (nd, 2, ns)) if ifc else np.zeros((nd, 2, ns)),
+                    np.complex128)
+    dp, pdp = _farr(np.reshape(dipoles, (nd, 3, ns)) if ifd else np.zeros((nd, 3, ns)),
+                    np.complex128)
+    pot, ppot = _out((nd, ns))
+    grad, pgrad = _out((nd, 3, ns))
+    hess, phess = _out((nd, 3, ns))
+    t1 = max(nt, 1)
+    pt, ppt = _out((nd, t1))
+    gt, pgt = _out((nd, 3, t1))
+    ht, pht = _out((nd, 3, t1))
+    ier = C.c_int(0)
+    L.bhfmm2d_(_i(nd), _d(eps), _i(ns), psrc, _i(ifc), pch, _i(ifd), pdp, _i(0), _i(ifpgh),
+               ppot, pgrad, phess, _i(nt), ptrg, _i(ifpghtarg), ppt, pgt, pht, C.byref(ier))
+    return dict(ier=ier.value, pot=pot, grad=grad, pottarg=pt[:, :nt], gradtarg=gt[:, :, :nt])
+
+
+def stfmm2d(nd, eps, sources, stoklet=None, strslet=None, strsvec=None, targets=None,
+            ifppreg=0, ifppregtarg=0):
+    """Reference stfmm2d (stokes/stfmm2d.f:27): pot (nd,2,ns), pre (nd,ns),
+    grad (nd,2,2,ns) and the target outputs."""
+    L = lib()
+    src, psrc = _farr(sources, np.float64)
+    ns = src.shape[1]
+    if targets is None:
+        nt = 0
+        trg, ptrg = _out((2, 1), np.float64)
+    else:
+        trg, ptrg = _farr(targets, np.float64)
+        nt = trg.shape[1]
+
+    def arr(a):
+        return _farr(np.reshape(a, (nd, 2, ns)) if a is not None else np.zeros((nd, 2, ns)),
+                     np.float64)
+    sk, psk = arr(stoklet)
+    sl, psl = arr(strslet)
+    sv, psv = arr(strsvec)
+    t1 = max(nt, 1)
+    pot, ppot = _out((nd, 2, ns), np.float64)
+    pre, ppre = _out((nd, ns), np.float64)
+    grad, pgrad = _out((nd, 2, 2, ns), np.float64)
+    pt, ppt = _out((nd, 2, t1), np.float64)
+    prt, pprt = _out((nd, t1), np.float64)
+    gt, pgt = _out((nd, 2, 2, t1), np.float64)
+    ier = C.c_int(0)
+    L.stfmm2d_(_i(nd), _d(eps), _i(ns), psrc, _i(1 if stoklet is not None else 0), psk,
+               _i(1 if strslet is not None else 0), psl, psv, _i(ifppreg), ppot, ppre, pgrad,
+               _i(nt), ptrg, _i(ifppregtarg), ppt, pprt, pgt, C.byref(ier))
+    return dict(ier=ier.value, pot=pot, pre=pre, grad=grad, pottarg=pt[..., :nt],
+                pretarg=prt[..., :nt], gradtarg=gt[..., :nt])

@@ -417,9 +417,86 @@ def mps_fixtures():
     np.savez_compressed(os.path.join(HERE, "mps_ref.npz"), **out)
 
 
+def bh_cases():
+    """(name, nd, eps, sources, targets, charges, dipoles) for the biharmonic FMM;
+    (name, nd, eps, sources, targets, stoklet, strslet, strsvec) for Stokes."""
+    rng = np.random.default_rng(23)
+    bh, st = [], []
+    for name, nd, eps, n, dist in (("u6", 1, 1e-6, 1200, "u"), ("c10", 1, 1e-10, 1200, "c"),
+                                   ("u9nd2", 2, 1e-9, 800, "u")):
+        if dist == "u":
+            src = rng.random((2, n))
+        else:  # 8 Gaussian clusters plus 10% uniform, offset from the origin
+            cen = rng.uniform(2.0, 3.0, (2, 8))
+            src = cen[:, rng.integers(0, 8, n)] + 0.02 * rng.standard_normal((2, n))
+            src[:, : n // 10] = rng.uniform(2.0, 3.0, (2, n // 10))
+        trg = rng.uniform(src.min(), src.max(), (2, 400))
+        ch = rng.standard_normal((nd, 2, n)) + 1j * rng.standard_normal((nd, 2, n))
+        dp = rng.standard_normal((nd, 3, n)) + 1j * rng.standard_normal((nd, 3, n))
+        bh.append((name, nd, eps, src, trg, ch, dp))
+        sk = rng.standard_normal((nd, 2, n))
+        sl = rng.standard_normal((nd, 2, n))
+        sv = rng.standard_normal((nd, 2, n))
+        st.append((name, nd, eps, src, trg, sk, sl, sv))
+    return bh, st
+
+
+def bh_fixtures():
+    """Reference biharmonic and Stokes outputs: the six bh2d_direct variants and
+    the three st2ddirect variants (nd 1 and 2; a coincident target), and bhfmm2d /
+    stfmm2d on bh_cases() (charges, dipoles or both; Stokeslets, stresslets or
+    both), with the inputs."""
+    out = {}
+    rng = np.random.default_rng(29)
+    ns, nt = 53, 37
+    for nd in (1, 2):
+        src = rng.random((2, ns))
+        trg = rng.random((2, nt))
+        trg[:, 0] = src[:, 4]  # coincident: excluded by thresh
+        ch = rng.standard_normal((nd, 2, ns)) + 1j * rng.standard_normal((nd, 2, ns))
+        dp = rng.standard_normal((nd, 3, ns)) + 1j * rng.standard_normal((nd, 3, ns))
+        sk, sl, sv = (rng.standard_normal((nd, 2, ns)) for _ in range(3))
+        pre = f"d{nd}_"
+        for key, val in dict(src=src, trg=trg, charge=ch, dip=dp, stoklet=sk, strslet=sl,
+                             strsvec=sv).items():
+            out[pre + key] = val
+        for ifc, ifd in ((1, 0), (0, 1), (1, 1)):
+            for pg in (1, 2):
+                r = R.bh_direct(nd, src, trg, ch if ifc else None, dp if ifd else None, pg,
+                                thresh=1e-14)
+                tag = f"{pre}bh_{'c' if ifc else ''}{'d' if ifd else ''}_{pg}_"
+                for k, v in r.items():
+                    out[tag + k] = v
+        for tag, args in (("s", (sk, None, None)), ("st", (sk, sl, sv)), ("t", (None, sl, sv))):
+            r = R.st_direct(nd, src, trg, *args, thresh=1e-14)
+            for k, v in r.items():
+                out[f"{pre}st_{tag}_{k}"] = v
+    bh, st = bh_cases()
+    for name, nd, eps, src, trg, ch, dp in bh:
+        out[f"fb_{name}_input_sha256"] = np.array(I.digest(src, trg, ch, dp))
+        for key, val in dict(nd=nd, eps=eps, src=src, trg=trg, charge=ch, dip=dp).items():
+            out[f"fb_{name}_{key}"] = np.asarray(val)
+        for tag, c, d in (("c", ch, None), ("d", None, dp), ("cd", ch, dp)):
+            r = R.bhfmm2d(nd, eps, src, c, d, trg, 2, 2)
+            assert r["ier"] == 0
+            for k in ("pot", "grad", "pottarg", "gradtarg"):
+                out[f"fb_{name}_{tag}_{k}"] = r[k]
+    for name, nd, eps, src, trg, sk, sl, sv in st:
+        out[f"fs_{name}_input_sha256"] = np.array(I.digest(src, trg, sk, sl, sv))
+        for key, val in dict(nd=nd, eps=eps, src=src, trg=trg, stoklet=sk, strslet=sl,
+                             strsvec=sv).items():
+            out[f"fs_{name}_{key}"] = np.asarray(val)
+        for tag, args in (("s", (sk, None, None)), ("st", (sk, sl, sv)), ("t", (None, sl, sv))):
+            r = R.stfmm2d(nd, eps, src, *args, trg, 3, 3)
+            assert r["ier"] == 0
+            for k in ("pot", "pre", "grad", "pottarg", "pretarg", "gradtarg"):
+                out[f"fs_{name}_{tag}_{k}"] = r[k]
+    np.savez_compressed(os.path.join(HERE, "bh_ref.npz"), **out)
+
+
 if __name__ == "__main__":
     which = sys.argv[1:] or ["direct", "hankel", "tree", "fmm", "hfmm", "lreal", "rfmm",
-                             "headline", "helmc2", "mps"]
+                             "headline", "helmc2", "mps", "bh"]
     if "direct" in which:
         direct_fixtures()
     if "hankel" in which:
@@ -440,5 +517,7 @@ if __name__ == "__main__":
         helm_c2_fixture()
     if "mps" in which:
         mps_fixtures()
+    if "bh" in which:
+        bh_fixtures()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -68,10 +68,31 @@ def test_shape_assertions():
         F.l2ddir(sources=src, targets=np.zeros((3, 2)), charges=np.ones(5), pgt=1)
 
 
-def test_out_of_scope_families():
-    for fn in (fmm2d_amd.bhfmm2d, fmm2d_amd.stfmm2d, fmm2d_amd.bh2ddir, fmm2d_amd.st2ddir):
-        with pytest.raises(NotImplementedError):
-            fn(eps=1e-6, sources=np.zeros((2, 3)))
+def test_biharmonic_stokes_host_checks(capsys):
+    """The reference wrappers' early returns and shape assertions
+    (python/fmm2dpy/fmm2d.py:1031-1054, 1172-1228) run before any device work."""
+    src = np.zeros((2, 5))
+    out = F.bhfmm2d(eps=1e-6, sources=src, charges=np.ones((2, 5), complex))
+    assert out.pot is None and "Nothing to compute" in capsys.readouterr().out
+    out = F.bh2ddir(sources=src, targets=src, charges=np.ones((2, 5), complex))
+    assert out.pottarg is None and "Nothing to compute" in capsys.readouterr().out
+    with pytest.raises(AssertionError, match="Charges must be of shape \\[2, number of source\\]"):
+        F.bhfmm2d(eps=1e-6, sources=src, charges=np.ones((3, 5), complex), pg=1)
+    with pytest.raises(AssertionError, match="Dipole must of shape \\[3, number of sources\\]"):
+        F.bhfmm2d(eps=1e-6, sources=src, dipoles=np.ones((2, 5), complex), pg=1)
+    with pytest.raises(AssertionError, match="Charges must be of shape \\[nd,2,ns\\]"):
+        F.bh2ddir(sources=src, targets=src, charges=np.ones((2, 2, 5), complex), pgt=1, nd=3)
+    out = F.stfmm2d(eps=1e-6, sources=src, stoklet=np.ones((2, 5)))
+    assert out.pot is None and "set either ifppreg or ifppregtarg" in capsys.readouterr().out
+    out = F.stfmm2d(eps=1e-6, sources=src, ifppreg=1)
+    assert out.pot is None and "set either stoklet or strslet+strsvec" in capsys.readouterr().out
+    out = F.st2ddir(eps=1e-6, sources=src, strslet=np.ones((2, 5)), ifppreg=1)
+    assert out.pot is None and "mush be both None" in capsys.readouterr().out
+    with pytest.raises(AssertionError, match="stoklet vectors must be of shape"):
+        F.stfmm2d(eps=1e-6, sources=src, stoklet=np.ones((3, 5)), ifppreg=1)
+    with pytest.raises(AssertionError, match="strsvec vectors must be of shape"):
+        F.st2ddir(eps=1e-6, sources=src, strslet=np.ones((2, 5)), strsvec=np.ones((2, 4)),
+                  ifppreg=1)
 
 
 def test_no_cpu_fallback_without_gpu():

@@ -217,3 +217,74 @@ def test_leaf_histogram_restatements_agree():
     # a window one source short (ndiv - 1) refines boxes of exactly ndiv sources
     src = rng.random((2, 3000))
     assert not np.array_equal(H.windows(src, None, 14), H.brute(src, None, 15))
+
+
+# ---------------------------------------------------------------- biharmonic / Stokes
+@pytest.mark.parametrize("nd", [1, 2])
+def test_oracle_bh_st_direct_match_reference(golden_dir, nd):
+    """orc_bh_direct / orc_st_direct against the reference's bh2d_direct* (all six
+    variants) and st2ddirect* (Stokeslet, both, stresslet), with a coincident
+    target excluded by thresh."""
+    g = np.load(f"{golden_dir}/bh_ref.npz")
+    pre = f"d{nd}_"
+    src, trg = g[pre + "src"], g[pre + "trg"]
+    ch, dp = g[pre + "charge"], g[pre + "dip"]
+    for ifc, ifd in ((1, 0), (0, 1), (1, 1)):
+        for pg in (1, 2):
+            r = O.bh_direct(nd, src, trg, ch if ifc else None, dp if ifd else None, pg, 1e-14)
+            tag = f"{pre}bh_{'c' if ifc else ''}{'d' if ifd else ''}_{pg}_"
+            for k, v in r.items():
+                assert _rel(v, g[tag + k]) < 1e-13, (tag, k)
+    sk, sl, sv = g[pre + "stoklet"], g[pre + "strslet"], g[pre + "strsvec"]
+    for tag, args in (("s", (sk, None, None)), ("st", (sk, sl, sv)), ("t", (None, sl, sv))):
+        r = O.st_direct(nd, src, trg, *args, thresh=1e-14)
+        for k, v in r.items():
+            assert _rel(v, g[f"{pre}st_{tag}_{k}"]) < 1e-13, (tag, k)
+
+
+def test_oracle_bh_derivatives_by_finite_differences():
+    """grad1 + grad2 = d vel/dz and grad3 = d vel/dzbar (central differences);
+    with c2 = d2 = d3 = 0 the velocity is analytic and grad2 = 0."""
+    rng = np.random.default_rng(3)
+    src = rng.random((2, 6)) * 0.5
+    x0 = np.array([[0.9], [0.8]])
+    ch = rng.standard_normal((1, 2, 6)) + 1j * rng.standard_normal((1, 2, 6))
+    dp = rng.standard_normal((1, 3, 6)) + 1j * rng.standard_normal((1, 3, 6))
+    h = 1e-5
+
+    def vel(p, c, d):
+        return O.bh_direct(1, src, p, c, d, 1)["pottarg"][0, 0]
+    for c, d in ((ch, dp), (ch * [[1], [0]], dp * [[1], [0], [0]])):
+        g = O.bh_direct(1, src, x0, c, d, 2)["gradtarg"][0, :, 0]
+        dx = (vel(x0 + [[h], [0]], c, d) - vel(x0 - [[h], [0]], c, d)) / (2 * h)
+        dy = (vel(x0 + [[0], [h]], c, d) - vel(x0 - [[0], [h]], c, d)) / (2 * h)
+        assert abs((dx - 1j * dy) / 2 - (g[0] + g[1])) < 1e-7 * abs(g).max()
+        assert abs((dx + 1j * dy) / 2 - g[2]) < 1e-7 * abs(g).max()
+    assert abs(g[1]) == 0.0
+
+
+@pytest.mark.parametrize("which", ["stokeslet", "stresslet"])
+def test_oracle_stokes_equations(which):
+    """The velocity gradient equals central differences of the velocity, the
+    velocity is divergence-free, and lap u = grad p (Stokes equations) to the
+    O(H^2) accuracy of the five-point stencil."""
+    rng = np.random.default_rng(4)
+    src = rng.random((2, 5)) * 0.5
+    x0 = np.array([[0.9], [0.8]])
+    sk, sl, sv = (rng.standard_normal((1, 2, 5)) for _ in range(3))
+    args = (sk, None, None) if which == "stokeslet" else (None, sl, sv)
+
+    def at(a, b, step):
+        o = O.st_direct(1, src, x0 + [[a * step], [b * step]], *args)
+        return o["pottarg"][0, :, 0], o["pretarg"][0, 0], o["gradtarg"][0, :, :, 0]
+    u0, p0, g0 = at(0, 0, 1.0)
+    h = 1e-5
+    fd = np.array([(at(1, 0, h)[0] - at(-1, 0, h)[0]) / (2 * h),
+                   (at(0, 1, h)[0] - at(0, -1, h)[0]) / (2 * h)])  # fd[a, b] = d_a u_b
+    assert np.abs(fd - g0).max() < 1e-7 * np.abs(g0).max()
+    assert abs(g0[0, 0] + g0[1, 1]) < 1e-12 * np.abs(g0).max()
+    H = 2e-3
+    lap = (at(1, 0, H)[0] + at(-1, 0, H)[0] + at(0, 1, H)[0] + at(0, -1, H)[0] - 4 * u0) / H ** 2
+    gp = np.array([(at(1, 0, H)[1] - at(-1, 0, H)[1]) / (2 * H),
+                   (at(0, 1, H)[1] - at(0, -1, H)[1]) / (2 * H)])
+    assert np.abs(lap - gp).max() < 1e-4 * np.abs(gp).max()
