@@ -27,7 +27,7 @@ HBM traffic from rocprofv3 counters (profiles/), and the reference library
 timed on the host cores on a bounded sample.
 
   python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--n 10000000]
-                  [--precision mixed|fp64] [--no-learned] [--no-cpu] [--validate 256]
+                  [--precision mixed|fp64] [--no-learned] [--no-cpu] [--validate 64]
 """
 import argparse
 import json
@@ -522,8 +522,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nerr", type=int, default=10_000)
-    ap.add_argument("--validate", type=int, default=256,
-                    help="sampled sources of the learned pipelines' in-step acceptance test")
+    ap.add_argument("--validate", type=int, default=64,
+                    help="sampled sources of the learned pipelines' in-step acceptance test "
+                         "(64: the relative L2 error of pot+grad over 192 complex values, "
+                         "about 10%% statistical spread; the FMM's error sits 3-4x below eps)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-parameter pipeline")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
