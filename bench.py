@@ -50,6 +50,7 @@ STAGES = {0: "p2m", 1: "list4", 2: "m2m", 3: "m2l plan", 13: "m2l kernel", 4: "l
 M2L_SLOT, NEAR_SLOT = 13, 7
 # keys of TRAFFIC_FILE for (M2L, near field) of each pipeline
 TRAFFIC_KEYS = {"fp64": ("m2l_fp64", "near_fp64"), "mixed": ("m2l_mixed", "near_mixed"),
+                "fp64+learned": ("m2l_fp64_learned", "near_fp64_learned"),
                 "mixed+learned": ("m2l_learned", "near_learned")}
 # HBM bytes per M2L launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
 # scaled by the counters' reading for a copy kernel of known size

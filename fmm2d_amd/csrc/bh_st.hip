@@ -36,6 +36,7 @@
 #include "devbuf.h"
 #include "fastmath.h"
 #include "lap_ops.h"
+#include "split_sum.h"
 
 namespace fmm2d {
 
@@ -231,16 +232,6 @@ __global__ __launch_bounds__(kBT) void k_st_direct(
     put(grad, sg, (int64_t)nd * (0 + 2 * (1 + 2 * it)) + d, g12);
     put(grad, sg, (int64_t)nd * (1 + 2 * (1 + 2 * it)) + d, g22);
   }
-}
-
-template <typename T>
-__global__ void k_add_split_copies(int64_t n, T* __restrict__ out, const T* __restrict__ part,
-                                   int nsplit) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T s = out[i];
-  for (int k = 0; k < nsplit; ++k) s += part[(int64_t)k * n + i];
-  out[i] = s;
 }
 
 // source-range split so that a launch has about 2048 blocks
@@ -494,8 +485,8 @@ int bh_direct(const BhDirectArgs& a, hipStream_t st) {
 #undef FMM2D_BHD
     hip_throw(hipGetLastError(), "kernel launch");
     if (nsplit > 1) {
-      launch1d(k_add_split_copies<cplx>, nv, st, a.vel, (const cplx*)V, (int)nsplit);
-      if (ng) launch1d(k_add_split_copies<cplx>, ng, st, a.grad, (const cplx*)G, (int)nsplit);
+      add_split_copies(a.vel, (const cplx*)V, nv, (int)nsplit, st);
+      if (ng) add_split_copies(a.grad, (const cplx*)G, ng, (int)nsplit, st);
     }
     return FMM2D_OK;
   } catch (const HipError& e) {
@@ -536,9 +527,9 @@ int st_direct(const StDirectArgs& a, hipStream_t st) {
 #undef FMM2D_STD
     hip_throw(hipGetLastError(), "kernel launch");
     if (nsplit > 1) {
-      if (np) launch1d(k_add_split_copies<double>, np, st, a.pot, (const double*)Pp, (int)nsplit);
-      if (nr) launch1d(k_add_split_copies<double>, nr, st, a.pre, (const double*)Pr, (int)nsplit);
-      if (ng) launch1d(k_add_split_copies<double>, ng, st, a.grad, (const double*)Pg, (int)nsplit);
+      if (np) add_split_copies(a.pot, (const double*)Pp, np, (int)nsplit, st);
+      if (nr) add_split_copies(a.pre, (const double*)Pr, nr, (int)nsplit, st);
+      if (ng) add_split_copies(a.grad, (const double*)Pg, ng, (int)nsplit, st);
     }
     return FMM2D_OK;
   } catch (const HipError& e) {

@@ -14,6 +14,7 @@
 #include "devbuf.h"
 #include "p2p.h"
 #include "p2p_math.h"
+#include "split_sum.h"
 
 namespace fmm2d {
 
@@ -154,16 +155,6 @@ __global__ __launch_bounds__(kTile) void direct_kernel(
   }
 }
 
-// out[i] += sum_s part[s*n + i]
-__global__ void add_splits_kernel(cplx* __restrict__ out, const cplx* __restrict__ part,
-                                  int64_t n, int nsplit) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  cplx s = out[i];
-  for (int k = 0; k < nsplit; ++k) s += part[(int64_t)k * n + i];
-  out[i] = s;
-}
-
 template <int FAM, int IFC, int IFD, int PG, int NDT>
 static void launch_direct(int ws, dim3 grid, hipStream_t st, const DirectArgs& a,
                           int64_t split_len, cplx* pot, cplx* grad, cplx* hess, int64_t sp,
@@ -276,11 +267,14 @@ int direct_eval(const DirectArgs& a, hipStream_t st, bool noalloc) {
   }
   FMM2D_HIP_TRY(hipGetLastError());
   if (nsplit > 1) {
-    hipLaunchKernelGGL(add_splits_kernel, dim3(ceil_div(np, 256)), dim3(256), 0, st, a.pot, p, np,
-                       (int)nsplit);
-    if (ng) hipLaunchKernelGGL(add_splits_kernel, dim3(ceil_div(ng, 256)), dim3(256), 0, st, a.grad, g, ng, (int)nsplit);
-    if (nh) hipLaunchKernelGGL(add_splits_kernel, dim3(ceil_div(nh, 256)), dim3(256), 0, st, a.hess, h, nh, (int)nsplit);
-    FMM2D_HIP_TRY(hipGetLastError());
+    try {
+      add_split_copies(a.pot, (const cplx*)p, np, (int)nsplit, st);
+      if (ng) add_split_copies(a.grad, (const cplx*)g, ng, (int)nsplit, st);
+      if (nh) add_split_copies(a.hess, (const cplx*)h, nh, (int)nsplit, st);
+    } catch (const HipError& e) {
+      if (!noalloc) (void)hipFreeAsync(work, st);
+      return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+    }
     if (!noalloc) FMM2D_HIP_TRY(hipFreeAsync(work, st));
   }
   return FMM2D_OK;
