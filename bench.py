@@ -48,6 +48,9 @@ STAGES = {0: "p2m", 1: "list4", 2: "m2m", 3: "m2l plan", 13: "m2l kernel", 4: "l
           6: "l2p+list3", 15: "near prep", 7: "near", 8: "tree+lists", 9: "sort+gather",
           10: "scatter"}
 M2L_SLOT, NEAR_SLOT = 13, 7
+# blocks of the learned steps' acceptance-test direct sums (16 per XCD), which run
+# on a second stream beside the FMM
+VAL_BLOCKS = 128
 # keys of TRAFFIC_FILE for (M2L, near field) of each pipeline
 TRAFFIC_KEYS = {"fp64": ("m2l_fp64", "near_fp64"), "mixed": ("m2l_mixed", "near_mixed"),
                 "fp64+learned": ("m2l_fp64_learned", "near_fp64_learned"),
@@ -450,7 +453,9 @@ def main_c5(args):
     def learned():
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
+            prev = L.fmm2d_set_direct_max_blocks(VAL_BLOCKS)
             pv, gv = direct_at(iv)
+            L.fmm2d_set_direct_max_blocks(prev)
         prop = PR.Proposer.propose_device(proposer, "h", nat.ptr(src), n, eps, zk=zk)
         o = nat.opts_struct({"precision": "mixed", "nterms": prop["nterms"], "ndiv": prop["ndiv"]})
         step(o)
@@ -618,9 +623,13 @@ def main():
             with torch.cuda.stream(side):
                 pv.zero_()
                 gv.zero_()
+                # a bounded grid: the check fills the CUs the FMM's latency-bound
+                # stages leave idle instead of holding all of them first
+                prev = L.fmm2d_set_direct_max_blocks(VAL_BLOCKS)
                 ier = L.fmm2d_l2d_direct(1, nat.ptr(src), n, 1, nat.ptr(chg), 0, None, None,
                                          nat.ptr(tsel_v), nval, 2, nat.ptr(pv), nat.ptr(gv),
                                          None, 1e-16, nat.stream_ptr())
+                L.fmm2d_set_direct_max_blocks(prev)
             if ier != 0:
                 raise RuntimeError(f"fmm2d_l2d_direct returned {ier}")
         prop = proposer.propose_device("l", nat.ptr(src), n, EPS)

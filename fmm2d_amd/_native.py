@@ -39,6 +39,7 @@ SIGNATURES["fmm2d_h2dterms"] = (_i, [_d, _d, _d, _d, C.POINTER(_i)])
 SIGNATURES["fmm2d_ndiv"] = (_i, [_d, _i64, _i64, _i, C.POINTER(_i), C.POINTER(_i)])
 SIGNATURES["fmm2d_release_workspace"] = (_i, [])
 SIGNATURES["fmm2d_set_workspace_limit"] = (_i, [_i64])
+SIGNATURES["fmm2d_set_direct_max_blocks"] = (_i, [_i])
 SIGNATURES["fmm2d_rfmm2d"] = (_i, [_i, _d, _i64, _vp, _i, _vp, _i, _vp, _vp, _i, _i, _vp, _vp,
                                    _vp, _i64, _vp, _i, _vp, _vp, _vp, _vp, _vp])
 SIGNATURES["fmm2d_r2d_direct"] = (_i, [_i, _vp, _i64, _i, _vp, _i, _vp, _vp, _vp, _i64, _i,

@@ -89,6 +89,11 @@ int fmm2d_r2d_direct(int nd, const double* sources, int64_t ns, int ifcharge, co
                           ifpgh, thresh, pot, grad, hess, reinterpret_cast<hipStream_t>(stream));
 }
 
+int fmm2d_set_direct_max_blocks(int nblocks) {
+  if (nblocks < 0) return FMM2D_ERR_ARG;
+  return g_direct_max_blocks.exchange(nblocks);
+}
+
 int fmm2d_bh2d_direct(int nd, const double* sources, int64_t ns, int ifcharge,
                       const fmm2d_cplx* charge, int ifdipole, const fmm2d_cplx* dip,
                       const double* targ, int64_t nt, int ifpgh, fmm2d_cplx* vel,

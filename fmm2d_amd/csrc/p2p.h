@@ -1,5 +1,6 @@
 // Direct (all-pairs) evaluation entry used by the C-ABI.
 #pragma once
+#include <atomic>
 #include "common.h"
 
 namespace fmm2d {
@@ -34,5 +35,9 @@ int direct_eval_real(int nd, const double* src, int64_t ns, int ifcharge, const 
                      int ifdipole, const double* dipstr, const double* dipvec, const double* targ,
                      int64_t nt, int pg, double thresh, double* pot, double* grad, double* hess,
                      hipStream_t st);
+
+// Bound on the blocks of one direct evaluation when the source range is split
+// (fmm2d_set_direct_max_blocks; 0 = the default of about 2048).
+extern std::atomic<int> g_direct_max_blocks;
 
 }  // namespace fmm2d

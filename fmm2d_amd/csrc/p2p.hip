@@ -9,6 +9,7 @@
 // 64 targets and split each source tile, and the source range is split across
 // blocks; each split writes a private copy of the outputs and a second kernel
 // adds the copies into the caller's buffers in split order (deterministic).
+#include <atomic>
 #include <mutex>
 
 #include "devbuf.h"
@@ -193,6 +194,8 @@ constexpr int64_t kScratchCplx = int64_t(1) << 21;
 constexpr int kMaxDevices = 64;
 static __device__ __align__(256) cplx scratch[kScratchCplx];
 
+std::atomic<int> g_direct_max_blocks{0};
+
 namespace {
 std::mutex g_scratch_mu;
 cplx* g_scratch_ptr[kMaxDevices] = {};
@@ -228,9 +231,11 @@ int direct_eval(const DirectArgs& a, hipStream_t st, bool noalloc) {
   const int64_t ng = (a.pg >= 2) ? np * NG : 0;
   const int64_t nh = (a.pg >= 3) ? np * NH : 0;
   const int64_t per = np + ng + nh;  // complex values of one split copy
-  // aim for >= 2048 blocks (8 per CU) before splitting the source range
+  // aim for >= 2048 blocks (8 per CU) before splitting the source range, or
+  // for the caller's bound (fmm2d_set_direct_max_blocks)
   int64_t nsplit = 1;
-  const int64_t want = 2048;
+  const int mb = g_direct_max_blocks.load(std::memory_order_relaxed);
+  const int64_t want = mb > 0 ? mb : 2048;
   if (tblocks * nchunk < want) nsplit = (want + tblocks * nchunk - 1) / (tblocks * nchunk);
   const int64_t max_split = (a.ns + kTile - 1) / kTile;
   if (nsplit > max_split) nsplit = max_split;

@@ -101,10 +101,47 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* in, T* out,
 
 // out[i] = in[0] + ... + in[i-1] for i < n; *total (device, may be null) =
 // the sum of all n.  Stream-ordered, no host synchronisation.
+// Short arrays in one launch: a single workgroup walks the array in tiles of
+// kScanTile, each thread scanning kScanItems consecutive elements in
+// registers and the workgroup scanning the thread totals.  The tree build
+// scans dozens of per-level arrays of a few thousand elements, where three
+// launches (and a partials buffer) cost more than the work.
+constexpr int64_t kScanSingleMax = 16 * kScanTile;
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(const T* in, T* out, int64_t n,
+                                                              T* total) {
+  __shared__ T lds[kScanWaves + 1];
+  T carry = 0;
+  for (int64_t base = 0; base < n; base += kScanTile) {
+    const int64_t j0 = base + (int64_t)threadIdx.x * kScanItems;
+    T loc[kScanItems];
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const T v = (j0 + i < n) ? in[j0 + i] : T(0);
+      loc[i] = s;
+      s += v;
+    }
+    T tot;
+    const T ex = block_exclusive_scan(s, lds, &tot);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i)
+      if (j0 + i < n) out[j0 + i] = carry + ex + loc[i];
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
 template <typename T>
 void exclusive_scan(const T* in, T* out, int64_t n, T* total, hipStream_t st) {
   if (n <= 0) {
     if (total) hip_throw(hipMemsetAsync(total, 0, sizeof(T), st), "hipMemsetAsync");
+    return;
+  }
+  if (n <= kScanSingleMax) {
+    hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kScanThreads), 0, st, in, out, n, total);
+    hip_throw(hipGetLastError(), "scan");
     return;
   }
   const int64_t nblk = ceil_div(n, (int64_t)kScanTile);

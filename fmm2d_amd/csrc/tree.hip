@@ -1271,7 +1271,9 @@ void key_leaf_histogram(const double* src, int64_t ns, const double* trg, int64_
   }
   const double bs = std::max(bb[1] - bb[0], bb[3] - bb[2]);
   if (boxsize0) *boxsize0 = bs;
-  const int d1 = kD1;
+  // levels up to nbins - 1 tell every bin apart (the last bin takes the leaf
+  // levels from nbins - 1 on): shorter keys, fewer sort passes
+  const int d1 = std::max(1, std::min(kD1, nbins - 1));
   // half[l] = boxsize(l) / 2 with boxsize(l + 1) = boxsize(l) / 2, as in build_tree
   std::vector<double> half(d1 + 2);
   double bsl = bs;

@@ -338,6 +338,17 @@ int fmm2d_release_workspace(void);
 int fmm2d_set_workspace_limit(int64_t bytes);
 
 /*
+ * Bound the blocks of the Laplace / Cauchy / Helmholtz direct evaluations
+ * (fmm2d_*2d_direct) whose targets are too few to fill the GPU: their source
+ * range is split over blocks until about nblocks run (0 = the default, about
+ * 2048, 8 per CU).  A bound of a few per XCD lets a sampled check run on a
+ * second stream beside an FMM without taking every CU from it.  Returns the
+ * previous bound.  No reference counterpart: for callers that overlap direct
+ * sums with other work on one GPU.
+ */
+int fmm2d_set_direct_max_blocks(int nblocks);
+
+/*
  * Parameter-proposer network (no reference counterpart: the reference picks
  * nterms and the leaf size by the fixed rules of laplace/l2dterms.f:38,
  * helmholtz/h2dterms.f:35 and laplace/lndiv2d.f; a proposal reaches the
