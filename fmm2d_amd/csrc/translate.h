@@ -326,8 +326,10 @@ struct TrBufs {
 // launch; launches are stream-ordered, so groups must not depend on each other).
 // fp32: operators (Afrag) and sources are float, accumulation FP32 on
 // v_mfma_f32_16x16x4f32 (gather mode only); otherwise FP64.
-// Launches of long operators (k loop not unrolled) with fewer wavefronts than
-// this split their slots and k steps over more wavefronts (atomic adds).
+// Launches with fewer wavefronts than this split their slots (and, for long
+// operators whose k loop is not unrolled, their k steps) over more wavefronts
+// (atomic adds): small problems and coarse levels would otherwise run a few
+// dozen wavefronts through 27 slots each.
 constexpr int64_t kTrSplitWaves = 4096;
 
 inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int nslots,
@@ -369,10 +371,11 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
   {                                                                                             \
     const int64_t nw0 = (int64_t)tiles * (KSU > 0 ? 1 : (RTT + RT - 1) / RT);                   \
     int ssplit = 1, ksplit = 1;                                                                 \
-    if (KSU == 0 && !LOGT && nw0 < kTrSplitWaves) {                                             \
+    if (!LOGT && nw0 < kTrSplitWaves) {                                                         \
       ssplit = (int)std::min<int64_t>(nslots, (kTrSplitWaves + nw0 - 1) / nw0);                 \
-      ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(KS / 16, (kTrSplitWaves + nw0 *     \
-                                        ssplit - 1) / (nw0 * ssplit)));                        \
+      if (KSU == 0)                                                                             \
+        ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(KS / 16, (kTrSplitWaves + nw0 *   \
+                                          ssplit - 1) / (nw0 * ssplit)));                      \
     }                                                                                           \
     const int64_t nw = nw0 * ssplit * ksplit;                                                   \
     const int64_t nblk = (nw + 3) / 4;                                                          \
