@@ -456,16 +456,20 @@ def main_c5(args):
             prev = L.fmm2d_set_direct_max_blocks(VAL_BLOCKS)
             pv, gv = direct_at(iv)
             L.fmm2d_set_direct_max_blocks(prev)
-        prop = PR.Proposer.propose_device(proposer, "h", nat.ptr(src), n, eps, zk=zk)
-        o = nat.opts_struct({"precision": "mixed", "nterms": prop["nterms"], "ndiv": prop["ndiv"]})
-        step(o)
-        torch.cuda.current_stream().wait_stream(side)
-        e = sample_err(iv, pv, gv)
-        state.update(proposal=prop, val_err=e)
-        if e > eps:
-            o = nat.opts_struct({"precision": "mixed"})
+        with PR.Prepared(nat.ptr(src), n) as prep:
+            prop = PR.Proposer.propose_device(proposer, "h", nat.ptr(src), n, eps, zk=zk,
+                                              prepared=prep)
+            o = nat.opts_struct({"precision": "mixed", "nterms": prop["nterms"],
+                                 "ndiv": prop["ndiv"], "prepared": prep})
             step(o)
-            state["fallbacks"] = state.get("fallbacks", 0) + 1
+            torch.cuda.current_stream().wait_stream(side)
+            e = sample_err(iv, pv, gv)
+            state.update(proposal=prop, val_err=e)
+            if e > eps:
+                o = nat.opts_struct({"precision": "mixed", "prepared": prep})
+                step(o)
+                state["fallbacks"] = state.get("fallbacks", 0) + 1
+            o.prepared = None  # the handle ends with this call
         return o
 
     pipelines = {}
@@ -632,21 +636,26 @@ def main():
                 L.fmm2d_set_direct_max_blocks(prev)
             if ier != 0:
                 raise RuntimeError(f"fmm2d_l2d_direct returned {ier}")
-        prop = proposer.propose_device("l", nat.ptr(src), n, EPS)
-        o = nat.opts_struct(dict(base, nterms=prop["nterms"], ndiv=prop["ndiv"]))
-        step(o)
-        st["proposal"], st["opts"] = prop, o
-        if nval:
-            torch.cuda.current_stream().wait_stream(side)
-            num = ((pot[iv] - pv).abs() ** 2).sum() + ((grad[iv] - gv).abs() ** 2).sum()
-            den_v = (pv.abs() ** 2).sum() + (gv.abs() ** 2).sum()
-            e = float(torch.sqrt(num / den_v))
-            st["val_err"] = e
-            if e > EPS:  # rejected: the reference's rules
-                o = nat.opts_struct(base)
-                step(o)
-                st["opts"] = o
-                st["fallbacks"] = st.get("fallbacks", 0) + 1
+        # the points' bounding square and sorted keys, made once for the
+        # proposal's features and the FMM's tree build
+        with PR.Prepared(nat.ptr(src), n) as prep:
+            prop = proposer.propose_device("l", nat.ptr(src), n, EPS, prepared=prep)
+            o = nat.opts_struct(dict(base, nterms=prop["nterms"], ndiv=prop["ndiv"],
+                                     prepared=prep))
+            step(o)
+            st["proposal"], st["opts"] = prop, o
+            if nval:
+                torch.cuda.current_stream().wait_stream(side)
+                num = ((pot[iv] - pv).abs() ** 2).sum() + ((grad[iv] - gv).abs() ** 2).sum()
+                den_v = (pv.abs() ** 2).sum() + (gv.abs() ** 2).sum()
+                e = float(torch.sqrt(num / den_v))
+                st["val_err"] = e
+                if e > EPS:  # rejected: the reference's rules
+                    o = nat.opts_struct(dict(base, prepared=prep))
+                    step(o)
+                    st["opts"] = o
+                    st["fallbacks"] = st.get("fallbacks", 0) + 1
+            o.prepared = None  # the handle ends with this step
         return st["opts"]
 
     def run(label, base, learned=False):

@@ -34,6 +34,9 @@ SIGNATURES["fmm2d_tree_free"] = (None, [_vp])
 SIGNATURES["fmm2d_tree_leaf_histogram"] = (_i, [_vp, _i64, _vp, _i64, _i, _i, _i, _vp, _vp,
                                                 _vp])
 SIGNATURES["fmm2d_key_leaf_histogram"] = (_i, [_vp, _i64, _vp, _i64, _i, _i, _vp, _vp, _vp])
+SIGNATURES["fmm2d_prepare"] = (_i, [_vp, _i64, _vp, _i64, C.POINTER(_vp), _vp])
+SIGNATURES["fmm2d_prepare_free"] = (None, [_vp])
+SIGNATURES["fmm2d_prepared_leaf_histogram"] = (_i, [_vp, _i, _i, _vp, _vp, _vp])
 SIGNATURES["fmm2d_l2dterms"] = (_i, [_d, C.POINTER(_i)])
 SIGNATURES["fmm2d_h2dterms"] = (_i, [_d, _d, _d, _d, C.POINTER(_i)])
 SIGNATURES["fmm2d_ndiv"] = (_i, [_d, _i64, _i64, _i, C.POINTER(_i), C.POINTER(_i)])
@@ -76,7 +79,7 @@ class FmmOpts(C.Structure):
                 ("nterms", C.POINTER(C.c_int32)), ("timeinfo", C.c_double * 16),
                 ("nlevels", C.c_int32), ("nboxes", C.c_int32), ("pad1", C.c_int32),
                 ("pad2", C.c_int32), ("work", C.c_double * 16), ("near_pairs", C.c_int64),
-                ("m2l_pairs", C.c_int64)]
+                ("m2l_pairs", C.c_int64), ("prepared", C.c_void_p)]
 
     def timeinfo_array(self):
         return np.array(self.timeinfo[:], dtype=np.float64)
@@ -97,8 +100,8 @@ class FmmJob(C.Structure):
 
 def opts_struct(opts=None):
     """Build an FmmOpts from a dict: ndiv, idivflag, ifnear, precision ("fp64" or
-    "mixed"), nterms (per-level list), count_work (0/1).  Missing keys take the
-    reference defaults."""
+    "mixed"), nterms (per-level list), count_work (0/1), prepared (an
+    fmm2d_prepare handle).  Missing keys take the reference defaults."""
     o = FmmOpts()
     opts = opts or {}
     o.ndiv = int(opts.get("ndiv", 0))
@@ -113,6 +116,9 @@ def opts_struct(opts=None):
         o._nterms_keep = arr
         o.nterms = C.cast(arr, C.POINTER(C.c_int32))
         o.n_nterms = len(nt)
+    prep = opts.get("prepared")
+    if prep is not None:
+        o.prepared = getattr(prep, "handle", prep)
     return o
 
 

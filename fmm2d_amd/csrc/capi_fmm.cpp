@@ -328,6 +328,49 @@ int fmm2d_key_leaf_histogram(const double* sources, int64_t ns, const double* ta
   return FMM2D_OK;
 }
 
+int fmm2d_prepare(const double* sources, int64_t ns, const double* targ, int64_t nt,
+                  void** handle, void* stream) {
+  if (!handle || !sources || ns < 1 || nt < 0 || (nt > 0 && !targ) || ns > INT32_MAX ||
+      nt > INT32_MAX)
+    return FMM2D_ERR_ARG;
+  *handle = nullptr;
+  try {
+    keep_pool_memory();
+    // nlmax 51: the depth limit of the FMM drivers' trees
+    *handle = tree_prepare(sources, ns, nt > 0 ? targ : nullptr, nt, 51,
+                           reinterpret_cast<hipStream_t>(stream));
+  } catch (const HipError& e) {
+    fprintf(stderr, "fmm2d_amd: %s failed: %s\n", e.what, hipGetErrorString(e.err));
+    return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    return FMM2D_ERR_HIP;
+  }
+  return FMM2D_OK;
+}
+
+void fmm2d_prepare_free(void* handle) { tree_prepare_free(static_cast<TreePrep*>(handle)); }
+
+int fmm2d_prepared_leaf_histogram(void* handle, int ndiv, int nbins, double* hist,
+                                  double* boxsize0, void* stream) {
+  if (!handle || ndiv < 1 || nbins < 1 || nbins > kLeafHistMax || !hist) return FMM2D_ERR_ARG;
+  try {
+    keep_pool_memory();
+    prepared_leaf_histogram(static_cast<const TreePrep*>(handle), ndiv, nbins, hist, boxsize0,
+                            reinterpret_cast<hipStream_t>(stream));
+  } catch (const HipError& e) {
+    fprintf(stderr, "fmm2d_amd: %s failed: %s\n", e.what, hipGetErrorString(e.err));
+    return e.err == hipErrorOutOfMemory ? FMM2D_ERR_ALLOC : FMM2D_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    return FMM2D_ERR_ALLOC;
+  } catch (...) {
+    return FMM2D_ERR_HIP;
+  }
+  return FMM2D_OK;
+}
+
+
 int fmm2d_tree_get(void* handle, int* laddr, int* ilevel, int* iparent, int* nchild,
                    int* ichild, int* ncoll, int* coll, double* centers, double* boxsize,
                    int* isrc, int* isrcse, int* itarg, int* itargse, int* nlist1, int* list1,

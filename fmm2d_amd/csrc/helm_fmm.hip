@@ -1373,7 +1373,8 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
     }
     // ---- tree and lists
     Tree t;
-    build_tree(a.src, ns, a.targ, nt, ndiv, idivflag, 51, t, st);
+    build_tree(a.src, ns, a.targ, nt, ndiv, idivflag, 51, t, st,
+               o ? static_cast<const TreePrep*>(o->prepared) : nullptr);
     Lists L;
     build_lists(t, L, st);
     tm.mark(8);

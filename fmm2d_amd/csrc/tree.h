@@ -54,11 +54,24 @@ struct Lists {
   DBuf<int> s2;  // [box][27]: list-2 source of each M2L offset slot, or -1
 };
 
+// The part of the tree build that does not depend on the leaf capacity: the
+// bounding square and the points' sorted path keys (opaque; tree.hip).
+struct TreePrep;
+TreePrep* tree_prepare(const double* src, int64_t ns, const double* trg, int64_t nt, int nlmax,
+                       hipStream_t st);
+void tree_prepare_free(TreePrep* p);
+// true when p was prepared for exactly these arrays and nlmax
+bool tree_prepare_matches(const TreePrep* p, const double* src, int64_t ns, const double* trg,
+                          int64_t nt, int nlmax);
+
 // Build the tree of the reference's pts_tree_build + pts_tree_sort
 // (common/pts_tree2d.f:406-655, 1222-1246) over the given points.
-// src/trg: (2,n) interleaved device arrays.  Throws HipError on HIP failure.
+// src/trg: (2,n) interleaved device arrays.  prep: a preparation of the same
+// points (used when tree_prepare_matches, else ignored), or null.  Throws
+// HipError on HIP failure.
 void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, int ndiv,
-                int idivflag, int nlmax, Tree& t, hipStream_t st);
+                int idivflag, int nlmax, Tree& t, hipStream_t st,
+                const TreePrep* prep = nullptr);
 
 // Lists 1-4 of computelists (common/tree_routs2d.f:486-710), as sets.
 void build_lists(const Tree& t, Lists& l, hipStream_t st);
@@ -73,5 +86,9 @@ void leaf_level_histogram(const Tree& t, int nbins, double* hist, hipStream_t st
 // only (no tree is built); *boxsize0 = the root box size.
 void key_leaf_histogram(const double* src, int64_t ns, const double* trg, int64_t nt, int ndiv,
                         int nbins, double* hist, double* boxsize0, hipStream_t st);
+// The same from a preparation's keys (min(20, nlmax) levels: equal to
+// key_leaf_histogram when nbins - 1 <= that depth).
+void prepared_leaf_histogram(const TreePrep* p, int ndiv, int nbins, double* hist,
+                             double* boxsize0, hipStream_t st);
 
 }  // namespace fmm2d

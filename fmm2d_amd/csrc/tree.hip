@@ -20,6 +20,8 @@
 // by level where the reference's loops carry a dependency between levels.
 
 #include <algorithm>
+#include <cmath>
+#include <memory>
 
 #include "radix_sort.h"
 #include "scan.h"
@@ -906,12 +908,31 @@ struct Builder {
 
 }  // namespace
 
-void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, int ndiv,
-                int idivflag, int nlmax, Tree& t, hipStream_t st) {
+// The steps of the tree build that do not depend on the leaf capacity: the
+// bounding square, the box sizes, the points' path keys sorted (sources and
+// targets separately).  Kept by a caller that needs the sorted keys before it
+// chooses the leaf capacity (the parameter proposer) and handed to the build.
+struct TreePrep {
+  const double* src = nullptr;
+  const double* trg = nullptr;
+  int64_t ns = 0, nt = 0;
+  int nlmax = 0;
+  std::vector<double> bsall;  // box size of every possible level
+  double root[2] = {0.0, 0.0};
+  bool walk_ok = false;
+  DBuf<double> dhalf;
+  PathKeys pk[2];
+};
+
+namespace {
+void prepare_into(TreePrep& P, const double* src, int64_t ns, const double* trg, int64_t nt,
+                  int nlmax, hipStream_t st) {
   if (nlmax > kMaxLevels) throw HipError{hipErrorInvalidValue, "nlmax beyond 51"};
-  Builder bld(st);
-  t.ns = ns;
-  t.nt = nt;
+  P.src = src;
+  P.trg = trg;
+  P.ns = ns;
+  P.nt = nt;
+  P.nlmax = nlmax;
   // ---- bounding square (pts_tree2d.f:455-517)
   double bb[4];
   {
@@ -923,27 +944,68 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
     hip_throw(hipStreamSynchronize(st), "sync");
   }
   // box sizes of every possible level (boxsize(l+1) = boxsize(l)/2)
-  std::vector<double> bsall(nlmax + 2);
+  std::vector<double>& bsall = P.bsall;
+  bsall.assign(nlmax + 2, 0.0);
   bsall[0] = bb[1] - bb[0];
   const double sizey = bb[3] - bb[2];
   if (sizey > bsall[0]) bsall[0] = sizey;
   for (int l = 1; l < (int)bsall.size(); ++l) bsall[l] = bsall[l - 1] / 2;
-  const double root[2] = {(bb[0] + bb[1]) / 2, (bb[2] + bb[3]) / 2};
-  bld.walk_ok = std::max(std::fabs(root[0]), std::fabs(root[1])) <= std::ldexp(bsall[0], 20);
+  P.root[0] = (bb[0] + bb[1]) / 2;
+  P.root[1] = (bb[2] + bb[3]) / 2;
+  P.walk_ok = std::max(std::fabs(P.root[0]), std::fabs(P.root[1])) <= std::ldexp(bsall[0], 20);
 
   // ---- point paths, sorted (sources and targets separately)
   std::vector<double> half(bsall.size());
   for (size_t l = 0; l < half.size(); ++l) half[l] = 0.5 * bsall[l];
-  DBuf<double> dhalf(half.size(), st);
-  hip_throw(hipMemcpyAsync(dhalf.p, half.data(), sizeof(double) * half.size(),
+  P.dhalf.alloc(half.size(), st);
+  hip_throw(hipMemcpyAsync(P.dhalf.p, half.data(), sizeof(double) * half.size(),
                            hipMemcpyHostToDevice, st), "copy");
   hip_throw(hipStreamSynchronize(st), "sync");
   const int d1 = std::min(kD1, nlmax);
-  PathKeys pk[2];
-  pk[0].build(src, ns, root[0], root[1], dhalf.p, d1, 0, st);
-  pk[1].build(trg, nt, root[0], root[1], dhalf.p, d1, 0, st);
-  bld.ks = pk[0].view();
-  bld.kt = pk[1].view();
+  P.pk[0].build(src, ns, P.root[0], P.root[1], P.dhalf.p, d1, 0, st);
+  P.pk[1].build(trg, nt, P.root[0], P.root[1], P.dhalf.p, d1, 0, st);
+}
+}  // namespace
+
+TreePrep* tree_prepare(const double* src, int64_t ns, const double* trg, int64_t nt, int nlmax,
+                       hipStream_t st) {
+  std::unique_ptr<TreePrep> P(new TreePrep());
+  prepare_into(*P, src, ns, trg, nt, nlmax, st);
+  return P.release();
+}
+
+void tree_prepare_free(TreePrep* p) { delete p; }
+
+bool tree_prepare_matches(const TreePrep* p, const double* src, int64_t ns, const double* trg,
+                          int64_t nt, int nlmax) {
+  return p && p->src == src && p->ns == ns && p->nt == nt && (nt == 0 || p->trg == trg) &&
+         p->nlmax == nlmax;
+}
+
+void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, int ndiv,
+                int idivflag, int nlmax, Tree& t, hipStream_t st, const TreePrep* prep) {
+  if (nlmax > kMaxLevels) throw HipError{hipErrorInvalidValue, "nlmax beyond 51"};
+  Builder bld(st);
+  t.ns = ns;
+  t.nt = nt;
+  // bounding square, box sizes and sorted path keys: the caller's, or made here
+  TreePrep own;
+  const TreePrep* P = prep;
+  if (!tree_prepare_matches(prep, src, ns, trg, nt, nlmax)) {
+    prepare_into(own, src, ns, trg, nt, nlmax, st);
+    P = &own;
+  }
+  const std::vector<double>& bsall = P->bsall;
+  const double* root = P->root;
+  bld.walk_ok = P->walk_ok;
+  const double* dhalf_p = P->dhalf.p;
+  const int d1 = std::min(kD1, nlmax);
+  // the keys in use: the prepared ones, or deeper ones made below (points
+  // closer than 2^-21 of the domain)
+  PathKeys pkx[2];
+  const PathKeys* pk[2] = {&P->pk[0], &P->pk[1]};
+  bld.ks = pk[0]->view();
+  bld.kt = pk[1]->view();
 
   size_t cap0 = std::max<size_t>(1024, (size_t)(8 * (ns + nt) / std::max(ndiv, 1)));
   bld.B.alloc(cap0, st);
@@ -969,7 +1031,7 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
     const int lo = ls[l], n = le[l] - ls[l];
     // the batch: levels l .. l+K-1 refined, sizes bounded by ub[k]
     int K = std::min(kBatch, nlmax - l);
-    if (pk[0].d2 == 0 && l < d1) K = std::min(K, d1 - l);
+    if (pk[0]->d2 == 0 && l < d1) K = std::min(K, d1 - l);
     // as many levels as fit the capacity a batch may reserve
     const int64_t limit = std::max<int64_t>(kBatchBoxes, 2 * (int64_t)bld.nboxes);
     std::vector<int64_t> ub{n};
@@ -983,20 +1045,22 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
       ub.push_back(u);
       reserve += u;
     }
-    const bool batch = (pk[0].d2 != 0 || l < d1) && K > 1;
+    const bool batch = (pk[0]->d2 != 0 || l < d1) && K > 1;
     if (!batch) {
       // one level, exact (pts_tree2d.f:551-631 as written)
       bld.ensure_flag(n);
       launch1d(k_refine_flags, n, st, lo, bld.B.ses.p, bld.B.set.p, idivflag, ndiv, bld.flag.p);
-      if (l + 1 > d1 && pk[0].d2 == 0) {
+      if (l + 1 > d1 && pk[0]->d2 == 0) {
         // a box at level d1 may need splitting: extend the paths (rare: only
         // for points closer than 2^-21 of the domain)
         if (bld.scan(n) > 0) {
           const int d2 = std::min(kD2Max, std::min(62, nlmax) - d1);
-          pk[0].build(src, ns, root[0], root[1], dhalf.p, d1, d2, st);
-          pk[1].build(trg, nt, root[0], root[1], dhalf.p, d1, d2, st);
-          bld.ks = pk[0].view();
-          bld.kt = pk[1].view();
+          pkx[0].build(src, ns, root[0], root[1], dhalf_p, d1, d2, st);
+          pkx[1].build(trg, nt, root[0], root[1], dhalf_p, d1, d2, st);
+          pk[0] = &pkx[0];
+          pk[1] = &pkx[1];
+          bld.ks = pk[0]->view();
+          bld.kt = pk[1]->view();
         }
       }
       const int base = bld.nboxes;
@@ -1145,11 +1209,11 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
   for (int k = 0; k < 2; ++k) {
     if (npts[k] == 0) continue;
     // big[k]: a leaf beyond kLeafSortMax; big[2 + k]: leaves beyond kLeafSmall
-    launch1d(k_leaf_sort_small, nb, st, bld.B.nchild.p, se[k], pk[k].perm.p, perm[k]->p, bigl.p,
+    launch1d(k_leaf_sort_small, nb, st, bld.B.nchild.p, se[k], pk[k]->perm.p, perm[k]->p, bigl.p,
              big.p + 2 + k);
     const unsigned grid = (unsigned)std::min<int64_t>(4096, ceil_div(nb, kLeafSortWaves));
     hipLaunchKernelGGL(k_leaf_sort, dim3(grid), dim3(kLeafSortWaves * 64), 0, st, bigl.p,
-                       big.p + 2 + k, se[k], pk[k].perm.p, perm[k]->p, big.p + k);
+                       big.p + 2 + k, se[k], pk[k]->perm.p, perm[k]->p, big.p + k);
     hip_throw(hipGetLastError(), "kernel launch");
   }
   int hbig[2] = {0, 0};
@@ -1160,7 +1224,7 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
     // a leaf too large for k_leaf_sort: stable sort of all points by leaf
     DBuf<unsigned> label(npts[k], st), lsorted(npts[k], st);
     DBuf<int> iota(npts[k], st);
-    launch1d(k_leaf_label, nb, st, bld.B.nchild.p, se[k], pk[k].perm.p, label.p);
+    launch1d(k_leaf_label, nb, st, bld.B.nchild.p, se[k], pk[k]->perm.p, label.p);
     int bits = 1;
     while ((int64_t(1) << bits) <= npts[k]) ++bits;
     // values: input indices 0..n-1 in input order
@@ -1257,6 +1321,26 @@ __global__ void k_key_leaf_hist(int64_t n, int64_t ns, int ndiv, int d1,
     if (h[k]) atomicAdd(&hist[k], h[k]);
 }
 
+namespace {
+// hist[l] from the sorted source keys ks and the target keys kt (any order),
+// both of d1 levels
+void hist_from_keys(const unsigned long long* ks, int64_t ns, const unsigned long long* kt,
+                    int64_t nt, int d1, int ndiv, int nbins, double* hist, hipStream_t st) {
+  const int64_t m = ns > ndiv ? ns - ndiv : 0;
+  DBuf<unsigned char> w(std::max<int64_t>(m, 1), st);
+  launch1d(k_window_levels, m, st, ndiv, d1, ks, w.p);
+  DBuf<unsigned long long> d(nbins, st);
+  d.zero();
+  launch1d(k_key_leaf_hist, ns + nt, st, ns, ndiv, d1, ks, (const unsigned char*)w.p, kt, nbins,
+           d.p);
+  std::vector<unsigned long long> h(nbins);
+  hip_throw(hipMemcpyAsync(h.data(), d.p, sizeof(unsigned long long) * nbins,
+                           hipMemcpyDeviceToHost, st), "copy");
+  hip_throw(hipStreamSynchronize(st), "sync");
+  for (int k = 0; k < nbins; ++k) hist[k] = (double)h[k];
+}
+}  // namespace
+
 void key_leaf_histogram(const double* src, int64_t ns, const double* trg, int64_t nt, int ndiv,
                         int nbins, double* hist, double* boxsize0, hipStream_t st) {
   if (nbins < 1 || nbins > kLeafHistMax) throw HipError{hipErrorInvalidValue, "histogram bins"};
@@ -1292,18 +1376,16 @@ void key_leaf_histogram(const double* src, int64_t ns, const double* trg, int64_
     launch1d(k_path_keys, nt, st, trg, cx, cy, (const double*)dhalf.p, d1, 0, kt.p,
              (unsigned long long*)nullptr, (int*)nullptr);
   radix_sort(k0.p, ks.p, (const int*)nullptr, (int*)nullptr, ns, 2 * d1, st);
-  const int64_t m = ns > ndiv ? ns - ndiv : 0;
-  DBuf<unsigned char> w(std::max<int64_t>(m, 1), st);
-  launch1d(k_window_levels, m, st, ndiv, d1, (const unsigned long long*)ks.p, w.p);
-  DBuf<unsigned long long> d(nbins, st);
-  d.zero();
-  launch1d(k_key_leaf_hist, ns + nt, st, ns, ndiv, d1, (const unsigned long long*)ks.p,
-           (const unsigned char*)w.p, (const unsigned long long*)kt.p, nbins, d.p);
-  std::vector<unsigned long long> h(nbins);
-  hip_throw(hipMemcpyAsync(h.data(), d.p, sizeof(unsigned long long) * nbins,
-                           hipMemcpyDeviceToHost, st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");
-  for (int k = 0; k < nbins; ++k) hist[k] = (double)h[k];
+  hist_from_keys(ks.p, ns, kt.p, nt, d1, ndiv, nbins, hist, st);
+}
+
+void prepared_leaf_histogram(const TreePrep* P, int ndiv, int nbins, double* hist,
+                             double* boxsize0, hipStream_t st) {
+  if (nbins < 1 || nbins > kLeafHistMax) throw HipError{hipErrorInvalidValue, "histogram bins"};
+  if (boxsize0) *boxsize0 = P->bsall[0];
+  // the prepared keys hold min(kD1, nlmax) levels; every level from
+  // nbins - 1 on falls in the last bin, so deeper keys give the same counts
+  hist_from_keys(P->pk[0].k1.p, P->ns, P->pk[1].k1.p, P->nt, P->pk[0].d1, ndiv, nbins, hist, st);
 }
 
 void leaf_level_histogram(const Tree& t, int nbins, double* hist, hipStream_t st) {

@@ -68,22 +68,60 @@ def tree_features(sources, targets, eps, zk, kind, ndiv):
     return device_features(nat.ptr(src), ns, nat.ptr(trg), nt, eps, zk, kind, ndiv)
 
 
-def device_features(src_ptr, ns, targ_ptr, nt, eps, zk, kind, ndiv, stream=None):
+class Prepared:
+    """An fmm2d_prepare handle: the bounding square and sorted path keys of
+    device-resident points, shared by the proposer's features
+    (fmm2d_prepared_leaf_histogram) and the FMM's tree build (opts
+    "prepared").  The point arrays must stay unchanged while it is open, and
+    it is used on the stream it was made on."""
+
+    def __init__(self, src_ptr, ns, targ_ptr=None, nt=0, stream=None):
+        import ctypes as C
+        self.L = nat.lib()
+        h = C.c_void_p()
+        self.stream = nat.stream_ptr() if stream is None else stream
+        nat.check(self.L.fmm2d_prepare(src_ptr, int(ns), targ_ptr, int(nt), C.byref(h),
+                                       self.stream), "prepare")
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            self.L.fmm2d_prepare_free(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+def device_features(src_ptr, ns, targ_ptr, nt, eps, zk, kind, ndiv, stream=None,
+                    prepared=None):
     """Feature vector (NIN floats) and the root box size for device-resident
     points (sources(2,ns), targets(2,nt) as device pointers): log10 N / 7,
     log10(1 + |zk| boxsize0) / 3, -log10 eps / 15, a Helmholtz flag, and the
     fraction of points held by the leaves of levels 0..15 of the tree refined
     while a box holds more than ndiv sources (fmm2d_key_leaf_histogram: from
     the sorted path keys, no tree built and nothing copied to the host but the
-    16 counts)."""
+    16 counts).  prepared: a Prepared of the same points, whose keys are used
+    instead of sorting them here (the same counts)."""
     import ctypes as C
     L = nat.lib()
     hist = np.zeros(NLEV, dtype=np.float64)
     bs0 = C.c_double(0.0)
-    nat.check(L.fmm2d_key_leaf_histogram(src_ptr, int(ns), targ_ptr, int(nt), int(ndiv), NLEV,
-                                         hist.ctypes.data, C.byref(bs0),
-                                         nat.stream_ptr() if stream is None else stream),
-              "key_leaf_histogram")
+    stream = nat.stream_ptr() if stream is None else stream
+    if prepared is not None:
+        nat.check(L.fmm2d_prepared_leaf_histogram(prepared.handle, int(ndiv), NLEV,
+                                                  hist.ctypes.data, C.byref(bs0), stream),
+                  "prepared_leaf_histogram")
+    else:
+        nat.check(L.fmm2d_key_leaf_histogram(src_ptr, int(ns), targ_ptr, int(nt), int(ndiv),
+                                             NLEV, hist.ctypes.data, C.byref(bs0), stream),
+                  "key_leaf_histogram")
     hist = hist / max(1.0, hist.sum())
     kz = 0.0 if kind != "h" else abs(complex(zk)) * bs0.value
     f = np.concatenate([[np.log10(ns + nt) / 7.0, np.log10(1.0 + kz) / 3.0,
@@ -142,13 +180,15 @@ class Proposer:
                                    targ_ptr=nat.ptr(trg), nt=nt, zk=zk)
 
     def propose_device(self, kind, src_ptr, ns, eps, targ_ptr=None, nt=0, zk=None,
-                       nlevels_max=52):
+                       nlevels_max=52, prepared=None):
         """propose() for device-resident points: features from the device leaf
         histogram, the network on the device, orders for levels
         0..nlevels_max-1 (the drivers read the entries of their tree's
-        levels)."""
+        levels).  prepared: a Prepared of the same points (the FMM call that
+        follows takes it too, in its opts)."""
         nd0 = default_ndiv(eps, ns, nt)
-        f, bs0 = device_features(src_ptr, ns, targ_ptr, nt, eps, zk, kind, nd0)
+        f, bs0 = device_features(src_ptr, ns, targ_ptr, nt, eps, zk, kind, nd0,
+                                 prepared=prepared)
         y = self.predict(f[None, :])[0]
         bsx = [bs0 / 2.0 ** l for l in range(nlevels_max)]
         p0 = default_nterms(kind, eps, bsx, zk)

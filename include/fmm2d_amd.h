@@ -69,6 +69,10 @@ typedef struct {
   double work[16];
   int64_t near_pairs;      /* out: source-target pairs in the near field */
   int64_t m2l_pairs;       /* out: list-2 translations performed */
+  /* in: a handle from fmm2d_prepare for the same sources/targets arrays (NULL:
+   * none); the tree build then starts from its sorted keys (no reference
+   * counterpart) */
+  void* prepared;
 } fmm2d_opts;
 
 /* Library ABI version (FMM2D_ABI_VERSION).  No reference counterpart. */
@@ -309,6 +313,30 @@ int fmm2d_tree_leaf_histogram(const double* sources, int64_t ns, const double* t
  */
 int fmm2d_key_leaf_histogram(const double* sources, int64_t ns, const double* targ, int64_t nt,
                              int ndiv, int nbins, double* hist, double* boxsize0, void* stream);
+
+/*
+ * Prepare a tree build: the bounding square of the points and their sorted
+ * path keys (the first steps of pts_tree_build, common/pts_tree2d.f:455-551,
+ * which do not depend on the leaf capacity), kept on the device under
+ * *handle.  A caller that chooses the leaf capacity and expansion orders from
+ * the point distribution (fmm2d_prepared_leaf_histogram, then the proposer)
+ * passes the handle in fmm2d_opts.prepared so the FMM does not sort the keys
+ * again.  The arrays must keep their address and contents until the handle is
+ * freed; the FMM uses the handle only when (sources, ns, targ, nt) are the
+ * ones prepared.  Equal results with and without it.  No reference
+ * counterpart: the reference has no parameter proposer.
+ */
+int fmm2d_prepare(const double* sources, int64_t ns, const double* targ, int64_t nt,
+                  void** handle, void* stream);
+
+void fmm2d_prepare_free(void* handle);
+
+/*
+ * fmm2d_key_leaf_histogram from a preparation's keys (no further bounding
+ * box or sort).  No reference counterpart (the proposer's features).
+ */
+int fmm2d_prepared_leaf_histogram(void* handle, int ndiv, int nbins, double* hist,
+                                  double* boxsize0, void* stream);
 
 /* Expansion-order and leaf-size rules on the host:
  *   l2dterms  laplace/l2dterms.f:38     h2dterms helmholtz/h2dterms.f:35

@@ -9,8 +9,9 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("via", ["keys", "prepared"])
 @pytest.mark.parametrize("case", ["uniform", "cluster", "curve", "targets", "tiny"])
-def test_key_leaf_histogram_matches_restatement(case):
+def test_key_leaf_histogram_matches_restatement(case, via):
     import torch
     from fmm2d_amd import _native as nat
     from oracle import leaf_hist_ref as H
@@ -38,9 +39,28 @@ def test_key_leaf_histogram_matches_restatement(case):
     t = torch.from_numpy(np.ascontiguousarray(trg.T)).to(dev) if nt else None
     hist = np.zeros(16)
     bs0 = C.c_double()
-    ier = nat.lib().fmm2d_key_leaf_histogram(nat.ptr(s), ns, nat.ptr(t), nt, ndiv, 16,
-                                            hist.ctypes.data, C.byref(bs0), nat.stream_ptr())
-    assert ier == 0
+    L = nat.lib()
+    if via == "keys":
+        ier = L.fmm2d_key_leaf_histogram(nat.ptr(s), ns, nat.ptr(t), nt, ndiv, 16,
+                                         hist.ctypes.data, C.byref(bs0), nat.stream_ptr())
+        assert ier == 0
+    else:
+        # the keys of fmm2d_prepare (20 levels) give the same counts
+        h = C.c_void_p()
+        assert L.fmm2d_prepare(nat.ptr(s), ns, nat.ptr(t), nt, C.byref(h), nat.stream_ptr()) == 0
+        try:
+            ier = L.fmm2d_prepared_leaf_histogram(h, ndiv, 16, hist.ctypes.data, C.byref(bs0),
+                                                  nat.stream_ptr())
+            assert ier == 0
+            h21 = np.zeros(21)
+            assert L.fmm2d_prepared_leaf_histogram(h, ndiv, 21, h21.ctypes.data, None,
+                                                   nat.stream_ptr()) == 0
+            k21 = np.zeros(21)
+            assert L.fmm2d_key_leaf_histogram(nat.ptr(s), ns, nat.ptr(t), nt, ndiv, 21,
+                                              k21.ctypes.data, None, nat.stream_ptr()) == 0
+            assert np.array_equal(h21, k21), (h21, k21)
+        finally:
+            L.fmm2d_prepare_free(h)
     want = H.windows(src, trg, ndiv)
     assert np.array_equal(hist, want), (hist, want)
     pts = src if trg is None else np.concatenate([src, trg], axis=1)

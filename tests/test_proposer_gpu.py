@@ -49,3 +49,42 @@ def test_proposals_meet_eps_and_beat_default_rules():
         t_prop += tp
         t_def += td
     assert t_prop < t_def, (t_prop, t_def)
+
+
+@pytest.mark.parametrize("kind", ["l", "h"])
+def test_prepared_handle_leaves_fmm_unchanged(kind):
+    """An fmm2d_prepare handle (the sorted keys the proposer's features use)
+    handed to the FMM in opts gives the same tree and results as the FMM's own
+    build; a handle for other arrays is ignored."""
+    from fmm2d_amd import _native as nat
+    rng = np.random.default_rng(3)
+    n, nt = 150000, 4000
+    src = make_points(rng, n, "cluster")
+    trg = rng.uniform(src.min(), src.max(), (2, nt))
+    ch = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    zk = 30.0 if kind == "h" else None
+    eps = 1e-6
+    print(f"prepared handle: kind {kind} n {n} nt {nt}", flush=True)
+    dev = F._DevInputs(src, n, 1, ch, None, None, trg, True)
+    other = F._DevInputs(src[:, : n // 2], n // 2, 1, ch[: n // 2], None, None, None, True)
+    runs = {}
+    for tag in ("own", "prepared", "other"):
+        prep = None
+        if tag == "prepared":
+            prep = PR.Prepared(nat.ptr(dev.src), n, nat.ptr(dev.targ), nt)
+        elif tag == "other":
+            prep = PR.Prepared(nat.ptr(other.src), n // 2)
+        opts = {"precision": "mixed", "prepared": prep}
+        bufs, ier = F._fmm_device(kind, eps, zk, dev, 2, 2, 1, opts)
+        assert ier == 0
+        if prep is not None:
+            prep.close()
+        runs[tag] = ({k: v.cpu().numpy() for k, v in bufs.items() if v is not None},
+                     (opts["nlevels"], opts["nboxes"]))
+    ref, shape = runs["own"]
+    for tag in ("prepared", "other"):
+        got, sh = runs[tag]
+        assert sh == shape, tag
+        for k, v in ref.items():
+            err = np.linalg.norm(got[k] - v) / np.linalg.norm(v)
+            assert err < 1e-13, (tag, k, err)
