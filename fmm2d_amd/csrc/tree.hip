@@ -1284,7 +1284,83 @@ __global__ void k_window_levels(int64_t m, int ndiv, int d1, const unsigned long
   if (j < m) w[j] = (unsigned char)key_common_levels(keys[j], keys[j + ndiv], d1);
 }
 
-__global__ void k_key_leaf_hist(int64_t n, int64_t ns, int ndiv, int d1,
+// h[b] += the lanes of the wave that hold bin b (valid lanes only): one LDS
+// add per distinct bin of the wave instead of one per lane
+__device__ __forceinline__ void wave_bin_add(unsigned long long* h, bool valid, int b) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long act = __ballot(valid);
+  while (act) {  // uniform
+    const int leader = __ffsll((long long)act) - 1;
+    const int bl = __shfl(b, leader);
+    const unsigned long long same = __ballot(valid && b == bl);
+    if (lane == leader) atomicAdd(&h[bl], (unsigned long long)__popcll(same));
+    act &= ~same;
+  }
+}
+
+// Sources' leaf levels for ndiv <= kWinMax, kHistTile sorted keys per block of
+// 256 threads.  Source i's leaf level is 1 + the largest w[j] =
+// common levels(ks[j], ks[j + ndiv]) over the windows j in [i - ndiv, i]
+// (0 outside [0, ns - ndiv - 1]): the maxima over ndiv + 1 consecutive w by
+// doubling in LDS (after t passes m[x] = max w[x .. x + 2^t - 1]; the window
+// is the union of two such ranges).
+constexpr int kHistTile = 2048;
+constexpr int kWinMax = 2048;
+__global__ __launch_bounds__(256) void k_src_leaf_hist(int64_t ns, int ndiv, int d1,
+                                                       const unsigned long long* __restrict__ ks,
+                                                       int nbins, unsigned long long* hist) {
+  __shared__ unsigned char m[kHistTile + kWinMax];
+  __shared__ unsigned long long h[kLeafHistMax];
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
+  const int64_t i0 = (int64_t)blockIdx.x * kHistTile;
+  const int64_t last = ns - ndiv - 1;  // last window start
+  const int len = kHistTile + ndiv;
+  for (int x = threadIdx.x; x < len; x += 256) {
+    const int64_t j = i0 - ndiv + x;
+    m[x] = (j >= 0 && j <= last) ? (unsigned char)key_common_levels(ks[j], ks[j + ndiv], d1)
+                                 : (unsigned char)0;
+  }
+  __syncthreads();
+  constexpr int kPer = (kHistTile + kWinMax) / 256;
+  int span = 1;
+  while (2 * span <= ndiv + 1) {  // uniform
+    unsigned char v[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int x = threadIdx.x + r * 256;
+      v[r] = x < len ? m[x] : (unsigned char)0;
+      if (x + span < len) v[r] = max(v[r], m[x + span]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int x = threadIdx.x + r * 256;
+      if (x < len) m[x] = v[r];
+    }
+    __syncthreads();
+    span *= 2;
+  }
+  // span <= ndiv + 1 < 2 span: window [xi - ndiv, xi] = ranges from xi - ndiv and xi - span + 1
+#pragma unroll
+  for (int r = 0; r < kHistTile / 256; ++r) {
+    const int64_t i = i0 + threadIdx.x + r * 256;
+    const bool valid = i < ns;
+    int b = 0;
+    if (valid) {
+      const int xi = (int)(i - i0) + ndiv;
+      const int best = max((int)m[xi - ndiv], (int)m[xi - span + 1]);
+      b = min(best + 1, nbins - 1);
+    }
+    wave_bin_add(h, valid, b);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nbins; k += blockDim.x)
+    if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+// Leaf levels of the cnt points from i0 on (sources below ns, then targets) by
+// direct window scans: the general path (targets; ndiv > kWinMax; ns <= ndiv).
+__global__ void k_key_leaf_hist(int64_t cnt, int64_t i0, int64_t ns, int ndiv, int d1,
                                 const unsigned long long* __restrict__ skeys,
                                 const unsigned char* __restrict__ w,
                                 const unsigned long long* __restrict__ tkeys, int nbins,
@@ -1292,8 +1368,10 @@ __global__ void k_key_leaf_hist(int64_t n, int64_t ns, int ndiv, int d1,
   __shared__ unsigned long long h[kLeafHistMax];
   for (int k = threadIdx.x; k < nbins; k += blockDim.x) h[k] = 0;
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = i0 + g;
+  int bin = 0;
+  if (g < cnt) {
     int lev = 0;
     if (ns > ndiv) {
       const int64_t last = ns - ndiv - 1;  // last window start
@@ -1314,8 +1392,9 @@ __global__ void k_key_leaf_hist(int64_t n, int64_t ns, int ndiv, int d1,
       }
       lev = best + 1;
     }
-    atomicAdd(&h[min(lev, nbins - 1)], 1ull);
+    bin = min(lev, nbins - 1);
   }
+  wave_bin_add(h, g < cnt, bin);
   __syncthreads();
   for (int k = threadIdx.x; k < nbins; k += blockDim.x)
     if (h[k]) atomicAdd(&hist[k], h[k]);
@@ -1326,13 +1405,21 @@ namespace {
 // both of d1 levels
 void hist_from_keys(const unsigned long long* ks, int64_t ns, const unsigned long long* kt,
                     int64_t nt, int d1, int ndiv, int nbins, double* hist, hipStream_t st) {
-  const int64_t m = ns > ndiv ? ns - ndiv : 0;
-  DBuf<unsigned char> w(std::max<int64_t>(m, 1), st);
-  launch1d(k_window_levels, m, st, ndiv, d1, ks, w.p);
   DBuf<unsigned long long> d(nbins, st);
   d.zero();
-  launch1d(k_key_leaf_hist, ns + nt, st, ns, ndiv, d1, ks, (const unsigned char*)w.p, kt, nbins,
-           d.p);
+  // sources by LDS window maxima when the window fits, the rest by scans
+  const bool tiled = ns > ndiv && ndiv <= kWinMax;
+  if (tiled)
+    hipLaunchKernelGGL(k_src_leaf_hist, dim3((unsigned)ceil_div(ns, (int64_t)kHistTile)),
+                       dim3(256), 0, st, ns, ndiv, d1, ks, nbins, d.p);
+  const int64_t i0 = tiled ? ns : 0;
+  if (ns + nt > i0) {
+    const int64_t m = ns > ndiv ? ns - ndiv : 0;
+    DBuf<unsigned char> w(std::max<int64_t>(m, 1), st);
+    launch1d(k_window_levels, m, st, ndiv, d1, ks, w.p);
+    launch1d(k_key_leaf_hist, ns + nt - i0, st, i0, ns, ndiv, d1, ks,
+             (const unsigned char*)w.p, kt, nbins, d.p);
+  }
   std::vector<unsigned long long> h(nbins);
   hip_throw(hipMemcpyAsync(h.data(), d.p, sizeof(unsigned long long) * nbins,
                            hipMemcpyDeviceToHost, st), "copy");

@@ -10,7 +10,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("via", ["keys", "prepared"])
-@pytest.mark.parametrize("case", ["uniform", "cluster", "curve", "targets", "tiny"])
+@pytest.mark.parametrize("case", ["uniform", "cluster", "curve", "targets", "tiny", "win2048",
+                                  "win2500"])
 def test_key_leaf_histogram_matches_restatement(case, via):
     import torch
     from fmm2d_amd import _native as nat
@@ -29,6 +30,10 @@ def test_key_leaf_histogram_matches_restatement(case, via):
     elif case == "targets":
         ns, nt = 50000, 30000
         src = 0.5 + 0.05 * rng.standard_normal((2, ns))
+    elif case in ("win2048", "win2500"):
+        # the largest window held in LDS (ndiv 2048), and one beyond it
+        ns, ndiv = 60000, int(case[3:])
+        src = 0.5 + 0.01 * rng.standard_normal((2, ns))
     else:
         ns, ndiv = 10, 15
         src = rng.random((2, ns))
