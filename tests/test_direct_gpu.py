@@ -41,6 +41,47 @@ def test_direct_matches_reference(golden_dir, kind, nd):
     assert worst < 1e-12, worst
 
 
+@pytest.mark.parametrize("kind", ["c", "l", "h"])
+def test_bounded_grid_matches_reference_and_unbounded(golden_dir, kind):
+    """fmm2d_set_direct_max_blocks: the bounded grid (fewer, longer split source
+    ranges) gives the reference's values on the golden cases and the unbounded
+    launch's values on 2e5 sources."""
+    from fmm2d_amd import _native as nat
+    L = nat.lib()
+    g = np.load(f"{golden_dir}/direct_ref.npz")
+    src, trg = g["nd1_src"], g["nd1_trg"]
+    ch, ds, dv = g["nd1_charge"], g["nd1_dipstr"], g["nd1_dipvec"]
+    zk = complex(g["nd1_zk"])
+    rng = np.random.default_rng(5)
+    n = 200_000
+    bsrc = rng.random((2, n))
+    btrg = rng.random((2, 64))
+    bch = rng.standard_normal((1, n)) + 1j * rng.standard_normal((1, n))
+    bds = rng.standard_normal((1, n)) + 1j * rng.standard_normal((1, n))
+    bdv = rng.standard_normal((1, 2, n))
+    zkb = 20.0 + 0.5j
+    free = F._direct(kind, zkb, bsrc, btrg, bch, bds, bdv, 3, 1, 1e-14)[0]
+    prev = L.fmm2d_set_direct_max_blocks(16)
+    try:
+        worst = 0.0
+        for ifc, ifd in ((1, 0), (0, 1), (1, 1)):
+            for pgt in (1, 2, 3):
+                tag = f"nd1_{kind}_{'c' if ifc else ''}{'d' if ifd else ''}_{pgt}_"
+                (p, gr, h), _ = F._direct(kind, zk, src, trg, ch if ifc else None,
+                                          ds if ifd else None, dv if ifd else None, pgt, 1, 1e-14)
+                worst = max(worst, _rel(p, g[tag + "pottarg"]))
+                if pgt >= 2:
+                    worst = max(worst, _rel(gr, g[tag + "gradtarg"]))
+                if pgt >= 3:
+                    worst = max(worst, _rel(h, g[tag + "hesstarg"]))
+        assert worst < 1e-12, worst
+        bound = F._direct(kind, zkb, bsrc, btrg, bch, bds, bdv, 3, 1, 1e-14)[0]
+    finally:
+        L.fmm2d_set_direct_max_blocks(prev)
+    for a, b in zip(bound, free):
+        assert _rel(a, b) < 1e-13
+
+
 def test_public_ddir_shapes_and_values(golden_dir):
     g = np.load(f"{golden_dir}/direct_ref.npz")
     src, trg = g["nd1_src"], g["nd1_trg"]
