@@ -405,8 +405,7 @@ int bh_fmm_impl(const BhArgs& a, hipStream_t st) {
   DBuf<double> dorg(2, st);
   hipLaunchKernelGGL(k_bbox_centre, dim3(1), dim3(256), 0, st, ns, a.src, nt, a.targ, dorg.p);
   double org[2];
-  hip_throw(hipMemcpyAsync(org, dorg.p, sizeof(org), hipMemcpyDeviceToHost, st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");
+  readback(org, dorg.p, sizeof(org), st);
   const int ndc = 5 * nd;
   DBuf<cplx> cch((size_t)ndc * ns, st), cdp((size_t)ndc * ns, st);
   launch1d(k_bh_densities, ns * nd, st, nd, a.ifcharge, a.ifdipole, a.src, org[0], org[1],

@@ -20,6 +20,18 @@ inline void hip_throw(hipError_t e, const char* what) {
   if (e != hipSuccess) throw HipError{e, what};
 }
 
+// Small transfers in stream order (hostio.hip).  readback copies `bytes` of
+// device memory to the host and waits for them (a kernel stores them into
+// pinned memory that the host polls: a few microseconds where a copy and a
+// stream synchronisation take about twenty); upload_bytes copies host memory
+// to the device without waiting (in a kernel's arguments; `src` may be reused
+// at once).  Sizes beyond the limits, or not a multiple of 4, copy and
+// synchronise.
+constexpr size_t kReadbackMax = 4096;
+constexpr size_t kUploadInline = 1024;
+void readback(void* dst, const void* src, size_t bytes, hipStream_t st);
+void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t st);
+
 // Workspace cache (ws.cpp).  ws_alloc returns at least `bytes` bytes usable
 // in stream order on `st`; ws_free returns the block to st's cache.
 void* ws_alloc(size_t bytes, hipStream_t st);

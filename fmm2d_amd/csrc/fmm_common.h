@@ -274,11 +274,9 @@ __global__ void k_counts(int64_t n, const int* ilevel, const int* iparent, const
 
 
 template <typename T> DBuf<T> upload(const std::vector<T>& v, hipStream_t st) {
+  static_assert(sizeof(T) % 4 == 0, "upload_bytes moves 4-byte words");
   DBuf<T> d(v.size(), st);
-  if (!v.empty())
-    hip_throw(hipMemcpyAsync(d.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st),
-              "upload");
-  hip_throw(hipStreamSynchronize(st), "sync");  // v may die after return
+  upload_bytes(d.p, v.data(), sizeof(T) * v.size(), st);  // v may die after return
   return d;
 }
 
@@ -337,21 +335,18 @@ void plan_children(const Tree& t, int pg, int pgt, ChildPlan& cp, hipStream_t st
     launch1d(k_compact_flagged, nb, st, (const int*)flag.p, (const int*)pos.p, cp.list.p);
   }
   int nnl = 0;
-  hip_throw(hipMemcpyAsync(&nnl, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");
+  readback(&nnl, cnt.p, sizeof(int), st);
   std::vector<int> lsx(t.lstart);
   lsx.push_back(nb);
   DBuf<int> d_lsx = upload(lsx, st);
   if (nl + 2 > 64) throw HipError{hipErrorInvalidValue, "tree levels beyond 62"};
   hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(64), 0, st, cp.list.p, nnl, d_lsx.p, nl,
                      dlb.p);
-  hip_throw(hipMemcpyAsync(cp.lev_pos.data(), dlb.p, sizeof(int) * (nl + 2),
-                           hipMemcpyDeviceToHost, st), "copy");
   cp.up.alloc(4 * (size_t)std::max(nnl, 1), st);
   cp.down.alloc(4 * (size_t)std::max(nnl, 1), st);
   launch1d(k_child_slots, nnl, st, cp.list.p, t.ichild.p, t.isrcse.p, t.itargse.p, pg, pgt,
            cp.up.p, cp.down.p);
-  hip_throw(hipStreamSynchronize(st), "sync");
+  readback(cp.lev_pos.data(), dlb.p, sizeof(int) * (nl + 2), st);
 }
 
 // seg[q * (nl + 2) + l]: first position in tlist of class q at level >= l
@@ -379,8 +374,7 @@ void plan_m2l(const Tree& t, const Lists& L, int pg, int pgt, M2LPlan& mp, hipSt
            t.itargse.p, pg, pgt, flag.p);
   exclusive_scan(flag.p, pos.p, nb, cnt.p, st);
   int np = 0;
-  hip_throw(hipMemcpyAsync(&np, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");
+  readback(&np, cnt.p, sizeof(int), st);
   DBuf<int> plist(std::max(np, 1), st);
   launch1d(k_compact_flagged, nb, st, (const int*)flag.p, (const int*)pos.p, plist.p);
   mp.tlist.alloc(4 * (size_t)std::max(np, 1), st);
@@ -397,9 +391,7 @@ void plan_m2l(const Tree& t, const Lists& L, int pg, int pgt, M2LPlan& mp, hipSt
   hipLaunchKernelGGL(k_level_bounds, dim3(1), dim3(64), 0, st, plist.p, np, d_lsx.p, nl,
                      dlb.p);
   std::vector<int> lb(nl + 2, 0);
-  hip_throw(hipMemcpyAsync(lb.data(), dlb.p, sizeof(int) * lb.size(), hipMemcpyDeviceToHost,
-                           st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");
+  readback(lb.data(), dlb.p, sizeof(int) * lb.size(), st);
   for (int q = 0; q <= 4; ++q) mp.cls_seg[q] = q * np;
   mp.lev_seg.assign(4 * (size_t)(nl + 2), 0);
   for (int q = 0; q < 4; ++q)

@@ -1439,9 +1439,7 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
       launch1d(k_hp2m_rowlen, ns, st, d_lev.p, leaf_s.p, t.ilevel.p, roff.p);
       exclusive_scan(roff.p, roff.p, ns, rtot.p, st);
       int64_t nrow = 0;
-      hip_throw(hipMemcpyAsync(&nrow, rtot.p, sizeof(int64_t), hipMemcpyDeviceToHost, st),
-                "copy");
-      hip_throw(hipStreamSynchronize(st), "sync");
+      readback(&nrow, rtot.p, sizeof(int64_t), st);
       DBuf<cplx> gJ(std::max<int64_t>(nrow, 1), st), uJ(ns, st);
       launch1d(k_hp2m_rows, ns, st, zk, d_lev.p, leaf_s.p, t.ilevel.p, t.centers.p, xs.p,
                (const int64_t*)roff.p, gJ.p, uJ.p);
@@ -1467,10 +1465,8 @@ int helmholtz_fmm(const HelmArgs& a, hipStream_t st) {
       exclusive_scan(roff4.p, roff4.p, nb, rtot4.p, st);
       int np4 = 0;
       int64_t nrow4 = 0;
-      hip_throw(hipMemcpyAsync(&np4, tot4.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
-      hip_throw(hipMemcpyAsync(&nrow4, rtot4.p, sizeof(int64_t), hipMemcpyDeviceToHost, st),
-                "copy");
-      hip_throw(hipStreamSynchronize(st), "sync");
+      readback(&np4, tot4.p, sizeof(int), st);
+      readback(&nrow4, rtot4.p, sizeof(int64_t), st);
       if (np4 > 0) {
         DBuf<cplx> gH(std::max<int64_t>(nrow4, 1), st), uH(np4, st);
         launch1d(k_hp2l_rows, np4, st, nb, zk, d_lev.p, t.ilevel.p, t.centers.p, L.l4.p, sse,
@@ -1608,10 +1604,9 @@ int helmholtz_mps(const HelmMpsArgs& a, hipStream_t st) {
     {
       DBuf<int> d(2, st);
       const int init[2] = {INT32_MIN, INT32_MAX};
-      hip_throw(hipMemcpyAsync(d.p, init, sizeof(init), hipMemcpyHostToDevice, st), "copy");
+      upload_bytes(d.p, init, sizeof(init), st);
       launch1d(k_mps_pmax, n, st, a.mterms, d.p);
-      hip_throw(hipMemcpyAsync(pm, d.p, sizeof(pm), hipMemcpyDeviceToHost, st), "copy");
-      hip_throw(hipStreamSynchronize(st), "sync");
+      readback(pm, d.p, sizeof(pm), st);
     }
     if (pm[1] < 0) return FMM2D_ERR_ARG;
     // ---- tree over the centres, one expansion per leaf by default

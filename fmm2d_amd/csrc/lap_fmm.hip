@@ -1236,8 +1236,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       if (a.ifcharge) launch1d(k_any_imag, a.ns * a.nd, st, a.charge, flag.p);
       if (a.ifdipole) launch1d(k_any_imag, a.ns * a.nd, st, a.dipstr, flag.p);
       int h = 0;
-      hip_throw(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
-      hip_throw(hipStreamSynchronize(st), "sync");
+      readback(&h, flag.p, sizeof(int), st);
       half = (h == 0) ? 1 : 0;
     }
     const int nd = (kind == 1 && !half) ? 2 * a.nd : a.nd;  // Cauchy densities

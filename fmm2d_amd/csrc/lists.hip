@@ -200,9 +200,7 @@ void build_lists(const Tree& t, Lists& L, hipStream_t st) {
   std::vector<double> bsl(t.boxsize);
   bsl.push_back(bsl.back() / 2);
   DBuf<double> dbs(bsl.size(), st);
-  hip_throw(hipMemcpyAsync(dbs.p, bsl.data(), sizeof(double) * bsl.size(), hipMemcpyHostToDevice,
-                           st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");  // bsl dies on return
+  upload_bytes(dbs.p, bsl.data(), sizeof(double) * bsl.size(), st);
   if (nb > 0) {
     const unsigned grid =
         (unsigned)std::min<int64_t>(8192, ceil_div(nb, kListWaves * kBoxesPerWave));

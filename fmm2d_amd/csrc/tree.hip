@@ -783,8 +783,7 @@ struct Builder {
     scan_only(n);
     hipLaunchKernelGGL(k_total, dim3(1), dim3(1), 0, st, n, flag.p, offs.p, total.p);
     int h = 0;
-    hip_throw(hipMemcpyAsync(&h, total.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
-    hip_throw(hipStreamSynchronize(st), "sync");
+    readback(&h, total.p, sizeof(int), st);
     return h;
   }
   // Refine several box ranges with one scan: segment k covers boxes
@@ -816,9 +815,7 @@ struct Builder {
     if (total.n < K + 1) total.alloc(K + 1, st);
     hipLaunchKernelGGL(k_seg_totals, dim3(1), dim3(64), 0, st, R, flag.p, offs.p, total.p);
     std::vector<int> cum(K + 1, 0);
-    hip_throw(hipMemcpyAsync(cum.data(), total.p, sizeof(int) * (K + 1), hipMemcpyDeviceToHost,
-                             st), "copy");
-    hip_throw(hipStreamSynchronize(st), "sync");
+    readback(cum.data(), total.p, sizeof(int) * (K + 1), st);
     const int nref = cum[K];
     if (nref == 0) return;
     B.grow(nboxes + 4 * (size_t)nref, nboxes);
@@ -940,8 +937,7 @@ void prepare_into(TreePrep& P, const double* src, int64_t ns, const double* trg,
     DBuf<double> part(4 * nb, st), out(4, st);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, st, src, ns, trg, nt, part.p);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, st, part.p, nb, out.p);
-    hip_throw(hipMemcpyAsync(bb, out.p, sizeof(bb), hipMemcpyDeviceToHost, st), "copy");
-    hip_throw(hipStreamSynchronize(st), "sync");
+    readback(bb, out.p, sizeof(bb), st);
   }
   // box sizes of every possible level (boxsize(l+1) = boxsize(l)/2)
   std::vector<double>& bsall = P.bsall;
@@ -958,9 +954,7 @@ void prepare_into(TreePrep& P, const double* src, int64_t ns, const double* trg,
   std::vector<double> half(bsall.size());
   for (size_t l = 0; l < half.size(); ++l) half[l] = 0.5 * bsall[l];
   P.dhalf.alloc(half.size(), st);
-  hip_throw(hipMemcpyAsync(P.dhalf.p, half.data(), sizeof(double) * half.size(),
-                           hipMemcpyHostToDevice, st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");
+  upload_bytes(P.dhalf.p, half.data(), sizeof(double) * half.size(), st);
   const int d1 = std::min(kD1, nlmax);
   P.pk[0].build(src, ns, P.root[0], P.root[1], P.dhalf.p, d1, 0, st);
   P.pk[1].build(trg, nt, P.root[0], P.root[1], P.dhalf.p, d1, 0, st);
@@ -1082,9 +1076,8 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
     bld.ensure_flag((int)ubmax);
     {
       const int h[3] = {lo, n, bld.nboxes};
-      hip_throw(hipMemcpyAsync(dlev.p + 2 * l, h, 2 * sizeof(int), hipMemcpyHostToDevice, st),
-                "copy");
-      hip_throw(hipMemcpyAsync(dnb.p, h + 2, sizeof(int), hipMemcpyHostToDevice, st), "copy");
+      upload_bytes(dlev.p + 2 * l, h, 2 * sizeof(int), st);
+      upload_bytes(dnb.p, h + 2, sizeof(int), st);
       for (int k = 0; k < K; ++k) {
         const int L = l + k;
         launch1d(k_refine_flags_lv, ub[k], st, dlev.p, L, bld.B.ses.p, bld.B.set.p, idivflag,
@@ -1099,10 +1092,8 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
       }
       std::vector<int> hl(2 * (K + 1));
       int hnb = 0;
-      hip_throw(hipMemcpyAsync(hl.data(), dlev.p + 2 * l, sizeof(int) * 2 * (K + 1),
-                               hipMemcpyDeviceToHost, st), "copy");
-      hip_throw(hipMemcpyAsync(&hnb, dnb.p, sizeof(int), hipMemcpyDeviceToHost, st), "copy");
-      hip_throw(hipStreamSynchronize(st), "sync");  // h, hl, hnb live until here
+      readback(hl.data(), dlev.p + 2 * l, sizeof(int) * 2 * (K + 1), st);
+      readback(&hnb, dnb.p, sizeof(int), st);
       bld.nboxes = hnb;
       for (int k = 1; k <= K; ++k) {
         if (hl[2 * k + 1] == 0) {
@@ -1217,8 +1208,7 @@ void build_tree(const double* src, int64_t ns, const double* trg, int64_t nt, in
     hip_throw(hipGetLastError(), "kernel launch");
   }
   int hbig[2] = {0, 0};
-  hip_throw(hipMemcpyAsync(hbig, big.p, sizeof(hbig), hipMemcpyDeviceToHost, st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");
+  readback(hbig, big.p, sizeof(hbig), st);
   for (int k = 0; k < 2; ++k) {
     if (npts[k] == 0 || !hbig[k]) continue;
     // a leaf too large for k_leaf_sort: stable sort of all points by leaf
@@ -1421,9 +1411,7 @@ void hist_from_keys(const unsigned long long* ks, int64_t ns, const unsigned lon
              (const unsigned char*)w.p, kt, nbins, d.p);
   }
   std::vector<unsigned long long> h(nbins);
-  hip_throw(hipMemcpyAsync(h.data(), d.p, sizeof(unsigned long long) * nbins,
-                           hipMemcpyDeviceToHost, st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");
+  readback(h.data(), d.p, sizeof(unsigned long long) * nbins, st);
   for (int k = 0; k < nbins; ++k) hist[k] = (double)h[k];
 }
 }  // namespace
@@ -1437,8 +1425,7 @@ void key_leaf_histogram(const double* src, int64_t ns, const double* trg, int64_
     DBuf<double> part(4 * nb, st), out(4, st);
     hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(256), 0, st, src, ns, trg, nt, part.p);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, st, part.p, nb, out.p);
-    hip_throw(hipMemcpyAsync(bb, out.p, sizeof(bb), hipMemcpyDeviceToHost, st), "copy");
-    hip_throw(hipStreamSynchronize(st), "sync");
+    readback(bb, out.p, sizeof(bb), st);
   }
   const double bs = std::max(bb[1] - bb[0], bb[3] - bb[2]);
   if (boxsize0) *boxsize0 = bs;
@@ -1453,8 +1440,7 @@ void key_leaf_histogram(const double* src, int64_t ns, const double* trg, int64_
     bsl = bsl / 2;
   }
   DBuf<double> dhalf(half.size(), st);
-  hip_throw(hipMemcpyAsync(dhalf.p, half.data(), sizeof(double) * half.size(),
-                           hipMemcpyHostToDevice, st), "copy");
+  upload_bytes(dhalf.p, half.data(), sizeof(double) * half.size(), st);
   const double cx = (bb[0] + bb[1]) / 2, cy = (bb[2] + bb[3]) / 2;
   DBuf<unsigned long long> k0(ns, st), ks(ns, st), kt(std::max<int64_t>(nt, 1), st);
   launch1d(k_path_keys, ns, st, src, cx, cy, (const double*)dhalf.p, d1, 0, k0.p,
@@ -1482,9 +1468,7 @@ void leaf_level_histogram(const Tree& t, int nbins, double* hist, hipStream_t st
   launch1d(k_leaf_hist, t.nboxes, st, t.nchild.p, t.ilevel.p, t.isrcse.p, t.itargse.p, nbins,
            d.p);
   std::vector<unsigned long long> h(nbins);
-  hip_throw(hipMemcpyAsync(h.data(), d.p, sizeof(unsigned long long) * nbins,
-                           hipMemcpyDeviceToHost, st), "copy");
-  hip_throw(hipStreamSynchronize(st), "sync");
+  readback(h.data(), d.p, sizeof(unsigned long long) * nbins, st);
   for (int k = 0; k < nbins; ++k) hist[k] = (double)h[k];
 }
 
