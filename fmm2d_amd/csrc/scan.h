@@ -19,10 +19,10 @@ constexpr int kScanWaves = kScanThreads / 64;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// Exclusive scan of v over the workgroup (256 threads); *total = the sum.
-// lds holds kScanWaves + 1 values.  Ends with a barrier, so lds may be reused.
-template <typename T>
-__device__ __forceinline__ T block_exclusive_scan(T v, T* lds, T* total) {
+// Exclusive scan of v over a workgroup of NW waves; *total = the sum.  lds
+// holds NW + 1 values.  Ends with a barrier, so lds may be reused.
+template <int NW, typename T>
+__device__ __forceinline__ T block_exclusive_scan_w(T v, T* lds, T* total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   T x = v;
 #pragma unroll
@@ -35,18 +35,24 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* lds, T* total) {
   if (threadIdx.x == 0) {
     T s = 0;
 #pragma unroll
-    for (int i = 0; i < kScanWaves; ++i) {
+    for (int i = 0; i < NW; ++i) {
       const T t = lds[i];
       lds[i] = s;
       s += t;
     }
-    lds[kScanWaves] = s;
+    lds[NW] = s;
   }
   __syncthreads();
   const T ex = x - v + lds[w];
-  *total = lds[kScanWaves];
+  *total = lds[NW];
   __syncthreads();
   return ex;
+}
+
+// the same over a workgroup of kScanThreads (256) threads
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* lds, T* total) {
+  return block_exclusive_scan_w<kScanWaves>(v, lds, total);
 }
 
 template <typename T>
@@ -106,31 +112,32 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const T* in, T* out,
 // registers and the workgroup scanning the thread totals.  The tree build
 // scans dozens of per-level arrays of a few thousand elements, where three
 // launches (and a partials buffer) cost more than the work.
+// Arrays up to kScanSingleMax elements: one workgroup of kScan1Threads, each
+// thread summing a contiguous chunk, one block scan of the chunk sums, then
+// each thread writing its chunk's prefix sums (reading each element before
+// writing it, so in place is allowed).  One launch where the three-launch scan
+// is bound by launch latency.
+constexpr int kScan1Threads = 1024;
 constexpr int64_t kScanSingleMax = 16 * kScanTile;
 
 template <typename T>
-__global__ __launch_bounds__(kScanThreads) void k_scan_single(const T* in, T* out, int64_t n,
-                                                              T* total) {
-  __shared__ T lds[kScanWaves + 1];
-  T carry = 0;
-  for (int64_t base = 0; base < n; base += kScanTile) {
-    const int64_t j0 = base + (int64_t)threadIdx.x * kScanItems;
-    T loc[kScanItems];
-    T s = 0;
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-      const T v = (j0 + i < n) ? in[j0 + i] : T(0);
-      loc[i] = s;
-      s += v;
-    }
-    T tot;
-    const T ex = block_exclusive_scan(s, lds, &tot);
-#pragma unroll
-    for (int i = 0; i < kScanItems; ++i)
-      if (j0 + i < n) out[j0 + i] = carry + ex + loc[i];
-    carry += tot;
+__global__ __launch_bounds__(kScan1Threads) void k_scan_single(const T* in, T* out, int64_t n,
+                                                               T* total) {
+  __shared__ T lds[kScan1Threads / 64 + 1];
+  const int64_t chunk = (n + kScan1Threads - 1) / kScan1Threads;
+  const int64_t b = (int64_t)threadIdx.x * chunk, e = min(n, b + chunk);
+  T s = 0;
+#pragma unroll 16
+  for (int64_t j = b; j < e; ++j) s += in[j];
+  T tot;
+  T run = block_exclusive_scan_w<kScan1Threads / 64>(s, lds, &tot);
+#pragma unroll 16
+  for (int64_t j = b; j < e; ++j) {
+    const T v = in[j];
+    out[j] = run;
+    run += v;
   }
-  if (threadIdx.x == 0 && total) *total = carry;
+  if (threadIdx.x == 0 && total) *total = tot;
 }
 
 template <typename T>
@@ -140,7 +147,7 @@ void exclusive_scan(const T* in, T* out, int64_t n, T* total, hipStream_t st) {
     return;
   }
   if (n <= kScanSingleMax) {
-    hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kScanThreads), 0, st, in, out, n, total);
+    hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kScan1Threads), 0, st, in, out, n, total);
     hip_throw(hipGetLastError(), "scan");
     return;
   }

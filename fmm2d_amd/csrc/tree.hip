@@ -37,6 +37,19 @@ constexpr int kMaxLevels = 51;  // nlmax of the reference (pts_tree2d.f)
 constexpr int kBatch = 8;             // refinement levels per host synchronisation
 constexpr int64_t kBatchBoxes = 1 << 24;  // box capacity a batch may reserve
 
+// copy the first n[k] 4-byte words of src[k] to dst[k] for k = blockIdx.y
+struct WordCopies {
+  const unsigned* src[10];
+  unsigned* dst[10];
+  int64_t n[10];
+};
+__global__ void k_copy_words(WordCopies c) {
+  const int k = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n[k];
+       i += (int64_t)gridDim.x * blockDim.x)
+    c.dst[k][i] = c.src[k][i];
+}
+
 struct Boxes {
   DBuf<double> centers;
   DBuf<int> ilevel, iparent, nchild, ichild, ncoll, coll, ses, set, iflag;
@@ -54,20 +67,35 @@ struct Boxes {
     set.alloc(2 * c, st);
     iflag.alloc(c, st);
   }
+  // capacity for `need` boxes, the first `keep` kept (one copy launch)
   void grow(size_t need, size_t keep) {
     if (need <= cap) return;
-    size_t c = std::max(need, cap * 2);
-    centers.grow(2 * c, 2 * keep);
-    ilevel.grow(c, keep);
-    iparent.grow(c, keep);
-    nchild.grow(c, keep);
-    ichild.grow(4 * c, 4 * keep);
-    ncoll.grow(c, keep);
-    coll.grow(9 * c, 9 * keep);
-    ses.grow(2 * c, 2 * keep);
-    set.grow(2 * c, 2 * keep);
-    iflag.grow(c, keep);
-    cap = c;
+    const hipStream_t st = ilevel.st;
+    Boxes b;
+    b.alloc(std::max(need, cap * 2), st);
+    if (keep > 0) {
+      WordCopies c{};
+      int k = 0;
+      auto add = [&](const void* from, void* to, size_t words) {
+        c.src[k] = static_cast<const unsigned*>(from);
+        c.dst[k] = static_cast<unsigned*>(to);
+        c.n[k++] = (int64_t)words;
+      };
+      add(centers.p, b.centers.p, 4 * keep);  // 2 doubles per box
+      add(ilevel.p, b.ilevel.p, keep);
+      add(iparent.p, b.iparent.p, keep);
+      add(nchild.p, b.nchild.p, keep);
+      add(ichild.p, b.ichild.p, 4 * keep);
+      add(ncoll.p, b.ncoll.p, keep);
+      add(coll.p, b.coll.p, 9 * keep);
+      add(ses.p, b.ses.p, 2 * keep);
+      add(set.p, b.set.p, 2 * keep);
+      add(iflag.p, b.iflag.p, keep);
+      const unsigned gx = (unsigned)std::min<int64_t>(ceil_div((int64_t)(9 * keep), 256), 1024);
+      hipLaunchKernelGGL(k_copy_words, dim3(gx, 10), dim3(256), 0, st, c);
+      hip_throw(hipGetLastError(), "box copy");
+    }
+    *this = std::move(b);
   }
 };
 
