@@ -525,11 +525,13 @@ __global__ __launch_bounds__(256) void k_heval(int64_t n, int nd, cplx zk, const
 }
 
 // nd = 1: a block of kHEvB consecutive points shares out their (point,
-// list-3 box) pairs evenly -- pair k to thread k mod kHEvB, its point found by
-// binary search in the block's scan of list-3 counts -- and sums each point's
-// multipole terms in LDS (FP64 atomic adds), so leaves with different list-3
-// lengths in one wavefront leave no lane idle; each thread then adds its own
-// point's local expansion.
+// list-3 box) pairs evenly -- in rounds of kHEvB pairs, pair k to thread k mod
+// kHEvB, its point found by binary search in the block's scan of list-3
+// counts -- so leaves with different list-3 lengths in one wavefront leave no
+// lane idle.  Each round's terms go to LDS, and each thread adds its own
+// point's terms from there in pair order (a fixed order: the sums do not
+// depend on scheduling); each thread then adds its own point's local
+// expansion.
 constexpr int kHEvB = 256;
 
 template <int PG>
@@ -545,7 +547,7 @@ __global__ __launch_bounds__(kHEvB) void k_heval_b(int64_t n, cplx zk, const HLe
   __shared__ int off[kHEvB];
   __shared__ int sb[kHEvB];
   __shared__ double sx[kHEvB], sy[kHEvB];
-  __shared__ double acc[2 * NC][kHEvB];  // re, im of each component's sum
+  __shared__ double part[2 * NC][kHEvB];  // re, im of each component: one round's terms
   const int tid = threadIdx.x;
   const int64_t i = xcd_block() * kHEvB + tid;
   const bool live = i < n;
@@ -555,41 +557,53 @@ __global__ __launch_bounds__(kHEvB) void k_heval_b(int64_t n, cplx zk, const HLe
   sb[tid] = b;
   sx[tid] = x;
   sy[tid] = y;
+  double my[2 * NC];
 #pragma unroll
-  for (int a = 0; a < 2 * NC; ++a) acc[a][tid] = 0.0;
+  for (int a = 0; a < 2 * NC; ++a) my[a] = 0.0;
   int o, m;
   o = block_exclusive_scan(cnt, scan_tmp, &m);
   off[tid] = o;
   __syncthreads();
-  for (int k = tid; k < m; k += kHEvB) {
-    int lo = 0, hi = kHEvB - 1;  // last slot t with off[t] <= k
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (off[mid] <= k) lo = mid;
-      else hi = mid - 1;
-    }
-    const int c = l3[(int64_t)kMaxList3 * sb[lo] + (k - off[lo])];
-    const HLev Lc = lev[ilevel[c]];
-    if (!Lc.on) continue;
-    const double dx = sx[lo] - centers[2 * c], dy = sy[lo] - centers[2 * c + 1];
-    const double r = hypot(dx, dy);
-    const cplx w = cplx{dx / r, dy / r};
-    const Deriv D{hexp(mp, Lc, c, 1, 0), Lc.p, 0.5 * zk, Lc.s, 1.0 / Lc.s, -1};
-    cplx res[NC];
+  for (int r0 = 0; r0 < m; r0 += kHEvB) {  // uniform
+    const int k = r0 + tid;
+    if (k < m) {
+      int lo = 0, hi = kHEvB - 1;  // last slot t with off[t] <= k
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= k) lo = mid;
+        else hi = mid - 1;
+      }
+      const int c = l3[(int64_t)kMaxList3 * sb[lo] + (k - off[lo])];
+      const HLev Lc = lev[ilevel[c]];
+      cplx res[NC];
 #pragma unroll
-    for (int a = 0; a < NC; ++a) res[a] = mkc(0.0);
-    eval_mpole<PG>(D, zk * r, Lc.s, w, res);
+      for (int a = 0; a < NC; ++a) res[a] = mkc(0.0);
+      if (Lc.on) {
+        const double dx = sx[lo] - centers[2 * c], dy = sy[lo] - centers[2 * c + 1];
+        const double r = hypot(dx, dy);
+        const cplx w = cplx{dx / r, dy / r};
+        const Deriv D{hexp(mp, Lc, c, 1, 0), Lc.p, 0.5 * zk, Lc.s, 1.0 / Lc.s, -1};
+        eval_mpole<PG>(D, zk * r, Lc.s, w, res);
+      }
 #pragma unroll
-    for (int a = 0; a < NC; ++a) {
-      atomicAdd(&acc[2 * a][lo], res[a].re);
-      atomicAdd(&acc[2 * a + 1][lo], res[a].im);
+      for (int a = 0; a < NC; ++a) {
+        part[2 * a][tid] = res[a].re;
+        part[2 * a + 1][tid] = res[a].im;
+      }
     }
+    __syncthreads();
+    // this point's pairs of the round, in pair order
+    const int kb = max(o, r0), ke = min(o + cnt, r0 + kHEvB);
+    for (int kk = kb; kk < ke; ++kk) {
+#pragma unroll
+      for (int a = 0; a < 2 * NC; ++a) my[a] += part[a][kk - r0];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   if (!live) return;
   cplx res[NC];
 #pragma unroll
-  for (int a = 0; a < NC; ++a) res[a] = cplx{acc[2 * a][tid], acc[2 * a + 1][tid]};
+  for (int a = 0; a < NC; ++a) res[a] = cplx{my[2 * a], my[2 * a + 1]};
   const HLev Lb = lev[ilevel[b]];
   if (Lb.on) {
     const double dx = x - centers[2 * b], dy = y - centers[2 * b + 1];

@@ -582,11 +582,12 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
 }
 
 // nd = 1: a block of kEvB consecutive points shares out their (point, list-3
-// box) pairs evenly -- pair k to thread k mod kEvB, its point found by binary
-// search in the block's scan of list-3 counts -- and sums each point's
-// multipole terms in LDS (FP64 atomic adds); leaves with different list-3
-// lengths in one wavefront then leave no lane idle.  Each thread then adds
-// its own point's local expansion.
+// box) pairs evenly -- in rounds of kEvB pairs, pair k to thread k mod kEvB,
+// its point found by binary search in the block's scan of list-3 counts --
+// so leaves with different list-3 lengths in one wavefront leave no lane
+// idle.  Each round's terms go to LDS, and each thread adds its own point's
+// terms from there in pair order (a fixed order: the sums do not depend on
+// scheduling).  Each thread then adds its own point's local expansion.
 constexpr int kEvB = 256;
 
 template <int PG>
@@ -602,7 +603,7 @@ __global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double*
   __shared__ int off[kEvB];
   __shared__ int sb[kEvB];
   __shared__ double sx[kEvB], sy[kEvB];
-  __shared__ double acc[2 * PG][kEvB];  // re, im of pot, grad, hess sums
+  __shared__ double part[2 * PG][kEvB];  // re, im of pot, grad, hess: one round's terms
   const int tid = threadIdx.x;
   const int64_t i = xcd_block() * kEvB + tid;
   const bool live = i < n;
@@ -612,41 +613,52 @@ __global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double*
   sb[tid] = b;
   sx[tid] = x;
   sy[tid] = y;
+  double my[2 * PG];
 #pragma unroll
-  for (int a = 0; a < 2 * PG; ++a) acc[a][tid] = 0.0;
+  for (int a = 0; a < 2 * PG; ++a) my[a] = 0.0;
   int o, m;
   o = block_exclusive_scan(cnt, scan_tmp, &m);
   off[tid] = o;
   __syncthreads();
-  for (int k = tid; k < m; k += kEvB) {
-    int lo = 0, hi = kEvB - 1;  // last slot t with off[t] <= k
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (off[mid] <= k) lo = mid;
-      else hi = mid - 1;
+  for (int r0 = 0; r0 < m; r0 += kEvB) {  // uniform
+    const int k = r0 + tid;
+    if (k < m) {
+      int lo = 0, hi = kEvB - 1;  // last slot t with off[t] <= k
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= k) lo = mid;
+        else hi = mid - 1;
+      }
+      const int bt = sb[lo];
+      cplx ap = mkc(0.0), ag = mkc(0.0), ah = mkc(0.0);
+      mp_eval<PG>(sx[lo], sy[lo], l3[(int64_t)kMaxList3 * bt + (k - off[lo])], 1, 0, P, ilevel,
+                  nterms, rsc, centers, mp, ap, ag, ah);
+      part[0][tid] = ap.re;
+      part[1][tid] = ap.im;
+      if (PG >= 2) {
+        part[2][tid] = ag.re;
+        part[3][tid] = ag.im;
+      }
+      if (PG >= 3) {
+        part[4][tid] = ah.re;
+        part[5][tid] = ah.im;
+      }
     }
-    const int bt = sb[lo];
-    cplx ap = mkc(0.0), ag = mkc(0.0), ah = mkc(0.0);
-    mp_eval<PG>(sx[lo], sy[lo], l3[(int64_t)kMaxList3 * bt + (k - off[lo])], 1, 0, P, ilevel,
-                nterms, rsc, centers, mp, ap, ag, ah);
-    atomicAdd(&acc[0][lo], ap.re);
-    atomicAdd(&acc[1][lo], ap.im);
-    if (PG >= 2) {
-      atomicAdd(&acc[2][lo], ag.re);
-      atomicAdd(&acc[3][lo], ag.im);
+    __syncthreads();
+    // this point's pairs of the round, in pair order
+    const int kb = max(o, r0), ke = min(o + cnt, r0 + kEvB);
+    for (int kk = kb; kk < ke; ++kk) {
+#pragma unroll
+      for (int a = 0; a < 2 * PG; ++a) my[a] += part[a][kk - r0];
     }
-    if (PG >= 3) {
-      atomicAdd(&acc[4][lo], ah.re);
-      atomicAdd(&acc[5][lo], ah.im);
-    }
+    __syncthreads();
   }
-  __syncthreads();
   if (!live) return;
   cplx ap, ag, ah;
   loc_eval<PG>(x, y, b, 1, 0, P, ilevel, nterms, rsc, centers, loc, ap, ag, ah);
-  pot[i] = ap + cplx{acc[0][tid], acc[1][tid]};
-  if (PG >= 2) grad[i] = ag + cplx{acc[2][tid], acc[3][tid]};
-  if (PG >= 3) hess[i] = ah + cplx{acc[4][tid], acc[5][tid]};
+  pot[i] = ap + cplx{my[0], my[1]};
+  if (PG >= 2) grad[i] = ag + cplx{my[2], my[3]};
+  if (PG >= 3) hess[i] = ah + cplx{my[4], my[5]};
 }
 
 // ---------------------------------------------------------- near field

@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_translate(
     const int* tlist, const int* slots, int slots_by_box, const int* slot_op, const T* Afrag,
     int64_t afs,
     int sbox0, int dbox0, const int* ilevel, const int2* band, const double* logr, const T* src,
-    cplx* dst, int ssplit, int ksplit) {
+    cplx* dst, int ssplit, int ksplit, double* part, int64_t* dstoff) {
   typedef typename Mfma<T>::acc_t acc_t;
   // Workgroups are dispatched round-robin over the 8 XCDs; block b works on
   // logical block (b % 8) * per_xcd + b / 8 so that each XCD walks a
@@ -102,8 +102,12 @@ __global__ __launch_bounds__(256) void k_translate(
   if (wv0 >= nwaves) return;
   const int lane = threadIdx.x & 63;
   // Split launches (few targets, long operators): wavefront wv0 also takes
-  // one of ssplit groups of slots and one of ksplit ranges of k steps, and
-  // adds its partial products with atomics.
+  // one of ssplit groups of slots and one of ksplit ranges of k steps.  MODE
+  // 0 (every slot adds to the column's own target) writes the partial sums of
+  // group sk to part[sk][column][row] (all 16 RTT rows, 0 outside the band)
+  // and the target's offset in dst to dstoff[column], for k_tr_split_sum to
+  // add in group order; MODE 1 (each slot's target is written by one slot of
+  // one column) takes slot groups only and adds directly.
   const int nsk = ssplit * ksplit;
   const int sk = (int)(wv0 % nsk);
   const int64_t wv = wv0 / nsk;
@@ -154,10 +158,7 @@ __global__ __launch_bounds__(256) void k_translate(
   const int s_hi = (int)((int64_t)nslots * (sg + 1) / ssplit);
   const int k_lo = (int)((int64_t)KS * kg / ksplit);
   const int k_hi = (int)((int64_t)KS * (kg + 1) / ksplit);
-  auto put = [&](double* a, double v) {
-    if (split) atomicAdd(a, v);
-    else *a += v;
-  };
+  auto put = [&](double* a, double v) { *a += v; };
   for (int rc = rc0; rc < (KSU > 0 ? RTT : rc0 + 1); rc += RT) {
     if (MODE == 0) {
       acc_t acc[RT];
@@ -208,7 +209,21 @@ __global__ __launch_bounds__(256) void k_translate(
         }
         if (STAGE) wave_lds_fence();  // the next slot overwrites the rows
       }
-      if (valid) {
+      if (split) {
+        const int64_t ncols = (int64_t)tab.tile0[tab.ng] * 16;
+        const int64_t colg = (int64_t)(tab.tile0[g] + tl) * 16 + (lane & 15);
+        double* Pp = part + ((int64_t)sk * ncols + colg) * (16 * RTT);
+        if (sk == 0 && rc == 0 && kr == 0)
+          dstoff[colg] = valid ? ((int64_t)(tb - dbox0) * nd + d) * 2 * Pout : -1;
+        const int2 bd = valid ? (band ? band[ilevel[tb]] : full_out) : int2{0, 0};
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = 16 * (rc + r) + Mfma<T>::row(kr, i);
+            if (rc + r < RTT) Pp[row] = (row >= bd.x && row < bd.y) ? (double)acc[r][i] : 0.0;
+          }
+      } else if (valid) {
         const int2 bd = band ? band[ilevel[tb]] : full_out;
         double* L = reinterpret_cast<double*>(dst + ((int64_t)(tb - dbox0) * nd + d) * Pout);
         // level-dependent part of L_0: lanes 0..15 hold Re M_0 (row 0), lanes
@@ -311,8 +326,22 @@ struct TrGroup {
   int opbase, cls;
 };
 
-// Launch translate passes over the given groups (at most kTrMaxGroups per
-// launch; launches are stream-ordered, so groups must not depend on each other).
+// dst[dstoff[c] + row] += sum over the nsk split groups, in group order, of
+// part[k][c][row] (rows 0 .. rows-1 of the ncols columns; dstoff -1: none)
+static __global__ void k_tr_split_sum(int64_t n, int64_t ncols, int rstride, int rows, int nsk,
+                               const double* __restrict__ part, const int64_t* __restrict__ dstoff,
+                               double* dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t c = t / rows;
+  const int row = (int)(t - c * rows);
+  const int64_t o = dstoff[c];
+  if (o < 0) return;
+  double s = 0.0;
+  for (int k = 0; k < nsk; ++k) s += part[((int64_t)k * ncols + c) * rstride + row];
+  dst[o + row] += s;
+}
+
 // Buffers of one translate launch.  src holds 2 Pin values per (box,
 // density) of the launch's element type (double: the cplx expansions).
 struct TrBufs {
@@ -327,9 +356,10 @@ struct TrBufs {
 // fp32: operators (Afrag) and sources are float, accumulation FP32 on
 // v_mfma_f32_16x16x4f32 (gather mode only); otherwise FP64.
 // Launches with fewer wavefronts than this split their slots (and, for long
-// operators whose k loop is not unrolled, their k steps) over more wavefronts
-// (atomic adds): small problems and coarse levels would otherwise run a few
-// dozen wavefronts through 27 slots each.
+// gather operators whose k loop is not unrolled, their k steps) over more
+// wavefronts, whose partial sums are added in a fixed order (no atomics: the
+// results do not depend on scheduling): small problems and coarse levels
+// would otherwise run a few dozen wavefronts through 27 slots each.
 constexpr int64_t kTrSplitWaves = 4096;
 
 inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int nslots,
@@ -373,18 +403,27 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     int ssplit = 1, ksplit = 1;                                                                 \
     if (!LOGT && nw0 < kTrSplitWaves) {                                                         \
       ssplit = (int)std::min<int64_t>(nslots, (kTrSplitWaves + nw0 - 1) / nw0);                 \
-      if (KSU == 0)                                                                             \
+      if (KSU == 0 && MODE == 0)                                                                \
         ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(KS / 16, (kTrSplitWaves + nw0 *   \
                                           ssplit - 1) / (nw0 * ssplit)));                      \
     }                                                                                           \
     const int64_t nw = nw0 * ssplit * ksplit;                                                   \
     const int64_t nblk = (nw + 3) / 4;                                                          \
     const int per_xcd = (int)((nblk + 7) / 8);                                                  \
+    const int nsk = ssplit * ksplit;                                                            \
+    const int64_t ncols = (int64_t)tiles * 16;                                                  \
+    const bool gsplit = MODE == 0 && nsk > 1;                                                   \
+    DBuf<double> part(gsplit ? (size_t)nsk * ncols * 16 * RTT : 0, st);                         \
+    DBuf<int64_t> dstoff(gsplit ? (size_t)ncols : 0, st);                                       \
     hipLaunchKernelGGL((k_translate<T, RT, KSU, MODE, LOGT>), dim3((unsigned)(8 * per_xcd)),    \
                        dim3(256), 0, st, nd, Pin, Pout, nslots, nw, per_xcd, tab, tlist, slots, \
                        (int)slots_by_box, slot_op, static_cast<const T*>(Afrag), afs, io.sbox0, \
                        io.dbox0, ilevel, band, logr, static_cast<const T*>(io.src), io.dst,     \
-                       ssplit, ksplit);                                                         \
+                       ssplit, ksplit, part.p, dstoff.p);                                       \
+    if (gsplit)                                                                                 \
+      launch1d(k_tr_split_sum, ncols * 2 * Pout, st, ncols, 16 * RTT, 2 * Pout, nsk,            \
+               (const double*)part.p, (const int64_t*)dstoff.p,                                 \
+               reinterpret_cast<double*>(io.dst));                                              \
   }
 #define FMM2D_TR_RT(T, MODE, LOGT)                 \
   switch (KS <= 4 * RTT ? RTT : 0) {               \
