@@ -30,6 +30,7 @@ timed on the host cores on a bounded sample.
                   [--precision mixed|fp64] [--no-learned] [--no-cpu] [--validate 64]
 """
 import argparse
+import collections
 import json
 import os
 import subprocess
@@ -241,11 +242,17 @@ def c4_problems(count, seed=4):
 
 def main_batch(args):
     """Batched workload (secondary): the C4 mix of independent lfmm2d and
-    hfmm2d problems through fmm2d_batch on --streams streams, against the same
-    problems one after another on one stream.  Rate = points per second of
-    the whole batch, inputs resident on the device."""
+    hfmm2d problems through fmm2d_batch on --streams streams.  Pipelines: the
+    reference's rules (FP64, lndiv2d/hndiv2d leaf capacity) and the tuned one
+    (batch_tune: leaf capacity per group of similar problems, mixed precision
+    for eps >= 1e-6), each checked instance by instance against FP64 direct
+    sums at sampled sources; the same reference-rule problems one after another
+    on one stream for comparison.  Rate = points per second of the whole batch,
+    inputs resident on the device; the tuning (a one-time cost per batch
+    shape) is timed separately."""
     import torch
     from fmm2d_amd import _native as nat
+    from fmm2d_amd import batch_tune as BT
     from fmm2d_amd import fmm2d as F
     L = nat.lib()
     count = args.batch
@@ -253,10 +260,18 @@ def main_batch(args):
     npts = sum(p["sources"].shape[1] for p in probs)
     print(f"bench: batch workload, {count} instances, {npts} points", file=sys.stderr,
           flush=True)
-    jobs, keep = F._stage_batch(probs)
+    t0 = time.perf_counter()
+    table = BT.tune(probs)
+    t_tune = time.perf_counter() - t0
+    print(f"bench[batch]: tuned {len(table)} groups in {t_tune:.2f} s; factors "
+          f"{sorted(collections.Counter(v['factor'] for v in table.values()).items())}",
+          file=sys.stderr, flush=True)
+    tuned = BT.apply(probs, table)
+    staged = {"rules": F._stage_batch(probs), "tuned": F._stage_batch(tuned)}
     torch.cuda.synchronize()
 
-    def run(nstreams):
+    def run(label, nstreams):
+        jobs = staged[label][0]
         t0 = time.perf_counter()
         ier = L.fmm2d_batch(count, jobs, nstreams)
         dt = time.perf_counter() - t0
@@ -264,24 +279,56 @@ def main_batch(args):
             raise RuntimeError(f"fmm2d_batch returned {ier}")
         return dt
 
+    def check(label, nsample=32, seed=9):
+        """max over instances of the relative L2 error of pot at nsample
+        sampled sources against the FP64 direct sum, and the instances above eps"""
+        rng = np.random.default_rng(seed)
+        worst, bad = 0.0, 0
+        for pb, (dev, bufs, nd, _) in zip(probs if label == "rules" else tuned,
+                                          staged[label][1]):
+            ns = pb["sources"].shape[1]
+            idx = np.sort(rng.choice(ns, size=min(nsample, ns), replace=False))
+            ex = F._direct(pb["kind"], pb.get("zk", 0.0), pb["sources"], pb["sources"][:, idx],
+                           pb["charges"], None, None, 1, 1, 1e-14)[0][0]
+            got = nat.fortran_to_numpy(bufs["pot"])[0][idx]
+            e = float(np.linalg.norm(got - ex[0]) / np.linalg.norm(ex[0]))
+            worst = max(worst, e)
+            if e > pb["eps"]:
+                bad += 1
+                print(f"bench[batch/{label}]: instance above eps: kind {pb['kind']} N {ns} eps "
+                      f"{pb['eps']:g} zk {pb.get('zk', 0):.3g} opts {pb.get('opts')} err {e:.2e}",
+                      file=sys.stderr, flush=True)
+        return worst, bad
+
     ns_ = args.streams
-    for _ in range(args.warmup):
-        run(ns_)
-    t_seq = min(run(1) for _ in range(args.steps))
-    t_bat = min(run(ns_) for _ in range(args.steps))
-    print(f"bench[batch]: one stream {t_seq*1e3:.1f} ms, {ns_} streams {t_bat*1e3:.1f} ms "
-          f"({t_seq / t_bat:.2f}x)", file=sys.stderr, flush=True)
+    res = {}
+    for label in ("rules", "tuned"):
+        for _ in range(args.warmup):
+            run(label, ns_)
+        dt = min(run(label, ns_) for _ in range(args.steps))
+        worst, bad = check(label)
+        res[label] = {"value": round(npts / dt, 1), "ms": round(dt * 1e3, 3),
+                      "max_rel_l2_err": worst, "instances_above_eps": bad}
+        print(f"bench[batch/{label}]: {ns_} streams {dt*1e3:.1f} ms, worst sampled error "
+              f"{worst:.2e}, {bad} instances above eps", file=sys.stderr, flush=True)
+    t_seq = min(run("rules", 1) for _ in range(args.steps))
+    print(f"bench[batch]: rules on one stream {t_seq*1e3:.1f} ms", file=sys.stderr, flush=True)
+    head = "tuned" if res["tuned"]["instances_above_eps"] == 0 else "rules"
     line = {
         "metric": f"batched FMM points/sec, {count} independent lfmm2d/hfmm2d instances "
                   "(secondary)",
-        "value": round(npts / t_bat, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(t_bat * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"C4 mix reduced to {count} instances: N log-uniform [1e3, 2e5], "
-                               "disk or perturbed curve, lfmm2d and hfmm2d (zk log-uniform "
-                               "[1, 300]) alternating, eps 1e-6/1e-9, pot at sources",
-                   "instances": count, "points": npts, "streams": ns_},
-        "one_stream": {"value": round(npts / t_seq, 1), "ms": round(t_seq * 1e3, 3)},
+        "value": res[head]["value"], "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res[head]["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if head == "tuned" else "f64", "data": "synthetic",
+        "config": {"workload": f"C4 mix, {count} instances: N log-uniform [1e3, 2e5], disk or "
+                               "perturbed curve, lfmm2d and hfmm2d (zk log-uniform [1, 300]) "
+                               "alternating, eps 1e-6/1e-9, pot at sources",
+                   "instances": count, "points": npts, "streams": ns_, "pipeline": head},
+        "pipelines": res,
+        "tuning": {"seconds": round(t_tune, 3), "groups": len(table),
+                   "factors": list(BT.FACTORS)},
+        "one_stream_rules": {"value": round(npts / t_seq, 1), "ms": round(t_seq * 1e3, 3)},
     }
     print(json.dumps(line), flush=True)
 
@@ -544,7 +591,7 @@ def main():
     ap.add_argument("--workload", default="lfmm2d", choices=["lfmm2d", "hfmm2d", "batch", "c5"],
                     help="lfmm2d: the headline workload; hfmm2d: the secondary Helmholtz one; "
                          "batch: many independent small FMMs through fmm2d_batch")
-    ap.add_argument("--batch", type=int, default=256, help="instances of the batch workload")
+    ap.add_argument("--batch", type=int, default=2048, help="instances of the batch workload")
     ap.add_argument("--streams", type=int, default=8, help="streams of the batch workload")
     args = ap.parse_args()
     if args.gpus != 1:

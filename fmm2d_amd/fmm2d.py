@@ -16,6 +16,8 @@ Reference quirks that are reproduced on purpose (python/fmm2dpy/fmm2d.py):
   * "Nothing to compute" is printed and an empty Output returned when no
     output is requested.
 """
+import ctypes
+
 import numpy as np
 import numpy.linalg as la
 
@@ -187,8 +189,10 @@ def fmm2d_batch(problems, nstreams=4):
 
     problems: list of dicts, each with "kind" ("h", "l", "r" or "c") and the
     keyword arguments of hfmm2d / lfmm2d / rfmm2d / cfmm2d (eps, zk, sources,
-    charges, dipstr, dipvec, targets, pg, pgt, nd).  When both pg and pgt are
-    set they must be equal, as the single-problem calls require.  The problems
+    charges, dipstr, dipvec, targets, pg, pgt, nd), and optionally "opts" (the
+    fmm2d_opts keys of _native.opts_struct: ndiv, precision, nterms, ...; see
+    fmm2d_amd.batch_tune for a leaf-capacity tuner that sets them).  When both
+    pg and pgt are set they must be equal, as the single-problem calls require.  The problems
     run on nstreams streams at once, largest first; returns one Output per
     problem, in order (ier set on each).
     """
@@ -217,7 +221,7 @@ def fmm2d_batch(problems, nstreams=4):
     ier = L.fmm2d_batch(len(problems), jobs, int(nstreams))
     nat.check(ier, "fmm2d_batch")
     outs = []
-    for k, (dev, bufs, nd) in enumerate(keep):
+    for k, (dev, bufs, nd, _) in enumerate(keep):
         out = Output()
         res = {}
         for name, t in bufs.items():
@@ -272,7 +276,11 @@ def _stage_batch(problems):
         j.nt, j.targ = dev.nt, nat.ptr(dev.targ)
         j.pottarg, j.gradtarg, j.hesstarg = (nat.ptr(bufs[n])
                                              for n in ("pottarg", "gradtarg", "hesstarg"))
-        keep.append((dev, bufs, nd))
+        o = None
+        if pb.get("opts"):
+            o = nat.opts_struct(pb["opts"])
+            j.opts = ctypes.pointer(o)
+        keep.append((dev, bufs, nd, o))
     return jobs, keep
 
 

@@ -136,23 +136,26 @@ def test_reference_fixtures_against_oracle_direct_sum(golden_dir):
     assert np.linalg.norm(g[tagh + "pottarg"] - exh["pottarg"]) / np.linalg.norm(exh["pottarg"]) < 1e-9
 
 
-def test_batch_job_layout(tmp_path):
-    """ctypes FmmJob has the field offsets of struct fmm2d_job in the header."""
-    names = [f[0] for f in nat.FmmJob._fields_]
-    src = tmp_path / "job.c"
+@pytest.mark.parametrize("cname,pytype", [("fmm2d_job", "FmmJob"), ("fmm2d_opts", "FmmOpts")])
+def test_struct_layouts(tmp_path, cname, pytype):
+    """ctypes FmmJob / FmmOpts have the size and field offsets of struct
+    fmm2d_job / fmm2d_opts in the header."""
+    cls = getattr(nat, pytype)
+    names = [f[0] for f in cls._fields_]
+    src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "fmm2d_amd.h"\nint main(void){\n'
-        + 'printf("%zu\\n", sizeof(fmm2d_job));\n'
-        + "".join(f'printf("%zu\\n", offsetof(fmm2d_job, {n}));\n' for n in names)
+        + f'printf("%zu\\n", sizeof({cname}));\n'
+        + "".join(f'printf("%zu\\n", offsetof({cname}, {n}));\n' for n in names)
         + "return 0;}\n")
-    exe = tmp_path / "job"
+    exe = tmp_path / "layout"
     import subprocess
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include",
                     str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
                                           text=True).stdout.split()]
-    assert got[0] == ctypes.sizeof(nat.FmmJob)
-    assert got[1:] == [getattr(nat.FmmJob, n).offset for n in names]
+    assert got[0] == ctypes.sizeof(cls)
+    assert got[1:] == [getattr(cls, n).offset for n in names]
 
 
 def test_batch_argument_checks():
