@@ -23,7 +23,7 @@ import numpy as np
 from . import _native as nat
 from . import fmm2d as F
 
-FACTORS = (0.5, 1.0, 2.0, 4.0)
+FACTORS = (1.0, 2.0, 4.0, 8.0, 16.0)
 
 
 def precision_for(eps):

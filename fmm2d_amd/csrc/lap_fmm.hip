@@ -343,32 +343,52 @@ __global__ __launch_bounds__(256) void k_p2m_reg(int64_t n, int nd, int P, const
   cplx acc[PT];
 #pragma unroll
   for (int k = 0; k < PT; ++k) acc[k] = mkc(0.0);
-  // sources two at a time: two independent power chains in flight
-  for (int j = s0; j < s1; j += 2) {
-    const bool two = j + 1 < s1;
-    const int j1 = two ? j + 1 : j;
-    const cplx w0 = cplx{(xs[2 * j] - cx) * rinv, (xs[2 * j + 1] - cy) * rinv};
-    const cplx w1 = cplx{(xs[2 * j1] - cx) * rinv, (xs[2 * j1 + 1] - cy) * rinv};
-    cplx c0 = IFC ? chs[(int64_t)j * nd + d] : mkc(0.0);
-    cplx c1 = IFC ? chs[(int64_t)j1 * nd + d] : mkc(0.0);
-    cplx e0 = IFD ? rinv * dps[(int64_t)j * nd + d] : mkc(0.0);
-    cplx e1 = IFD ? rinv * dps[(int64_t)j1 * nd + d] : mkc(0.0);
-    if (!two) c1 = e1 = mkc(0.0);
-    if (IFC) acc[0] += c0 + c1;
-    cplx v0 = mkc(1.0), v1 = mkc(1.0);  // w^(k-1)
+  // U sources at a time: U independent power chains in flight (4 while the
+  // coefficients leave the registers for them, else 2)
+  constexpr int U = PT <= 24 ? 4 : 2;
+  for (int j = s0; j < s1; j += U) {
+    cplx w[U], c[U], e[U], v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = j + u < s1;
+      const int jj = ok ? j + u : j;
+      w[u] = cplx{(xs[2 * jj] - cx) * rinv, (xs[2 * jj + 1] - cy) * rinv};
+      c[u] = (IFC && ok) ? chs[(int64_t)jj * nd + d] : mkc(0.0);
+      e[u] = (IFD && ok) ? rinv * dps[(int64_t)jj * nd + d] : mkc(0.0);
+      v[u] = mkc(1.0);  // w^(k-1)
+    }
+    if (IFC) {
+      cplx s = c[0];
+#pragma unroll
+      for (int u = 1; u < U; ++u) s += c[u];
+      acc[0] += s;
+    }
 #pragma unroll
     for (int k = 1; k < PT; ++k) {
       if (k <= p) {
-        if (IFD) acc[k] += v0 * e0 + v1 * e1;
-        v0 = v0 * w0;
-        v1 = v1 * w1;
+        if (IFD) {
+          cplx s = v[0] * e[0];
+#pragma unroll
+          for (int u = 1; u < U; ++u) s += v[u] * e[u];
+          acc[k] += s;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = v[u] * w[u];
         if (IFC) {
+          const double ck = -1.0 / k;
           if (RE) {
-            const double ck = -1.0 / k;
-            const double a0 = ck * c0.re, a1 = ck * c1.re;
-            acc[k] += cplx{v0.re * a0 + v1.re * a1, v0.im * a0 + v1.im * a1};
+            double sr = v[0].re * c[0].re, si = v[0].im * c[0].re;
+#pragma unroll
+            for (int u = 1; u < U; ++u) {
+              sr += v[u].re * c[u].re;
+              si += v[u].im * c[u].re;
+            }
+            acc[k] += cplx{ck * sr, ck * si};
           } else {
-            acc[k] += (-1.0 / k) * (v0 * c0 + v1 * c1);
+            cplx s = v[0] * c[0];
+#pragma unroll
+            for (int u = 1; u < U; ++u) s += v[u] * c[u];
+            acc[k] += ck * s;
           }
         }
       }
