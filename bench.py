@@ -48,7 +48,7 @@ PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
 STAGES = {0: "p2m", 1: "list4", 2: "m2m", 3: "m2l plan", 13: "m2l kernel", 4: "l2l",
           6: "l2p+list3", 15: "near prep", 7: "near", 8: "tree+lists", 9: "sort+gather",
           10: "scatter"}
-M2L_SLOT, NEAR_SLOT = 13, 7
+M2L_SLOT, NEAR_SLOT, TREE_SLOT = 13, 7, 8
 # blocks of the learned steps' acceptance-test direct sums (16 per XCD), which run
 # on a second stream beside the FMM
 VAL_BLOCKS = 128
@@ -808,6 +808,16 @@ def main():
                  "traffic": read_traffic(TRAFFIC_KEYS[head][1]),
                  "launch_ms": round(t_near * 1e3, 4), "flops_per_launch": work[7]},
     }
+    # the tree stage (build, level restriction, final order, lists) against
+    # HBM: 40 bytes per point + 80 per box; in a learned step the bounding box
+    # and key sort come from the prepare call, which is timed in the step but
+    # outside this stage
+    t_tree = stage[TREE_SLOT]
+    a_tree = work[TREE_SLOT] / t_tree / 1e9 if t_tree > 0 else 0.0
+    rl["tree"] = {"bound": "hbm", "kernel": "tree build + lists (stage)",
+                  "achieved": round(a_tree, 2), "peak": PEAK["hbm_gbs"], "unit": "GB/s",
+                  "frac": round(a_tree / PEAK["hbm_gbs"], 4), "traffic": None,
+                  "launch_ms": round(t_tree * 1e3, 4), "bytes_per_call": work[TREE_SLOT]}
     roofline = rl["near"] if t_near >= t_m2l else rl["m2l"]
 
     cpu = None
