@@ -106,8 +106,10 @@ __global__ __launch_bounds__(256) void k_translate(
   // 0 (every slot adds to the column's own target) writes the partial sums of
   // group sk to part[sk][column][row] (all 16 RTT rows, 0 outside the band)
   // and the target's offset in dst to dstoff[column], for k_tr_split_sum to
-  // add in group order; MODE 1 (each slot's target is written by one slot of
-  // one column) takes slot groups only and adds directly.
+  // add in group order.  MODE 1 (each slot's target is written by one slot of
+  // one column): slot groups add directly; with k ranges split, the partial
+  // sums go to part[kg][column][slot][row] and dstoff[column][slot], for
+  // k_tr_split_sum to add in k-range order.
   const int nsk = ssplit * ksplit;
   const int sk = (int)(wv0 % nsk);
   const int64_t wv = wv0 / nsk;
@@ -277,7 +279,22 @@ __global__ __launch_bounds__(256) void k_translate(
             step(ks, (k >= klo && k < khi) ? M[k] : T(0));
           }
         }
-        if (c >= 0) {
+        if (ksplit > 1) {
+          // partial over k range kg of (column, slot): part[kg][column][slot][row]
+          const int64_t ncs = (int64_t)tab.tile0[tab.ng] * 16 * nslots;
+          const int64_t cs = ((int64_t)(tab.tile0[g] + tl) * 16 + (lane & 15)) * nslots + s;
+          double* Pp = part + ((int64_t)kg * ncs + cs) * (16 * RTT);
+          if (kg == 0 && rc == 0 && kr == 0)
+            dstoff[cs] = c >= 0 ? ((int64_t)(c - dbox0) * nd + d) * 2 * Pout : -1;
+          const int2 bd = c >= 0 ? (band ? band[ilevel[c]] : full_out) : int2{0, 0};
+#pragma unroll
+          for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int row = 16 * (rc + r) + Mfma<T>::row(kr, i);
+              if (rc + r < RTT) Pp[row] = (row >= bd.x && row < bd.y) ? (double)acc[r][i] : 0.0;
+            }
+        } else if (c >= 0) {
           const int2 bd = band ? band[ilevel[c]] : full_out;
           double* L = reinterpret_cast<double*>(dst + ((int64_t)(c - dbox0) * nd + d) * Pout);
 #pragma unroll
@@ -356,7 +373,7 @@ struct TrBufs {
 // fp32: operators (Afrag) and sources are float, accumulation FP32 on
 // v_mfma_f32_16x16x4f32 (gather mode only); otherwise FP64.
 // Launches with fewer wavefronts than this split their slots (and, for long
-// gather operators whose k loop is not unrolled, their k steps) over more
+// operators whose k loop is not unrolled, their k steps) over more
 // wavefronts, whose partial sums are added in a fixed order (no atomics: the
 // results do not depend on scheduling): small problems and coarse levels
 // would otherwise run a few dozen wavefronts through 27 slots each.
@@ -403,7 +420,7 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     int ssplit = 1, ksplit = 1;                                                                 \
     if (!LOGT && nw0 < kTrSplitWaves) {                                                         \
       ssplit = (int)std::min<int64_t>(nslots, (kTrSplitWaves + nw0 - 1) / nw0);                 \
-      if (KSU == 0 && MODE == 0)                                                                \
+      if (KSU == 0)                                                                             \
         ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(KS / 16, (kTrSplitWaves + nw0 *   \
                                           ssplit - 1) / (nw0 * ssplit)));                      \
     }                                                                                           \
@@ -411,9 +428,11 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
     const int64_t nblk = (nw + 3) / 4;                                                          \
     const int per_xcd = (int)((nblk + 7) / 8);                                                  \
     const int nsk = ssplit * ksplit;                                                            \
-    const int64_t ncols = (int64_t)tiles * 16;                                                  \
-    const bool gsplit = MODE == 0 && nsk > 1;                                                   \
-    DBuf<double> part(gsplit ? (size_t)nsk * ncols * 16 * RTT : 0, st);                         \
+    /* partial sums: MODE 0 per (split group, column), MODE 1 per (k range, column, slot) */    \
+    const int64_t ncols = (int64_t)tiles * 16 * (MODE == 0 ? 1 : nslots);                       \
+    const int npart = MODE == 0 ? nsk : ksplit;                                                 \
+    const bool gsplit = npart > 1;                                                              \
+    DBuf<double> part(gsplit ? (size_t)npart * ncols * 16 * RTT : 0, st);                       \
     DBuf<int64_t> dstoff(gsplit ? (size_t)ncols : 0, st);                                       \
     hipLaunchKernelGGL((k_translate<T, RT, KSU, MODE, LOGT>), dim3((unsigned)(8 * per_xcd)),    \
                        dim3(256), 0, st, nd, Pin, Pout, nslots, nw, per_xcd, tab, tlist, slots, \
@@ -421,7 +440,7 @@ inline void launch_translate(int mode, int logt, int nd, const TrBufs& io, int n
                        io.dbox0, ilevel, band, logr, static_cast<const T*>(io.src), io.dst,     \
                        ssplit, ksplit, part.p, dstoff.p);                                       \
     if (gsplit)                                                                                 \
-      launch1d(k_tr_split_sum, ncols * 2 * Pout, st, ncols, 16 * RTT, 2 * Pout, nsk,            \
+      launch1d(k_tr_split_sum, ncols * 2 * Pout, st, ncols, 16 * RTT, 2 * Pout, npart,          \
                (const double*)part.p, (const int64_t*)dstoff.p,                                 \
                reinterpret_cast<double*>(io.dst));                                              \
   }
