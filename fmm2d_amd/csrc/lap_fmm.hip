@@ -576,6 +576,89 @@ __device__ __forceinline__ void loc_eval(double x, double y, int b, int nd, int 
   if (PG >= 3) ah = (2.0 * rinv * rinv) * f2;
 }
 
+// Mixed precision (nd = 1): the same evaluations from FP32 copies of the
+// expansions, in FP32 arithmetic.  The displacement from the box centre is
+// formed in FP64 and rounded once; each evaluation's result is added to the
+// point's FP64 sums.
+struct fcx {
+  float re, im;
+};
+__device__ __forceinline__ fcx fmulc(fcx a, fcx b) {
+  return fcx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ fcx faddc(fcx a, fcx b) { return fcx{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ fcx fscl(float s, fcx a) { return fcx{s * a.re, s * a.im}; }
+__device__ __forceinline__ cplx todbl(fcx a) { return cplx{(double)a.re, (double)a.im}; }
+
+// horner<PG> over interleaved FP32 coefficients a[2k], a[2k+1]
+template <int PG>
+__device__ __forceinline__ void horner32(const float* a, int kmax, bool zero0, fcx t, fcx& f,
+                                         fcx& f1, fcx& f2) {
+  f = f1 = f2 = fcx{0.f, 0.f};
+  const float2* a2 = reinterpret_cast<const float2*>(a);
+  for (int k0 = kmax; k0 >= 0; k0 -= kEvU) {
+    float2 q[kEvU];
+#pragma unroll
+    for (int u = 0; u < kEvU; ++u) q[u] = a2[max(k0 - u, 0)];
+#pragma unroll
+    for (int u = 0; u < kEvU; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        if (PG >= 3) f2 = faddc(fmulc(f2, t), f1);
+        if (PG >= 2) f1 = faddc(fmulc(f1, t), f);
+        f = faddc(fmulc(f, t), (zero0 && k == 0) ? fcx{0.f, 0.f} : fcx{q[u].x, q[u].y});
+      }
+    }
+  }
+}
+
+template <int PG>
+__device__ __forceinline__ void mp_eval32(double x, double y, int c, int P, const int* ilevel,
+                                          const int* nterms, const double* rsc,
+                                          const double* centers, const float* mp, cplx& ap,
+                                          cplx& ag, cplx& ah) {
+  const int lc = ilevel[c];
+  const int pc = nterms[lc];
+  const double rc = rsc[lc];
+  const double zx = x - centers[2 * c], zy = y - centers[2 * c + 1];
+  const double rr = zx * zx + zy * zy, rcr = rc / rr;
+  const fcx u = fcx{(float)(zx * rcr), (float)(-zy * rcr)};  // r/z
+  const float rci = (float)(1.0 / rc);
+  const float* M = mp + (int64_t)c * 2 * P;
+  const fcx m0 = fcx{M[0], M[1]};
+  fcx g, g1, g2;
+  horner32<PG>(M, pc, true, u, g, g1, g2);
+  const float lg = 0.5f * __builtin_amdgcn_logf((float)rr) * 0.69314718055994531f;
+  ap += todbl(faddc(fscl(lg, m0), g));
+  if (PG >= 2) {
+    const fcx u2 = fmulc(u, u);
+    const fcx t0 = fmulc(fscl(rci, u), m0), t1 = fmulc(fscl(rci, u2), g1);
+    ag += todbl(fcx{t0.re - t1.re, t0.im - t1.im});
+    if (PG >= 3) {
+      const fcx u3 = fmulc(u2, u);
+      const fcx a0 = fmulc(u2, m0);
+      const fcx b0 = faddc(fmulc(fmulc(u3, u), g2), fmulc(u3, g1));
+      ah += todbl(fscl(rci * rci, fcx{2.f * b0.re - a0.re, 2.f * b0.im - a0.im}));
+    }
+  }
+}
+
+template <int PG>
+__device__ __forceinline__ void loc_eval32(double x, double y, int b, int P, const int* ilevel,
+                                           const int* nterms, const double* rsc,
+                                           const double* centers, const float* loc, cplx& ap,
+                                           cplx& ag, cplx& ah) {
+  const int lev = ilevel[b];
+  const double r = rsc[lev], rinv = 1.0 / r;
+  const fcx w = fcx{(float)((x - centers[2 * b]) * rinv), (float)((y - centers[2 * b + 1]) * rinv)};
+  fcx f, f1, f2;
+  horner32<PG>(loc + (int64_t)b * 2 * P, nterms[lev], false, w, f, f1, f2);
+  ap = todbl(f);
+  ag = ah = mkc(0.0);
+  if (PG >= 2) ag = todbl(fscl((float)rinv, f1));
+  if (PG >= 3) ah = todbl(fscl((float)(2.0 * rinv * rinv), f2));
+}
+
 // Thread per point (any nd): local expansion, then the point's list-3 boxes.
 // The evaluation kernels are the first to write the sorted outputs: they
 // assign, and the near field adds after them.
@@ -610,13 +693,14 @@ __global__ void k_eval(int64_t n, int nd, int P, const double* xy, const int* le
 // scheduling).  Each thread then adds its own point's local expansion.
 constexpr int kEvB = 256;
 
-template <int PG>
+template <int PG, int F32>
 __global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double* xy,
                                                  const int* leaf, const int* ilevel,
                                                  const int* nterms, const double* rsc,
                                                  const double* centers, const int* n3,
                                                  const int* l3, const cplx* mp,
-                                                 const cplx* loc, cplx* pot, cplx* grad,
+                                                 const cplx* loc, const float* mp32,
+                                                 const float* loc32, cplx* pot, cplx* grad,
                                                  cplx* hess) {
   static_assert(kEvB == kScanThreads, "block_exclusive_scan spans the workgroup");
   __shared__ int scan_tmp[kScanWaves + 1];
@@ -651,8 +735,11 @@ __global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double*
       }
       const int bt = sb[lo];
       cplx ap = mkc(0.0), ag = mkc(0.0), ah = mkc(0.0);
-      mp_eval<PG>(sx[lo], sy[lo], l3[(int64_t)kMaxList3 * bt + (k - off[lo])], 1, 0, P, ilevel,
-                  nterms, rsc, centers, mp, ap, ag, ah);
+      const int c = l3[(int64_t)kMaxList3 * bt + (k - off[lo])];
+      if (F32)
+        mp_eval32<PG>(sx[lo], sy[lo], c, P, ilevel, nterms, rsc, centers, mp32, ap, ag, ah);
+      else
+        mp_eval<PG>(sx[lo], sy[lo], c, 1, 0, P, ilevel, nterms, rsc, centers, mp, ap, ag, ah);
       part[0][tid] = ap.re;
       part[1][tid] = ap.im;
       if (PG >= 2) {
@@ -675,7 +762,8 @@ __global__ __launch_bounds__(kEvB) void k_eval_b(int64_t n, int P, const double*
   }
   if (!live) return;
   cplx ap, ag, ah;
-  loc_eval<PG>(x, y, b, 1, 0, P, ilevel, nterms, rsc, centers, loc, ap, ag, ah);
+  if (F32) loc_eval32<PG>(x, y, b, P, ilevel, nterms, rsc, centers, loc32, ap, ag, ah);
+  else loc_eval<PG>(x, y, b, 1, 0, P, ilevel, nterms, rsc, centers, loc, ap, ag, ah);
   pot[i] = ap + cplx{my[0], my[1]};
   if (PG >= 2) grad[i] = ag + cplx{my[2], my[3]};
   if (PG >= 3) hess[i] = ah + cplx{my[4], my[5]};
@@ -1212,14 +1300,21 @@ __global__ void k_re_to_f32(int64_t n, const cplx* in, float* out) {
   if (i < n) out[i] = (float)in[i].re;
 }
 
+// mp32, loc32: FP32 copies of the expansions (nd = 1, mixed precision), or null
 void launch_eval(int pg, int64_t n, hipStream_t st, int nd, int P, const double* xy,
                  const int* leaf, const Tree& t, const int* nterms, const double* rsc,
-                 const Lists& L, const cplx* mp, const cplx* loc, cplx* p, cplx* g, cplx* h) {
+                 const Lists& L, const cplx* mp, const cplx* loc, const float* mp32,
+                 const float* loc32, cplx* p, cplx* g, cplx* h) {
+  const bool f32 = nd == 1 && mp32 && loc32;
 #define FMM2D_EVAL(PG)                                                                         \
   do {                                                                                         \
-    if (nd == 1)                                                                               \
-      launch1d_xcd(k_eval_b<PG>, n, st, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p,     \
-                   L.n3.p, L.l3.p, mp, loc, p, g, h);                                          \
+    if (f32)                                                                                   \
+      launch1d_xcd(k_eval_b<PG, 1>, n, st, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p,  \
+                   L.n3.p, L.l3.p, mp, loc, mp32, loc32, p, g, h);                             \
+    else if (nd == 1)                                                                          \
+      launch1d_xcd(k_eval_b<PG, 0>, n, st, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p,  \
+                   L.n3.p, L.l3.p, mp, loc, (const float*)nullptr, (const float*)nullptr, p,   \
+                   g, h);                                                                      \
     else                                                                                       \
       launch1d_xcd(k_eval<PG>, n, st, nd, P, xy, leaf, t.ilevel.p, nterms, rsc, t.centers.p,   \
                    L.n3.p, L.l3.p, mp, loc, p, g, h);                                          \
@@ -1442,6 +1537,13 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
                        d_op4.p, d_af, t.ilevel.p, d_band.p, d_logr.p, st);
     tm.mark(2);
     // ---- step 4: M2L, targets grouped by child position (box order within a group)
+    // (mixed precision: FP32 copies of the multipoles, for the M2L and the
+    // list-3 evaluations, and of the locals after L2L, for L2P)
+    DBuf<float> mp32, loc32;
+    if (mixed) {
+      mp32.alloc(2 * nexp, st);
+      launch1d(k_to_f32, 2 * nexp, st, reinterpret_cast<const double*>(mp.p), mp32.p);
+    }
     {
       M2LPlan m2l;
       plan_m2l(t, L, pg, pgt, m2l, st);
@@ -1451,9 +1553,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
         for (int q = 0; q < 4; ++q)
           groups.push_back(TrGroup{m2l.cls_seg[q], m2l.cls_seg[q + 1], 0, q});
         if (mixed) {
-          // FP32 operators and an FP32 copy of the multipoles
-          DBuf<float> mp32(2 * nexp, st);
-          launch1d(k_to_f32, 2 * nexp, st, reinterpret_cast<const double*>(mp.p), mp32.p);
+          // FP32 operators and the FP32 copy of the multipoles
           tm.mark(3);  // planning
           launch_translate(0, 1, nd, TrBufs{mp32.p, 0, P, loc.p, 0, P}, kSlots, groups,
                            m2l.tlist.p, L.s2.p, d_soff.p, ops.af32, t.ilevel.p, d_band.p,
@@ -1476,12 +1576,16 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
                        cp.down.p, d_op4.p, d_af, t.ilevel.p, d_band.p, d_logr.p, st);
     tm.mark(4);
     // ---- steps 6+7: list 3 and L2P
+    if (mixed && nd == 1) {
+      loc32.alloc(2 * nexp, st);
+      launch1d(k_to_f32, 2 * nexp, st, reinterpret_cast<const double*>(loc.p), loc32.p);
+    }
     if (pg > 0)
       launch_eval(pg, ns, st, nd, P, xs.p, leaf_s.p, t, d_nterms.p, d_rsc.p, L, mp.p, loc.p,
-                  ps.p, gs.p, hs.p);
+                  mp32.p, loc32.p, ps.p, gs.p, hs.p);
     if (pgt > 0)
       launch_eval(pgt, nt, st, nd, P, xt.p, leaf_t.p, t, d_nterms.p, d_rsc.p, L, mp.p, loc.p,
-                  pt.p, gt.p, ht.p);
+                  mp32.p, loc32.p, pt.p, gt.p, ht.p);
     tm.mark(6);
     // ---- step 8: near field, thresh = boxsize(0) 2^-51 (cfmm2d.f:1276)
     const int ifnear = o ? o->ifnear : 1;
