@@ -409,6 +409,22 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// FP32 complex of the mixed-precision P2L and evaluation kernels
+struct fcx {
+  float re, im;
+};
+__device__ __forceinline__ fcx fmulc(fcx a, fcx b) {
+  return fcx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ fcx faddc(fcx a, fcx b) { return fcx{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ fcx fscl(float s, fcx a) { return fcx{s * a.re, s * a.im}; }
+__device__ __forceinline__ cplx todbl(fcx a) { return cplx{(double)a.re, (double)a.im}; }
+
+// log|z| from |z|^2 in FP32 (hardware log2)
+__device__ __forceinline__ float half_log_f32(float rr) {
+  return 0.5f * __builtin_amdgcn_logf(rr) * 0.69314718055994531f;
+}
+
 // List-4 P2L (l2dformta{c,d,cd}, laprouts2d.f:501-690): the sources of a
 // box's list-4 boxes into its local expansion.  Each wavefront takes boxes of
 // `boxes` in a grid-stride loop (4 wavefronts per block, each with its own
@@ -418,10 +434,12 @@ __device__ __forceinline__ void wave_lds_sync() {
 // column n.  With u = r/(z - c) and e = -d/(z - c) (q: charge, d: Cauchy
 // dipole strength)
 //   T_0 = log|z - c| q + e,   T_n = -(1/n) (u^n q - n u^n e),
-// the 1/n applied once to the column sum.  RE: real charges.
+// the 1/n applied once to the column sum.  RE: real charges.  F32 (mixed
+// precision): the terms in FP32 (displacement formed in FP64 and rounded
+// once), rows of FP32 in LDS, each batch's column sums added to FP64 sums.
 constexpr int kP2lWaves = 4;
 
-template <int IFC, int IFD, int RE>
+template <int IFC, int IFD, int RE, int F32>
 __global__ __launch_bounds__(kP2lWaves * kW) void k_p2l_w(
     const int* boxes, const int* nboxes, int nd, int P, const int* ilevel, const int* nterms,
     const double* rsc, const double* centers, const int* n4, const int* l4, const int* sse,
@@ -429,6 +447,7 @@ __global__ __launch_bounds__(kP2lWaves * kW) void k_p2l_w(
   extern __shared__ cplx shall[];  // [kP2lWaves][bat][P]
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   cplx* sh = shall + (size_t)wv * bat * P;
+  float2* shf = reinterpret_cast<float2*>(sh);  // F32 rows, in the wave's own part
   const int nbx = *nboxes;
   for (int ib = blockIdx.x * kP2lWaves + wv; ib < nbx; ib += gridDim.x * kP2lWaves) {
     const int b = boxes[ib];
@@ -463,28 +482,63 @@ __global__ __launch_bounds__(kP2lWaves * kW) void k_p2l_w(
             q = chs[j * nd + d];
             if (RE) q.im = 0.0;
           }
-          cplx* row = sh + lane * P;
           // z0 = source - centre of a well-separated box (no overflow guards)
           const double zx = xs[2 * j] - cx, zy = xs[2 * j + 1] - cy;
-          const double rr = zx * zx + zy * zy, ri = fast_rcp(rr);
-          const cplx zi = cplx{zx * ri, -zy * ri};
-          const cplx u = r * zi;
-          if (IFD) e = -(zi * dps[j * nd + d]);
-          cplx t0 = e;
-          if (IFC) t0 += (0.5 * fast_log(rr)) * q;
-          row[0] = t0;
-          cplx v = mkc(1.0);  // u^m
-          for (int m = 1; m <= p; ++m) {
-            v = v * u;
-            cplx s = mkc(0.0);
-            if (IFC) s = RE ? cplx{v.re * q.re, v.im * q.re} : v * q;
-            if (IFD) s -= (double)m * (v * e);
-            row[m] = s;
+          if (F32) {
+            float2* row = shf + lane * P;
+            const float fx = (float)zx, fy = (float)zy;
+            const float frr = fx * fx + fy * fy, fri = __builtin_amdgcn_rcpf(frr);
+            const fcx zi = fcx{fx * fri, -fy * fri};
+            const fcx u = fscl((float)r, zi);
+            const fcx qf = fcx{(float)q.re, (float)q.im};
+            fcx ef = fcx{0.f, 0.f};
+            if (IFD) {
+              const cplx dq = dps[j * nd + d];
+              ef = fscl(-1.f, fmulc(zi, fcx{(float)dq.re, (float)dq.im}));
+            }
+            fcx t0 = ef;
+            if (IFC) t0 = faddc(t0, fscl(half_log_f32(frr), qf));
+            row[0] = float2{t0.re, t0.im};
+            fcx v = fcx{1.f, 0.f};  // u^m
+            for (int m = 1; m <= p; ++m) {
+              v = fmulc(v, u);
+              fcx s = fcx{0.f, 0.f};
+              if (IFC) s = RE ? fcx{v.re * qf.re, v.im * qf.re} : fmulc(v, qf);
+              if (IFD) s = faddc(s, fscl(-(float)m, fmulc(v, ef)));
+              row[m] = float2{s.re, s.im};
+            }
+          } else {
+            cplx* row = sh + lane * P;
+            const double rr = zx * zx + zy * zy, ri = fast_rcp(rr);
+            const cplx zi = cplx{zx * ri, -zy * ri};
+            const cplx u = r * zi;
+            if (IFD) e = -(zi * dps[j * nd + d]);
+            cplx t0 = e;
+            if (IFC) t0 += (0.5 * fast_log(rr)) * q;
+            row[0] = t0;
+            cplx v = mkc(1.0);  // u^m
+            for (int m = 1; m <= p; ++m) {
+              v = v * u;
+              cplx s = mkc(0.0);
+              if (IFC) s = RE ? cplx{v.re * q.re, v.im * q.re} : v * q;
+              if (IFD) s -= (double)m * (v * e);
+              row[m] = s;
+            }
           }
         }
         wave_lds_sync();
-        if (act)
-          for (int jj = g; jj < cnt; jj += G) acc += sh[jj * P + n];
+        if (act) {
+          if (F32) {
+            fcx a32 = fcx{0.f, 0.f};
+            for (int jj = g; jj < cnt; jj += G) {
+              const float2 t = shf[jj * P + n];
+              a32 = faddc(a32, fcx{t.x, t.y});
+            }
+            acc += todbl(a32);
+          } else {
+            for (int jj = g; jj < cnt; jj += G) acc += sh[jj * P + n];
+          }
+        }
         wave_lds_sync();
       }
       if (act) sh[g * P + n] = acc;
@@ -580,15 +634,6 @@ __device__ __forceinline__ void loc_eval(double x, double y, int b, int nd, int 
 // expansions, in FP32 arithmetic.  The displacement from the box centre is
 // formed in FP64 and rounded once; each evaluation's result is added to the
 // point's FP64 sums.
-struct fcx {
-  float re, im;
-};
-__device__ __forceinline__ fcx fmulc(fcx a, fcx b) {
-  return fcx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-__device__ __forceinline__ fcx faddc(fcx a, fcx b) { return fcx{a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ fcx fscl(float s, fcx a) { return fcx{s * a.re, s * a.im}; }
-__device__ __forceinline__ cplx todbl(fcx a) { return cplx{(double)a.re, (double)a.im}; }
 
 // horner<PG> over interleaved FP32 coefficients a[2k], a[2k+1]
 template <int PG>
@@ -1449,15 +1494,20 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
       int dev = 0, ncu = 0, per_cu = 0;
       hip_throw(hipGetDevice(&dev), "device");
       hip_throw(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "cu");
-      hip_throw(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2l_w<1, 1, 1>,
+      hip_throw(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2l_w<1, 1, 1, 0>,
                                                              kP2lWaves * kW, lds_p2x),
                 "occupancy");
       p2x_grid = std::max(1, ncu * std::max(1, per_cu));
     }
-#define FMM2D_P2L_W(C, D, RE)                                                                 \
-  hipLaunchKernelGGL((k_p2l_w<C, D, RE>), dim3(p2x_grid), dim3(kP2lWaves * kW), lds_p2x, st,  \
-                     p2x_list.p, p2x_cnt.p, nd, P, t.ilevel.p, d_nterms.p, d_rsc.p,            \
+#define FMM2D_P2L_W32(C, D, RE, F)                                                              \
+  hipLaunchKernelGGL((k_p2l_w<C, D, RE, F>), dim3(p2x_grid), dim3(kP2lWaves * kW), lds_p2x,   \
+                     st, p2x_list.p, p2x_cnt.p, nd, P, t.ilevel.p, d_nterms.p, d_rsc.p,        \
                      t.centers.p, L.n4.p, L.l4.p, sse, xs.p, chs.p, dps.p, bat_x, loc.p)
+#define FMM2D_P2L_W(C, D, RE)                  \
+  do {                                         \
+    if (mixed) FMM2D_P2L_W32(C, D, RE, 1);     \
+    else FMM2D_P2L_W32(C, D, RE, 0);           \
+  } while (0)
 #define FMM2D_P2M_REG(PT, C, D, RE)                                                           \
   launch1d_xcd(k_p2m_reg<PT, C, D, RE>, nbd, st, nd, P, t.ilevel.p, t.nchild.p, d_nterms.p,   \
                d_rsc.p, t.centers.p, sse, xs.p, chs.p, dps.p, mp.p)
@@ -1520,6 +1570,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
 #undef FMM2D_P2M_PT
 #undef FMM2D_P2M_REG
 #undef FMM2D_P2L_W
+#undef FMM2D_P2L_W32
     tm.mark(1);
     // ---- translation operators as MFMA fragments: 49 M2L offsets, 4 M2M, 4 L2L
     const int opM2M = kOff * kOff, opL2L = opM2M + 4;
