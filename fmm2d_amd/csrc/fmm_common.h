@@ -65,10 +65,22 @@ __global__ void k_scatter(int64_t n, int m, const int* perm, const cplx* in, cpl
 }
 
 
+// leaf[i] = the leaf holding sorted point i.  A wavefront takes 64 boxes and
+// writes their point ranges one box after the other with all 64 lanes
+// (consecutive lanes on consecutive points).
 __global__ void k_leaf_of(int64_t n, const int* nchild, const int* se, int* leaf) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n || nchild[b] != 0) return;
-  for (int i = se[2 * b]; i < se[2 * b + 1]; ++i) leaf[i] = (int)b;
+  const int lane = threadIdx.x & 63;
+  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  const int64_t b = b0 + lane;
+  int s = 0, e = 0;
+  if (b < n && nchild[b] == 0) {
+    s = se[2 * b];
+    e = se[2 * b + 1];
+  }
+  for (int k = 0; k < 64; ++k) {
+    const int sk = __shfl(s, k), ek = __shfl(e, k);
+    for (int i = sk + lane; i < ek; i += 64) leaf[i] = (int)(b0 + k);
+  }
 }
 
 
