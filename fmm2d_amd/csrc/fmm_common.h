@@ -78,7 +78,7 @@ __global__ void k_leaf_of(int64_t n, const int* nchild, const int* se, int* leaf
     e = se[2 * b + 1];
   }
   for (int k = 0; k < 64; ++k) {
-    const int sk = __shfl(s, k), ek = __shfl(e, k);
+    const int sk = __builtin_amdgcn_readlane(s, k), ek = __builtin_amdgcn_readlane(e, k);
     for (int i = sk + lane; i < ek; i += 64) leaf[i] = (int)(b0 + k);
   }
 }

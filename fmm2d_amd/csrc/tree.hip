@@ -732,7 +732,8 @@ __global__ __launch_bounds__(kLeafSortWaves * 64) void k_leaf_sort(const int* bi
     if (m <= 64) {
       const int v = lane < m ? perm[s + lane] : 0x7fffffff;
       int rank = 0;
-      for (int j = 0; j < m; ++j) rank += (__shfl(v, j) < v) ? 1 : 0;
+      // j is uniform: read lane j through the scalar unit (v_readlane)
+      for (int j = 0; j < m; ++j) rank += (__builtin_amdgcn_readlane(v, j) < v) ? 1 : 0;
       if (lane < m) out[s + rank] = v;
       continue;
     }
@@ -1309,7 +1310,7 @@ __device__ __forceinline__ void wave_bin_add(unsigned long long* h, bool valid, 
   unsigned long long act = __ballot(valid);
   while (act) {  // uniform
     const int leader = __ffsll((long long)act) - 1;
-    const int bl = __shfl(b, leader);
+    const int bl = __builtin_amdgcn_readlane(b, leader);
     const unsigned long long same = __ballot(valid && b == bl);
     if (lane == leader) atomicAdd(&h[bl], (unsigned long long)__popcll(same));
     act &= ~same;
