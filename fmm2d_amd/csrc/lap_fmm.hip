@@ -939,8 +939,18 @@ __device__ __forceinline__ void pair_f32(float fx, float fy, float frr, bool w,
       const float cr = w ? s.cr[d] : 0.0f;
       f.lr[d] += l2 * cr;
       if (RE) {
-        if (PG >= 2) { f.gr[d] += z1r * cr; f.gi[d] += z1i * cr; }
-        if (PG >= 3) { f.hr[d] += z2r * cr; f.hi[d] += z2i * cr; }
+        if (PG == 2) {
+          // c/z = (c / |z|^2) conj(z): one product fewer than through 1/z
+          const float cri = cr * ri;
+          f.gr[d] += fx * cri;
+          f.gi[d] -= fy * cri;
+        }
+        if (PG >= 3) {
+          f.gr[d] += z1r * cr;
+          f.gi[d] += z1i * cr;
+          f.hr[d] += z2r * cr;
+          f.hi[d] += z2i * cr;
+        }
       } else {
         const float ci = w ? s.ci[d] : 0.0f;
         f.li[d] += l2 * ci;
