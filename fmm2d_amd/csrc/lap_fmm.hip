@@ -1065,11 +1065,11 @@ constexpr int kFoldT = 32;
 template <int IFC, int IFD, int PG, int NDT, int F32, int RE>
 __global__ __launch_bounds__(256) void k_near(int64_t n, int nd, int d0, const double* xy,
                                               const int* leaf, const int2* seg, int self,
-                                              const double* xs, const cplx* chs,
-                                              const cplx* dps, const float* ch32,
-                                              const float* dp32, double thresh, float lo,
-                                              int* slow, int* nslow, cplx* pot, cplx* grad,
-                                              cplx* hess) {
+                                              const double* __restrict__ xs, const cplx* chs,
+                                              const cplx* dps, const float* __restrict__ ch32,
+                                              const float* __restrict__ dp32, double thresh,
+                                              float lo, int* slow, int* nslow, cplx* pot,
+                                              cplx* grad, cplx* hess) {
   const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = leaf[i];
@@ -1087,7 +1087,28 @@ __global__ __launch_bounds__(256) void k_near(int64_t n, int nd, int d0, const d
   for (int t = 0; t < T;) {
     const int j = sc.j;
     const double* px = xs + 2 * (int64_t)j;
-    if (wave_all(sc.e - j >= kNearU)) {
+    const int j0 = __builtin_amdgcn_readfirstlane(j);
+    if (F32 && wave_all(j == j0 && sc.e - j >= kNearU)) {
+      // every lane at the same source (a wavefront within one leaf): the
+      // source loads are wave-uniform and go through the scalar unit instead
+      // of every lane's vector load
+      const double* p0 = xs + 2 * (int64_t)j0;
+      double sx[kNearU], sy[kNearU];
+      StrF<NDT> sf[kNearU];
+#pragma unroll
+      for (int u = 0; u < kNearU; ++u) {
+        sx[u] = p0[2 * u];
+        sy[u] = p0[2 * u + 1];
+        load_strf<IFC, IFD, NDT, RE>(j0 + u, nd, d0, ch32, dp32, sf[u]);
+      }
+      const int ds = iself - j0;
+#pragma unroll
+      for (int u = 0; u < kNearU; ++u)
+        any_slow |= near_pair32<IFC, IFD, PG, NDT, RE>(x - sx[u], y - sy[u], ds != u, sf[u], lo,
+                                                       f);
+      sc.j = j + kNearU;
+      t += kNearU;
+    } else if (wave_all(sc.e - j >= kNearU)) {
       if (F32) {
         double sx[kNearU], sy[kNearU];
         StrF<NDT> sf[kNearU];
