@@ -66,12 +66,14 @@ namespace {
 // has zero imaginary part, the imaginary Cauchy densities vanish identically
 // and only the real parts are kept (Cauchy density d).
 // (with xy given, the point coordinates are gathered into oxy in the same pass)
+// inv (if given) receives the inverse permutation: inv[perm[i]] = i.
 __global__ void k_gather_split(int64_t n, int nd, int half, const int* perm, const cplx* ch,
                                const cplx* ds, const double* dv, cplx* och, cplx* ods,
-                               const double* xy, double* oxy) {
+                               const double* xy, double* oxy, int* inv) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t j = perm[i];
+  if (inv) inv[j] = (int)i;
   if (xy) {
     oxy[2 * i] = xy[2 * j];
     oxy[2 * i + 1] = xy[2 * j + 1];
@@ -150,6 +152,37 @@ __global__ void k_scatter_unpack(int64_t n, int nd, int half, int pg, const int*
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t j = perm[i];
+  const int np = half ? 1 : 2;
+  const cplx z = mkc(0.0);
+  for (int d = 0; d < nd; ++d) {
+    const int64_t a = i * np * nd + np * d;
+    const cplx pr = p1[a], pi = half ? z : p1[a + 1];
+    pot[j * nd + d] = cplx{pr.re, pi.re};
+    if (pg >= 2) {
+      const cplx gr = g1[a], gi = half ? z : g1[a + 1];
+      grad[nd * (2 * j) + d] = cplx{gr.re, gi.re};
+      grad[nd * (2 * j + 1) + d] = cplx{-gr.im, -gi.im};
+    }
+    if (pg >= 3) {
+      const cplx hr = h1[a], hi = half ? z : h1[a + 1];
+      const cplx hxx = cplx{hr.re, hi.re};
+      hess[nd * (3 * j) + d] = hxx;
+      hess[nd * (3 * j + 1) + d] = cplx{-hr.im, -hi.im};
+      hess[nd * (3 * j + 2) + d] = -hxx;
+    }
+  }
+}
+
+// k_scatter_unpack as a gather: thread k takes caller's point k from sorted
+// position inv[k], so the outputs are written in the caller's order
+// (consecutive threads, consecutive addresses) and the sorted outputs are read
+// at scattered positions.  Same values as k_scatter_unpack.
+__global__ void k_gather_unpack(int64_t n, int nd, int half, int pg, const int* inv,
+                                const cplx* p1, const cplx* g1, const cplx* h1, cplx* pot,
+                                cplx* grad, cplx* hess) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int64_t i = inv[j];
   const int np = half ? 1 : 2;
   const cplx z = mkc(0.0);
   for (int d = 0; d < nd; ++d) {
@@ -1472,6 +1505,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
     // ---- sort and gather
     DBuf<double> xs(2 * ns, st), xt(2 * std::max<int64_t>(nt, 1), st);
     DBuf<cplx> chs(a.ifcharge ? ns * nd : 1, st), dps(a.ifdipole ? ns * nd : 1, st);
+    DBuf<int> invs;  // lfmm2d: sorted position of each caller's source
     if (kind != 1) launch1d(k_gather_xy, ns, st, t.isrc.p, a.src, xs.p);
     if (nt > 0) launch1d(k_gather_xy, nt, st, t.itarg.p, a.targ, xt.p);
     if (kind == 0) {
@@ -1483,8 +1517,10 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
                a.ifdipole ? reinterpret_cast<const double*>(a.dipstr) : nullptr, a.dipvec,
                chs.p, dps.p);
     } else {
+      if (a.ifpgh > 0) invs.alloc(ns, st);
       launch1d(k_gather_split, ns, st, a.nd, half, t.isrc.p, a.ifcharge ? a.charge : nullptr,
-               a.ifdipole ? a.dipstr : nullptr, a.dipvec, chs.p, dps.p, a.src, xs.p);
+               a.ifdipole ? a.dipstr : nullptr, a.dipvec, chs.p, dps.p, a.src, xs.p,
+               a.ifpgh > 0 ? invs.p : nullptr);
     }
     DBuf<int> leaf_s(ns, st), leaf_t(std::max<int64_t>(nt, 1), st);
     launch1d(k_leaf_of, nb, st, t.nchild.p, t.isrcse.p, leaf_s.p);
@@ -1721,7 +1757,7 @@ int laplace_fmm(int kind, const CauchyArgs& a, hipStream_t st) {
                  reinterpret_cast<double*>(a.hesstarg));
     } else {
       if (pg >= 1)
-        launch1d(k_scatter_unpack, ns, st, a.nd, half, pg, t.isrc.p, ps.p, gs.p, hs.p, a.pot, a.grad,
+        launch1d(k_gather_unpack, ns, st, a.nd, half, pg, invs.p, ps.p, gs.p, hs.p, a.pot, a.grad,
                  a.hess);
       if (pgt >= 1)
         launch1d(k_scatter_unpack, nt, st, a.nd, half, pgt, t.itarg.p, pt.p, gt.p, ht.p, a.pottarg,
