@@ -47,7 +47,7 @@ PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
 # timeinfo slots of fmm2d_opts (include/fmm2d_amd.h)
 STAGES = {0: "p2m", 1: "list4", 2: "m2m", 3: "m2l plan", 13: "m2l kernel", 4: "l2l",
           6: "l2p+list3", 15: "near prep", 7: "near", 8: "tree+lists", 9: "sort+gather",
-          10: "scatter"}
+          10: "unsort"}
 M2L_SLOT, NEAR_SLOT, TREE_SLOT = 13, 7, 8
 # blocks of the learned steps' acceptance-test direct sums (16 per XCD), which run
 # on a second stream beside the FMM
