@@ -446,6 +446,84 @@ def c5_points(n, seed=5):
     return np.ascontiguousarray(pts.T), ch
 
 
+def main_direct(args):
+    """Secondary workload (not the headline metric): the direct evaluators
+    (f2py l2ddir and h2ddir, laplace/lapkernels2d.f, helmholtz/helmkernels2d.f)
+    on device-resident inputs: l2d with 1e5 sources x 1e5 targets, charges,
+    pot+grad (1e10 pairs); h2d at zk = 100 with 2e4 x 2e4 (4e8 pairs).  Rate =
+    source-target pairs per second from HIP events around the call; roofline
+    against the FP64 vector peak with algorithmic flops per pair (a
+    transcendental counted as one flop): l2d 20 (displacement 2, |z|^2 3, log
+    1 and its scaling 1, 1/z 4, complex charge times log and 1/z 8, sums 1),
+    h2d 40 (displacement and |z| 6, H0 and H1 counted 2, 1/|z| scaling 8,
+    products and sums 24).  CPU baseline: the reference's own l2d_directcg /
+    h2d_directcg (oracle/_ref, one core) on the same sources and the first 4000
+    (l2d) or 2000 (h2d) targets."""
+    import torch
+    from fmm2d_amd import _native as nat
+    L = nat.lib()
+    dev = nat.torch_device()
+    rng = np.random.default_rng(11)
+    res = {}
+    for kind, n, zk, fpp in (("l", 100000, None, 20.0), ("h", 20000, 100.0, 40.0)):
+        s_ = rng.random((n, 2))
+        t_ = rng.random((n, 2))
+        q_ = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        src, trg = torch.from_numpy(s_).to(dev), torch.from_numpy(t_).to(dev)
+        chg = torch.from_numpy(q_).to(dev)
+        pot = torch.zeros(n, dtype=torch.complex128, device=dev)
+        grad = torch.zeros((n, 2), dtype=torch.complex128, device=dev)
+
+        def call():
+            if kind == "l":
+                return L.fmm2d_l2d_direct(1, nat.ptr(src), n, 1, nat.ptr(chg), 0, None, None,
+                                          nat.ptr(trg), n, 2, nat.ptr(pot), nat.ptr(grad), None,
+                                          1e-16, nat.stream_ptr())
+            return L.fmm2d_h2d_direct(1, zk, 0.0, nat.ptr(src), n, 1, nat.ptr(chg), 0, None, None,
+                                      nat.ptr(trg), n, 2, nat.ptr(pot), nat.ptr(grad), None,
+                                      1e-16, nat.stream_ptr())
+        for _ in range(args.warmup):
+            assert call() == 0
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            assert call() == 0
+        e1.record()
+        torch.cuda.synchronize()
+        dt = e0.elapsed_time(e1) * 1e-3 / args.steps
+        pairs = float(n) * n
+        tf = pairs * fpp / dt / 1e12
+        cpu = None
+        if not args.no_cpu:
+            from oracle import ref_binding as R
+            m = 4000 if kind == "l" else 2000
+            t0 = time.perf_counter()
+            R.direct(kind, 1, s_.T.copy(), t_[:m].T.copy(), charges=q_[None, :], pgt=2,
+                     zk=zk)
+            secs = time.perf_counter() - t0
+            cpu = {"value": round(float(n) * m / secs, 1), "unit": "pairs/s", "cores": 1,
+                   "kind": "reference", "sample": f"reference {kind}2d_directcg (oracle/_ref) "
+                   f"on {n} sources x {m} targets, {secs:.2f} s"}
+        res[kind] = {"pairs": pairs, "ms": round(dt * 1e3, 4), "value": round(pairs / dt, 1),
+                     "roofline": {"bound": "valu", "achieved": round(tf, 3),
+                                  "peak": PEAK["fp64_tflops"], "unit": "TFLOP/s",
+                                  "frac": round(tf / PEAK["fp64_tflops"], 4), "traffic": None,
+                                  "flops_per_pair": fpp},
+                     "cpu_baseline": cpu}
+        print(f"bench[direct/{kind}]: {n} x {n}: {dt*1e3:.2f} ms, {pairs/dt:.3e} pairs/s, "
+              f"{tf:.2f} TFLOP/s", file=sys.stderr, flush=True)
+    r = res["l"]
+    print(json.dumps({
+        "metric": "l2ddir source-target pairs/sec, 1e5 x 1e5, charges, pot+grad (secondary)",
+        "value": r["value"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "l2ddir 1e5 sources x 1e5 targets uniform in [0,1]^2, complex "
+                               "charges, pot+grad; h2ddir zk=100 2e4 x 2e4 alongside"},
+        "roofline": r["roofline"], "cpu_baseline": r["cpu_baseline"], "h2ddir": res["h"]}))
+
+
 def main_c5(args):
     """The learned proposer's evaluation configuration: hfmm2d, N = 2e6 points on a
     curve plus clusters, zk = 500, eps = 1e-6, pot+grad at the sources.  Pipelines:
@@ -588,7 +666,8 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run one elementwise copy kernel over MIB MiB (reads and "
                          "writes MIB MiB), the reference kernel for scaling HBM counters")
-    ap.add_argument("--workload", default="lfmm2d", choices=["lfmm2d", "hfmm2d", "batch", "c5"],
+    ap.add_argument("--workload", default="lfmm2d",
+                    choices=["lfmm2d", "hfmm2d", "batch", "c5", "direct"],
                     help="lfmm2d: the headline workload; hfmm2d: the secondary Helmholtz one; "
                          "batch: many independent small FMMs through fmm2d_batch")
     ap.add_argument("--batch", type=int, default=2048, help="instances of the batch workload")
@@ -606,6 +685,8 @@ def main():
         return main_batch(args)
     if args.workload == "c5":
         return main_c5(args)
+    if args.workload == "direct":
+        return main_direct(args)
     import ctypes as C
     import torch
     from fmm2d_amd import _native as nat
