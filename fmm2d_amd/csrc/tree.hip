@@ -159,7 +159,10 @@ __global__ void k_bbox_final(const double* part, int nb, double* out) {
 // refined, so it is computed once per point; digits are child indices 0..3 in
 // the reference's order (1=(-,-), 2=(+,-), 3=(-,+), 4=(+,+)), most significant
 // digit = level 1.  Levels 1..20 go to key1 (a 40-bit radix sort), levels
-// 21..51 to key2 (only computed when the tree is that deep).
+// 21..51 to key2 (only computed when the tree is that deep).  x - cx < 0 is
+// tested as x < cx: with gradual underflow the difference of two distinct
+// finite doubles is never rounded to zero and keeps its sign, and both forms
+// are false for NaN.
 constexpr int kD1 = 20;
 constexpr int kD2Max = 31;
 
@@ -172,7 +175,7 @@ __global__ void k_path_keys(int64_t n, const double* __restrict__ xy, double cx0
   double cx = cx0, cy = cy0;
   unsigned long long k = 0;
   for (int l = 1; l <= d1; ++l) {
-    const int qx = (x - cx < 0.0) ? 0 : 1, qy = (y - cy < 0.0) ? 0 : 1;
+    const int qx = (x < cx) ? 0 : 1, qy = (y < cy) ? 0 : 1;
     k = (k << 2) | (unsigned long long)(2 * qy + qx);
     const double h = half[l];
     cx += qx ? h : -h;
@@ -182,7 +185,7 @@ __global__ void k_path_keys(int64_t n, const double* __restrict__ xy, double cx0
   if (d2 > 0) {
     k = 0;
     for (int l = d1 + 1; l <= d1 + d2; ++l) {
-      const int qx = (x - cx < 0.0) ? 0 : 1, qy = (y - cy < 0.0) ? 0 : 1;
+      const int qx = (x < cx) ? 0 : 1, qy = (y < cy) ? 0 : 1;
       k = (k << 2) | (unsigned long long)(2 * qy + qx);
       const double h = half[l];
       cx += qx ? h : -h;
